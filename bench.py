@@ -1,0 +1,192 @@
+"""Benchmark of the WCSPH dam-break step on one MI355X.
+
+Workload (the project's WCSPH dam-break configuration): a 0.4 x 0.4 x 0.2 m
+water column at dx = 5 mm (80 x 80 x 40 = 256,000 fluid particles) in a
+1.6 x 0.4 x 0.6 m open tank with a 3-layer dummy shell, Wendland C2 with
+h = 1.3 dx, rho0 = 1000, Tait EOS gamma = 7, c0 = 10 sqrt(2 g H), Riemann
+pressure with limiter, dt = 0.2 h / c0.  One step is one iteration of the
+solver loop: hash, sort, reorder, cell ranges, neighbour lists, then the
+three pair passes (momentum + kick/drift, continuity + EOS, momentum + kick).
+All state is resident in HBM before the timed region starts.
+
+Prints one JSON line:
+  value      = fluid particle-steps per second over the timed steps
+  roofline   = the dominant kernel's algorithmic bytes / its mean launch time
+               (HIP events on the scene stream, inside the timed region)
+  cpu_baseline = the FP64 oracle on the host cores over a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> float:
+    """Compulsory HBM bytes of one launch (each array once per launch;
+    neighbour re-reads served by caches are not counted).  Layout: position
+    3 x f64, velocity 3 x f64, drho/rho/p f32, phase u8, ELL list 4 B/entry,
+    counts 4 B."""
+    state_in = 24 + 24 + 8 + 1          # x, v, rho+p, phase
+    lists = 4 * (pairs_ff + pairs_fd) + 8 * nf
+    if kernel == "sph_momentum_first":
+        return nf * (state_in + 24 + 24) + lists       # + write v, x
+    if kernel == "sph_momentum_second":
+        return nf * (state_in + 24) + lists            # + write v
+    if kernel == "sph_continuity":
+        return nf * (state_in + 4 + 12) + lists        # + read drho, write drho/rho/p
+    if kernel == "sph_lists":
+        return nf * (24 + 1) + lists                   # read x, phase; write lists + counts
+    if kernel == "sph_reorder":
+        return nf * 2 * (24 + 24 + 4 + 1 + 4) + nf * (4 + 4 + 8 + 12)
+    return 0.0
+
+
+def algorithmic_flops(kernel: str, pairs_ff: int, pairs_fd: int) -> float:
+    """FP32 flops of a pair pass (add/mul/sub = 1, fma = 2, div/sqrt = 1):
+    momentum ~100 per pair, continuity ~75 per pair."""
+    if kernel.startswith("sph_momentum"):
+        return 100.0 * (pairs_ff + pairs_fd)
+    if kernel == "sph_continuity":
+        return 75.0 * (pairs_ff + pairs_fd)
+    return 0.0
+
+
+def cpu_baseline(spec, budget_s: float = 15.0) -> dict:
+    import oracle
+
+    t0 = time.perf_counter()
+    o = oracle.OracleSPH(spec)
+    o.step(spec.dt, 1)  # warm-up (first allocation of the lists)
+    steps = 0
+    t1 = time.perf_counter()
+    while True:
+        o.step(spec.dt, 1)
+        steps += 1
+        el = time.perf_counter() - t1
+        if el >= budget_s or steps >= 50:
+            break
+    threads = int(oracle.lib().o_num_threads())
+    o.close()
+    return {"value": spec.nf * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (C, OpenMP) on the same {spec.nf}-fluid dam break, {steps} steps after "
+                      f"1 warm-up step ({el:.1f} s; setup {t1 - t0:.1f} s excluded)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dx", type=float, default=0.005)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build measures the single-GPU kernel "
+              f"(--gpus {args.gpus} requested; multi-GPU sharding is a design, not a run)", file=sys.stderr)
+        return 2
+
+    import this_studio_amd as ts
+    from this_studio_amd import scenes
+
+    spec = scenes.dam_break(dx=args.dx)
+    s = ts.SPHScene()
+    s.set_eos(spec.eos, spec.gamma)
+    s.add_fluid(spec.fluid_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
+    s.add_dummy(spec.dummy_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
+    s.initialize(spec.min_bound, spec.max_bound, spec.gravity)
+    print(f"[bench] wcsph dam break: {spec.nf} fluid, {spec.nd} dummies, dt={spec.dt:.3e}", file=sys.stderr,
+          flush=True)
+    s.step(spec.dt, args.warmup)
+    s.synchronize()
+    print(f"[bench] warm-up done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    s.enable_timing(True)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(spec.dt, args.steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    timing = s.timing()
+    s.enable_timing(False)
+    cnt_ff = s.get("neighborCount")
+    cnt_fd = s.get("dummyNeighborCount")
+    pairs_ff, pairs_fd = int(cnt_ff.sum()), int(cnt_fd.sum())
+    print(f"[bench] timed {args.steps} steps in {el:.3f} s; pairs ff={pairs_ff} fd={pairs_fd}", file=sys.stderr,
+          flush=True)
+
+    nf = spec.nf
+    value = nf * args.steps / el
+    kernels = {}
+    for name, (launches, total_ms) in timing.items():
+        avg_ms = total_ms / max(launches, 1)
+        b = algorithmic_bytes(name, nf, pairs_ff, pairs_fd)
+        fl = algorithmic_flops(name, pairs_ff, pairs_fd)
+        kernels[name] = {"launches": launches, "avg_us": avg_ms * 1e3,
+                         "share": total_ms / (el * 1e3),
+                         "GBps": (b / (avg_ms * 1e-3) / 1e9) if b else None,
+                         "TFLOPs": (fl / (avg_ms * 1e-3) / 1e12) if fl else None}
+        print(f"[bench] {name:22s} {launches:6d} x {avg_ms * 1e3:9.1f} us  "
+              f"share {kernels[name]['share'] * 100:5.1f}%  "
+              f"{(kernels[name]['GBps'] or 0):8.1f} GB/s  {(kernels[name]['TFLOPs'] or 0):6.2f} TF/s",
+              file=sys.stderr, flush=True)
+    dom = max(timing, key=lambda k: timing[k][1])
+    d_launches, d_total = timing[dom]
+    d_avg_s = d_total / d_launches * 1e-3
+    d_bytes = algorithmic_bytes(dom, nf, pairs_ff, pairs_fd)
+    achieved = d_bytes / d_avg_s / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "avg_launch_us": round(d_avg_s * 1e6, 2), "algorithmic_bytes": int(d_bytes)}
+    fl = algorithmic_flops(dom, pairs_ff, pairs_fd)
+    if fl:
+        roofline["fp32_valu_tflops"] = round(fl / d_avg_s / 1e12, 2)
+        roofline["fp32_valu_frac"] = round(fl / d_avg_s / 1e12 / FP32_PEAK_TFLOPS, 4)
+
+    # whole-step algorithmic HBM rate (sum over the step's kernels)
+    step_bytes = sum(algorithmic_bytes(k, nf, pairs_ff, pairs_fd) * timing[k][0] for k in timing) / args.steps
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(spec, args.cpu_budget)
+    out = {
+        "metric": "particle-steps/s",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "wcsph_dam_break", "fluid_particles": nf, "dummy_particles": spec.nd,
+                   "dx": spec.dx, "h": 1.3 * spec.dx, "eos": "tait_gamma7", "c0": round(spec.c0, 4),
+                   "dt": spec.dt, "pairs_ff": pairs_ff, "pairs_fd": pairs_fd,
+                   "step_hbm_GBps": round(step_bytes / (el / args.steps) / 1e9, 1),
+                   "precision": "fp32 pair arithmetic, fp64 position/velocity, fp32 drho"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

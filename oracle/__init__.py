@@ -1,0 +1,196 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product path never loads
+it.  Build with `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing; run `make -C oracle`")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+class Grid(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("minx", "miny", "minz", "maxx", "maxy", "maxz", "cx", "cy", "cz")] + \
+               [(n, C.c_int) for n in ("nx", "ny", "nz")]
+
+    @property
+    def num_grids(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+def _declare(L):
+    L.o_grid_set.argtypes = [C.POINTER(Grid), _dp, _dp, C.c_double]
+    L.o_grid_set.restype = C.c_int
+    L.o_hash.argtypes = [C.POINTER(Grid), _dp, C.c_int, _ip]
+    L.o_sort_by_hash.argtypes = [_ip, C.c_int, C.c_int, _ip, _ip]
+    L.o_cell_ranges.argtypes = [_ip, C.c_int, C.c_int, _ip, _ip]
+    L.o_stencil.argtypes = [C.POINTER(Grid), _dp, _ip, _ip, _ip]
+    L.o_sph_count.argtypes = [C.POINTER(Grid), _dp, _dp, C.c_int, _dp, _dp, _ip, _ip, _ip, C.c_int, _ip]
+    L.o_sph_count.restype = C.c_long
+    L.o_sph_write.argtypes = [C.POINTER(Grid), _dp, _dp, C.c_int, _dp, _dp, _ip, _ip, _ip, C.c_int, _ip, _ip, _ip]
+    L.o_grad_wendland.argtypes = [_dp, C.c_double, _dp]
+    L.o_wendland.argtypes = [C.c_double, C.c_double]
+    L.o_wendland.restype = C.c_double
+    L.o_sph_create.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                               _dp, _dp, _dp, C.c_int, C.c_double, C.c_int]
+    L.o_sph_create.restype = _vp
+    L.o_sph_destroy.argtypes = [_vp]
+    L.o_sph_initialize.argtypes = [_vp]
+    L.o_sph_initialize.restype = C.c_int
+    L.o_sph_neighbor_search.argtypes = [_vp]
+    L.o_sph_first_half.argtypes = [_vp, C.c_double]
+    L.o_sph_second_half.argtypes = [_vp, C.c_double]
+    L.o_sph_step.argtypes = [_vp, C.c_double, C.c_int]
+    L.o_sph_get.argtypes = [_vp, C.c_char_p, _vp]
+    L.o_sph_get.restype = C.c_long
+    L.o_sph_set.argtypes = [_vp, C.c_char_p, _vp]
+    L.o_sph_set.restype = C.c_int
+    L.o_sph_grid.argtypes = [_vp, C.POINTER(Grid)]
+    L.o_num_threads.restype = C.c_int
+
+
+def d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_dp)
+
+
+def i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def grid_set(min_bound, max_bound, cell: float) -> Grid:
+    g = Grid()
+    mn = np.ascontiguousarray(min_bound, dtype=np.float64)
+    mx = np.ascontiguousarray(max_bound, dtype=np.float64)
+    if lib().o_grid_set(C.byref(g), d(mn), d(mx), float(cell)) != 0:
+        raise ValueError("degenerate grid")
+    return g
+
+
+def hash_points(g: Grid, pos: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(pos.shape[0], dtype=np.int32)
+    lib().o_hash(C.byref(g), d(pos), pos.shape[0], i(out))
+    return out
+
+
+def sort_by_hash(h: np.ndarray, num_grids: int):
+    h = np.ascontiguousarray(h, dtype=np.int32)
+    sh = np.empty_like(h)
+    si = np.empty_like(h)
+    lib().o_sort_by_hash(i(h), h.size, int(num_grids), i(sh), i(si))
+    return sh, si
+
+
+def cell_ranges(sh: np.ndarray, num_grids: int):
+    s = np.empty(num_grids, dtype=np.int32)
+    e = np.empty(num_grids, dtype=np.int32)
+    lib().o_cell_ranges(i(np.ascontiguousarray(sh, dtype=np.int32)), sh.size, int(num_grids), i(s), i(e))
+    return s, e
+
+
+def grad_wendland(rij, h: float) -> np.ndarray:
+    out = np.zeros(3)
+    r = np.ascontiguousarray(rij, dtype=np.float64)
+    lib().o_grad_wendland(r.ctypes.data_as(_dp), float(h), out.ctypes.data_as(_dp))
+    return out
+
+
+def wendland(r: float, h: float) -> float:
+    return lib().o_wendland(float(r), float(h))
+
+
+SPH_INT = {"hash", "index", "dindex", "cellStart", "cellEnd", "count_ff", "count_fd", "count_dd",
+           "pointing_ff", "pointing_fd", "pointing_dd"}
+
+
+class OracleSPH:
+    """The oracle's WCSPHSolver (WCSPHSolver.h:18-636) in FP64."""
+
+    def __init__(self, spec, nthreads: int = 0):
+        nf, nd = spec.nf, spec.nd
+        self._keep = []
+
+        def arr(x, n, comps=1):
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (n, comps) if comps > 1 else (n,)))
+            a = np.ascontiguousarray(a.reshape(-1))
+            self._keep.append(a)
+            return d(a)
+
+        fv = spec.fluid_vel if spec.fluid_vel is not None else np.zeros((nf, 3))
+        dv = spec.dummy_vel if spec.dummy_vel is not None else np.zeros((nd, 3))
+        self.h = lib().o_sph_create(
+            nf, arr(spec.fluid_pos, nf, 3), arr(fv, nf, 3), arr(spec.c0, nf), arr(spec.dx, nf), arr(spec.rho0, nf),
+            arr(spec.nu, nf), nd, arr(spec.dummy_pos, max(nd, 1), 3) if nd else None, arr(dv, max(nd, 1), 3) if nd else None,
+            arr(spec.c0, max(nd, 1)), arr(spec.dx, max(nd, 1)), arr(spec.rho0, max(nd, 1)), arr(spec.nu, max(nd, 1)),
+            arr(spec.min_bound, 3), arr(spec.max_bound, 3), arr(spec.gravity, 3), int(spec.eos), float(spec.gamma),
+            int(nthreads))
+        if lib().o_sph_initialize(self.h) != 0:
+            raise ValueError("oracle initialize failed")
+
+    def close(self):
+        if self.h:
+            lib().o_sph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def grid(self) -> Grid:
+        g = Grid()
+        lib().o_sph_grid(self.h, C.byref(g))
+        return g
+
+    def neighbor_search(self):
+        lib().o_sph_neighbor_search(self.h)
+
+    def first_half(self, dt):
+        lib().o_sph_first_half(self.h, float(dt))
+
+    def second_half(self, dt):
+        lib().o_sph_second_half(self.h, float(dt))
+
+    def step(self, dt, n=1):
+        lib().o_sph_step(self.h, float(dt), int(n))
+
+    def get(self, name: str) -> np.ndarray:
+        n = lib().o_sph_get(self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.int32 if name in SPH_INT else np.float64)
+        lib().o_sph_get(self.h, name.encode(), out.ctypes.data_as(_vp))
+        if name in ("pos", "vel", "acc", "dnormal"):
+            out = out.reshape(-1, 3)
+        return out
+
+    def set(self, name: str, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if lib().o_sph_set(self.h, name.encode(), v.ctypes.data_as(_vp)) != 0:
+            raise KeyError(name)

@@ -1,0 +1,190 @@
+"""GPU parity of the WCSPH path (libdemsph.so on the MI355X) against the
+FP64 oracle and the reference's own output frames.
+
+Bit-exact: cell hashes, sort permutation, cell ranges, neighbour counts and
+pair lists (as the reference's objectPointing arrays, in its order).
+Tolerances (FP32 pair arithmetic, FP64 integration) are stated per test.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+import this_studio_amd as ts
+from this_studio_amd import scenes
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def make_gpu(spec, capacity=None):
+    s = ts.SPHScene()
+    s.set_eos(spec.eos, spec.gamma)
+    if capacity:
+        s.set_capacity(*capacity)
+    s.add_fluid(spec.fluid_pos, spec.fluid_vel, spec.c0, spec.dx, spec.rho0, spec.nu)
+    if spec.nd:
+        s.add_dummy(spec.dummy_pos, spec.dummy_vel, spec.c0, spec.dx, spec.rho0, spec.nu)
+    s.initialize(spec.min_bound, spec.max_bound, spec.gravity)
+    return s
+
+
+def test_build_cells_raw_matches_oracle():
+    import torch
+
+    rng = np.random.default_rng(11)
+    n = 50000
+    pos = rng.uniform(-0.1, 1.1, (n, 3))  # some points outside the domain exercise the clamps
+    g = oracle.grid_set([0, 0, 0], [1.0, 0.7, 0.4], 0.021)
+    ng = g.num_grids
+    hv = oracle.hash_points(g, pos)
+    sh, si = oracle.sort_by_hash(hv, ng)
+    cs, ce = oracle.cell_ranges(sh, ng)
+    dev = torch.device("cuda:0")
+    dpos = torch.from_numpy(pos.reshape(-1)).to(dev)
+    hidx = torch.empty(n, dtype=torch.int32, device=dev)
+    hval = torch.empty(n, dtype=torch.int32, device=dev)
+    dcs = torch.empty(ng, dtype=torch.int32, device=dev)
+    dce = torch.empty(ng, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    ts.build_cells(dpos.data_ptr(), hidx.data_ptr(), hval.data_ptr(), dcs.data_ptr(), dce.data_ptr(),
+                   [g.minx, g.miny, g.minz], [g.maxx, g.maxy, g.maxz], [g.cx, g.cy, g.cz], [g.nx, g.ny, g.nz], ng, n)
+    torch.cuda.synchronize()
+    assert np.array_equal(hval.cpu().numpy(), sh)
+    assert np.array_equal(hidx.cpu().numpy(), si)
+    assert np.array_equal(dcs.cpu().numpy(), cs)
+    assert np.array_equal(dce.cpu().numpy(), ce)
+
+
+@pytest.fixture(scope="module")
+def small():
+    spec = scenes.small_dam_break()
+    return spec
+
+
+def test_neighbor_search_bit_exact(small):
+    o = oracle.OracleSPH(small)
+    s = make_gpu(small)
+    o.neighbor_search()
+    s.neighbor_search()
+    assert np.array_equal(s.get("hashValue"), np.sort(o.get("hash"), kind="stable"))
+    assert np.array_equal(s.get("hashIndex"), o.get("index"))
+    assert np.array_equal(s.get("cellHashStart"), o.get("cellStart"))
+    assert np.array_equal(s.get("cellHashEnd"), o.get("cellEnd"))
+    assert np.array_equal(s.get("neighborCount"), o.get("count_ff"))
+    assert np.array_equal(s.get("objectPointing"), o.get("pointing_ff"))
+    assert np.array_equal(s.get("dummyNeighborCount"), o.get("count_fd"))
+    assert np.array_equal(s.get("dummyObjectPointing"), o.get("pointing_fd"))
+
+
+def test_dummy_normals_match_oracle(small):
+    o = oracle.OracleSPH(small)
+    s = make_gpu(small)
+    # normals are stored in FP32: 1e-6 absolute on unit vectors
+    np.testing.assert_allclose(s.get("dummyNormal"), o.get("dnormal"), atol=1e-6)
+
+
+def _compare(s, o, pos_tol, vel_tol, rho_tol):
+    dp = np.abs(s.get("position") - o.get("pos")).max()
+    dv = np.abs(s.get("velocity") - o.get("vel")).max()
+    dr = np.abs(s.get("density") - o.get("rho")).max()
+    assert dp < pos_tol, dp
+    assert dv < vel_tol, dv
+    assert dr < rho_tol, dr
+    return dp, dv, dr
+
+
+def test_one_step_matches_oracle(small):
+    o = oracle.OracleSPH(small)
+    s = make_gpu(small)
+    o.step(small.dt, 1)
+    s.step(small.dt, 1)
+    # one step: FP32 pair sums against FP64 -> |dv| <= 1e-6 m/s, |dx| <= dt*|dv|
+    _compare(s, o, 1e-9, 1e-6, 1e-4)
+
+
+def test_split_halves_equal_fused_step(small):
+    a = make_gpu(small)
+    b = make_gpu(small)
+    for _ in range(3):
+        a.neighbor_search()
+        a.first_half(small.dt)
+        a.second_half(small.dt)
+    b.step(small.dt, 3)
+    assert np.array_equal(a.get("position"), b.get("position"))
+    assert np.array_equal(a.get("velocity"), b.get("velocity"))
+
+
+def test_200_steps_match_oracle(small):
+    o = oracle.OracleSPH(small)
+    s = make_gpu(small)
+    o.step(small.dt, 200)
+    s.step(small.dt, 200)
+    # 200 steps (0.037 s of collapse): relative to dx = 1e-2 m, <= 1e-5
+    dp, dv, dr = _compare(s, o, 1e-7, 1e-4, 1e-2)
+    assert dp / small.dx < 1e-5
+
+
+def test_tait_eos_matches_oracle():
+    spec = scenes.small_dam_break(eos=1)
+    o = oracle.OracleSPH(spec)
+    s = make_gpu(spec)
+    o.step(spec.dt, 100)
+    s.step(spec.dt, 100)
+    _compare(s, o, 1e-7, 1e-4, 1e-2)
+    p_o, p_s = o.get("p"), s.get("pressure")
+    assert np.abs(p_s - p_o).max() < 1e-3 * max(1.0, np.abs(p_o).max())
+
+
+def test_list_overflow_replays_exactly(small):
+    a = make_gpu(small)
+    b = make_gpu(small, capacity=(4, 2))  # forces overflow, regrow and replay from the checkpoint
+    a.step(small.dt, 20)
+    b.step(small.dt, 20)
+    assert np.array_equal(a.get("position"), b.get("position"))
+    assert np.array_equal(a.get("velocity"), b.get("velocity"))
+    assert np.array_equal(a.get("densityChange"), b.get("densityChange"))
+    # neighbour_search alone also regrows
+    c = make_gpu(small, capacity=(3, 3))
+    c.neighbor_search()
+    o = oracle.OracleSPH(small)
+    o.neighbor_search()
+    assert np.array_equal(c.get("objectPointing"), o.get("pointing_ff"))
+
+
+def test_reference_dam_break_frame1():
+    """The reference's damBreak.cpp scene, 670 steps, against the reference's
+    own frame at step 670 (tests/golden/dam_break_frames.npz)."""
+    spec = scenes.reference_dam_break()
+    ref = np.load(GOLDEN / "dam_break_frames.npz")
+    s = make_gpu(spec)
+    s.step(spec.dt, 670)
+    err = np.abs(s.get("position") - ref["pos1"]).max()
+    # FP32 pair arithmetic over 670 steps: within 1e-5 of the spacing (5e-3 m)
+    assert err < 5e-8 + 1e-5 * spec.dx, err
+    assert np.abs(s.get("velocity") - ref["vel1"]).max() < 1e-3
+    assert np.abs(s.get("density") - ref["rho1"]).max() < 0.05
+
+
+def test_full_size_dam_break_cells_and_counts():
+    """256,000-particle dam break: hash, permutation, cell ranges and
+    per-particle neighbour counts are bit-exact against the oracle."""
+    spec = scenes.dam_break()
+    s = make_gpu(spec)
+    s.neighbor_search()
+    g = oracle.grid_set(spec.min_bound, spec.max_bound, 2.0 * 1.3 * spec.dx)
+    hv = oracle.hash_points(g, spec.fluid_pos)
+    sh, si = oracle.sort_by_hash(hv, g.num_grids)
+    cs, ce = oracle.cell_ranges(sh, g.num_grids)
+    assert np.array_equal(s.get("hashIndex"), si)
+    assert np.array_equal(s.get("cellHashStart"), cs)
+    assert np.array_equal(s.get("cellHashEnd"), ce)
+    cnt = s.get("neighborCount")
+    assert cnt.max() == 80 and cnt.sum() > 70 * spec.nf
+    h = np.full(spec.nf, 1.3 * spec.dx)
+    ocnt = np.zeros(spec.nf, dtype=np.int32)
+    oracle.lib().o_sph_count(oracle.Grid.from_buffer_copy(g), oracle.d(spec.fluid_pos), oracle.d(h), spec.nf,
+                             oracle.d(spec.fluid_pos), oracle.d(h), oracle.i(si), oracle.i(cs), oracle.i(ce), 1,
+                             oracle.i(ocnt))
+    assert np.array_equal(cnt, ocnt)

@@ -1,0 +1,173 @@
+// Shared device/host helpers for the MI355X (gfx950) DEMSPH kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <stdexcept>
+
+#include "../../include/demsph.h"
+
+namespace demsph {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+#define DEMSPH_HIP(call)                                                                  \
+    do {                                                                                  \
+        hipError_t _e = (call);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            throw ::demsph::HipFailure(std::string(#call) + ": " + hipGetErrorString(_e)); \
+        }                                                                                 \
+    } while (0)
+
+struct HipFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidArg : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Runs f, translating exceptions to C-ABI status codes.
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return DEMSPH_OK;
+    } catch (const InvalidArg& e) {
+        set_error(e.what());
+        return DEMSPH_ERR_INVALID;
+    } catch (const HipFailure& e) {
+        set_error(e.what());
+        return DEMSPH_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return DEMSPH_ERR_HIP;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return DEMSPH_ERR_INTERNAL;
+    }
+}
+
+// ---------------------------------------------------------------- memory
+template <class T>
+T* dmalloc(size_t n)
+{
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    DEMSPH_HIP(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------- geometry
+// Uniform grid geometry of the reference's spatialGrid (boundary.h:5-106).
+struct GridGeom {
+    double minx, miny, minz;
+    double maxx, maxy, maxz;
+    double cx, cy, cz;
+    int nx, ny, nz;
+    __host__ __device__ long long numGrids() const { return (long long)nx * ny * nz; }
+};
+
+// spatialGrid::set (boundary.h:70-94).  Returns false when the reference
+// would ignore the call.
+bool grid_set(GridGeom& g, const double* minb, const double* maxb, double cellOneDim);
+
+// Number of bits needed to hold hashes in [0, numGrids].
+int key_bits(long long numGrids);
+
+// calculateHash kernel body (neighborSearchKernel.cu:6-27 with
+// neighborSearchKernel.cuh:4-14).  Contraction is disabled so that the
+// clamp and the cell truncation round exactly as the FP64 oracle does.
+__device__ __forceinline__ int cell_hash(const GridGeom& g, double px, double py, double pz)
+{
+#pragma clang fp contract(off)
+    if (px < g.minx) px = g.minx;
+    else if (px >= g.maxx) px = g.maxx - 0.5 * g.cx;
+    if (py < g.miny) py = g.miny;
+    else if (py >= g.maxy) py = g.maxy - 0.5 * g.cy;
+    if (pz < g.minz) pz = g.minz;
+    else if (pz >= g.maxz) pz = g.maxz - 0.5 * g.cz;
+    int ix = (int)((px - g.minx) / g.cx);
+    int iy = (int)((py - g.miny) / g.cy);
+    int iz = (int)((pz - g.minz) / g.cz);
+    return iz * g.ny * g.nx + iy * g.nx + ix;
+}
+
+// 27-cell stencil of a query point (ballNeighborSearchKernel.cu:23-33,
+// SPHNeighborSearchKernel.cu:21-31): unclamped truncation, grid clamp and
+// face shrink.
+struct Stencil {
+    int cx, cy, cz;
+    int lox, loy, loz, hix, hiy, hiz;
+};
+__device__ __forceinline__ Stencil make_stencil(const GridGeom& g, double px, double py, double pz)
+{
+#pragma clang fp contract(off)
+    Stencil s;
+    int gx = (int)((px - g.minx) / g.cx);
+    int gy = (int)((py - g.miny) / g.cy);
+    int gz = (int)((pz - g.minz) / g.cz);
+    s.lox = s.loy = s.loz = -1;
+    s.hix = s.hiy = s.hiz = 1;
+    if (gx <= 0) { gx = 0; s.lox = 0; }
+    if (gx >= g.nx - 1) { gx = g.nx - 1; s.hix = 0; }
+    if (gy <= 0) { gy = 0; s.loy = 0; }
+    if (gy >= g.ny - 1) { gy = g.ny - 1; s.hiy = 0; }
+    if (gz <= 0) { gz = 0; s.loz = 0; }
+    if (gz >= g.nz - 1) { gz = g.nz - 1; s.hiz = 0; }
+    s.cx = gx; s.cy = gy; s.cz = gz;
+    return s;
+}
+
+// Exact FP64 neighbour cut test of the reference: (cut*cut - |r|^2) >= 0
+// with |r|^2 = x*x + y*y + z*z evaluated left to right without fusion
+// (SPHNeighborSearchKernel.cu:47-49, ballNeighborSearchKernel.cu:51-53).
+__device__ __forceinline__ bool within_cut(double cut, double rx, double ry, double rz)
+{
+#pragma clang fp contract(off)
+    double d2 = rx * rx + ry * ry + rz * rz;
+    return (cut * cut - d2) >= 0.0;
+}
+
+// ---------------------------------------------------------------- launch
+constexpr int kNumXcd = 8;
+
+// Maps the hardware block id to a logical block id so that each XCD (which
+// receives blocks round-robin) works on one contiguous range of logical
+// blocks: neighbouring particles then share one L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int b = blockIdx.x, G = gridDim.x;
+    const int q = G / kNumXcd, r = G % kNumXcd;
+    const int xcd = b % kNumXcd, pos = b / kNumXcd;
+    return (xcd < r) ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+}
+
+inline int blocks_for(long long n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// ---------------------------------------------------------------- sort/scan
+// Bytes of temporary storage radix_sort_pairs needs for n keys.
+size_t radix_sort_temp_bytes(int n);
+// Stable LSD radix sort of (keys, vals) on bits [0, nbits).  vals == nullptr
+// sorts an implicit iota.  Results end in keys_out / vals_out.  All
+// buffers hold n elements; temp holds radix_sort_temp_bytes(n) bytes.
+void radix_sort_pairs(const uint32_t* keys_in, const int* vals_in, uint32_t* keys_out, int* vals_out,
+                      uint32_t* keys_tmp, int* vals_tmp, void* temp, int n, int nbits, hipStream_t s);
+// Exclusive scan of n int32 values; temp holds scan_temp_bytes(n) bytes.
+// Writes the total to *total_dev when non-null.
+size_t scan_temp_bytes(int n);
+void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, hipStream_t s);
+
+// Fill n ints with v.
+void fill_i32(int* p, long long n, int v, hipStream_t s);
+
+}  // namespace demsph

@@ -1,0 +1,391 @@
+// Stable LSD radix sort, device-wide scan and grid helpers for gfx950.
+//
+// The reference sorts (cell hash, particle index) pairs with Thrust's
+// radix sort_by_key over an iota value array (buildHashStartEnd.cu:50-76),
+// so its permutation is the lexicographic (hash, index) order.  This file
+// implements that order natively: 8-bit digits, one histogram kernel and one
+// scatter kernel per digit, a wave64 ballot "match-any" to rank equal digits
+// stably inside a wavefront, and LDS prefix sums across the wavefronts of a
+// workgroup.  Only the bits that hold hashes in [0, numGrids] are sorted.
+#include "common.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace demsph {
+
+namespace {
+constexpr int kRadixBits = 8;
+constexpr int kBins = 1 << kRadixBits;
+constexpr int kSortThreads = 256;             // 4 wavefronts
+constexpr int kWaves = kSortThreads / 64;
+constexpr int kItems = 8;                     // keys per lane
+constexpr int kTile = kSortThreads * kItems;  // keys per workgroup
+
+int sort_blocks(int n) { return (n + kTile - 1) / kTile; }
+
+// Histogram of one digit per tile, written digit-major so that one
+// exclusive scan over [digit][block] yields every scatter base.
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                             uint32_t* __restrict__ hist, int nblocks)
+{
+    __shared__ uint32_t h[kBins];
+    for (int i = threadIdx.x; i < kBins; i += kSortThreads) h[i] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTile;
+#pragma unroll
+    for (int it = 0; it < kItems; it++) {
+        const int e = base + it * kSortThreads + threadIdx.x;
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & (kBins - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kBins; d += kSortThreads) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
+}
+
+// Stable scatter.  Tile element order is wave-major, then item, then lane,
+// which is the global index order inside the tile.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
+                                                                const int* __restrict__ vals_in,
+                                                                uint32_t* __restrict__ keys_out,
+                                                                int* __restrict__ vals_out, int n, int shift,
+                                                                const uint32_t* __restrict__ offsets, int nblocks)
+{
+    __shared__ uint32_t wave_count[kWaves][kBins];
+    __shared__ uint32_t block_base[kBins];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < kWaves * kBins; i += kSortThreads) (&wave_count[0][0])[i] = 0;
+    for (int d = threadIdx.x; d < kBins; d += kSortThreads) block_base[d] = offsets[(size_t)d * nblocks + blockIdx.x];
+    __syncthreads();
+
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const int wbase = blockIdx.x * kTile + wave * (64 * kItems);
+    uint32_t key[kItems];
+    int val[kItems];
+    uint32_t pos[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; it++) {
+        const int e = wbase + it * 64 + lane;
+        const bool valid = e < n;
+        key[it] = valid ? keys_in[e] : 0u;
+        val[it] = valid ? (vals_in ? vals_in[e] : e) : 0;
+        const uint32_t digit = (key[it] >> shift) & (kBins - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; b++) {
+            const bool bit = (digit >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t before = wave_count[wave][digit];
+        pos[it] = before + (uint32_t)__popcll(peers & lt_mask);
+        // The lowest lane of each group of equal digits advances the counter.
+        const bool leader = valid && ((peers & lt_mask) == 0ull);
+        __builtin_amdgcn_wave_barrier();
+        if (leader) wave_count[wave][digit] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        if (!valid) pos[it] = 0xffffffffu;
+    }
+    __syncthreads();
+    // Exclusive prefix of the per-wave counts, per digit.
+    for (int d = threadIdx.x; d < kBins; d += kSortThreads) {
+        uint32_t run = block_base[d];
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            const uint32_t c = wave_count[w][d];
+            wave_count[w][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kItems; it++) {
+        if (pos[it] == 0xffffffffu) continue;
+        const uint32_t digit = (key[it] >> shift) & (kBins - 1);
+        const uint32_t dst = wave_count[wave][digit] + pos[it];
+        keys_out[dst] = key[it];
+        vals_out[dst] = val[it];
+    }
+}
+
+// ----------------------------------------------------------------- scan
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int wave_inclusive_sum(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ int block_exclusive(int v, int& total)
+{
+    __shared__ int wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inc = wave_inclusive_sum(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int wprefix = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; w++) {
+        if (w < wave) wprefix += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    total = tot;
+    return wprefix + inc - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ partial)
+{
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++)
+        if (base + i < n) s += in[base + i];
+    int total;
+    (void)block_exclusive(s, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Single workgroup: exclusive scan of the per-tile partial sums.
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(int* __restrict__ partial, int m, int* __restrict__ total_out)
+{
+    int carry = 0;
+    for (int base = 0; base < m; base += kScanThreads) {
+        const int i = base + threadIdx.x;
+        const int v = i < m ? partial[i] : 0;
+        int total;
+        const int ex = block_exclusive(v, total);
+        if (i < m) partial[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_final(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                             const int* __restrict__ partial)
+{
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    int total;
+    int run = block_exclusive(s, total) + partial[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+}
+
+__global__ void k_fill_i32(int* p, long long n, int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Radix histogram offsets are uint32; scan them as int (n < 2^31).
+}  // namespace
+
+size_t scan_temp_bytes(int n)
+{
+    const int m = (n + kScanTile - 1) / kScanTile;
+    return sizeof(int) * (size_t)(m + 1);
+}
+
+void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, hipStream_t s)
+{
+    if (n <= 0) {
+        if (total_dev) DEMSPH_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), s));
+        return;
+    }
+    const int m = (n + kScanTile - 1) / kScanTile;
+    int* partial = static_cast<int*>(temp);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(m), dim3(kScanThreads), 0, s, in, n, partial);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, partial, m, total_dev);
+    hipLaunchKernelGGL(k_scan_final, dim3(m), dim3(kScanThreads), 0, s, in, out, n, partial);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+size_t radix_sort_temp_bytes(int n)
+{
+    const int nb = sort_blocks(n > 0 ? n : 1);
+    const size_t hist = sizeof(uint32_t) * (size_t)kBins * nb;
+    return hist * 2 + scan_temp_bytes(kBins * nb) + 256;
+}
+
+void radix_sort_pairs(const uint32_t* keys_in, const int* vals_in, uint32_t* keys_out, int* vals_out,
+                      uint32_t* keys_tmp, int* vals_tmp, void* temp, int n, int nbits, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (nbits < 1) nbits = 1;
+    const int nb = sort_blocks(n);
+    uint32_t* hist = static_cast<uint32_t*>(temp);
+    uint32_t* offs = hist + (size_t)kBins * nb;
+    void* scan_tmp = offs + (size_t)kBins * nb;
+    const int passes = (nbits + kRadixBits - 1) / kRadixBits;
+    // Ping-pong so that the last pass writes keys_out / vals_out.
+    const uint32_t* ksrc = keys_in;
+    const int* vsrc = vals_in;
+    for (int p = 0; p < passes; p++) {
+        const bool last = (p == passes - 1);
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t* kdst = to_out ? keys_out : keys_tmp;
+        int* vdst = to_out ? vals_out : vals_tmp;
+        (void)last;
+        const int shift = p * kRadixBits;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
+        exclusive_scan(reinterpret_cast<const int*>(hist), reinterpret_cast<int*>(offs), kBins * nb, scan_tmp, nullptr, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst, vdst, n, shift, offs, nb);
+        DEMSPH_HIP(hipGetLastError());
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+}
+
+void fill_i32(int* p, long long n, int v, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+bool grid_set(GridGeom& g, const double* minb, const double* maxb, double c)
+{
+    if (maxb[0] <= minb[0] || maxb[1] <= minb[1] || maxb[2] <= minb[2]) return false;
+    if (c < 1.e-20) return false;
+    const double dx = maxb[0] - minb[0], dy = maxb[1] - minb[1], dz = maxb[2] - minb[2];
+    g.minx = minb[0]; g.miny = minb[1]; g.minz = minb[2];
+    g.maxx = maxb[0]; g.maxy = maxb[1]; g.maxz = maxb[2];
+    g.nx = dx > c ? (int)(dx / c) : 1;
+    g.ny = dy > c ? (int)(dy / c) : 1;
+    g.nz = dz > c ? (int)(dz / c) : 1;
+    g.cx = dx / (double)g.nx;
+    g.cy = dy / (double)g.ny;
+    g.cz = dz / (double)g.nz;
+    return true;
+}
+
+int key_bits(long long numGrids)
+{
+    int b = 1;
+    while (b < 32 && (1ll << b) <= numGrids) b++;
+    return b;
+}
+
+// ----------------------------------------------------------------- errors
+namespace {
+thread_local std::string g_last_error = "no error";
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// ----------------------------------------------------------------- raw cells
+namespace {
+__global__ void k_hash_aos(const double* __restrict__ pos, int n, GridGeom g, uint32_t* __restrict__ hash)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    hash[i] = (uint32_t)cell_hash(g, pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]);
+}
+
+// findStartAndEnd (buildHashStartEnd.cu:14-34) over sorted hashes.
+__global__ void k_cell_ranges(const uint32_t* __restrict__ sh, int n, long long numGrids, int* __restrict__ start,
+                              int* __restrict__ end)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long h = (int)sh[i];
+    if (h < 0 || h >= numGrids) return;
+    if (i == 0 || sh[i - 1] != sh[i]) start[h] = i;
+    if (i == n - 1 || sh[i + 1] != sh[i]) end[h] = i + 1;
+}
+
+struct RawCellCache {
+    std::mutex mu;
+    int device = -1;
+    size_t cap = 0;
+    uint32_t* keys = nullptr;
+    uint32_t* keys_tmp = nullptr;
+    int* vals_tmp = nullptr;
+    void* temp = nullptr;
+    size_t temp_bytes = 0;
+};
+RawCellCache g_raw;
+}  // namespace
+
+}  // namespace demsph
+
+using namespace demsph;
+
+extern "C" const char* demsph_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* demsph_version(void) { return "demsph-mi355x 0.1 (gfx950)"; }
+
+extern "C" int demsph_grid_geometry(const double* minBound, const double* maxBound, double cellSizeOneDim,
+                                    double* cellSize, int* gridSize)
+{
+    return guarded([&] {
+        if (!minBound || !maxBound || !cellSize || !gridSize) throw InvalidArg("null argument");
+        GridGeom g;
+        if (!grid_set(g, minBound, maxBound, cellSizeOneDim)) throw InvalidArg("degenerate domain or cell size");
+        cellSize[0] = g.cx; cellSize[1] = g.cy; cellSize[2] = g.cz;
+        gridSize[0] = g.nx; gridSize[1] = g.ny; gridSize[2] = g.nz;
+    });
+}
+
+extern "C" int demsph_build_cells(const double* position, int* hashIndex, int* hashValue, int* cellHashStart,
+                                  int* cellHashEnd, const double* minBound, const double* maxBound,
+                                  const double* cellSize, const int* gridSize, size_t numGrids, size_t numObject,
+                                  void* stream)
+{
+    return guarded([&] {
+        if (!minBound || !maxBound || !cellSize || !gridSize) throw InvalidArg("null grid argument");
+        if (numObject > 0 && (!position || !hashIndex || !hashValue)) throw InvalidArg("null particle array");
+        if (numGrids > 0 && (!cellHashStart || !cellHashEnd)) throw InvalidArg("null cell array");
+        if (numObject > (size_t)INT32_MAX || numGrids > (size_t)INT32_MAX) throw InvalidArg("size exceeds int32 range");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        GridGeom g;
+        g.minx = minBound[0]; g.miny = minBound[1]; g.minz = minBound[2];
+        g.maxx = maxBound[0]; g.maxy = maxBound[1]; g.maxz = maxBound[2];
+        g.cx = cellSize[0]; g.cy = cellSize[1]; g.cz = cellSize[2];
+        g.nx = gridSize[0]; g.ny = gridSize[1]; g.nz = gridSize[2];
+        fill_i32(cellHashStart, (long long)numGrids, -1, s);
+        fill_i32(cellHashEnd, (long long)numGrids, -1, s);
+        const int n = (int)numObject;
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lock(g_raw.mu);
+        int dev = 0;
+        DEMSPH_HIP(hipGetDevice(&dev));
+        if (g_raw.device != dev || g_raw.cap < numObject) {
+            // Older buffers may still be read by queued work on another stream.
+            DEMSPH_HIP(hipDeviceSynchronize());
+            dfree(g_raw.keys); dfree(g_raw.keys_tmp); dfree(g_raw.vals_tmp);
+            if (g_raw.temp) (void)hipFree(g_raw.temp);
+            g_raw.cap = numObject + numObject / 4 + 1024;
+            g_raw.keys = dmalloc<uint32_t>(g_raw.cap);
+            g_raw.keys_tmp = dmalloc<uint32_t>(g_raw.cap);
+            g_raw.vals_tmp = dmalloc<int>(g_raw.cap);
+            g_raw.temp_bytes = radix_sort_temp_bytes((int)g_raw.cap);
+            DEMSPH_HIP(hipMalloc(&g_raw.temp, g_raw.temp_bytes));
+            g_raw.device = dev;
+        }
+        hipLaunchKernelGGL(k_hash_aos, dim3(blocks_for(n, 256)), dim3(256), 0, s, position, n, g, g_raw.keys);
+        radix_sort_pairs(g_raw.keys, nullptr, reinterpret_cast<uint32_t*>(hashValue), hashIndex, g_raw.keys_tmp,
+                         g_raw.vals_tmp, g_raw.temp, n, key_bits((long long)numGrids), s);
+        hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                           reinterpret_cast<const uint32_t*>(hashValue), n, (long long)numGrids, cellHashStart,
+                           cellHashEnd);
+        DEMSPH_HIP(hipGetLastError());
+    });
+}

@@ -1,0 +1,90 @@
+// Per-kernel timing with HIP events on the stream the kernels run on.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace demsph {
+
+class KernelTimer {
+public:
+    bool enabled = false;
+
+    // Records an event pair around one launch.  Call begin(), launch, end().
+    void begin(const char* name, hipStream_t s)
+    {
+        if (!enabled) return;
+        Rec r;
+        r.name = name;
+        r.a = take();
+        r.b = take();
+        DEMSPH_HIP(hipEventRecord(r.a, s));
+        pending_.push_back(r);
+    }
+    void end(hipStream_t s)
+    {
+        if (!enabled || pending_.empty()) return;
+        DEMSPH_HIP(hipEventRecord(pending_.back().b, s));
+    }
+    // Converts recorded event pairs into totals; the stream must be idle.
+    void harvest()
+    {
+        for (auto& r : pending_) {
+            float ms = 0.f;
+            DEMSPH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            auto& t = totals_[r.name];
+            t.first += 1;
+            t.second += ms;
+            pool_.push_back(r.a);
+            pool_.push_back(r.b);
+        }
+        pending_.clear();
+    }
+    void reset()
+    {
+        totals_.clear();
+    }
+    const std::map<std::string, std::pair<long, double>>& totals() const { return totals_; }
+    ~KernelTimer()
+    {
+        for (auto& r : pending_) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool_) (void)hipEventDestroy(e);
+    }
+
+private:
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    hipEvent_t take()
+    {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        DEMSPH_HIP(hipEventCreate(&e));
+        return e;
+    }
+    std::vector<Rec> pending_;
+    std::vector<hipEvent_t> pool_;
+    std::map<std::string, std::pair<long, double>> totals_;
+};
+
+// Launches a kernel with optional event timing.
+#define DEMSPH_LAUNCH(timer, name, kernel, grid, block, shmem, stream, ...)          \
+    do {                                                                             \
+        (timer).begin(name, stream);                                                 \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);         \
+        (timer).end(stream);                                                         \
+        DEMSPH_HIP(hipGetLastError());                                               \
+    } while (0)
+
+}  // namespace demsph
