@@ -70,8 +70,14 @@ def test_neighbor_search_bit_exact(small):
     s.neighbor_search()
     assert np.array_equal(s.get("hashValue"), np.sort(o.get("hash"), kind="stable"))
     assert np.array_equal(s.get("hashIndex"), o.get("index"))
-    assert np.array_equal(s.get("cellHashStart"), o.get("cellStart"))
-    assert np.array_equal(s.get("cellHashEnd"), o.get("cellEnd"))
+    # The reference re-hashes the dummies into the same cell arrays after the
+    # fluid list is written (WCSPHSolver.h:218-231), so the oracle's final
+    # cell arrays hold dummy ranges; the fluid ranges come from its fluid sort.
+    g = o.grid()
+    sh, _ = oracle.sort_by_hash(o.get("hash"), g.num_grids)
+    cs, ce = oracle.cell_ranges(sh, g.num_grids)
+    assert np.array_equal(s.get("cellHashStart"), cs)
+    assert np.array_equal(s.get("cellHashEnd"), ce)
     assert np.array_equal(s.get("neighborCount"), o.get("count_ff"))
     assert np.array_equal(s.get("objectPointing"), o.get("pointing_ff"))
     assert np.array_equal(s.get("dummyNeighborCount"), o.get("count_fd"))
