@@ -92,7 +92,21 @@ def main() -> int:
     ap.add_argument("--dx", type=float, default=0.005)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 launches) to calibrate HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n,
+                          "launches": 5}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build measures the single-GPU kernel "
               f"(--gpus {args.gpus} requested; multi-GPU sharding is a design, not a run)", file=sys.stderr)
