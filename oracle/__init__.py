@@ -194,3 +194,176 @@ class OracleSPH:
         v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         if lib().o_sph_set(self.h, name.encode(), v.ctypes.data_as(_vp)) != 0:
             raise KeyError(name)
+
+
+# ----------------------------------------------------------------- DEM
+_DEM_DECLARED = False
+
+
+def _declare_dem(L):
+    global _DEM_DECLARED
+    if _DEM_DECLARED:
+        return
+    L.o_dem_create.restype = _vp
+    L.o_dem_destroy.argtypes = [_vp]
+    for f in ("o_dem_set_linear", "o_dem_set_hertz", "o_dem_set_bonded"):
+        getattr(L, f).argtypes = [_vp, C.c_int, C.c_int, _dp]
+    L.o_dem_add_ball.argtypes = [_vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.o_dem_add_fixed_ball.argtypes = [_vp, _dp, C.c_double, C.c_int, C.c_int]
+    L.o_dem_add_clump.argtypes = [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_int]
+    L.o_dem_add_fixed_clump.argtypes = [_vp, C.c_int, _dp, _dp, _dp, C.c_int]
+    L.o_dem_add_mesh_wall.argtypes = [_vp, C.c_int, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, C.c_int]
+    L.o_dem_add_mesh_wall.restype = C.c_int
+    L.o_dem_add_bonds.argtypes = [_vp, C.c_int, _ip, _ip, C.c_int, _dp]
+    L.o_dem_initialize.argtypes = [_vp, _dp, _dp]
+    L.o_dem_initialize.restype = C.c_int
+    for f in ("o_dem_neighbor_search", "o_dem_solve_begin", "o_dem_hook_clear"):
+        getattr(L, f).argtypes = [_vp]
+    L.o_dem_contact.argtypes = [_vp, C.c_double]
+    L.o_dem_triangle_contact.argtypes = [_vp, C.c_double]
+    L.o_dem_first_half.argtypes = [_vp, _dp, C.c_double]
+    L.o_dem_second_half.argtypes = [_vp, _dp, C.c_double]
+    L.o_dem_step.argtypes = [_vp, _dp, C.c_double, C.c_int]
+    L.o_dem_hook_constant_force.argtypes = [_vp, _dp, C.c_double]
+    L.o_dem_hook_damping.argtypes = [_vp, C.c_double]
+    L.o_dem_hook_buoyancy.argtypes = [_vp, C.c_double, C.c_double, _dp, _dp, C.c_double]
+    L.o_dem_set_policy.argtypes = [_vp, C.c_int]
+    L.o_dem_get.argtypes = [_vp, C.c_char_p, _vp]
+    L.o_dem_get.restype = C.c_long
+    L.o_dem_set.argtypes = [_vp, C.c_char_p, _vp]
+    L.o_dem_set.restype = C.c_int
+    L.o_dem_grids.argtypes = [_vp, C.POINTER(Grid), C.POINTER(Grid)]
+    L.o_dem_step_count.restype = C.c_size_t
+    L.o_dem_step_count.argtypes = [_vp]
+    L.o_sphere_triangle.argtypes = [_dp, C.c_double, _dp, _dp, _dp, _dp]
+    L.o_sphere_triangle.restype = C.c_int
+    _DEM_DECLARED = True
+
+
+DEM_INT = {"pointed", "pointing", "neighborCount", "tri_pointed", "tri_pointing", "tri_cancel", "bond_pointed",
+           "bond_pointing", "bond_isBonded", "hash", "index", "cellStart", "cellEnd", "tri_index"}
+DEM_VEC = {"pos", "vel", "angVel", "force", "torque", "sliding", "rolling", "torsion", "contactForce",
+           "contactTorque", "contactPoint", "tri_point", "tri_normal", "tri_force", "tri_sliding", "bond_shearForce",
+           "bond_bendingTorque", "bond_point", "bond_normal", "clump_pos", "clump_vel", "clump_angVel",
+           "clump_force", "wall_pos", "vertex_global", "tri_circumcenter"}
+
+
+def _a(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+    return a
+
+
+def sphere_triangle(c, r, v0, v1, v2):
+    L = lib()
+    _declare_dem(L)
+    cp = np.zeros(3)
+    t = L.o_sphere_triangle(d(_a(c)), float(r), d(_a(v0)), d(_a(v1)), d(_a(v2)), cp.ctypes.data_as(_dp))
+    return t, cp
+
+
+class OracleDEM:
+    """The oracle's DEMSolver (DEMSolver.h:144-2079) in FP64, driven by a
+    this_studio_amd.scenes.DEMScene recording."""
+
+    def __init__(self, scene, solve_begin: bool = True, policy_d1: int = 0):
+        L = lib()
+        _declare_dem(L)
+        self.L = L
+        self.scene = scene
+        self.h = L.o_dem_create()
+        L.o_dem_set_policy(self.h, int(policy_d1))
+        for kind, a, b, p in scene.materials:
+            fn = {"linear": L.o_dem_set_linear, "hertz": L.o_dem_set_hertz, "bonded": L.o_dem_set_bonded}[kind]
+            fn(self.h, int(a), int(b), d(_a(p)))
+        for kind, o in scene.objects:
+            if kind == "balls":
+                for k in range(o["pos"].shape[0]):
+                    L.o_dem_add_ball(self.h, d(o["pos"][k]), d(o["vel"][k]), d(o["w"][k]), float(o["r"][k]),
+                                     float(o["density"][k]), int(o["mat"][k]), int(o["clump"][k]))
+            elif kind == "fixed":
+                for k in range(o["pos"].shape[0]):
+                    L.o_dem_add_fixed_ball(self.h, d(o["pos"][k]), float(o["r"][k]), int(o["mat"][k]),
+                                           int(o["clump"][k]))
+            elif kind == "clump":
+                L.o_dem_add_clump(self.h, o["points"].shape[0], d(o["points"]), d(o["radii"]), d(o["centroid"]),
+                                  d(o["vel"]), d(o["w"]), o["mass"], d(o["inertia"]), o["mat"])
+            elif kind == "fixed_clump":
+                L.o_dem_add_fixed_clump(self.h, o["points"].shape[0], d(o["points"]), d(o["radii"]),
+                                        d(o["centroid"]), o["mat"])
+            elif kind == "wall":
+                t0, t1, t2 = (np.ascontiguousarray(o[k], dtype=np.int32) for k in ("t0", "t1", "t2"))
+                L.o_dem_add_mesh_wall(self.h, o["verts"].shape[0], d(o["verts"]), t0.size, i(t0), i(t1), i(t2),
+                                      d(o["pos"]), d(o["vel"]), d(o["w"]), o["mat"])
+        if L.o_dem_initialize(self.h, d(_a(scene.min_bound)), d(_a(scene.max_bound))) != 0:
+            raise ValueError("oracle DEM initialize failed")
+        for h in scene.hooks:
+            if h[0] == "const":
+                L.o_dem_hook_constant_force(self.h, d(_a(h[1])), float(h[2]))
+            elif h[0] == "damping":
+                L.o_dem_hook_damping(self.h, float(h[1]))
+            elif h[0] == "buoyancy":
+                L.o_dem_hook_buoyancy(self.h, float(h[1]), float(h[2]), d(_a(h[3])), d(_a(h[4])), float(h[5]))
+        if solve_begin:
+            self.solve_begin()
+
+    def solve_begin(self):
+        """solve() up to the loop (DEMSolver.h:1994-1998), with
+        addInitialCondition's bonds from the initial pair list when asked."""
+        L = self.L
+        L.o_dem_solve_begin(self.h)
+        bonds = None
+        if self.scene.bonds_from_initial_pairs:
+            bonds = (self.get("pointed"), self.get("pointing"))
+        if self.scene.explicit_bonds is not None:
+            bonds = self.scene.explicit_bonds
+        if bonds is not None:
+            a = np.ascontiguousarray(bonds[0], dtype=np.int32)
+            b = np.ascontiguousarray(bonds[1], dtype=np.int32)
+            pos = self.get("pos")
+            L.o_dem_add_bonds(self.h, a.size, i(a), i(b), pos.shape[0], d(pos))
+            L.o_dem_initialize(self.h, d(_a(self.scene.min_bound)), d(_a(self.scene.max_bound)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.o_dem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n: int = 1, dt: float | None = None):
+        self.L.o_dem_step(self.h, d(_a(self.scene.gravity)), float(dt or self.scene.dt), int(n))
+
+    def neighbor_search(self):
+        self.L.o_dem_neighbor_search(self.h)
+
+    def contact(self, dt):
+        self.L.o_dem_contact(self.h, float(dt))
+
+    def triangle_contact(self, dt):
+        self.L.o_dem_triangle_contact(self.h, float(dt))
+
+    def grids(self):
+        gb, gt = Grid(), Grid()
+        self.L.o_dem_grids(self.h, C.byref(gb), C.byref(gt))
+        return gb, gt
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.L.o_dem_get(self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.int32 if name in DEM_INT else np.float64)
+        if n:
+            self.L.o_dem_get(self.h, name.encode(), out.ctypes.data_as(_vp))
+        if name in DEM_VEC:
+            out = out.reshape(-1, 3)
+        return out
+
+    def set(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if self.L.o_dem_set(self.h, name.encode(), v.ctypes.data_as(_vp)) != 0:
+            raise KeyError(name)
+
+    @property
+    def step_count(self) -> int:
+        return int(self.L.o_dem_step_count(self.h))

@@ -68,6 +68,8 @@ void o_sort_by_hash(const int* hash, int n, int numGrids, int* sortedHash, int* 
     int inRange = 1;
     for (int i = 0; i < n; i++)
         if (hash[i] < 0 || hash[i] >= numGrids) { inRange = 0; break; }
+    /* sparse keys: a comparison sort is cheaper than a pass over every cell */
+    if (inRange && (long)n * 8 < (long)numGrids) inRange = 0;
     if (inRange) {
         int* cnt = (int*)calloc((size_t)numGrids + 1, sizeof(int));
         for (int i = 0; i < n; i++) cnt[hash[i] + 1]++;
@@ -93,7 +95,8 @@ void o_sort_by_hash(const int* hash, int n, int numGrids, int* sortedHash, int* 
 /* buildHashStartEnd.cu:14-34 after the 0xFF memsets (neighborSearchKernel.cu:105-106) */
 void o_cell_ranges(const int* sh, int n, int numGrids, int* start, int* end)
 {
-    for (int h = 0; h < numGrids; h++) { start[h] = -1; end[h] = -1; }
+    memset(start, 0xFF, sizeof(int) * (size_t)numGrids);
+    memset(end, 0xFF, sizeof(int) * (size_t)numGrids);
     for (int i = 0; i < n; i++) {
         int h = sh[i];
         if (h < 0 || h >= numGrids) continue;

@@ -135,3 +135,241 @@ def small_dam_break(nx: int = 12, ny: int = 6, nz: int = 10, dx: float = 0.01, e
                      lattice_counts=(nx, ny, nz))
     spec.name = "small_dam_break"
     return spec
+
+
+# ---------------------------------------------------------------------------
+# DEM scenes: a recorder of the reference's DEMSolver setup calls
+# (DEMSolver.h:1637-1942) that can be replayed into the GPU scene or the
+# oracle, so both see identical inputs in identical order.
+# ---------------------------------------------------------------------------
+class DEMScene:
+    def __init__(self, name: str = ""):
+        self.name = name
+        self.materials = []      # ("linear"|"hertz"|"bonded", a, b, params)
+        self.objects = []        # ordered ball-producing calls and walls
+        self.min_bound = np.zeros(3)
+        self.max_bound = np.ones(3)
+        self.gravity = np.array([0.0, 0.0, -9.81])
+        self.dt = 1e-5
+        self.hooks = []          # ("const", f[n,3], ramp) | ("damping", c) | ("buoyancy", rho, cd, g, vf, level)
+        self.bonds_from_initial_pairs = False
+        self.explicit_bonds = None
+
+    # material rows ------------------------------------------------------
+    def set_linear_pair(self, a, b, kn, ks, kr, kt, dn, ds, dr, dt, mus, mur, mut):
+        self.materials.append(("linear", a, b, [kn, ks, kr, kt, dn, ds, dr, dt, mus, mur, mut]))
+
+    def set_hertzian_pair(self, a, b, E, G, e, krks, ktks, mus, mur, mut):
+        self.materials.append(("hertz", a, b, [E, G, e, krks, ktks, mus, mur, mut]))
+
+    def set_bonded_pair(self, a, b, gamma, E, knks, sigt, C, mu):
+        self.materials.append(("bonded", a, b, [gamma, E, knks, sigt, C, mu]))
+
+    # objects --------------------------------------------------------------
+    def add_balls(self, pos, vel, w, radius, density, mat, clump=-1):
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        b = lambda x: np.broadcast_to(np.asarray(x, dtype=np.float64), (n, 3)).copy()
+        s = lambda x, t=np.float64: np.broadcast_to(np.asarray(x, dtype=t), (n,)).copy()
+        self.objects.append(("balls", dict(pos=pos, vel=b(vel), w=b(w), r=s(radius), density=s(density),
+                                           mat=s(mat, np.int32), clump=s(clump, np.int32))))
+
+    def add_fixed_balls(self, pos, radius, mat, clump=-1):
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        s = lambda x, t=np.float64: np.broadcast_to(np.asarray(x, dtype=t), (n,)).copy()
+        self.objects.append(("fixed", dict(pos=pos, r=s(radius), mat=s(mat, np.int32), clump=s(clump, np.int32))))
+
+    def add_clump(self, points, radii, centroid, vel, w, mass, inertia, mat):
+        self.objects.append(("clump", dict(points=np.asarray(points, dtype=np.float64).reshape(-1, 3),
+                                           radii=np.asarray(radii, dtype=np.float64).reshape(-1),
+                                           centroid=np.asarray(centroid, dtype=np.float64), vel=np.asarray(vel, float),
+                                           w=np.asarray(w, float), mass=float(mass),
+                                           inertia=np.asarray(inertia, float).reshape(6), mat=int(mat))))
+
+    def add_fixed_clump(self, points, radii, centroid, mat):
+        self.objects.append(("fixed_clump", dict(points=np.asarray(points, dtype=np.float64).reshape(-1, 3),
+                                                 radii=np.asarray(radii, dtype=np.float64).reshape(-1),
+                                                 centroid=np.asarray(centroid, dtype=np.float64), mat=int(mat))))
+
+    def add_mesh_wall(self, verts, t0, t1, t2, pos, vel, w, mat):
+        self.objects.append(("wall", dict(verts=np.asarray(verts, dtype=np.float64).reshape(-1, 3),
+                                          t0=np.asarray(t0, dtype=np.int32), t1=np.asarray(t1, dtype=np.int32),
+                                          t2=np.asarray(t2, dtype=np.int32), pos=np.asarray(pos, float),
+                                          vel=np.asarray(vel, float), w=np.asarray(w, float), mat=int(mat))))
+
+    @property
+    def num_balls(self) -> int:
+        n = 0
+        for kind, o in self.objects:
+            if kind in ("balls", "fixed"):
+                n += o["pos"].shape[0]
+            elif kind in ("clump", "fixed_clump"):
+                n += o["points"].shape[0]
+        return n
+
+    def initial_positions(self) -> np.ndarray:
+        out = []
+        for kind, o in self.objects:
+            if kind in ("balls", "fixed"):
+                out.append(o["pos"])
+            elif kind in ("clump", "fixed_clump"):
+                out.append(o["points"])
+        return np.concatenate(out) if out else np.zeros((0, 3))
+
+
+def box_mesh(lo, hi):
+    """A closed axis-aligned box as 8 vertices and 12 triangles (inward or
+    outward orientation does not matter to the sphere-triangle test)."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    v = np.array([[lo[0], lo[1], lo[2]], [hi[0], lo[1], lo[2]], [hi[0], hi[1], lo[2]], [lo[0], hi[1], lo[2]],
+                  [lo[0], lo[1], hi[2]], [hi[0], lo[1], hi[2]], [hi[0], hi[1], hi[2]], [lo[0], hi[1], hi[2]]])
+    quads = [(0, 1, 2, 3), (4, 7, 6, 5), (0, 4, 5, 1), (1, 5, 6, 2), (2, 6, 7, 3), (3, 7, 4, 0)]
+    t0, t1, t2 = [], [], []
+    for a, b, c, d in quads:
+        t0 += [a, a]
+        t1 += [b, c]
+        t2 += [c, d]
+    return v, np.array(t0), np.array(t1), np.array(t2)
+
+
+def cylinder_mesh(center, axis, radius, height, n_around, n_height, caps=True):
+    """makeCylinderMesh (pointCloudGeneration.h:230-327): rings of n_around
+    vertices, two triangles per side quad, optional cap fans."""
+    w = np.asarray(axis, float)
+    w = w / np.linalg.norm(w)
+    tmp = np.array([0.0, 0.0, 1.0]) if abs(w[2]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    u = np.cross(tmp, w)
+    u /= np.linalg.norm(u)
+    v = np.cross(w, u)
+    v /= np.linalg.norm(v)
+    center = np.asarray(center, float)
+    verts = []
+    for iz in range(n_height + 1):
+        z = (iz / n_height - 0.5) * height
+        c = center + z * w
+        for ia in range(n_around):
+            ang = 2.0 * math.pi * (ia / n_around)
+            verts.append(c + radius * (math.cos(ang) * u + math.sin(ang) * v))
+    vid = lambda iz, ia: iz * n_around + (ia % n_around)
+    t0, t1, t2 = [], [], []
+    for iz in range(n_height):
+        for ia in range(n_around):
+            i00, i01, i10, i11 = vid(iz, ia), vid(iz, ia + 1), vid(iz + 1, ia), vid(iz + 1, ia + 1)
+            t0 += [i00, i00]
+            t1 += [i10, i11]
+            t2 += [i11, i01]
+    if caps:
+        bc = len(verts)
+        verts.append(center - 0.5 * height * w)
+        tc = len(verts)
+        verts.append(center + 0.5 * height * w)
+        for ia in range(n_around):
+            t0.append(bc); t1.append(vid(0, ia + 1)); t2.append(vid(0, ia))
+        for ia in range(n_around):
+            t0.append(tc); t1.append(vid(n_height, ia)); t2.append(vid(n_height, ia + 1))
+    return np.array(verts), np.array(t0), np.array(t1), np.array(t2)
+
+
+def hex_sheet(min_bound, max_bound, spacing):
+    """generateHexPointCloud without periodic axes (pointCloudGeneration.h:
+    92-195): rows dy = a sqrt(3)/2 apart, odd rows shifted by a/2."""
+    dx = spacing
+    dy = spacing * math.sqrt(3.0) * 0.5
+    lx = max(0.0, max_bound[0] - min_bound[0])
+    ly = max(0.0, max_bound[1] - min_bound[1])
+    nx = max(1, int(math.floor(lx / dx + 1e-12)))
+    ny = max(1, int(math.floor(ly / dy + 1e-12)))
+    pts = []
+    for j in range(ny):
+        y = min_bound[1] + j * dy
+        if y >= max_bound[1]:
+            break
+        shift = 0.5 * dx if (j & 1) else 0.0
+        for i in range(nx):
+            x = min_bound[0] + i * dx + shift
+            if x >= max_bound[0]:
+                break
+            pts.append((x, y, min_bound[2]))
+    return np.array(pts)
+
+
+def elastic_bounce() -> DEMScene:
+    """elasticBounce.cpp:3-58: one ball above a two-triangle plane, linear
+    spring with contact time 5 ms, dt = t_c / 50."""
+    s = DEMScene("elastic_bounce")
+    r, tc = 0.01, 0.005
+    s.add_balls([0.0, 0.0, 0.0], [0, 0, 0], [0, 0, 0], r, 2000, 0)
+    verts = np.array([[-1.0, -1.0, -1.0], [1.0, -1.0, -1.0], [1.0, 1.0, -1.0], [-1.0, 1.0, -1.0]])
+    s.add_mesh_wall(verts, [0, 1], [1, 2], [3, 3], [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    mass = 4.0 / 3.0 * r ** 3 * math.pi * 2000
+    s.set_linear_pair(0, 0, 0.5 * mass * math.pi ** 2 / tc / tc, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    s.min_bound = np.array([-1.0, -1.0, -1.0])
+    s.max_bound = np.array([1.0, 1.0, 1.0])
+    s.gravity = np.array([0.0, 0.0, -9.81])
+    s.dt = tc / 50.0
+    return s
+
+
+def bonded_beam() -> DEMScene:
+    """bondedParticleBeam.cpp: 11-ball cantilever, fixed first ball, bonds from
+    the initial pair list, tip load 1e5 N ramped over 1 s, global damping 0.1."""
+    s = DEMScene("bonded_beam")
+    L, nb = 4.0, 10
+    r = L / nb / 2.0
+    s.set_bonded_pair(0, 0, 1.0, 200e9, 2.6, 0.0, 0.0, 0.0)
+    s.add_fixed_balls([0.0, 0.0, 0.0], r, 0)
+    s.add_balls([[(i + 1) * 2.0 * r, 0.0, 0.0] for i in range(nb)], [0, 0, 0], [0, 0, 0], r, 7800, 0)
+    f = np.zeros((nb + 1, 3))
+    f[nb] = [0, 0, 100e3]
+    s.hooks = [("const", f, 1.0), ("damping", 0.1)]
+    s.bonds_from_initial_pairs = True
+    s.min_bound = np.array([0.0, -r, -r])
+    s.max_bound = np.array([L, r, r])
+    s.gravity = np.zeros(3)
+    s.dt = 1e-5
+    return s
+
+
+def granular_collapse(n_side: int = 16, seed: int = 20261202) -> DEMScene:
+    """DEM granular collapse: n_side^3 spheres r = 5 mm, rho = 2500 on an
+    11 mm lattice with seeded U(-1 mm, +1 mm) jitter per axis, in a 0.3 m box
+    of 12 triangles, Hertz-Mindlin E = 1e7, nu = 0.3 (E* = E/(2(1-nu^2)),
+    G* = E/(4(2-nu)(1+nu))), e = 0.5, mu_s = 0.3, dt = 1e-5.  A jitter that
+    would overlap an earlier sphere is redrawn (up to 64 times, then zero)."""
+    s = DEMScene("granular_collapse")
+    r, sp, L = 0.005, 0.011, 0.3
+    E, nu = 1e7, 0.3
+    Es = E / (2 * (1 - nu * nu))
+    Gs = E / (4 * (2 - nu) * (1 + nu))
+    s.set_hertzian_pair(0, 0, Es, Gs, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    start = np.array([(L - (n_side - 1) * sp) / 2, (L - (n_side - 1) * sp) / 2, 0.01])
+    g = splitmix64(seed)
+    u = lambda: (next(g) >> 11) * (1.0 / 9007199254740992.0)
+    base = np.array([[i, j, k] for k in range(n_side) for j in range(n_side) for i in range(n_side)], float) * sp + start
+    pos = np.empty_like(base)
+    grid = {}
+    for idx, p0 in enumerate(base):
+        cell = tuple(np.floor(p0 / sp).astype(int))
+        for attempt in range(65):
+            jit = np.array([u(), u(), u()]) * 0.002 - 0.001 if attempt < 64 else np.zeros(3)
+            p = p0 + jit
+            ok = True
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dxx in (-1, 0, 1):
+                        q = grid.get((cell[0] + dxx, cell[1] + dy, cell[2] + dz))
+                        if q is not None and np.sum((pos[q] - p) ** 2) < (2 * r) ** 2:
+                            ok = False
+            if ok:
+                break
+        pos[idx] = p
+        grid[cell] = idx
+    s.add_balls(pos, [0, 0, 0], [0, 0, 0], r, 2500, 0)
+    v, t0, t1, t2 = box_mesh([0, 0, 0], [L, L, L])
+    s.add_mesh_wall(v, t0, t1, t2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    s.min_bound = np.zeros(3)
+    s.max_bound = np.array([L, L, L])
+    s.dt = 1e-5
+    return s
