@@ -132,6 +132,141 @@ int demsph_sph_set(demsph_sph* s, const char* field, const void* in, size_t coun
 int demsph_sph_enable_timing(demsph_sph* s, int enable);
 int demsph_sph_timing(demsph_sph* s, int index, char* name, size_t nameCap, long* launches, double* totalMs);
 
+/* ------------------------------------------------------------------ DEM
+ * Opaque DEMSolver scene (kernel/DEMSolver.h:144-2079). */
+typedef struct demsph_dem demsph_dem;
+
+/* DEMSolver(cudaStream_t) (DEMSolver.h:147-151).  stream may be NULL (the
+ * legacy default stream, as the reference's apps use). */
+int demsph_dem_create(demsph_dem** out, void* stream);
+int demsph_dem_destroy(demsph_dem* d);
+
+/* Material-pair rows, appended (DEMSolver.h:1637-1726):
+ * setLinearPair(a, b, kn, ks, kr, kt, dn, ds, dr, dt, mus, mur, mut),
+ * setHertzianPair(a, b, E*, G*, e, kr/ks, kt/ks, mus, mur, mut),
+ * setBondedPair(a, b, gamma, E, kn/ks, sigma_t, C, mu). */
+int demsph_dem_set_linear_pair(demsph_dem* d, int a, int b, double kn, double ks, double kr, double kt, double dn,
+                               double ds, double dr, double dt, double mus, double mur, double mut);
+int demsph_dem_set_hertzian_pair(demsph_dem* d, int a, int b, double E, double G, double e, double krks,
+                                 double ktks, double mus, double mur, double mut);
+int demsph_dem_set_bonded_pair(demsph_dem* d, int a, int b, double gamma, double E, double knks, double sigmat,
+                               double C, double mu);
+
+/* addBall (DEMSolver.h:1728-1750), batched: mass = 4/3 pi r^3 density,
+ * inverse mass 0 when mass <= 0.  clumpID may be NULL (all -1). */
+int demsph_dem_add_balls(demsph_dem* d, size_t n, const double* position, const double* velocity,
+                         const double* angularVelocity, const double* radius, const double* density,
+                         const int* materialID, const int* clumpID);
+/* addFixedBall (DEMSolver.h:1752-1768), batched. */
+int demsph_dem_add_fixed_balls(demsph_dem* d, size_t n, const double* position, const double* radius,
+                               const int* materialID, const int* clumpID);
+/* addClump (DEMSolver.h:1796-1845): pebbles get the clump's mean density,
+ * velocity v + w x (p - centroid) when mass > 1e-20; inertia = xx yy zz xy xz yz. */
+int demsph_dem_add_clump(demsph_dem* d, size_t numPebbles, const double* points, const double* radii,
+                         const double* centroid, const double* velocity, const double* angularVelocity,
+                         double mass, const double* inertia, int materialID);
+/* addFixedClump (DEMSolver.h:1847-1875). */
+int demsph_dem_add_fixed_clump(demsph_dem* d, size_t numPebbles, const double* points, const double* radii,
+                               const double* centroid, int materialID);
+/* addMeshWall (DEMSolver.h:1882-1935).  As in the reference the wall starts
+ * at rest (velocity arguments are not stored) and an index out of range
+ * makes the call a no-op (return DEMSPH_OK, nothing added). */
+int demsph_dem_add_mesh_wall(demsph_dem* d, size_t numVertices, const double* vertices, size_t numTriangles,
+                             const int* tri0, const int* tri1, const int* tri2, const double* position,
+                             const double* velocity, const double* angularVelocity, int materialID);
+/* addBondedInteraction (DEMSolver.h:1937-1942, bondedInteraction::add
+ * :93-134): (min,max) deduplicated, point = midpoint, normal = unit(pa-pb).
+ * ballPosition: numPositions interleaved doubles x3. */
+int demsph_dem_add_bonded_interactions(demsph_dem* d, size_t n, const int* objectPointed,
+                                       const int* objectPointing, size_t numPositions, const double* ballPosition);
+
+/* Contact force arithmetic: 0 = FP32 forces with FP64 overlap, contact
+ * geometry and spring history (default); 1 = FP64 throughout. */
+int demsph_dem_set_precision(demsph_dem* d, int fp64Forces);
+
+/* initialize() (DEMSolver.h:245-276): grids, device upload.  May be called
+ * again after adding bonds (the reference's addInitialCondition path). */
+int demsph_dem_initialize(demsph_dem* d, const double* minBound, const double* maxBound);
+/* neighborSearch() (DEMSolver.h:698-777). */
+int demsph_dem_neighbor_search(demsph_dem* d);
+/* calculateBallContactForceTorque(timeStep) (DEMSolver.h:599-695): ball
+ * pairs, bonds and the per-ball reduction (forces are overwritten). */
+int demsph_dem_contact(demsph_dem* d, double timeStep);
+/* addTriangleContactForceTorque(timeStep) (DEMSolver.h:428-597): ball-wall
+ * contacts added to the forces, then rigid wall motion over timeStep. */
+int demsph_dem_triangle_contact(demsph_dem* d, double timeStep);
+/* interaction1stHalf / 2ndHalf (DEMSolver.h:779-906). */
+int demsph_dem_first_half(demsph_dem* d, const double* gravity, double halfTimeStep);
+int demsph_dem_second_half(demsph_dem* d, const double* gravity, double halfTimeStep);
+/* The start of solve() (DEMSolver.h:1994-1997): neighbour search and one
+ * force evaluation with zero time step; resets the step counter. */
+int demsph_dem_solve_begin(demsph_dem* d);
+/* nsteps iterations of loop_ball / loop_ball_wall (DEMSolver.h:1376-1405,
+ * 1452-1485): rebuild every 5th step, two force evaluations per step,
+ * external forces (hook) at t - dt/2 and t, kick-drift-kick. */
+int demsph_dem_step(demsph_dem* d, const double* gravity, double timeStep, int nsteps);
+int demsph_dem_synchronize(demsph_dem* d);
+/* Steps taken since solve_begin, and the simulated time. */
+int demsph_dem_clock(demsph_dem* d, size_t* iStep, double* time);
+
+/* External force hook (DEMSolver.h:1372-1374 addExternalForceTorque): the
+ * callback runs on the host twice per step with device views valid (see
+ * demsph_dem_views); forces/torques it adds are integrated.  NULL removes it. */
+typedef void (*demsph_dem_hook_fn)(void* user, size_t iStep, double time);
+int demsph_dem_set_hook(demsph_dem* d, demsph_dem_hook_fn fn, void* user);
+
+/* Device views in insertion order, FP64 interleaved x3 (the reference's
+ * getBall().force()/torque()/position()/velocity()/angularVelocity(),
+ * radius(), inverseMass(), particle.h:234-241).  Valid during the hook; the
+ * library scatters them before and gathers force/torque after the hook. */
+typedef struct {
+    double* force;
+    double* torque;
+    double* position;
+    double* velocity;
+    double* angularVelocity;
+    double* radius;
+    double* inverseMass;
+    size_t numBalls;
+} demsph_dem_ball_views;
+int demsph_dem_views(demsph_dem* d, demsph_dem_ball_views* views);
+
+/* Built-in external force program, applied in order at each hook point on
+ * the device (DEMSPH/externalForceTorque.cu:4-161): a per-ball constant force
+ * (n x3, insertion order) scaled by min(time/ramp, 1) (ramp <= 0: 1), global
+ * damping F -= c|F| v/|v|, T -= c|T| w/|w|, and buoyancy + quadratic drag
+ * against the plane z = waterLevel. */
+int demsph_dem_hook_clear(demsph_dem* d);
+int demsph_dem_hook_constant_force(demsph_dem* d, const double* force, double ramp);
+int demsph_dem_hook_global_damping(demsph_dem* d, double coefficient);
+int demsph_dem_hook_buoyancy_drag(demsph_dem* d, double rhoFluid, double Cd, const double* gravity,
+                                  const double* fluidVelocity, double waterLevel);
+
+/* Sizes: balls, clumps, walls, triangles, ball pairs, bonds. */
+int demsph_dem_sizes(demsph_dem* d, size_t* numBalls, size_t* numClumps, size_t* numWalls, size_t* numTriangles,
+                     size_t* numPairs, size_t* numBonds);
+/* Copies a field in insertion order (see the DEM field table in
+ * INTEGRATION.md): "position" "velocity" "angularVelocity" "force" "torque"
+ * (double 3n), "radius" "inverseMass" (double n), "objectPointed"
+ * "objectPointing" (int, ball pairs as the reference lists them),
+ * "slidingSpring" "rollingSpring" "torsionSpring" (double 3/pair),
+ * "overlap" (double/pair), "triPointed" "triPointing" "triCancel" (int),
+ * "triSlidingSpring" (double 3/pair), "bondPointed" "bondPointing"
+ * "bondIsBonded" (int), "bondNormalForce" "bondTorsionTorque" (double),
+ * "bondShearForce" "bondBendingTorque" "bondPoint" "bondNormal" (double 3),
+ * "clumpPosition" "clumpVelocity" "clumpAngularVelocity" (double 3/clump),
+ * "clumpOrientation" (double 4/clump), "wallPosition" (double 3/wall),
+ * "wallOrientation" (double 4/wall), "vertexGlobalPosition" (double 3/vertex),
+ * "hashValue" "hashIndex" "cellHashStart" "cellHashEnd" (int). */
+int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity, size_t* count);
+/* Overwrites "position" "velocity" "angularVelocity" (double 3n, insertion
+ * order) or "wallVelocity" "wallAngularVelocity" (double 3/wall; the
+ * reference's getMeshWallBody() write path). */
+int demsph_dem_set(demsph_dem* d, const char* field, const void* in, size_t count);
+
+int demsph_dem_enable_timing(demsph_dem* d, int enable);
+int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long* launches, double* totalMs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,208 @@
+"""GPU parity of the DEM path (libdemsph.so on the MI355X) against the FP64
+oracle, closed forms and the reference's own output frames.
+
+Bit-exact: hashes, sort permutation, cell ranges, ball-pair lists (as the
+reference's objectPointed/objectPointing arrays, in its order) and
+ball-triangle candidate lists.  Forces run in FP32 with FP64 geometry and
+spring history (default) or fully FP64; tolerances are stated per test.
+"""
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+import this_studio_amd as ts
+from this_studio_amd import scenes
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def random_scene(n=3000, seed=7, box=0.4):
+    rng = np.random.default_rng(seed)
+    s = scenes.DEMScene("random")
+    pos = rng.uniform(0.02, box - 0.02, (n, 3))
+    rad = rng.uniform(0.005, 0.01, n)
+    clump = np.where(rng.uniform(size=n) < 0.1, rng.integers(0, 50, n), -1).astype(np.int32)
+    fixed = rng.uniform(size=n) < 0.1
+    vel = rng.normal(scale=0.1, size=(n, 3))
+    for k in range(n):
+        if fixed[k]:
+            s.add_fixed_balls(pos[k], rad[k], 0, clump[k])
+        else:
+            s.add_balls(pos[k], vel[k], [0, 0, 0], rad[k], 1000.0, 0, clump[k])
+    s.set_hertzian_pair(0, 0, 5e6, 1e6, 0.5, 0.1, 0.1, 0.3, 0.05, 0.05)
+    s.min_bound = np.zeros(3)
+    s.max_bound = box * np.ones(3)
+    s.dt = 1e-5
+    return s
+
+
+@pytest.mark.parametrize("name", ["random", "collapse"])
+def test_neighbor_search_bit_exact(name):
+    s = random_scene() if name == "random" else scenes.granular_collapse()
+    o = oracle.OracleDEM(s, solve_begin=False)
+    g = ts.DEMSceneGPU(s, solve_begin=False)
+    o.neighbor_search()
+    g.neighbor_search()
+    assert np.array_equal(g.get("hashIndex"), o.get("index"))
+    assert np.array_equal(g.get("cellHashStart"), o.get("cellStart"))
+    assert np.array_equal(g.get("cellHashEnd"), o.get("cellEnd"))
+    assert np.array_equal(g.get("objectPointed"), o.get("pointed"))
+    assert np.array_equal(g.get("objectPointing"), o.get("pointing"))
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_forces_after_solve_begin(fp64):
+    s = random_scene()
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s, fp64=fp64)
+    # FP32 forces: 1e-5 of the largest force; FP64: 1e-10
+    tol = 1e-10 if fp64 else 1e-5
+    assert _rel(g.get("force"), o.get("force")) < tol
+    assert _rel(g.get("torque"), o.get("torque")) < tol
+    np.testing.assert_allclose(g.get("overlap"), o.get("overlap"), atol=1e-15)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_random_scene_100_steps(fp64):
+    s = random_scene()
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s, fp64=fp64)
+    o.step(100)
+    g.step(100)
+    d = 2 * 0.005
+    err = np.abs(g.get("position") - o.get("pos")).max() / d
+    assert err < (1e-12 if fp64 else 1e-7), err
+    assert np.array_equal(g.get("objectPointing"), o.get("pointing"))
+    # spring history carried through 20 rebuilds
+    assert _rel(g.get("slidingSpring"), o.get("sliding")) < (1e-9 if fp64 else 1e-4)
+
+
+def test_elastic_bounce_matches_reference_frames():
+    s = scenes.elastic_bounce()
+    g = ts.DEMSceneGPU(s)
+    fx = np.load(GOLDEN / "elastic_bounce_frames.npz")["frames"]
+    z, vz = [], []
+    for _ in range(200):
+        g.step(100)
+        z.append(g.get("position")[0, 2])
+        vz.append(g.get("velocity")[0, 2])
+    # FP32 contact force arithmetic: positions within 1e-6 m, velocities 1e-5 m/s
+    assert np.abs(np.array(z) - fx[1:, 4]).max() < 1e-6
+    assert np.abs(np.array(vz) - fx[1:, 7]).max() < 1e-5
+
+
+def test_bonded_beam_matches_oracle_and_closed_form():
+    s = scenes.bonded_beam()
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s)
+    assert g.get("bondPointed").tolist() == list(range(10))
+    o.step(20000)
+    g.step(20000)
+    assert _rel(g.get("position")[:, 2], o.get("pos")[:, 2]) < 1e-5
+    assert _rel(g.get("bondNormalForce"), o.get("bond_normalForce")) < 1e-4
+    g.step(100000)
+    tip = g.get("position")[-1, 2]
+    eb = 1e5 * 4.0 ** 3 / (3 * 200e9 * math.pi * 0.2 ** 4 / 4)
+    assert abs(tip - eb) / eb < 0.01
+
+
+def test_hertz_collision_closed_form():
+    E, r, rho, v0 = 1e8, 0.01, 2500.0, 0.5
+    s = scenes.DEMScene("two_balls")
+    s.set_hertzian_pair(0, 0, E, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    s.add_balls([[-r - 1e-5, 0.0, 0.0], [r + 1e-5, 0.0, 0.0]], [[v0, 0, 0], [-v0, 0, 0]], [0, 0, 0], r, rho, 0)
+    s.min_bound, s.max_bound, s.gravity, s.dt = -0.1 * np.ones(3), 0.1 * np.ones(3), np.zeros(3), 1e-6
+    g = ts.DEMSceneGPU(s)
+    steps, dmax = 0, 0.0
+    for k in range(1000):
+        g.step(1)
+        ov = g.get("overlap")
+        if ov.size and ov[0] > 0:
+            steps += 1
+            dmax = max(dmax, ov[0])
+        elif steps:
+            break
+    m = 4 / 3 * math.pi * r ** 3 * rho
+    ms, Rs, v = m / 2, r / 2, 2 * v0
+    tc = 2.868 * (ms ** 2 / (Rs * E ** 2 * v)) ** 0.2
+    assert abs(steps * s.dt - tc) / tc < 0.01
+    assert abs(dmax - (15 * ms * v ** 2 / (16 * E * math.sqrt(Rs))) ** 0.4) / dmax < 0.005
+    assert abs(g.get("velocity")[1, 0] - v0) / v0 < 2e-3
+
+
+def test_granular_collapse_matches_oracle():
+    s = scenes.granular_collapse()
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s)
+    o.step(300)
+    g.step(300)
+    err = np.abs(g.get("position") - o.get("pos")).max() / 0.01
+    assert err < 1e-5, err
+    assert np.array_equal(g.get("objectPointing"), o.get("pointing"))
+    assert np.array_equal(g.get("triPointing"), o.get("tri_pointing"))
+    assert np.array_equal(g.get("triCancel"), o.get("tri_cancel"))
+
+
+def test_clumps_on_plane_match_oracle():
+    s = scenes.DEMScene("clumps")
+    rng = np.random.default_rng(5)
+    a = 0.02 / math.sqrt(8.0)
+    tet = np.array([[a, a, a], [a, -a, -a], [-a, a, -a], [-a, -a, a]])
+    for k in range(24):
+        c = np.array([0.05 + 0.06 * (k % 6), 0.05 + 0.06 * (k // 6), 0.05 + 0.01 * (k % 3)])
+        s.add_clump(tet + c, np.full(4, 0.008), c, rng.normal(scale=0.05, size=3), rng.normal(size=3), 0.02,
+                    [2e-6, 2.5e-6, 3e-6, 0, 0, 0], 0)
+    s.set_linear_pair(0, 0, 2e3, 2e3 / 2.6, 0, 0, 0.2, 0.2, 0, 0, 0.4, 0, 0)
+    v = np.array([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [1.0, 1.0, 0.0], [-1.0, 1.0, 0.0]])
+    s.add_mesh_wall(v, [0, 1], [1, 2], [3, 3], [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    s.min_bound = np.array([-0.05, -0.05, -0.05])
+    s.max_bound = np.array([0.45, 0.25, 0.2])
+    s.dt = 2e-5
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s)
+    o.step(1500)
+    g.step(1500)
+    assert np.abs(g.get("clumpPosition") - o.get("clump_pos")).max() < 1e-7
+    assert np.abs(g.get("position") - o.get("pos")).max() < 1e-7
+
+
+def test_user_hook_through_views_equals_builtin_hook():
+    """addExternalForceTorque through the C-ABI callback, calling the HIP
+    launcher on the insertion-ordered FP64 views, against the built-in
+    device hook program: bit-identical trajectories."""
+    s = scenes.bonded_beam()
+    builtin = ts.DEMSceneGPU(s)
+    s2 = scenes.bonded_beam()
+    s2.hooks = []
+    user = ts.DEMSceneGPU(s2)
+    L = ts.lib()
+    L.launchAddConstantForce.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p]
+    L.launchAddGlobalDampingForceTorque.argtypes = [C.c_void_p] * 4 + [C.c_double, C.c_size_t, C.c_size_t,
+                                                                         C.c_size_t, C.c_void_p]
+    import torch
+
+    f_e = torch.zeros((11, 3), dtype=torch.float64, device="cuda:0")
+
+    def hook(i, t):
+        v = user.views()
+        c = min(t, 1.0)
+        f_e[10, 2] = 1e5 * c
+        torch.cuda.synchronize()
+        L.launchAddConstantForce(v.force, f_e.data_ptr(), v.numBalls, 1, 64, None)
+        L.launchAddGlobalDampingForceTorque(v.force, v.torque, v.velocity, v.angularVelocity, 0.1, v.numBalls, 1, 64,
+                                            None)
+
+    user.set_hook(hook)
+    builtin.step(300)
+    user.step(300)
+    assert np.array_equal(builtin.get("position"), user.get("position"))
+    assert np.array_equal(builtin.get("velocity"), user.get("velocity"))

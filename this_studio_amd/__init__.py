@@ -212,3 +212,237 @@ def grid_geometry(min_bound, max_bound, cell_one_dim: float):
     gs = np.zeros(3, dtype=np.int32)
     check(lib().demsph_grid_geometry(_d(mn), _d(mx), float(cell_one_dim), _d(cs), gs.ctypes.data_as(_ip)))
     return cs, gs
+
+
+# --------------------------------------------------------------------- DEM
+_DEM_DECLARED = False
+HOOK_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, C.c_double)
+
+
+class BallViews(C.Structure):
+    _fields_ = [("force", C.c_void_p), ("torque", C.c_void_p), ("position", C.c_void_p), ("velocity", C.c_void_p),
+                ("angularVelocity", C.c_void_p), ("radius", C.c_void_p), ("inverseMass", C.c_void_p),
+                ("numBalls", C.c_size_t)]
+
+
+def _declare_dem(L):
+    global _DEM_DECLARED
+    if _DEM_DECLARED:
+        return
+    d = C.c_double
+    sig = {
+        "demsph_dem_create": [C.POINTER(_vp), _vp],
+        "demsph_dem_destroy": [_vp],
+        "demsph_dem_set_linear_pair": [_vp, C.c_int, C.c_int] + [d] * 11,
+        "demsph_dem_set_hertzian_pair": [_vp, C.c_int, C.c_int] + [d] * 8,
+        "demsph_dem_set_bonded_pair": [_vp, C.c_int, C.c_int] + [d] * 6,
+        "demsph_dem_add_balls": [_vp, _sz, _dp, _dp, _dp, _dp, _dp, _ip, _ip],
+        "demsph_dem_add_fixed_balls": [_vp, _sz, _dp, _dp, _ip, _ip],
+        "demsph_dem_add_clump": [_vp, _sz, _dp, _dp, _dp, _dp, _dp, d, _dp, C.c_int],
+        "demsph_dem_add_fixed_clump": [_vp, _sz, _dp, _dp, _dp, C.c_int],
+        "demsph_dem_add_mesh_wall": [_vp, _sz, _dp, _sz, _ip, _ip, _ip, _dp, _dp, _dp, C.c_int],
+        "demsph_dem_add_bonded_interactions": [_vp, _sz, _ip, _ip, _sz, _dp],
+        "demsph_dem_set_precision": [_vp, C.c_int],
+        "demsph_dem_initialize": [_vp, _dp, _dp],
+        "demsph_dem_neighbor_search": [_vp],
+        "demsph_dem_contact": [_vp, d],
+        "demsph_dem_triangle_contact": [_vp, d],
+        "demsph_dem_first_half": [_vp, _dp, d],
+        "demsph_dem_second_half": [_vp, _dp, d],
+        "demsph_dem_solve_begin": [_vp],
+        "demsph_dem_step": [_vp, _dp, d, C.c_int],
+        "demsph_dem_synchronize": [_vp],
+        "demsph_dem_clock": [_vp, C.POINTER(_sz), C.POINTER(d)],
+        "demsph_dem_set_hook": [_vp, HOOK_FN, _vp],
+        "demsph_dem_views": [_vp, C.POINTER(BallViews)],
+        "demsph_dem_hook_clear": [_vp],
+        "demsph_dem_hook_constant_force": [_vp, _dp, d],
+        "demsph_dem_hook_global_damping": [_vp, d],
+        "demsph_dem_hook_buoyancy_drag": [_vp, d, d, _dp, _dp, d],
+        "demsph_dem_sizes": [_vp] + [C.POINTER(_sz)] * 6,
+        "demsph_dem_get": [_vp, C.c_char_p, _vp, _sz, C.POINTER(_sz)],
+        "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
+        "demsph_dem_enable_timing": [_vp, C.c_int],
+        "demsph_dem_timing": [_vp, C.c_int, C.c_char_p, _sz, C.POINTER(C.c_long), C.POINTER(C.c_double)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _DEM_DECLARED = True
+
+
+DEM_INT_FIELDS = {"objectPointed", "objectPointing", "triPointed", "triPointing", "triCancel", "bondPointed",
+                  "bondPointing", "bondIsBonded", "hashValue", "hashIndex", "cellHashStart", "cellHashEnd"}
+DEM_VEC_FIELDS = {"position", "velocity", "angularVelocity", "force", "torque", "slidingSpring", "rollingSpring",
+                  "torsionSpring", "triSlidingSpring", "bondShearForce", "bondBendingTorque", "bondPoint",
+                  "bondNormal", "clumpPosition", "clumpVelocity", "clumpAngularVelocity", "wallPosition",
+                  "vertexGlobalPosition"}
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
+class DEMSceneGPU:
+    """One DEM scene on the GPU (the reference's DEMSolver,
+    DEMSPH/kernel/DEMSolver.h:144-2079) through the C ABI, built from a
+    this_studio_amd.scenes.DEMScene recording."""
+
+    def __init__(self, scene=None, fp64: bool = False, solve_begin: bool = True, stream: int | None = None):
+        L = lib()
+        _declare_dem(L)
+        self.L = L
+        self._h = _vp()
+        self._hook_ref = None
+        check(L.demsph_dem_create(C.byref(self._h), _vp(stream) if stream else None))
+        check(L.demsph_dem_set_precision(self._h, 1 if fp64 else 0))
+        self.scene = scene
+        if scene is not None:
+            self._replay(scene)
+            if solve_begin:
+                self.solve_begin()
+
+    def _replay(self, s):
+        L, h = self.L, self._h
+        for kind, a, b, p in s.materials:
+            fn = {"linear": L.demsph_dem_set_linear_pair, "hertz": L.demsph_dem_set_hertzian_pair,
+                  "bonded": L.demsph_dem_set_bonded_pair}[kind]
+            check(fn(h, int(a), int(b), *[float(x) for x in p]))
+        for kind, o in s.objects:
+            if kind == "balls":
+                n = o["pos"].shape[0]
+                check(L.demsph_dem_add_balls(h, n, _d(_f64(o["pos"], n, 3)), _d(_f64(o["vel"], n, 3)),
+                                             _d(_f64(o["w"], n, 3)), _d(_f64(o["r"], n)), _d(_f64(o["density"], n)),
+                                             _i32(o["mat"]).ctypes.data_as(_ip), _i32(o["clump"]).ctypes.data_as(_ip)))
+            elif kind == "fixed":
+                n = o["pos"].shape[0]
+                check(L.demsph_dem_add_fixed_balls(h, n, _d(_f64(o["pos"], n, 3)), _d(_f64(o["r"], n)),
+                                                   _i32(o["mat"]).ctypes.data_as(_ip),
+                                                   _i32(o["clump"]).ctypes.data_as(_ip)))
+            elif kind == "clump":
+                n = o["points"].shape[0]
+                check(L.demsph_dem_add_clump(h, n, _d(_f64(o["points"], n, 3)), _d(_f64(o["radii"], n)),
+                                             _d(_f64(o["centroid"], 3)), _d(_f64(o["vel"], 3)), _d(_f64(o["w"], 3)),
+                                             float(o["mass"]), _d(_f64(o["inertia"], 6)), int(o["mat"])))
+            elif kind == "fixed_clump":
+                n = o["points"].shape[0]
+                check(L.demsph_dem_add_fixed_clump(h, n, _d(_f64(o["points"], n, 3)), _d(_f64(o["radii"], n)),
+                                                   _d(_f64(o["centroid"], 3)), int(o["mat"])))
+            elif kind == "wall":
+                nv = o["verts"].shape[0]
+                t0, t1, t2 = _i32(o["t0"]), _i32(o["t1"]), _i32(o["t2"])
+                check(L.demsph_dem_add_mesh_wall(h, nv, _d(_f64(o["verts"], nv, 3)), t0.size,
+                                                 t0.ctypes.data_as(_ip), t1.ctypes.data_as(_ip),
+                                                 t2.ctypes.data_as(_ip), _d(_f64(o["pos"], 3)), _d(_f64(o["vel"], 3)),
+                                                 _d(_f64(o["w"], 3)), int(o["mat"])))
+        check(L.demsph_dem_initialize(h, _d(_f64(s.min_bound, 3)), _d(_f64(s.max_bound, 3))))
+        n = self.sizes()[0]
+        for hk in s.hooks:
+            if hk[0] == "const":
+                check(L.demsph_dem_hook_constant_force(h, _d(_f64(hk[1], n, 3)), float(hk[2])))
+            elif hk[0] == "damping":
+                check(L.demsph_dem_hook_global_damping(h, float(hk[1])))
+            elif hk[0] == "buoyancy":
+                check(L.demsph_dem_hook_buoyancy_drag(h, float(hk[1]), float(hk[2]), _d(_f64(hk[3], 3)),
+                                                      _d(_f64(hk[4], 3)), float(hk[5])))
+
+    def solve_begin(self):
+        """solve() up to the loop, with addInitialCondition's bonds from the
+        initial pair list when the scene asks for them (DEMSolver.h:1994-1998)."""
+        check(self.L.demsph_dem_solve_begin(self._h))
+        s = self.scene
+        bonds = None
+        if s is not None and s.bonds_from_initial_pairs:
+            bonds = (self.get("objectPointed"), self.get("objectPointing"))
+        if s is not None and s.explicit_bonds is not None:
+            bonds = s.explicit_bonds
+        if bonds is not None:
+            pos = self.get("position")
+            a, b = _i32(bonds[0]), _i32(bonds[1])
+            check(self.L.demsph_dem_add_bonded_interactions(self._h, a.size, a.ctypes.data_as(_ip),
+                                                            b.ctypes.data_as(_ip), pos.shape[0],
+                                                            _d(_f64(pos, pos.shape[0], 3))))
+            check(self.L.demsph_dem_initialize(self._h, _d(_f64(s.min_bound, 3)), _d(_f64(s.max_bound, 3))))
+
+    def close(self):
+        if self._h:
+            check(self.L.demsph_dem_destroy(self._h))
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1, dt: float | None = None, gravity=None):
+        g = _f64(self.scene.gravity if gravity is None else gravity, 3)
+        check(self.L.demsph_dem_step(self._h, _d(g), float(dt or self.scene.dt), int(n)))
+
+    def neighbor_search(self):
+        check(self.L.demsph_dem_neighbor_search(self._h))
+
+    def contact(self, dt):
+        check(self.L.demsph_dem_contact(self._h, float(dt)))
+
+    def triangle_contact(self, dt):
+        check(self.L.demsph_dem_triangle_contact(self._h, float(dt)))
+
+    def synchronize(self):
+        check(self.L.demsph_dem_synchronize(self._h))
+
+    def clock(self):
+        i, t = _sz(), C.c_double()
+        check(self.L.demsph_dem_clock(self._h, C.byref(i), C.byref(t)))
+        return i.value, t.value
+
+    def sizes(self):
+        v = [_sz() for _ in range(6)]
+        check(self.L.demsph_dem_sizes(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_hook(self, fn):
+        """fn(iStep, time) runs on the host at each addExternalForceTorque
+        point with views() valid; None removes it."""
+        if fn is None:
+            self._hook_ref = None
+            check(self.L.demsph_dem_set_hook(self._h, HOOK_FN(), None))
+            return
+        self._hook_ref = HOOK_FN(lambda user, i, t: fn(i, t))
+        check(self.L.demsph_dem_set_hook(self._h, self._hook_ref, None))
+
+    def views(self) -> BallViews:
+        v = BallViews()
+        check(self.L.demsph_dem_views(self._h, C.byref(v)))
+        return v
+
+    def get(self, field: str) -> np.ndarray:
+        cnt = _sz()
+        check(self.L.demsph_dem_get(self._h, field.encode(), None, 0, C.byref(cnt)))
+        dt = np.int32 if field in DEM_INT_FIELDS else np.float64
+        out = np.empty(cnt.value, dtype=dt)
+        check(self.L.demsph_dem_get(self._h, field.encode(), out.ctypes.data_as(_vp), cnt.value, C.byref(cnt)))
+        if field in DEM_VEC_FIELDS:
+            out = out.reshape(-1, 3)
+        if field in ("clumpOrientation", "wallOrientation"):
+            out = out.reshape(-1, 4)
+        return out
+
+    def set(self, field: str, values):
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
+        check(self.L.demsph_dem_set(self._h, field.encode(), v.ctypes.data_as(_vp), v.size))
+
+    def enable_timing(self, on: bool = True):
+        check(self.L.demsph_dem_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> dict:
+        out, i = {}, 0
+        buf = C.create_string_buffer(128)
+        while True:
+            n, ms = C.c_long(), C.c_double()
+            if self.L.demsph_dem_timing(self._h, i, buf, 128, C.byref(n), C.byref(ms)) != 0:
+                break
+            out[buf.value.decode()] = (n.value, ms.value)
+            i += 1
+        return out

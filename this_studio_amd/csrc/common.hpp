@@ -169,5 +169,14 @@ void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, 
 
 // Fill n ints with v.
 void fill_i32(int* p, long long n, int v, hipStream_t s);
+// p[i] = i.
+void iota_i32(int* p, int n, hipStream_t s);
+// key[orig[s]] = cell hash of SoA point s (orig == nullptr: key[s]).
+void hash_soa_by_orig(int n, const GridGeom& g, const double* px, const double* py, const double* pz, const int* orig,
+                      uint32_t* key, hipStream_t s);
+// key[i] = cell hash of interleaved point i.
+void hash_aos(const double* pos, int n, const GridGeom& g, uint32_t* key, hipStream_t s);
+// findStartAndEnd over sorted keys; start/end must hold -1 beforehand.
+void cell_ranges(const uint32_t* sorted_keys, int n, long long numGrids, int* start, int* end, hipStream_t s);
 
 }  // namespace demsph

@@ -1,0 +1,2010 @@
+// DEM scene for MI355X: balls in cell-sorted SoA order (reordered at each
+// pair-list rebuild), CSR half pair lists per owner with spring history
+// carried by insertion indices, a counting-sort reverse index for the
+// partner-side reduction, per-bond state, per-ball wall contacts, rigid
+// clumps and mesh walls, and the external-force hook.
+//
+// Reference: DEMSPH/kernel/DEMSolver.h (sequencing :428-906, :1376-1485,
+// :1961-2049), ballNeighborSearchKernel.cu, contactKernel.cu(h),
+// ballIntegrationKernel.cu, wallIntegrationKernel.cu(h), and the hook
+// kernels of DEMSPH/externalForceTorque.cu.
+//
+// Precision: positions, velocities, angular velocities, overlaps, contact
+// geometry, spring history, bond state, clump and wall state are FP64; the
+// contact force model runs in FP32 (default) or FP64, selected per scene.
+#include "common.hpp"
+#include "dem_kernels.hpp"
+#include "timing.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <set>
+#include <utility>
+#include <vector>
+
+namespace demsph {
+namespace {
+
+using namespace dem;
+constexpr int kT = 256;
+
+struct Balls {
+    int n;
+    double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *r, *im;
+    int *mat, *clump, *orig;
+};
+
+struct Forces {
+    double *fx, *fy, *fz, *tx, *ty, *tz;
+};
+
+struct Pairs {
+    int* start;    // n + 1 (owner CSR)
+    int* partner;  // cap
+    double *sx, *sy, *sz, *rx, *ry, *rz, *ox, *oy, *oz;  // sliding, rolling, torsion springs
+    float *Fx, *Fy, *Fz, *Rx, *Ry, *Rz;                  // pair force, partner-side torque
+    double* ov;                                           // overlap
+};
+
+__device__ __forceinline__ D3 ld3(const double* x, const double* y, const double* z, int i) { return {x[i], y[i], z[i]}; }
+__device__ __forceinline__ void st3(double* x, double* y, double* z, int i, D3 v) { x[i] = v.x; y[i] = v.y; z[i] = v.z; }
+
+// ----------------------------------------------------------------- rebuild
+struct ReorderArgs {
+    Balls src, dst;
+    const uint32_t* skey;
+    const int* perm;
+    int* rank;
+    long long numGrids;
+    int *cellStart, *cellEnd;
+};
+
+__global__ __launch_bounds__(kT) void k_reorder(ReorderArgs a)
+{
+    const int k = blockIdx.x * kT + threadIdx.x;
+    if (k >= a.src.n) return;
+    const int o = a.perm[k];
+    const int s = a.rank[o];
+    a.rank[o] = k;
+    a.dst.orig[k] = o;
+    a.dst.px[k] = a.src.px[s]; a.dst.py[k] = a.src.py[s]; a.dst.pz[k] = a.src.pz[s];
+    a.dst.vx[k] = a.src.vx[s]; a.dst.vy[k] = a.src.vy[s]; a.dst.vz[k] = a.src.vz[s];
+    a.dst.wx[k] = a.src.wx[s]; a.dst.wy[k] = a.src.wy[s]; a.dst.wz[k] = a.src.wz[s];
+    a.dst.r[k] = a.src.r[s]; a.dst.im[k] = a.src.im[s];
+    a.dst.mat[k] = a.src.mat[s]; a.dst.clump[k] = a.src.clump[s];
+    const uint32_t h = a.skey[k];
+    if ((long long)h < a.numGrids) {
+        if (k == 0 || a.skey[k - 1] != h) a.cellStart[h] = k;
+        if (k == a.src.n - 1 || a.skey[k + 1] != h) a.cellEnd[h] = k + 1;
+    }
+}
+
+// Walks the 27-cell stencil of ball A and visits every ball B that the
+// reference pairs with A (ballNeighborSearchKernel.cu:6-59): insertion
+// index of A below that of B, not both fixed, not one clump, within
+// 1.1 (rA + rB).  Visit order = the reference's write order.
+template <class F>
+__device__ __forceinline__ void walk_pairs(const Balls& b, const GridGeom& g, const int* cs, const int* ce, int A,
+                                           F&& visit)
+{
+    const double xa = b.px[A], ya = b.py[A], za = b.pz[A], ra = b.r[A];
+    const int oa = b.orig[A];
+    const bool fa = b.im[A] == 0.0;
+    const int ca = b.clump[A];
+    const Stencil st = make_stencil(g, xa, ya, za);
+    for (int zz = st.loz; zz <= st.hiz; zz++)
+        for (int yy = st.loy; yy <= st.hiy; yy++) {
+            const int row = (st.cz + zz) * g.ny * g.nx + (st.cy + yy) * g.nx + st.cx;
+            for (int xx = st.lox; xx <= st.hix; xx++) {
+                const int hb = row + xx;
+                const int s0 = cs[hb];
+                if (s0 == -1) continue;
+                const int s1 = ce[hb];
+                for (int j = s0; j < s1; j++) {
+                    if (oa >= b.orig[j]) continue;
+                    if (fa && b.im[j] == 0.0) continue;
+                    const int cb = b.clump[j];
+                    if (cb >= 0 && ca == cb) continue;
+                    const double cut = 1.1 * (ra + b.r[j]);
+                    if (within_cut(cut, xa - b.px[j], ya - b.py[j], za - b.pz[j])) visit(j);
+                }
+            }
+        }
+}
+
+__global__ __launch_bounds__(kT) void k_count_pairs(Balls b, GridGeom g, const int* cs, const int* ce, int* cnt)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= b.n) return;
+    int c = 0;
+    walk_pairs(b, g, cs, ce, i, [&](int) { c++; });
+    cnt[i] = c;
+}
+
+struct WritePairsArgs {
+    Balls b;
+    GridGeom g;
+    const int *cs, *ce;
+    Pairs cur, old;
+    const int* old_orig;  // old slot -> insertion index
+    const int* rank_old;  // insertion index -> old slot
+    int has_old;
+};
+
+// writeBallInteractionsKernel (ballNeighborSearchKernel.cu:61-151) with the
+// history carried by insertion indices: a new pair (A,B) takes the springs
+// of the old pair (A,B) if it existed, else zero.
+__global__ __launch_bounds__(kT) void k_write_pairs(WritePairsArgs a)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    int w = a.cur.start[i];
+    const int oa = a.b.orig[i];
+    int os0 = 0, os1 = 0;
+    if (a.has_old) {
+        const int s = a.rank_old[oa];
+        os0 = a.old.start[s];
+        os1 = a.old.start[s + 1];
+    }
+    walk_pairs(a.b, a.g, a.cs, a.ce, i, [&](int j) {
+        a.cur.partner[w] = j;
+        double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int ob = a.b.orig[j];
+        for (int p = os0; p < os1; p++) {
+            if (a.old_orig[a.old.partner[p]] == ob) {
+                s[0] = a.old.sx[p]; s[1] = a.old.sy[p]; s[2] = a.old.sz[p];
+                s[3] = a.old.rx[p]; s[4] = a.old.ry[p]; s[5] = a.old.rz[p];
+                s[6] = a.old.ox[p]; s[7] = a.old.oy[p]; s[8] = a.old.oz[p];
+                break;
+            }
+        }
+        a.cur.sx[w] = s[0]; a.cur.sy[w] = s[1]; a.cur.sz[w] = s[2];
+        a.cur.rx[w] = s[3]; a.cur.ry[w] = s[4]; a.cur.rz[w] = s[5];
+        a.cur.ox[w] = s[6]; a.cur.oy[w] = s[7]; a.cur.oz[w] = s[8];
+        w++;
+    });
+}
+
+// Reverse index: pairs grouped by partner, ascending pair id within each
+// partner (the reference sorts pairs by pointing object, DEMSolver.h:59-78).
+__global__ __launch_bounds__(kT) void k_rev_count(int n, const int* start, const int* partner, int* rcnt)
+{
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    for (int p = start[i]; p < start[i + 1]; p++) atomicAdd(&rcnt[partner[p]], 1);
+}
+__global__ __launch_bounds__(kT) void k_rev_fill(int n, const int* start, const int* partner, const int* rstart,
+                                                 int* rcursor, int* rpair)
+{
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    for (int p = start[i]; p < start[i + 1]; p++) {
+        const int j = partner[p];
+        rpair[rstart[j] + atomicAdd(&rcursor[j], 1)] = p;
+    }
+}
+__global__ __launch_bounds__(kT) void k_rev_sort(int n, const int* rstart, int* rpair)
+{
+    const int j = blockIdx.x * kT + threadIdx.x;
+    if (j >= n) return;
+    const int s0 = rstart[j], s1 = rstart[j + 1];
+    for (int a = s0 + 1; a < s1; a++) {
+        const int v = rpair[a];
+        int b = a - 1;
+        while (b >= s0 && rpair[b] > v) {
+            rpair[b + 1] = rpair[b];
+            b--;
+        }
+        rpair[b + 1] = v;
+    }
+}
+
+// ----------------------------------------------------------------- forces
+struct PairForceArgs {
+    Balls b;
+    Pairs p;
+    Tables tb;
+    double dt;
+    float *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner-side sums per slot
+};
+
+// updateBallContactKernel + calBallContactForceTorqueKernel
+// (contactKernel.cu:135-259), one thread per owner ball over its pairs, and
+// the owner half of sumObjectPointedForceTorque (:657-681).
+template <class T>
+__global__ __launch_bounds__(kT) void k_pair_forces(PairForceArgs a)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i);
+    const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i);
+    const D3 wi = ld3(a.b.wx, a.b.wy, a.b.wz, i);
+    const double ai = a.b.r[i], imi = a.b.im[i];
+    const int mi = a.b.mat[i];
+    V3<T> Fo{0, 0, 0}, To{0, 0, 0};
+    const int p0 = a.p.start[i], p1 = a.p.start[i + 1];
+    for (int p = p0; p < p1; p++) {
+        const int j = a.p.partner[p];
+        const D3 rj = ld3(a.b.px, a.b.py, a.b.pz, j);
+        const double aj = a.b.r[j];
+        const D3 rij = ri - rj;
+        const D3 n = normalize(rij);
+        const double delta = ai + aj - len(rij);
+        const D3 rc = rj + n * (aj - 0.5 * delta);
+        a.p.ov[p] = delta;
+        D3 es{a.p.sx[p], a.p.sy[p], a.p.sz[p]}, er{a.p.rx[p], a.p.ry[p], a.p.rz[p]}, et{a.p.ox[p], a.p.oy[p], a.p.oz[p]};
+        V3<T> F{0, 0, 0}, Tq{0, 0, 0};
+        if (delta > 0) {
+            const double R = ai * aj / (ai + aj);
+            const double m = 1. / (imi + a.b.im[j]);
+            const D3 vj = ld3(a.b.vx, a.b.vy, a.b.vz, j), wj = ld3(a.b.wx, a.b.wy, a.b.wz, j);
+            const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
+            const int ip = pair_index(mi, a.b.mat[j], a.tb.nMat, a.tb.cap);
+            contact_model<T>(a.tb, ip, m, R, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es, er, et, F, Tq);
+        } else {
+            es = er = et = D3{0, 0, 0};
+        }
+        a.p.sx[p] = es.x; a.p.sy[p] = es.y; a.p.sz[p] = es.z;
+        a.p.rx[p] = er.x; a.p.ry[p] = er.y; a.p.rz[p] = er.z;
+        a.p.ox[p] = et.x; a.p.oy[p] = et.y; a.p.oz[p] = et.z;
+        const V3<T> li = cvt<T>(rc - ri), lj = cvt<T>(rc - rj);
+        const V3<T> Ti = Tq + cross(li, F);
+        const V3<T> Tj = Tq + cross(lj, F);
+        a.p.Fx[p] = (float)F.x; a.p.Fy[p] = (float)F.y; a.p.Fz[p] = (float)F.z;
+        a.p.Rx[p] = (float)Tj.x; a.p.Ry[p] = (float)Tj.y; a.p.Rz[p] = (float)Tj.z;
+        Fo = Fo + F;
+        To = To + Ti;
+    }
+    a.ofx[i] = (float)Fo.x; a.ofy[i] = (float)Fo.y; a.ofz[i] = (float)Fo.z;
+    a.otx[i] = (float)To.x; a.oty[i] = (float)To.y; a.otz[i] = (float)To.z;
+}
+
+struct BondArgs {
+    int nb;
+    Balls b;
+    const int* rank;
+    const int *bi, *bj;  // insertion indices
+    int* bonded;
+    double *bpx, *bpy, *bpz, *bnx, *bny, *bnz, *bsx, *bsy, *bsz, *btx, *bty, *btz, *bFn, *bTt, *bsig, *btau;
+    float *Fx, *Fy, *Fz, *Tix, *Tiy, *Tiz, *Tjx, *Tjy, *Tjz;  // per-bond contributions to i and j
+    Tables tb;
+    double dt;
+};
+
+// calBondedForceTorqueKernel (contactKernel.cu:535-655).  The bond force
+// goes to both balls through the per-ball gather (no atomics); the lever
+// arm of the shear force is taken about each ball centre.
+__global__ __launch_bounds__(kT) void k_bond_forces(BondArgs a)
+{
+    const int k = blockIdx.x * kT + threadIdx.x;
+    if (k >= a.nb) return;
+    if (a.bonded[k] == 0) {
+        a.bFn[k] = 0; a.bTt[k] = 0;
+        a.bsx[k] = a.bsy[k] = a.bsz[k] = 0;
+        a.btx[k] = a.bty[k] = a.btz[k] = 0;
+        a.Fx[k] = a.Fy[k] = a.Fz[k] = 0.f;
+        a.Tix[k] = a.Tiy[k] = a.Tiz[k] = 0.f;
+        a.Tjx[k] = a.Tjy[k] = a.Tjz[k] = 0.f;
+        return;
+    }
+    const int i = a.rank[a.bi[k]], j = a.rank[a.bj[k]];
+    const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i), rj = ld3(a.b.px, a.b.py, a.b.pz, j);
+    const double ai = a.b.r[i], aj = a.b.r[j];
+    const D3 n0 = ld3(a.bnx, a.bny, a.bnz, k);
+    const D3 n = normalize(ri - rj);
+    const double delta = ai + aj - len(ri - rj);
+    const D3 rc = rj + n * (aj - 0.5 * delta);
+    st3(a.bpx, a.bpy, a.bpz, k, rc);
+    st3(a.bnx, a.bny, a.bnz, k, n);
+    const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i), vj = ld3(a.b.vx, a.b.vy, a.b.vz, j);
+    const D3 wi = ld3(a.b.wx, a.b.wy, a.b.wz, i), wj = ld3(a.b.wx, a.b.wy, a.b.wz, j);
+    const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
+    const int ip = pair_index(a.b.mat[i], a.b.mat[j], a.tb.nMat, a.tb.cap);
+    double Fn = a.bFn[k], Tt = a.bTt[k], sig = a.bsig[k], tau = a.btau[k];
+    D3 Fs = ld3(a.bsx, a.bsy, a.bsz, k), Tb = ld3(a.btx, a.bty, a.btz, k);
+    a.bonded[k] = bond_model(Fn, Tt, Fs, Tb, sig, tau, n0, n, vc, wi, wj, ai, aj, a.dt, Bp(a.tb, ip, B_GAMMA),
+                             Bp(a.tb, ip, B_E), Bp(a.tb, ip, B_KNKS), Bp(a.tb, ip, B_SIG), Bp(a.tb, ip, B_C),
+                             Bp(a.tb, ip, B_MU));
+    a.bFn[k] = Fn; a.bTt[k] = Tt; a.bsig[k] = sig; a.btau[k] = tau;
+    st3(a.bsx, a.bsy, a.bsz, k, Fs);
+    st3(a.btx, a.bty, a.btz, k, Tb);
+    const D3 Fb = n * Fn + Fs;
+    const D3 Tn = n * Tt + Tb;
+    const D3 Ti = Tn + cross(rc - ri, Fs);
+    const D3 Tj = -Tn + cross(rc - rj, -Fs);
+    a.Fx[k] = (float)Fb.x; a.Fy[k] = (float)Fb.y; a.Fz[k] = (float)Fb.z;
+    a.Tix[k] = (float)Ti.x; a.Tiy[k] = (float)Ti.y; a.Tiz[k] = (float)Ti.z;
+    a.Tjx[k] = (float)Tj.x; a.Tjy[k] = (float)Tj.y; a.Tjz[k] = (float)Tj.z;
+}
+
+// ----------------------------------------------------------------- walls
+struct WallArgs {
+    Balls b;
+    Tables tb;
+    double dt;
+    GridGeom tg;
+    const int *tcs, *tce, *tsorted;  // triangle cells over circumcentres
+    const int *t0, *t1, *t2, *twall;
+    const double *vgx, *vgy, *vgz;     // vertex global positions
+    const double *wpx, *wpy, *wpz, *wvx, *wvy, *wvz, *wwx, *wwy, *wwz;
+    const int* wmat;
+    int KW, nballs;
+    // per-ball history keyed by insertion index, entry k at k*nballs + o
+    int* hcnt;
+    int *htri, *hcancel;
+    double *hsx, *hsy, *hsz, *hrx, *hry, *hrz, *hox, *hoy, *hoz;
+    double *hov;
+    float *ofx, *ofy, *ofz, *otx, *oty, *otz;  // wall force/torque per slot
+    int* overflow;
+};
+
+// count/writeBallTriangleInteractions (ballNeighborSearchKernel.cu:153-339),
+// updateBallTriangleContact (contactKernel.cu:261-418) and
+// calBallWallContactForceTorqueKernel (:420-533), fused per ball.  The
+// candidate list, its spring history and cancel flags live in registers
+// and are written back as the new history.
+template <class T>
+__global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
+{
+    constexpr int KMAX = 16;
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    const int o = a.b.orig[i];
+    const int N = a.nballs;
+    const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i);
+    const double rad = a.b.r[i];
+    int tri[KMAX];
+    int cnt = 0;
+    // candidate walk over the triangle grid, AABB distance test
+    const Stencil st = make_stencil(a.tg, ri.x, ri.y, ri.z);
+    for (int zz = st.loz; zz <= st.hiz; zz++)
+        for (int yy = st.loy; yy <= st.hiy; yy++)
+            for (int xx = st.lox; xx <= st.hix; xx++) {
+                const int hb = (st.cz + zz) * a.tg.ny * a.tg.nx + (st.cy + yy) * a.tg.nx + st.cx + xx;
+                const int s0 = a.tcs[hb];
+                if (s0 == -1) continue;
+                const int s1 = a.tce[hb];
+                for (int s = s0; s < s1; s++) {
+                    const int t = a.tsorted[s];
+                    const int i0 = a.t0[t], i1 = a.t1[t], i2 = a.t2[t];
+                    const double minx = fmin(a.vgx[i0], fmin(a.vgx[i1], a.vgx[i2]));
+                    const double miny = fmin(a.vgy[i0], fmin(a.vgy[i1], a.vgy[i2]));
+                    const double minz = fmin(a.vgz[i0], fmin(a.vgz[i1], a.vgz[i2]));
+                    const double maxx = fmax(a.vgx[i0], fmax(a.vgx[i1], a.vgx[i2]));
+                    const double maxy = fmax(a.vgy[i0], fmax(a.vgy[i1], a.vgy[i2]));
+                    const double maxz = fmax(a.vgz[i0], fmax(a.vgz[i1], a.vgz[i2]));
+                    const double cx = fmin(fmax(ri.x, minx), maxx), cy = fmin(fmax(ri.y, miny), maxy),
+                                 cz = fmin(fmax(ri.z, minz), maxz);
+                    const double dx = ri.x - cx, dy = ri.y - cy, dz = ri.z - cz;
+                    if (dx * dx + dy * dy + dz * dz <= rad * rad) {
+                        if (cnt < KMAX && cnt < a.KW) tri[cnt] = t;
+                        cnt++;
+                    }
+                }
+            }
+    if (cnt > KMAX || cnt > a.KW) {
+        atomicMax(a.overflow, cnt);
+        cnt = min(KMAX, a.KW);
+    }
+    // spring history carry by triangle id
+    D3 es[KMAX], er[KMAX], et[KMAX];
+    int cancel[KMAX], type[KMAX];
+    D3 cp[KMAX];
+    const int oldc = a.hcnt[o];
+    for (int c = 0; c < cnt; c++) {
+        es[c] = er[c] = et[c] = D3{0, 0, 0};
+        for (int q = 0; q < oldc; q++) {
+            const size_t h = (size_t)q * N + o;
+            if (a.htri[h] == tri[c]) {
+                es[c] = D3{a.hsx[h], a.hsy[h], a.hsz[h]};
+                er[c] = D3{a.hrx[h], a.hry[h], a.hrz[h]};
+                et[c] = D3{a.hox[h], a.hoy[h], a.hoz[h]};
+                break;
+            }
+        }
+    }
+    // classification and cancellation of duplicate edge/vertex contacts
+    for (int c = 0; c < cnt; c++) {
+        const int t = tri[c];
+        const D3 p0 = ld3(a.vgx, a.vgy, a.vgz, a.t0[t]), p1 = ld3(a.vgx, a.vgy, a.vgz, a.t1[t]),
+                 p2 = ld3(a.vgx, a.vgy, a.vgz, a.t2[t]);
+        type[c] = classify(ri, rad, p0, p1, p2, cp[c]);
+        cancel[c] = 0;
+    }
+    for (int c = 0; c < cnt; c++) {
+        if (type[c] == ST_NONE) {
+            es[c] = er[c] = et[c] = D3{0, 0, 0};
+            cancel[c] = 1;
+            continue;
+        }
+        if (type[c] == ST_FACE) continue;
+        for (int c1 = 0; c1 < cnt; c1++) {
+            if (c1 == c) continue;
+            const int t1 = type[c1];
+            if (t1 == ST_NONE) continue;
+            bool hit = false;
+            if (t1 == ST_FACE) {
+                const int t = tri[c1];
+                const D3 q0 = ld3(a.vgx, a.vgy, a.vgz, a.t0[t]), q1 = ld3(a.vgx, a.vgy, a.vgz, a.t1[t]),
+                         q2 = ld3(a.vgx, a.vgy, a.vgz, a.t2[t]);
+                const D3 e01 = cross(cp[c] - q0, q1 - q0), e12 = cross(cp[c] - q1, q2 - q1), e02 = cross(cp[c] - q0, q2 - q0);
+                hit = dot(e01, e01) < 1.e-20 || dot(e12, e12) < 1.e-20 || dot(e02, e02) < 1.e-20;
+            } else {
+                const D3 d = cp[c] - cp[c1];
+                const bool same = dot(d, d) < 1.e-20;
+                if (t1 == ST_EDGE) hit = (type[c] == t1) ? (c1 < c && same) : same;
+                else hit = (type[c] == t1) && c1 < c && same;
+            }
+            if (hit) {
+                es[c] = es[c1]; er[c] = er[c1]; et[c] = et[c1];
+                cancel[c] = 1;
+                break;
+            }
+        }
+    }
+    // forces (balls with inverse mass below 1e-20 skip, their springs keep)
+    V3<T> Fw{0, 0, 0}, Tw{0, 0, 0};
+    const double im = a.b.im[i];
+    if (!(im < 1.e-20)) {
+        const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i), wi = ld3(a.b.wx, a.b.wy, a.b.wz, i);
+        const double m = 1. / im;
+        for (int c = 0; c < cnt; c++) {
+            if (cancel[c]) continue;
+            const int t = tri[c];
+            const int w = a.twall[t];
+            const D3 rc = cp[c];
+            const D3 n = normalize(ri - rc);
+            const double delta = rad - len(ri - rc);
+            const D3 rj = ld3(a.wpx, a.wpy, a.wpz, w), vj = ld3(a.wvx, a.wvy, a.wvz, w), wj = ld3(a.wwx, a.wwy, a.wwz, w);
+            const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
+            const int ip = pair_index(a.b.mat[i], a.wmat[w], a.tb.nMat, a.tb.cap);
+            V3<T> F{0, 0, 0}, Tq{0, 0, 0};
+            contact_model<T>(a.tb, ip, m, rad, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es[c], er[c], et[c],
+                             F, Tq);
+            Fw = Fw + F;
+            Tw = Tw + Tq + cross(cvt<T>(rc - ri), F);
+        }
+    }
+    // new history
+    a.hcnt[o] = cnt;
+    for (int c = 0; c < cnt; c++) {
+        const size_t h = (size_t)c * N + o;
+        a.htri[h] = tri[c];
+        a.hcancel[h] = cancel[c];
+        a.hsx[h] = es[c].x; a.hsy[h] = es[c].y; a.hsz[h] = es[c].z;
+        a.hrx[h] = er[c].x; a.hry[h] = er[c].y; a.hrz[h] = er[c].z;
+        a.hox[h] = et[c].x; a.hoy[h] = et[c].y; a.hoz[h] = et[c].z;
+        a.hov[h] = rad - len(ri - cp[c]);
+    }
+    a.ofx[i] = (float)Fw.x; a.ofy[i] = (float)Fw.y; a.ofz[i] = (float)Fw.z;
+    a.otx[i] = (float)Tw.x; a.oty[i] = (float)Tw.y; a.otz[i] = (float)Tw.z;
+}
+
+// ----------------------------------------------------------------- gather
+struct GatherArgs {
+    Balls b;
+    const float *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner sums
+    const float *pFx, *pFy, *pFz, *pRx, *pRy, *pRz;  // pair results
+    const int *rstart, *rpair;
+    const int *istart, *ibond;                     // bond incidence by insertion index (id*2 + side)
+    const float *bFx, *bFy, *bFz, *bTix, *bTiy, *bTiz, *bTjx, *bTjy, *bTjz;
+    const float *wfx, *wfy, *wfz, *wtx, *wty, *wtz;  // walls (nullable)
+    Forces out;
+};
+
+// sumObjectPointingForceTorqueFromInteraction (contactKernel.cu:683-714) and
+// the per-ball totals: owner sums - partner-side pair terms + bonds + walls.
+__global__ __launch_bounds__(kT) void k_gather(GatherArgs a)
+{
+    const int j = xcd_block() * kT + threadIdx.x;
+    if (j >= a.b.n) return;
+    float fx = a.ofx[j], fy = a.ofy[j], fz = a.ofz[j];
+    float tx = a.otx[j], ty = a.oty[j], tz = a.otz[j];
+    for (int q = a.rstart[j]; q < a.rstart[j + 1]; q++) {
+        const int p = a.rpair[q];
+        fx -= a.pFx[p]; fy -= a.pFy[p]; fz -= a.pFz[p];
+        tx -= a.pRx[p]; ty -= a.pRy[p]; tz -= a.pRz[p];
+    }
+    if (a.istart) {
+        const int o = a.b.orig[j];
+        for (int q = a.istart[o]; q < a.istart[o + 1]; q++) {
+            const int e = a.ibond[q];
+            const int k = e >> 1;
+            if ((e & 1) == 0) {
+                fx += a.bFx[k]; fy += a.bFy[k]; fz += a.bFz[k];
+                tx += a.bTix[k]; ty += a.bTiy[k]; tz += a.bTiz[k];
+            } else {
+                fx -= a.bFx[k]; fy -= a.bFy[k]; fz -= a.bFz[k];
+                tx += a.bTjx[k]; ty += a.bTjy[k]; tz += a.bTjz[k];
+            }
+        }
+    }
+    if (a.wfx) {
+        fx += a.wfx[j]; fy += a.wfy[j]; fz += a.wfz[j];
+        tx += a.wtx[j]; ty += a.wty[j]; tz += a.wtz[j];
+    }
+    a.out.fx[j] = fx; a.out.fy[j] = fy; a.out.fz[j] = fz;
+    a.out.tx[j] = tx; a.out.ty[j] = ty; a.out.tz[j] = tz;
+}
+
+// ----------------------------------------------------------------- integration
+struct ClumpArgs {
+    int nc;
+    double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *q0, *q1, *q2, *q3;
+    const double *ixx, *iyy, *izz, *ixy, *ixz, *iyz, *im;
+    const int *ps, *pe;  // pebble insertion-index ranges
+    double *Fx, *Fy, *Fz, *Tx, *Ty, *Tz;
+    Balls b;
+    Forces f;
+    const int* rank;
+    double gx, gy, gz, h;
+};
+
+// sumClumpForceTorqueKernel (ballIntegrationKernel.cu:3-28, pebble forces),
+// clumpVelocityAngularVelocityIntegrationKernel (:58-79) and, in the first
+// half, the clump drift and orientation update (:101-121, :230-238).
+template <bool FIRST>
+__global__ __launch_bounds__(kT) void k_clump_half(ClumpArgs a)
+{
+    const int c = blockIdx.x * kT + threadIdx.x;
+    if (c >= a.nc) return;
+    const D3 rc = ld3(a.px, a.py, a.pz, c);
+    D3 F{0, 0, 0}, T{0, 0, 0};
+    for (int o = a.ps[c]; o < a.pe[c]; o++) {
+        const int s = a.rank[o];
+        const D3 r = ld3(a.b.px, a.b.py, a.b.pz, s);
+        const D3 Fi = ld3(a.f.fx, a.f.fy, a.f.fz, s);
+        F = F + Fi;
+        T = T + (ld3(a.f.tx, a.f.ty, a.f.tz, s) + cross(r - rc, Fi));
+    }
+    st3(a.Fx, a.Fy, a.Fz, c, F);
+    st3(a.Tx, a.Ty, a.Tz, c, T);
+    const double im = a.im[c];
+    D3 v = ld3(a.vx, a.vy, a.vz, c), w = ld3(a.wx, a.wy, a.wz, c);
+    Quat q{a.q0[c], a.q1[c], a.q2[c], a.q3[c]};
+    if (im > 1.e-20) {
+        v = v + (F * im + D3{a.gx, a.gy, a.gz}) * a.h;
+        const Sym I{a.ixx[c], a.iyy[c], a.izz[c], a.ixy[c], a.ixz[c], a.iyz[c]};
+        w = w + sym_mul(sym_rotate(q, I), T) * a.h;
+        st3(a.vx, a.vy, a.vz, c, v);
+        st3(a.wx, a.wy, a.wz, c, w);
+    }
+    if (FIRST) {
+        st3(a.px, a.py, a.pz, c, rc + v * (2.0 * a.h));
+        q = qintegrate(q, w, 2.0 * a.h);
+        a.q0[c] = q.q0; a.q1[c] = q.q1; a.q2[c] = q.q2; a.q3[c] = q.q3;
+    }
+}
+
+struct BallHalfArgs {
+    Balls b;
+    Forces f;
+    int nc;
+    const double *cpx, *cpy, *cpz, *cvx, *cvy, *cvz, *cwx, *cwy, *cwz;  // clumps (nullable)
+    double gx, gy, gz, h;
+};
+
+// setPebbleVelocityAngularVelocityKernel (ballIntegrationKernel.cu:81-99),
+// ballVelocityAngularVelocityIntegrationKernel (:30-56) and, in the first
+// half, positionIntegrationKernel (:112-121) for every ball.
+template <bool FIRST>
+__global__ __launch_bounds__(kT) void k_ball_half(BallHalfArgs a)
+{
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    D3 v = ld3(a.b.vx, a.b.vy, a.b.vz, i);
+    const int c = a.b.clump[i];
+    // A clump id without a clump object only excludes contacts; such a ball
+    // is neither kicked nor driven by a clump (ballIntegrationKernel.cu:44).
+    if (c >= 0 && c < a.nc) {
+        const D3 w = ld3(a.cwx, a.cwy, a.cwz, c);
+        st3(a.b.wx, a.b.wy, a.b.wz, i, w);
+        v = ld3(a.cvx, a.cvy, a.cvz, c) + cross(w, ld3(a.b.px, a.b.py, a.b.pz, i) - ld3(a.cpx, a.cpy, a.cpz, c));
+        st3(a.b.vx, a.b.vy, a.b.vz, i, v);
+    } else if (c < 0) {
+        const double im = a.b.im[i];
+        if (!(im < 1.e-20)) {
+            v = v + (ld3(a.f.fx, a.f.fy, a.f.fz, i) * im + D3{a.gx, a.gy, a.gz}) * a.h;
+            st3(a.b.vx, a.b.vy, a.b.vz, i, v);
+            const double r = a.b.r[i];
+            if (!(r < 1.e-20)) {
+                const double I = 0.4 * r * r / im;
+                D3 w = ld3(a.b.wx, a.b.wy, a.b.wz, i);
+                w = w + divide(ld3(a.f.tx, a.f.ty, a.f.tz, i), I) * a.h;
+                st3(a.b.wx, a.b.wy, a.b.wz, i, w);
+            }
+        }
+    }
+    if (FIRST) {
+        const double dt = 2.0 * a.h;
+        a.b.px[i] += dt * v.x;
+        a.b.py[i] += dt * v.y;
+        a.b.pz[i] += dt * v.z;
+    }
+}
+
+// launchWallIntegration / vertex update / circumcentres
+// (wallIntegrationKernel.cu:3-53), full dt per call.
+struct WallMoveArgs {
+    int nw, nv, nt;
+    double *px, *py, *pz, *q0, *q1, *q2, *q3;
+    const double *vx, *vy, *vz, *wx, *wy, *wz;
+    const double *lx, *ly, *lz;
+    double *gx, *gy, *gz;
+    const int* vwall;
+    const int *t0, *t1, *t2;
+    double* tcc;  // interleaved x3
+    double dt;
+};
+__global__ void k_wall_move(WallMoveArgs a)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.nw) return;
+    a.px[w] += a.dt * a.vx[w];
+    a.py[w] += a.dt * a.vy[w];
+    a.pz[w] += a.dt * a.vz[w];
+    Quat q{a.q0[w], a.q1[w], a.q2[w], a.q3[w]};
+    q = qintegrate(q, ld3(a.wx, a.wy, a.wz, w), a.dt);
+    a.q0[w] = q.q0; a.q1[w] = q.q1; a.q2[w] = q.q2; a.q3[w] = q.q3;
+}
+__global__ void k_vertex_update(WallMoveArgs a)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.nv) return;
+    const int w = a.vwall[v];
+    const Quat q{a.q0[w], a.q1[w], a.q2[w], a.q3[w]};
+    st3(a.gx, a.gy, a.gz, v, ld3(a.px, a.py, a.pz, w) + qrotate(q, ld3(a.lx, a.ly, a.lz, v)));
+}
+__global__ void k_circumcenters(WallMoveArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nt) return;
+    const int i0 = a.t0[t], i1 = a.t1[t], i2 = a.t2[t];
+    const double p0[3] = {a.gx[i0], a.gy[i0], a.gz[i0]}, p1[3] = {a.gx[i1], a.gy[i1], a.gz[i1]},
+                 p2[3] = {a.gx[i2], a.gy[i2], a.gz[i2]};
+    circumcenter(p0, p1, p2, &a.tcc[3 * (size_t)t]);
+}
+
+// ----------------------------------------------------------------- hooks
+// addConstantForce (externalForceTorque.cu:4-12), per insertion index.
+__global__ void k_hook_const(Balls b, Forces f, const double* fo, double s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const int o = b.orig[i];
+    f.fx[i] += fo[3 * (size_t)o] * s;
+    f.fy[i] += fo[3 * (size_t)o + 1] * s;
+    f.fz[i] += fo[3 * (size_t)o + 2] * s;
+}
+// addGlobalDampingForceTorque (externalForceTorque.cu:26-37)
+__global__ void k_hook_damping(Balls b, Forces f, double c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const D3 F = ld3(f.fx, f.fy, f.fz, i), T = ld3(f.tx, f.ty, f.tz, i);
+    st3(f.fx, f.fy, f.fz, i, F - normalize(ld3(b.vx, b.vy, b.vz, i)) * (c * len(F)));
+    st3(f.tx, f.ty, f.tz, i, T - normalize(ld3(b.wx, b.wy, b.wz, i)) * (c * len(T)));
+}
+// addBuoyancyDragKernel (externalForceTorque.cu:66-133)
+__global__ void k_hook_buoyancy(Balls b, Forces f, double rho, double cd, D3 g, D3 vf, double level)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    if (b.im[i] <= 0.0) return;
+    const double zc = b.pz[i], r = b.r[i];
+    const double dd = level - zc;
+    double Vs;
+    if (dd <= -r) return;
+    if (dd >= r) Vs = (4.0 / 3.0) * M_PI * r * r * r;
+    else {
+        const double hh = dd + r;
+        Vs = (M_PI * hh * hh * (3.0 * r - hh)) / 3.0;
+    }
+    if (Vs <= 0.0) return;
+    const D3 Fb{-rho * Vs * g.x, -rho * Vs * g.y, -rho * Vs * g.z};
+    const double Vb = (4.0 / 3.0) * M_PI * r * r * r;
+    const double alpha = (Vb > 1.e-30) ? (Vs / Vb) : 0.0;
+    const D3 vr = ld3(b.vx, b.vy, b.vz, i) - vf;
+    const double v2 = vr.x * vr.x + vr.y * vr.y + vr.z * vr.z;
+    D3 Fd{0, 0, 0};
+    if (v2 > 1.e-30 && cd > 0.0) {
+        const double vm = sqrt(v2);
+        const double A = alpha * (M_PI * r * r);
+        const double k = 0.5 * rho * cd * A * vm;
+        Fd = D3{-k * vr.x, -k * vr.y, -k * vr.z};
+    }
+    f.fx[i] += Fb.x + Fd.x;
+    f.fy[i] += Fb.y + Fd.y;
+    f.fz[i] += Fb.z + Fd.z;
+}
+
+// Views in insertion order for a user hook (AoS x3 doubles).
+struct ViewArgs {
+    Balls b;
+    Forces f;
+    double *F, *T, *P, *V, *W, *R, *IM;
+};
+__global__ void k_views_scatter(ViewArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.b.n) return;
+    const size_t o = a.b.orig[i];
+    a.F[3 * o] = a.f.fx[i]; a.F[3 * o + 1] = a.f.fy[i]; a.F[3 * o + 2] = a.f.fz[i];
+    a.T[3 * o] = a.f.tx[i]; a.T[3 * o + 1] = a.f.ty[i]; a.T[3 * o + 2] = a.f.tz[i];
+    a.P[3 * o] = a.b.px[i]; a.P[3 * o + 1] = a.b.py[i]; a.P[3 * o + 2] = a.b.pz[i];
+    a.V[3 * o] = a.b.vx[i]; a.V[3 * o + 1] = a.b.vy[i]; a.V[3 * o + 2] = a.b.vz[i];
+    a.W[3 * o] = a.b.wx[i]; a.W[3 * o + 1] = a.b.wy[i]; a.W[3 * o + 2] = a.b.wz[i];
+    a.R[o] = a.b.r[i];
+    a.IM[o] = a.b.im[i];
+}
+__global__ void k_views_gather(ViewArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.b.n) return;
+    const size_t o = a.b.orig[i];
+    a.f.fx[i] = a.F[3 * o]; a.f.fy[i] = a.F[3 * o + 1]; a.f.fz[i] = a.F[3 * o + 2];
+    a.f.tx[i] = a.T[3 * o]; a.f.ty[i] = a.T[3 * o + 1]; a.f.tz[i] = a.T[3 * o + 2];
+}
+
+}  // namespace
+}  // namespace demsph
+
+// ====================================================================== host
+using namespace demsph;
+using namespace demsph::dem;
+
+namespace {
+
+struct HostBalls {
+    std::vector<double> pos, vel, w, r, im;
+    std::vector<int> mat, clump;
+    size_t size() const { return r.size(); }
+    void add(const double* p, const double* v, const double* ww, double rr, double inv, int m, int c)
+    {
+        for (int k = 0; k < 3; k++) {
+            pos.push_back(p[k]);
+            vel.push_back(v ? v[k] : 0.0);
+            w.push_back(ww ? ww[k] : 0.0);
+        }
+        r.push_back(rr);
+        im.push_back(inv);
+        mat.push_back(m);
+        clump.push_back(c);
+    }
+};
+
+struct HostClumps {
+    std::vector<double> pos, vel, w, invI, im;  // invI: 6 per clump (xx yy zz xy xz yz)
+    std::vector<int> mat, ps, pe;
+    size_t size() const { return im.size(); }
+};
+
+struct HostWalls {
+    std::vector<double> pos;  // 3 per wall
+    std::vector<int> mat;
+    std::vector<double> vlocal;  // 3 per vertex
+    std::vector<int> vwall;
+    std::vector<int> t0, t1, t2, twall;
+    std::vector<double> tcc;  // 3 per triangle
+};
+
+struct Row {
+    int kind, a, b;
+    double p[11];
+};
+
+// Owned device array helper.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t m)
+    {
+        dfree(p);
+        p = dmalloc<T>(m);
+        n = m;
+    }
+    void free_() { dfree(p); n = 0; }
+    ~DBuf() { dfree(p); }
+};
+
+}  // namespace
+
+struct demsph_dem {
+    hipStream_t stream = nullptr;
+    bool initialized = false;
+    int fp64 = 0;
+    HostBalls hb;
+    HostClumps hc;
+    HostWalls hw;
+    std::vector<Row> rows;
+    std::vector<int> bond_a, bond_b;
+    std::vector<double> bond_point, bond_normal;  // host staging of new bonds
+    std::set<std::pair<int, int>> bond_set;
+    int n = 0, nc = 0, nw = 0, nv = 0, nt = 0, nbond = 0, nbond_dev = 0;
+    GridGeom bg{}, tg{};
+    long long bng = 0, tng = 0;
+    bool bgValid = false, tgValid = false;
+    int nMat = 0, cap = 0;
+    KernelTimer timer;
+
+    // balls: two buffers, cur is the live one
+    DBuf<double> bpx[2], bpy[2], bpz[2], bvx[2], bvy[2], bvz[2], bwx[2], bwy[2], bwz[2], br[2], bim[2];
+    DBuf<int> bmat[2], bclump[2], borig[2];
+    int cb = 0;
+    DBuf<int> rank, rank_old;
+    DBuf<double> fx, fy, fz, tx, ty, tz;
+    // sort / cells
+    DBuf<uint32_t> key, skey, ktmp;
+    DBuf<int> perm, vtmp;
+    void* sort_temp = nullptr;
+    DBuf<int> cellStart, cellEnd;
+    // pairs: two lists, cp is the live one
+    int pcap[2] = {0, 0};
+    DBuf<int> pstart[2], ppartner[2];
+    DBuf<double> psx[2], psy[2], psz[2], prx[2], pry[2], prz[2], pox[2], poy[2], poz[2], pov[2];
+    DBuf<float> pFx[2], pFy[2], pFz[2], pRx[2], pRy[2], pRz[2];
+    int cp = 0;
+    bool has_list = false;
+    int npairs = 0;
+    DBuf<int> cnt, rcnt, rstart, rcursor, rpair;
+    void* scan_temp = nullptr;
+    DBuf<int> total;  // device int
+    int* total_h = nullptr;
+    DBuf<float> ofx, ofy, ofz, otx, oty, otz;
+    // material tables
+    DBuf<double> tH, tL, tB;
+    // bonds
+    DBuf<int> dbi, dbj, dbonded, dinc_start, dinc_bond;
+    DBuf<double> dbpx, dbpy, dbpz, dbnx, dbny, dbnz, dbsx, dbsy, dbsz, dbtx, dbty, dbtz, dbFn, dbTt, dbsig, dbtau;
+    DBuf<float> dbFx, dbFy, dbFz, dbTix, dbTiy, dbTiz, dbTjx, dbTjy, dbTjz;
+    // clumps
+    DBuf<double> cpx, cpy, cpz, cvx, cvy, cvz, cwx, cwy, cwz, cq0, cq1, cq2, cq3, cixx, ciyy, cizz, cixy, cixz, ciyz,
+        cim, cFx, cFy, cFz, cTx, cTy, cTz;
+    DBuf<int> cps, cpe;
+    // walls
+    DBuf<double> wpx, wpy, wpz, wvx, wvy, wvz, wwx, wwy, wwz, wq0, wq1, wq2, wq3;
+    DBuf<int> wmat;
+    DBuf<double> vlx, vly, vlz, vgx, vgy, vgz;
+    DBuf<int> vwall, dt0, dt1, dt2, dtwall;
+    DBuf<double> tcc;
+    DBuf<uint32_t> tkey, tskey, tktmp;
+    DBuf<int> tperm, tvtmp, tcs, tce;
+    void* tsort_temp = nullptr;
+    int KW = 16;
+    DBuf<int> hcnt, htri, hcancel;
+    DBuf<double> hsx, hsy, hsz, hrx, hry, hrz, hox, hoy, hoz, hov;
+    DBuf<float> wfx, wfy, wfz, wtx, wty, wtz;
+    DBuf<int> overflow;
+    int* overflow_h = nullptr;
+    bool walls_valid = false;  // wall forces of the last triangle evaluation are held
+    // hooks
+    struct HookOp {
+        int kind;
+        double c, rho, cd, level, ramp;
+        double g[3], vf[3];
+        DBuf<double>* f;
+    };
+    std::vector<HookOp> hooks;
+    std::vector<DBuf<double>*> hook_bufs;
+    demsph_dem_hook_fn hook_fn = nullptr;
+    void* hook_user = nullptr;
+    DBuf<double> vF, vT, vP, vV, vW, vR, vIM;
+    // clock
+    size_t iStep = 0;
+    double time = 0.0;
+
+    ~demsph_dem()
+    {
+        (void)hipStreamSynchronize(stream);
+        if (sort_temp) (void)hipFree(sort_temp);
+        if (tsort_temp) (void)hipFree(tsort_temp);
+        if (scan_temp) (void)hipFree(scan_temp);
+        if (total_h) (void)hipHostFree(total_h);
+        if (overflow_h) (void)hipHostFree(overflow_h);
+        for (auto* b : hook_bufs) delete b;
+    }
+
+    Balls balls(int which) const
+    {
+        Balls b;
+        b.n = n;
+        b.px = bpx[which].p; b.py = bpy[which].p; b.pz = bpz[which].p;
+        b.vx = bvx[which].p; b.vy = bvy[which].p; b.vz = bvz[which].p;
+        b.wx = bwx[which].p; b.wy = bwy[which].p; b.wz = bwz[which].p;
+        b.r = br[which].p; b.im = bim[which].p;
+        b.mat = bmat[which].p; b.clump = bclump[which].p; b.orig = borig[which].p;
+        return b;
+    }
+    Balls cur() const { return balls(cb); }
+    Forces forces() const { return Forces{fx.p, fy.p, fz.p, tx.p, ty.p, tz.p}; }
+    Pairs pairs(int which) const
+    {
+        Pairs p;
+        p.start = pstart[which].p; p.partner = ppartner[which].p;
+        p.sx = psx[which].p; p.sy = psy[which].p; p.sz = psz[which].p;
+        p.rx = prx[which].p; p.ry = pry[which].p; p.rz = prz[which].p;
+        p.ox = pox[which].p; p.oy = poy[which].p; p.oz = poz[which].p;
+        p.Fx = pFx[which].p; p.Fy = pFy[which].p; p.Fz = pFz[which].p;
+        p.Rx = pRx[which].p; p.Ry = pRy[which].p; p.Rz = pRz[which].p;
+        p.ov = pov[which].p;
+        return p;
+    }
+    Tables tables() const { return Tables{nMat, cap, tH.p, tL.p, tB.p}; }
+
+    void alloc_pairs(int which, int c)
+    {
+        c = std::max(c, 1);
+        ppartner[which].alloc(c);
+        psx[which].alloc(c); psy[which].alloc(c); psz[which].alloc(c);
+        prx[which].alloc(c); pry[which].alloc(c); prz[which].alloc(c);
+        pox[which].alloc(c); poy[which].alloc(c); poz[which].alloc(c);
+        pov[which].alloc(c);
+        pFx[which].alloc(c); pFy[which].alloc(c); pFz[which].alloc(c);
+        pRx[which].alloc(c); pRy[which].alloc(c); pRz[which].alloc(c);
+        pcap[which] = c;
+    }
+
+    // contactModelParameters::buildFromTables (contactKernel.cu:6-104)
+    void build_tables()
+    {
+        int mx = -1;
+        for (auto& r : rows) mx = std::max(mx, std::max(r.a, r.b));
+        nMat = mx < 0 ? 0 : mx + 1;
+        cap = nMat > 0 ? nMat * (nMat + 1) / 2 + 1 : 0;
+        std::vector<double> H((size_t)H_N * std::max(cap, 1), 0.0), L((size_t)L_N * std::max(cap, 1), 0.0),
+            B((size_t)B_N * std::max(cap, 1), 0.0);
+        for (int i = 0; i < cap; i++) {
+            H[H_RES * cap + i] = 1.0;
+            B[B_GAMMA * cap + i] = 1.0;
+            B[B_KNKS * cap + i] = 1.0;
+        }
+        for (auto& r : rows) {
+            const int i = pair_index(r.a, r.b, nMat, cap);
+            if (r.kind == 0) for (int q = 0; q < L_N; q++) L[q * cap + i] = r.p[q];
+            if (r.kind == 1) for (int q = 0; q < 8; q++) H[q * cap + i] = r.p[q];
+            if (r.kind == 2) for (int q = 0; q < B_N; q++) B[q * cap + i] = r.p[q];
+        }
+        for (int i = 0; i < cap; i++) {
+            // dissipation of the Hertz model (contactKernel.cu:239-240)
+            const double logR = std::log(H[H_RES * cap + i]);
+            H[H_D * cap + i] = -logR / std::sqrt(logR * logR + M_PI * M_PI);
+        }
+        tH.alloc(H.size()); tL.alloc(L.size()); tB.alloc(B.size());
+        DEMSPH_HIP(hipMemcpy(tH.p, H.data(), H.size() * 8, hipMemcpyHostToDevice));
+        DEMSPH_HIP(hipMemcpy(tL.p, L.data(), L.size() * 8, hipMemcpyHostToDevice));
+        DEMSPH_HIP(hipMemcpy(tB.p, B.data(), B.size() * 8, hipMemcpyHostToDevice));
+    }
+
+    template <class T>
+    void up(DBuf<T>& d, const std::vector<T>& h, size_t count)
+    {
+        d.alloc(count);
+        if (count) DEMSPH_HIP(hipMemcpy(d.p, h.data(), count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void up3(DBuf<double>& x, DBuf<double>& y, DBuf<double>& z, const std::vector<double>& h, size_t count)
+    {
+        std::vector<double> a(count), b(count), c(count);
+        for (size_t i = 0; i < count; i++) { a[i] = h[3 * i]; b[i] = h[3 * i + 1]; c[i] = h[3 * i + 2]; }
+        up(x, a, count); up(y, b, count); up(z, c, count);
+    }
+
+    void upload_bonds()
+    {
+        // new bonds are appended to the device arrays (existing state kept)
+        const int nb_new = (int)bond_a.size();
+        if (nb_new == nbond_dev) return;
+        for (int k = nbond_dev; k < nb_new; k++)
+            if (bond_a[k] >= n || bond_b[k] >= n) throw InvalidArg("bond references a ball index beyond the scene");
+        std::vector<double> keep[16];
+        std::vector<int> keep_bonded;
+        DBuf<double>* st[16] = {&dbpx, &dbpy, &dbpz, &dbnx, &dbny, &dbnz, &dbsx, &dbsy, &dbsz, &dbtx, &dbty, &dbtz,
+                                &dbFn, &dbTt, &dbsig, &dbtau};
+        for (int k = 0; k < 16; k++) {
+            keep[k].assign(nb_new, 0.0);
+            if (nbond_dev) DEMSPH_HIP(hipMemcpy(keep[k].data(), st[k]->p, nbond_dev * 8, hipMemcpyDeviceToHost));
+        }
+        keep_bonded.assign(nb_new, 1);
+        if (nbond_dev) DEMSPH_HIP(hipMemcpy(keep_bonded.data(), dbonded.p, nbond_dev * 4, hipMemcpyDeviceToHost));
+        for (int k = nbond_dev; k < nb_new; k++) {
+            keep[0][k] = bond_point[3 * k]; keep[1][k] = bond_point[3 * k + 1]; keep[2][k] = bond_point[3 * k + 2];
+            keep[3][k] = bond_normal[3 * k]; keep[4][k] = bond_normal[3 * k + 1]; keep[5][k] = bond_normal[3 * k + 2];
+        }
+        for (int k = 0; k < 16; k++) up(*st[k], keep[k], nb_new);
+        up(dbonded, keep_bonded, nb_new);
+        up(dbi, bond_a, nb_new);
+        up(dbj, bond_b, nb_new);
+        for (auto* b : {&dbFx, &dbFy, &dbFz, &dbTix, &dbTiy, &dbTiz, &dbTjx, &dbTjy, &dbTjz}) b->alloc(nb_new);
+        // incidence by insertion index: (bond id, side) in bond order
+        std::vector<int> ist(n + 1, 0), ib(2 * (size_t)nb_new);
+        for (int k = 0; k < nb_new; k++) { ist[bond_a[k] + 1]++; ist[bond_b[k] + 1]++; }
+        for (int i = 0; i < n; i++) ist[i + 1] += ist[i];
+        std::vector<int> cur(ist.begin(), ist.end() - 1);
+        for (int k = 0; k < nb_new; k++) {
+            ib[cur[bond_a[k]]++] = 2 * k;
+            ib[cur[bond_b[k]]++] = 2 * k + 1;
+        }
+        up(dinc_start, ist, n + 1);
+        up(dinc_bond, ib, ib.size());
+        nbond_dev = nb_new;
+        nbond = nb_new;
+    }
+
+    void initialize(const double* minb, const double* maxb)
+    {
+        DEMSPH_HIP(hipStreamSynchronize(stream));
+        build_tables();
+        if (initialized) {
+            upload_bonds();
+            return;
+        }
+        n = (int)hb.size();
+        nc = (int)hc.size();
+        nw = (int)hw.mat.size();
+        nv = (int)hw.vwall.size();
+        nt = (int)hw.t0.size();
+        if (n == 0) throw InvalidArg("no balls");
+        // grids (DEMSolver.h:245-263)
+        double c = 0.0;
+        for (double r : hb.r) c = std::max(c, r * 2.0);
+        bgValid = grid_set(bg, minb, maxb, c);
+        if (!bgValid) throw InvalidArg("degenerate domain or ball radius");
+        for (int t = 0; t < nt; t++) {
+            const double* vl = hw.vlocal.data();
+            const double* a = &vl[3 * (size_t)hw.t0[t]];
+            const double* b = &vl[3 * (size_t)hw.t1[t]];
+            const double* cc = &vl[3 * (size_t)hw.t2[t]];
+            double m[3];
+            circumcenter(a, b, cc, m);
+            for (const double* p : {a, b, cc}) {
+                const double d = 2.0 * std::sqrt((p[0] - m[0]) * (p[0] - m[0]) + (p[1] - m[1]) * (p[1] - m[1]) +
+                                                 (p[2] - m[2]) * (p[2] - m[2]));
+                c = std::max(c, d);
+            }
+        }
+        if (nt > 0) {
+            tgValid = grid_set(tg, minb, maxb, c);
+            if (!tgValid) throw InvalidArg("degenerate triangle grid");
+        }
+        bng = bg.numGrids();
+        tng = nt > 0 ? tg.numGrids() : 0;
+        if (bng > INT32_MAX / 2 || tng > INT32_MAX / 2) throw InvalidArg("grid too large");
+        // balls in insertion order (slot == index)
+        for (int b = 0; b < 2; b++) {
+            for (auto* d : {&bpx[b], &bpy[b], &bpz[b], &bvx[b], &bvy[b], &bvz[b], &bwx[b], &bwy[b], &bwz[b], &br[b], &bim[b]})
+                d->alloc(n);
+            for (auto* d : {&bmat[b], &bclump[b], &borig[b]}) d->alloc(n);
+        }
+        cb = 0;
+        up3(bpx[0], bpy[0], bpz[0], hb.pos, n);
+        up3(bvx[0], bvy[0], bvz[0], hb.vel, n);
+        up3(bwx[0], bwy[0], bwz[0], hb.w, n);
+        up(br[0], hb.r, n);
+        up(bim[0], hb.im, n);
+        up(bmat[0], hb.mat, n);
+        up(bclump[0], hb.clump, n);
+        rank.alloc(n); rank_old.alloc(n);
+        iota_i32(borig[0].p, n, stream);
+        iota_i32(rank.p, n, stream);
+        for (auto* d : {&fx, &fy, &fz, &tx, &ty, &tz}) d->alloc(n);
+        for (auto* d : {&ofx, &ofy, &ofz, &otx, &oty, &otz}) d->alloc(n);
+        key.alloc(n); skey.alloc(n); ktmp.alloc(n); perm.alloc(n); vtmp.alloc(n);
+        DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(n)));
+        cellStart.alloc(bng); cellEnd.alloc(bng);
+        cnt.alloc(n + 1); rcnt.alloc(n + 1); rstart.alloc(n + 1); rcursor.alloc(n);
+        DEMSPH_HIP(hipMalloc(&scan_temp, scan_temp_bytes(n + 1)));
+        total.alloc(2);
+        DEMSPH_HIP(hipHostMalloc((void**)&total_h, 2 * sizeof(int), hipHostMallocDefault));
+        for (int w = 0; w < 2; w++) {
+            pstart[w].alloc(n + 1);
+            alloc_pairs(w, 6 * n);  // initial capacity as DEMSolver.h:174
+        }
+        rpair.alloc(6 * n);
+        has_list = false;
+        // clumps
+        if (nc > 0) {
+            up3(cpx, cpy, cpz, hc.pos, nc);
+            up3(cvx, cvy, cvz, hc.vel, nc);
+            up3(cwx, cwy, cwz, hc.w, nc);
+            std::vector<double> q0(nc, 1.0), qz(nc, 0.0);
+            up(cq0, q0, nc); up(cq1, qz, nc); up(cq2, qz, nc); up(cq3, qz, nc);
+            std::vector<double> comp[6];
+            for (int k = 0; k < 6; k++) {
+                comp[k].resize(nc);
+                for (int c2 = 0; c2 < nc; c2++) comp[k][c2] = hc.invI[6 * c2 + k];
+            }
+            up(cixx, comp[0], nc); up(ciyy, comp[1], nc); up(cizz, comp[2], nc);
+            up(cixy, comp[3], nc); up(cixz, comp[4], nc); up(ciyz, comp[5], nc);
+            up(cim, hc.im, nc);
+            up(cps, hc.ps, nc); up(cpe, hc.pe, nc);
+            for (auto* d : {&cFx, &cFy, &cFz, &cTx, &cTy, &cTz}) d->alloc(nc);
+        }
+        // walls
+        if (nt > 0) {
+            up3(wpx, wpy, wpz, hw.pos, nw);
+            std::vector<double> z(nw, 0.0), one(nw, 1.0);
+            up(wvx, z, nw); up(wvy, z, nw); up(wvz, z, nw);
+            up(wwx, z, nw); up(wwy, z, nw); up(wwz, z, nw);
+            up(wq0, one, nw); up(wq1, z, nw); up(wq2, z, nw); up(wq3, z, nw);
+            up(wmat, hw.mat, nw);
+            up3(vlx, vly, vlz, hw.vlocal, nv);
+            std::vector<double> vg(3 * (size_t)nv);
+            for (int v = 0; v < nv; v++)
+                for (int k = 0; k < 3; k++) vg[3 * v + k] = hw.vlocal[3 * v + k] + hw.pos[3 * hw.vwall[v] + k];
+            up3(vgx, vgy, vgz, vg, nv);
+            up(vwall, hw.vwall, nv);
+            up(dt0, hw.t0, nt); up(dt1, hw.t1, nt); up(dt2, hw.t2, nt); up(dtwall, hw.twall, nt);
+            up(tcc, hw.tcc, 3 * (size_t)nt);
+            tkey.alloc(nt); tskey.alloc(nt); tktmp.alloc(nt); tperm.alloc(nt); tvtmp.alloc(nt);
+            DEMSPH_HIP(hipMalloc(&tsort_temp, radix_sort_temp_bytes(nt)));
+            tcs.alloc(tng); tce.alloc(tng);
+            const size_t hn = (size_t)KW * n;
+            hcnt.alloc(n);
+            DEMSPH_HIP(hipMemset(hcnt.p, 0, n * sizeof(int)));
+            htri.alloc(hn); hcancel.alloc(hn);
+            for (auto* d : {&hsx, &hsy, &hsz, &hrx, &hry, &hrz, &hox, &hoy, &hoz, &hov}) d->alloc(hn);
+            for (auto* d : {&wfx, &wfy, &wfz, &wtx, &wty, &wtz}) d->alloc(n);
+        }
+        overflow.alloc(1);
+        DEMSPH_HIP(hipMemset(overflow.p, 0, sizeof(int)));
+        DEMSPH_HIP(hipHostMalloc((void**)&overflow_h, sizeof(int), hipHostMallocDefault));
+        nbond_dev = 0;
+        upload_bonds();
+        DEMSPH_HIP(hipStreamSynchronize(stream));
+        DEMSPH_HIP(hipDeviceSynchronize());
+        initialized = true;
+    }
+
+    void require_init() const
+    {
+        if (!initialized) throw InvalidArg("scene not initialized");
+    }
+
+    // ---------------------------------------------------------- stages
+    // neighborSearch (DEMSolver.h:698-777)
+    void neighbor_search()
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        Balls b = cur();
+        timer.begin("dem_hash_sort", stream);
+        hash_soa_by_orig(n, bg, b.px, b.py, b.pz, b.orig, key.p, stream);
+        radix_sort_pairs(key.p, nullptr, skey.p, perm.p, ktmp.p, vtmp.p, sort_temp, n, key_bits(bng), stream);
+        fill_i32(cellStart.p, bng, -1, stream);
+        fill_i32(cellEnd.p, bng, -1, stream);
+        timer.end(stream);
+        DEMSPH_HIP(hipMemcpyAsync(rank_old.p, rank.p, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+        ReorderArgs ra{};
+        ra.src = b;
+        ra.dst = balls(1 - cb);
+        ra.skey = skey.p; ra.perm = perm.p; ra.rank = rank.p; ra.numGrids = bng;
+        ra.cellStart = cellStart.p; ra.cellEnd = cellEnd.p;
+        DEMSPH_LAUNCH(timer, "dem_reorder", k_reorder, gb, tb, 0, stream, ra);
+        const int old_cb = cb;
+        cb = 1 - cb;
+        Balls nb = cur();
+        DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs, gb, tb, 0, stream, nb, bg, cellStart.p, cellEnd.p,
+                      cnt.p);
+        // the owner CSR goes into the list that becomes current
+        const int np = 1 - cp;
+        exclusive_scan(cnt.p, pstart[np].p, n, scan_temp, total.p, stream);
+        // start[n] = total
+        DEMSPH_HIP(hipMemcpyAsync(pstart[np].p + n, total.p, sizeof(int), hipMemcpyDeviceToDevice, stream));
+        DEMSPH_HIP(hipMemcpyAsync(total_h, total.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        DEMSPH_HIP(hipStreamSynchronize(stream));  // resizePair (interaction.h:164-175)
+        timer.harvest();
+        const int tot = total_h[0];
+        if (tot > pcap[np]) alloc_pairs(np, tot + tot / 4);
+        if (tot > (int)rpair.n) rpair.alloc(pcap[np]);
+        WritePairsArgs wa{};
+        wa.b = nb; wa.g = bg; wa.cs = cellStart.p; wa.ce = cellEnd.p;
+        wa.cur = pairs(np);
+        wa.old = pairs(cp);
+        wa.old_orig = borig[old_cb].p;
+        wa.rank_old = rank_old.p;
+        wa.has_old = has_list ? 1 : 0;
+        DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs, gb, tb, 0, stream, wa);
+        cp = np;
+        has_list = true;
+        npairs = tot;
+        // reverse index by partner (DEMSolver.h:59-78, reset every build)
+        timer.begin("dem_reverse_index", stream);
+        DEMSPH_HIP(hipMemsetAsync(rcnt.p, 0, (n + 1) * sizeof(int), stream));
+        DEMSPH_HIP(hipMemsetAsync(rcursor.p, 0, n * sizeof(int), stream));
+        hipLaunchKernelGGL(k_rev_count, gb, tb, 0, stream, n, pstart[cp].p, ppartner[cp].p, rcnt.p);
+        exclusive_scan(rcnt.p, rstart.p, n + 1, scan_temp, nullptr, stream);
+        hipLaunchKernelGGL(k_rev_fill, gb, tb, 0, stream, n, pstart[cp].p, ppartner[cp].p, rstart.p, rcursor.p,
+                           rpair.p);
+        hipLaunchKernelGGL(k_rev_sort, gb, tb, 0, stream, n, rstart.p, rpair.p);
+        DEMSPH_HIP(hipGetLastError());
+        timer.end(stream);
+    }
+
+    // Ball-pair and bond forces plus the reduction: calculateBallContactForceTorque
+    // (DEMSolver.h:599-695); with walls, ball-wall contacts are added and the
+    // walls then move (addTriangleContactForceTorque, :428-597).
+    void evaluate(double springDt, double wallDt, bool do_pairs = true, bool do_walls = true)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        Balls b = cur();
+        const bool walls = do_walls && nt > 0;
+        if (do_pairs) evaluate_pairs(springDt);
+        if (walls) evaluate_walls(wallDt);
+        gather(nt > 0 && (walls || walls_valid));
+        if (walls) {
+            walls_valid = true;
+            move_walls(wallDt);
+        }
+    }
+
+    void evaluate_pairs(double springDt)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        Balls b = cur();
+        PairForceArgs pa{};
+        pa.b = b; pa.p = pairs(cp); pa.tb = tables(); pa.dt = springDt;
+        pa.ofx = ofx.p; pa.ofy = ofy.p; pa.ofz = ofz.p; pa.otx = otx.p; pa.oty = oty.p; pa.otz = otz.p;
+        if (fp64) DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<double>, gb, tb, 0, stream, pa);
+        else DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<float>, gb, tb, 0, stream, pa);
+        if (nbond > 0) {
+            BondArgs ba{};
+            ba.nb = nbond; ba.b = b; ba.rank = rank.p; ba.bi = dbi.p; ba.bj = dbj.p; ba.bonded = dbonded.p;
+            ba.bpx = dbpx.p; ba.bpy = dbpy.p; ba.bpz = dbpz.p; ba.bnx = dbnx.p; ba.bny = dbny.p; ba.bnz = dbnz.p;
+            ba.bsx = dbsx.p; ba.bsy = dbsy.p; ba.bsz = dbsz.p; ba.btx = dbtx.p; ba.bty = dbty.p; ba.btz = dbtz.p;
+            ba.bFn = dbFn.p; ba.bTt = dbTt.p; ba.bsig = dbsig.p; ba.btau = dbtau.p;
+            ba.Fx = dbFx.p; ba.Fy = dbFy.p; ba.Fz = dbFz.p;
+            ba.Tix = dbTix.p; ba.Tiy = dbTiy.p; ba.Tiz = dbTiz.p; ba.Tjx = dbTjx.p; ba.Tjy = dbTjy.p; ba.Tjz = dbTjz.p;
+            ba.tb = tables(); ba.dt = springDt;
+            DEMSPH_LAUNCH(timer, "dem_bond_forces", k_bond_forces, dim3(blocks_for(nbond, kT)), tb, 0, stream, ba);
+        }
+    }
+
+    void evaluate_walls(double wallDt)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        Balls b = cur();
+        {
+            timer.begin("dem_triangle_cells", stream);
+            hash_aos(tcc.p, nt, tg, tkey.p, stream);
+            radix_sort_pairs(tkey.p, nullptr, tskey.p, tperm.p, tktmp.p, tvtmp.p, tsort_temp, nt, key_bits(tng), stream);
+            fill_i32(tcs.p, tng, -1, stream);
+            fill_i32(tce.p, tng, -1, stream);
+            cell_ranges(tskey.p, nt, tng, tcs.p, tce.p, stream);
+            timer.end(stream);
+            WallArgs wa{};
+            wa.b = b; wa.tb = tables(); wa.dt = wallDt; wa.tg = tg; wa.tcs = tcs.p; wa.tce = tce.p; wa.tsorted = tperm.p;
+            wa.t0 = dt0.p; wa.t1 = dt1.p; wa.t2 = dt2.p; wa.twall = dtwall.p;
+            wa.vgx = vgx.p; wa.vgy = vgy.p; wa.vgz = vgz.p;
+            wa.wpx = wpx.p; wa.wpy = wpy.p; wa.wpz = wpz.p; wa.wvx = wvx.p; wa.wvy = wvy.p; wa.wvz = wvz.p;
+            wa.wwx = wwx.p; wa.wwy = wwy.p; wa.wwz = wwz.p; wa.wmat = wmat.p;
+            wa.KW = KW; wa.nballs = n;
+            wa.hcnt = hcnt.p; wa.htri = htri.p; wa.hcancel = hcancel.p;
+            wa.hsx = hsx.p; wa.hsy = hsy.p; wa.hsz = hsz.p; wa.hrx = hrx.p; wa.hry = hry.p; wa.hrz = hrz.p;
+            wa.hox = hox.p; wa.hoy = hoy.p; wa.hoz = hoz.p; wa.hov = hov.p;
+            wa.ofx = wfx.p; wa.ofy = wfy.p; wa.ofz = wfz.p; wa.otx = wtx.p; wa.oty = wty.p; wa.otz = wtz.p;
+            wa.overflow = overflow.p;
+            if (fp64) DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<double>, gb, tb, 0, stream, wa);
+            else DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<float>, gb, tb, 0, stream, wa);
+        }
+    }
+
+    void gather(bool with_walls)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        Balls b = cur();
+        GatherArgs ga{};
+        ga.b = b;
+        ga.ofx = ofx.p; ga.ofy = ofy.p; ga.ofz = ofz.p; ga.otx = otx.p; ga.oty = oty.p; ga.otz = otz.p;
+        ga.pFx = pFx[cp].p; ga.pFy = pFy[cp].p; ga.pFz = pFz[cp].p; ga.pRx = pRx[cp].p; ga.pRy = pRy[cp].p; ga.pRz = pRz[cp].p;
+        ga.rstart = rstart.p; ga.rpair = rpair.p;
+        if (nbond > 0) {
+            ga.istart = dinc_start.p; ga.ibond = dinc_bond.p;
+            ga.bFx = dbFx.p; ga.bFy = dbFy.p; ga.bFz = dbFz.p;
+            ga.bTix = dbTix.p; ga.bTiy = dbTiy.p; ga.bTiz = dbTiz.p; ga.bTjx = dbTjx.p; ga.bTjy = dbTjy.p; ga.bTjz = dbTjz.p;
+        }
+        if (with_walls) {
+            ga.wfx = wfx.p; ga.wfy = wfy.p; ga.wfz = wfz.p; ga.wtx = wtx.p; ga.wty = wty.p; ga.wtz = wtz.p;
+        }
+        ga.out = forces();
+        DEMSPH_LAUNCH(timer, "dem_gather", k_gather, gb, tb, 0, stream, ga);
+    }
+
+    void move_walls(double dt)
+    {
+        WallMoveArgs m{};
+        m.nw = nw; m.nv = nv; m.nt = nt;
+        m.px = wpx.p; m.py = wpy.p; m.pz = wpz.p; m.q0 = wq0.p; m.q1 = wq1.p; m.q2 = wq2.p; m.q3 = wq3.p;
+        m.vx = wvx.p; m.vy = wvy.p; m.vz = wvz.p; m.wx = wwx.p; m.wy = wwy.p; m.wz = wwz.p;
+        m.lx = vlx.p; m.ly = vly.p; m.lz = vlz.p; m.gx = vgx.p; m.gy = vgy.p; m.gz = vgz.p;
+        m.vwall = vwall.p; m.t0 = dt0.p; m.t1 = dt1.p; m.t2 = dt2.p; m.tcc = tcc.p; m.dt = dt;
+        timer.begin("dem_wall_motion", stream);
+        hipLaunchKernelGGL(k_wall_move, dim3(blocks_for(nw, 64)), dim3(64), 0, stream, m);
+        hipLaunchKernelGGL(k_vertex_update, dim3(blocks_for(nv, 256)), dim3(256), 0, stream, m);
+        hipLaunchKernelGGL(k_circumcenters, dim3(blocks_for(nt, 256)), dim3(256), 0, stream, m);
+        DEMSPH_HIP(hipGetLastError());
+        timer.end(stream);
+    }
+
+    void run_hooks(double t)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        Balls b = cur();
+        for (auto& h : hooks) {
+            if (h.kind == 0) {
+                double s = 1.0;
+                if (h.ramp > 0) s = std::min(t / h.ramp, 1.0);
+                hipLaunchKernelGGL(k_hook_const, gb, tb, 0, stream, b, forces(), (const double*)h.f->p, s);
+            } else if (h.kind == 1) {
+                hipLaunchKernelGGL(k_hook_damping, gb, tb, 0, stream, b, forces(), h.c);
+            } else {
+                hipLaunchKernelGGL(k_hook_buoyancy, gb, tb, 0, stream, b, forces(), h.rho, h.cd, D3{h.g[0], h.g[1], h.g[2]},
+                                   D3{h.vf[0], h.vf[1], h.vf[2]}, h.level);
+            }
+            DEMSPH_HIP(hipGetLastError());
+        }
+        if (hook_fn) {
+            ensure_views();
+            ViewArgs v = view_args();
+            hipLaunchKernelGGL(k_views_scatter, gb, tb, 0, stream, v);
+            DEMSPH_HIP(hipGetLastError());
+            hook_fn(hook_user, iStep, t);
+            hipLaunchKernelGGL(k_views_gather, gb, tb, 0, stream, v);
+            DEMSPH_HIP(hipGetLastError());
+        }
+    }
+
+    void ensure_views()
+    {
+        if (vF.p) return;
+        for (auto* d : {&vF, &vT, &vP, &vV, &vW}) d->alloc(3 * (size_t)n);
+        vR.alloc(n);
+        vIM.alloc(n);
+    }
+    ViewArgs view_args() const
+    {
+        return ViewArgs{cur(), forces(), vF.p, vT.p, vP.p, vV.p, vW.p, vR.p, vIM.p};
+    }
+
+    void half(bool first, const double* g, double h)
+    {
+        const dim3 tb(kT);
+        if (nc > 0) {
+            ClumpArgs c{};
+            c.nc = nc;
+            c.px = cpx.p; c.py = cpy.p; c.pz = cpz.p; c.vx = cvx.p; c.vy = cvy.p; c.vz = cvz.p;
+            c.wx = cwx.p; c.wy = cwy.p; c.wz = cwz.p; c.q0 = cq0.p; c.q1 = cq1.p; c.q2 = cq2.p; c.q3 = cq3.p;
+            c.ixx = cixx.p; c.iyy = ciyy.p; c.izz = cizz.p; c.ixy = cixy.p; c.ixz = cixz.p; c.iyz = ciyz.p;
+            c.im = cim.p; c.ps = cps.p; c.pe = cpe.p;
+            c.Fx = cFx.p; c.Fy = cFy.p; c.Fz = cFz.p; c.Tx = cTx.p; c.Ty = cTy.p; c.Tz = cTz.p;
+            c.b = cur(); c.f = forces(); c.rank = rank.p;
+            c.gx = g[0]; c.gy = g[1]; c.gz = g[2]; c.h = h;
+            if (first) DEMSPH_LAUNCH(timer, "dem_clump_half", k_clump_half<true>, dim3(blocks_for(nc, kT)), tb, 0, stream, c);
+            else DEMSPH_LAUNCH(timer, "dem_clump_half", k_clump_half<false>, dim3(blocks_for(nc, kT)), tb, 0, stream, c);
+        }
+        BallHalfArgs a{};
+        a.b = cur(); a.f = forces();
+        if (nc > 0) {
+            a.cpx = cpx.p; a.cpy = cpy.p; a.cpz = cpz.p; a.cvx = cvx.p; a.cvy = cvy.p; a.cvz = cvz.p;
+            a.cwx = cwx.p; a.cwy = cwy.p; a.cwz = cwz.p;
+        }
+        a.nc = nc;
+        a.gx = g[0]; a.gy = g[1]; a.gz = g[2]; a.h = h;
+        if (first) DEMSPH_LAUNCH(timer, "dem_ball_half", k_ball_half<true>, dim3(blocks_for(n, kT)), tb, 0, stream, a);
+        else DEMSPH_LAUNCH(timer, "dem_ball_half", k_ball_half<false>, dim3(blocks_for(n, kT)), tb, 0, stream, a);
+    }
+
+    void check_overflow()
+    {
+        if (nt == 0) return;
+        DEMSPH_HIP(hipMemcpyAsync(overflow_h, overflow.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        DEMSPH_HIP(hipStreamSynchronize(stream));
+        timer.harvest();
+        if (overflow_h[0] > 0) {
+            const int c = overflow_h[0];
+            DEMSPH_HIP(hipMemset(overflow.p, 0, sizeof(int)));
+            throw std::length_error("a ball had " + std::to_string(c) +
+                                    " wall-triangle candidates, above the per-ball capacity of 16");
+        }
+    }
+
+    void solve_begin()
+    {
+        require_init();
+        neighbor_search();
+        evaluate(0.0, 0.0);
+        iStep = 0;
+        time = 0.0;
+        check_overflow();
+    }
+
+    // loop_ball / loop_ball_wall (DEMSolver.h:1376-1405, 1452-1485)
+    void step(const double* g, double dt, int nsteps)
+    {
+        require_init();
+        if (!has_list) throw InvalidArg("solve_begin() (or neighbor_search) must precede step()");
+        if (!(dt > 0)) throw InvalidArg("time step must be positive");
+        const double h = 0.5 * dt;
+        for (int k = 0; k < nsteps; k++) {
+            iStep++;
+            time += dt;
+            if (iStep % 5 == 0) neighbor_search();
+            evaluate(h, dt);
+            run_hooks(time - h);
+            half(true, g, h);
+            evaluate(h, dt);
+            run_hooks(time);
+            half(false, g, h);
+        }
+        check_overflow();
+    }
+};
+
+namespace {
+
+template <class T>
+std::vector<T> fetch(const T* d, size_t n, hipStream_t s)
+{
+    std::vector<T> h(n);
+    if (n) DEMSPH_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    DEMSPH_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+template <class T>
+void emit(const std::vector<T>& v, void* out, size_t cap, size_t* count)
+{
+    if (count) *count = v.size();
+    if (!out) return;
+    if (cap < v.size()) throw InvalidArg("output capacity too small");
+    if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(T));
+}
+
+// symMatrix inverse (myMat.h:163-179), host side for addClump.
+void sym_inverse(const double* A, double* r)
+{
+    const double xx = A[0], yy = A[1], zz = A[2], xy = A[3], xz = A[4], yz = A[5];
+    const double det_inv = 1.0 / (xx * (yy * zz - yz * yz) + xy * (yz * xz - xy * zz) + xz * (xy * yz - yy * xz));
+    r[0] = (yy * zz - yz * yz) * det_inv;
+    r[1] = (xx * zz - xz * xz) * det_inv;
+    r[2] = (xx * yy - xy * xy) * det_inv;
+    r[3] = -(xy * zz - xz * yz) * det_inv;
+    r[4] = (xy * yz - xz * yy) * det_inv;
+    r[5] = -(xx * yz - xz * xy) * det_inv;
+}
+
+}  // namespace
+
+#define DEM_GUARD(...) return guarded([&] { if (!d) throw InvalidArg("null scene"); __VA_ARGS__; })
+
+extern "C" int demsph_dem_create(demsph_dem** out, void* stream)
+{
+    return guarded([&] {
+        if (!out) throw InvalidArg("null output handle");
+        auto* d = new demsph_dem();
+        d->stream = static_cast<hipStream_t>(stream);
+        *out = d;
+    });
+}
+
+extern "C" int demsph_dem_destroy(demsph_dem* d) { return guarded([&] { delete d; }); }
+
+static void add_row(demsph_dem* d, int kind, int a, int b, std::initializer_list<double> v)
+{
+    if (d->initialized) throw InvalidArg("material rows must be set before initialize()");
+    Row r{};
+    r.kind = kind; r.a = a; r.b = b;
+    int k = 0;
+    for (double x : v) r.p[k++] = x;
+    d->rows.push_back(r);
+}
+
+extern "C" int demsph_dem_set_linear_pair(demsph_dem* d, int a, int b, double kn, double ks, double kr, double kt,
+                                          double dn, double ds, double dr, double dt, double mus, double mur, double mut)
+{
+    DEM_GUARD(add_row(d, 0, a, b, {kn, ks, kr, kt, dn, ds, dr, dt, mus, mur, mut}));
+}
+extern "C" int demsph_dem_set_hertzian_pair(demsph_dem* d, int a, int b, double E, double G, double e, double krks,
+                                            double ktks, double mus, double mur, double mut)
+{
+    DEM_GUARD(add_row(d, 1, a, b, {E, G, e, krks, ktks, mus, mur, mut}));
+}
+extern "C" int demsph_dem_set_bonded_pair(demsph_dem* d, int a, int b, double gamma, double E, double knks,
+                                          double sigmat, double C, double mu)
+{
+    DEM_GUARD(add_row(d, 2, a, b, {gamma, E, knks, sigmat, C, mu}));
+}
+
+extern "C" int demsph_dem_add_balls(demsph_dem* d, size_t n, const double* position, const double* velocity,
+                                    const double* angularVelocity, const double* radius, const double* density,
+                                    const int* materialID, const int* clumpID)
+{
+    DEM_GUARD({
+        if (d->initialized) throw InvalidArg("balls must be added before initialize()");
+        if (n && (!position || !radius || !density || !materialID)) throw InvalidArg("null array");
+        for (size_t i = 0; i < n; i++) {
+            const double r = radius[i];
+            const double mass = 4. / 3. * r * r * r * M_PI * density[i];
+            const double inv = mass > 0.0 ? 1.0 / mass : 0.0;
+            d->hb.add(&position[3 * i], velocity ? &velocity[3 * i] : nullptr,
+                      angularVelocity ? &angularVelocity[3 * i] : nullptr, r, inv, materialID[i],
+                      clumpID ? clumpID[i] : -1);
+        }
+    });
+}
+
+extern "C" int demsph_dem_add_fixed_balls(demsph_dem* d, size_t n, const double* position, const double* radius,
+                                          const int* materialID, const int* clumpID)
+{
+    DEM_GUARD({
+        if (d->initialized) throw InvalidArg("balls must be added before initialize()");
+        if (n && (!position || !radius || !materialID)) throw InvalidArg("null array");
+        for (size_t i = 0; i < n; i++)
+            d->hb.add(&position[3 * i], nullptr, nullptr, radius[i], 0.0, materialID[i], clumpID ? clumpID[i] : -1);
+    });
+}
+
+extern "C" int demsph_dem_add_clump(demsph_dem* d, size_t np, const double* points, const double* radii,
+                                    const double* centroid, const double* velocity, const double* angularVelocity,
+                                    double mass, const double* inertia, int materialID)
+{
+    DEM_GUARD({
+        if (d->initialized) throw InvalidArg("clumps must be added before initialize()");
+        if (!points || !radii || !centroid || !velocity || !angularVelocity || !inertia) throw InvalidArg("null array");
+        const int cid = (int)d->hc.size();
+        const int ps = (int)d->hb.size();
+        double volume = 0.0, inv = 0.0;
+        std::vector<double> pv(3 * np);
+        for (size_t i = 0; i < np; i++)
+            for (int k = 0; k < 3; k++) pv[3 * i + k] = velocity[k];
+        const double* w = angularVelocity;
+        if (mass > 1.e-20) {
+            inv = 1.0 / mass;
+            for (size_t i = 0; i < np; i++) {
+                volume += 4.0 / 3.0 * M_PI * std::pow(radii[i], 3.0);
+                const double rx = points[3 * i] - centroid[0], ry = points[3 * i + 1] - centroid[1],
+                             rz = points[3 * i + 2] - centroid[2];
+                pv[3 * i] += w[1] * rz - w[2] * ry;
+                pv[3 * i + 1] += w[2] * rx - w[0] * rz;
+                pv[3 * i + 2] += w[0] * ry - w[1] * rx;
+            }
+        }
+        const double dens = volume > 0. ? mass / volume : 0.0;
+        for (size_t i = 0; i < np; i++) {
+            const double r = radii[i];
+            const double m = 4. / 3. * r * r * r * M_PI * dens;
+            d->hb.add(&points[3 * i], &pv[3 * i], w, r, m > 0.0 ? 1.0 / m : 0.0, materialID, cid);
+        }
+        for (int k = 0; k < 3; k++) {
+            d->hc.pos.push_back(centroid[k]);
+            d->hc.vel.push_back(velocity[k]);
+            d->hc.w.push_back(w[k]);
+        }
+        double ii[6];
+        sym_inverse(inertia, ii);
+        for (int k = 0; k < 6; k++) d->hc.invI.push_back(ii[k]);
+        d->hc.im.push_back(inv);
+        d->hc.mat.push_back(materialID);
+        d->hc.ps.push_back(ps);
+        d->hc.pe.push_back(ps + (int)np);
+    });
+}
+
+extern "C" int demsph_dem_add_fixed_clump(demsph_dem* d, size_t np, const double* points, const double* radii,
+                                          const double* centroid, int materialID)
+{
+    DEM_GUARD({
+        if (d->initialized) throw InvalidArg("clumps must be added before initialize()");
+        if (!points || !radii || !centroid) throw InvalidArg("null array");
+        const int cid = (int)d->hc.size();
+        const int ps = (int)d->hb.size();
+        for (size_t i = 0; i < np; i++) d->hb.add(&points[3 * i], nullptr, nullptr, radii[i], 0.0, materialID, cid);
+        for (int k = 0; k < 3; k++) {
+            d->hc.pos.push_back(centroid[k]);
+            d->hc.vel.push_back(0.0);
+            d->hc.w.push_back(0.0);
+        }
+        for (int k = 0; k < 6; k++) d->hc.invI.push_back(0.0);
+        d->hc.im.push_back(0.0);
+        d->hc.mat.push_back(materialID);
+        d->hc.ps.push_back(ps);
+        d->hc.pe.push_back(ps + (int)np);
+    });
+}
+
+extern "C" int demsph_dem_add_mesh_wall(demsph_dem* d, size_t nv, const double* vertices, size_t nt, const int* tri0,
+                                        const int* tri1, const int* tri2, const double* position,
+                                        const double* velocity, const double* angularVelocity, int materialID)
+{
+    DEM_GUARD({
+        (void)velocity;
+        (void)angularVelocity;  // the reference stores zero wall velocities (DEMSolver.h:1901-1911)
+        if (d->initialized) throw InvalidArg("walls must be added before initialize()");
+        if (nt == 0) return;
+        if (!vertices || !tri0 || !tri1 || !tri2 || !position) throw InvalidArg("null array");
+        int mx = -1;
+        for (size_t i = 0; i < nt; i++) mx = std::max(mx, std::max(tri0[i], std::max(tri1[i], tri2[i])));
+        if (mx >= (int)nv) return;  // DEMSolver.h:1891-1897: ignored
+        const int vs = (int)d->hw.vwall.size();
+        const int wi = (int)d->hw.mat.size();
+        for (int k = 0; k < 3; k++) d->hw.pos.push_back(position[k]);
+        d->hw.mat.push_back(materialID);
+        for (size_t i = 0; i < nv; i++) {
+            for (int k = 0; k < 3; k++) d->hw.vlocal.push_back(vertices[3 * i + k]);
+            d->hw.vwall.push_back(wi);
+        }
+        for (size_t i = 0; i < nt; i++) {
+            double a[3], b[3], c[3], m[3];
+            for (int k = 0; k < 3; k++) {
+                a[k] = vertices[3 * (size_t)tri0[i] + k] + position[k];
+                b[k] = vertices[3 * (size_t)tri1[i] + k] + position[k];
+                c[k] = vertices[3 * (size_t)tri2[i] + k] + position[k];
+            }
+            circumcenter(a, b, c, m);
+            d->hw.t0.push_back(tri0[i] + vs);
+            d->hw.t1.push_back(tri1[i] + vs);
+            d->hw.t2.push_back(tri2[i] + vs);
+            d->hw.twall.push_back(wi);
+            for (int k = 0; k < 3; k++) d->hw.tcc.push_back(m[k]);
+        }
+    });
+}
+
+extern "C" int demsph_dem_add_bonded_interactions(demsph_dem* d, size_t n, const int* ob0, const int* ob1,
+                                                  size_t npos, const double* pos)
+{
+    DEM_GUARD({
+        if (n && (!ob0 || !ob1 || !pos)) throw InvalidArg("null array");
+        for (size_t k = 0; k < n; k++) {
+            int a = ob0[k], b = ob1[k];
+            if (a < 0 || b < 0) continue;
+            if ((size_t)a >= npos || (size_t)b >= npos) continue;
+            if (a == b) continue;
+            if (a > b) std::swap(a, b);
+            if (!d->bond_set.insert({a, b}).second) continue;
+            d->bond_a.push_back(a);
+            d->bond_b.push_back(b);
+            const double* pa = &pos[3 * (size_t)a];
+            const double* pb = &pos[3 * (size_t)b];
+            for (int q = 0; q < 3; q++) d->bond_point.push_back(0.5 * (pa[q] + pb[q]));
+            const double nx = pa[0] - pb[0], ny = pa[1] - pb[1], nz = pa[2] - pb[2];
+            const double l2 = nx * nx + ny * ny + nz * nz;
+            const double inv = l2 < 1.0e-20 ? 0.0 : 1.0 / std::sqrt(l2);
+            d->bond_normal.push_back(nx * inv);
+            d->bond_normal.push_back(ny * inv);
+            d->bond_normal.push_back(nz * inv);
+        }
+    });
+}
+
+extern "C" int demsph_dem_set_precision(demsph_dem* d, int fp64Forces) { DEM_GUARD(d->fp64 = fp64Forces ? 1 : 0); }
+
+extern "C" int demsph_dem_initialize(demsph_dem* d, const double* minBound, const double* maxBound)
+{
+    DEM_GUARD({
+        if (!minBound || !maxBound) throw InvalidArg("null bounds");
+        d->initialize(minBound, maxBound);
+    });
+}
+
+extern "C" int demsph_dem_neighbor_search(demsph_dem* d) { DEM_GUARD(d->require_init(); d->neighbor_search()); }
+
+extern "C" int demsph_dem_contact(demsph_dem* d, double timeStep)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!d->has_list) throw InvalidArg("neighbor_search() must precede contact()");
+        // ball pairs and bonds; forces are overwritten (the reference clears them first)
+        d->walls_valid = false;
+        d->evaluate(timeStep, 0.0, true, false);
+    });
+}
+
+extern "C" int demsph_dem_triangle_contact(demsph_dem* d, double timeStep)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!d->has_list) throw InvalidArg("neighbor_search() must precede triangle_contact()");
+        d->evaluate(0.0, timeStep, false, true);
+    });
+}
+
+extern "C" int demsph_dem_first_half(demsph_dem* d, const double* gravity, double halfTimeStep)
+{
+    DEM_GUARD(d->require_init(); d->half(true, gravity, halfTimeStep));
+}
+extern "C" int demsph_dem_second_half(demsph_dem* d, const double* gravity, double halfTimeStep)
+{
+    DEM_GUARD(d->require_init(); d->half(false, gravity, halfTimeStep));
+}
+extern "C" int demsph_dem_solve_begin(demsph_dem* d) { DEM_GUARD(d->solve_begin()); }
+extern "C" int demsph_dem_step(demsph_dem* d, const double* gravity, double timeStep, int nsteps)
+{
+    DEM_GUARD({
+        if (!gravity) throw InvalidArg("null gravity");
+        d->step(gravity, timeStep, nsteps);
+    });
+}
+extern "C" int demsph_dem_synchronize(demsph_dem* d)
+{
+    DEM_GUARD(DEMSPH_HIP(hipStreamSynchronize(d->stream)); d->timer.harvest());
+}
+extern "C" int demsph_dem_clock(demsph_dem* d, size_t* iStep, double* time)
+{
+    DEM_GUARD({
+        if (iStep) *iStep = d->iStep;
+        if (time) *time = d->time;
+    });
+}
+extern "C" int demsph_dem_set_hook(demsph_dem* d, demsph_dem_hook_fn fn, void* user)
+{
+    DEM_GUARD(d->hook_fn = fn; d->hook_user = user);
+}
+extern "C" int demsph_dem_views(demsph_dem* d, demsph_dem_ball_views* v)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!v) throw InvalidArg("null views");
+        d->ensure_views();
+        v->force = d->vF.p; v->torque = d->vT.p; v->position = d->vP.p; v->velocity = d->vV.p;
+        v->angularVelocity = d->vW.p; v->radius = d->vR.p; v->inverseMass = d->vIM.p; v->numBalls = d->n;
+    });
+}
+extern "C" int demsph_dem_hook_clear(demsph_dem* d)
+{
+    DEM_GUARD({
+        for (auto* b : d->hook_bufs) delete b;
+        d->hook_bufs.clear();
+        d->hooks.clear();
+    });
+}
+extern "C" int demsph_dem_hook_constant_force(demsph_dem* d, const double* force, double ramp)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!force) throw InvalidArg("null force");
+        auto* b = new DBuf<double>();
+        d->hook_bufs.push_back(b);
+        b->alloc(3 * (size_t)d->n);
+        DEMSPH_HIP(hipMemcpy(b->p, force, 3 * (size_t)d->n * 8, hipMemcpyHostToDevice));
+        demsph_dem::HookOp h{};
+        h.kind = 0; h.ramp = ramp; h.f = b;
+        d->hooks.push_back(h);
+    });
+}
+extern "C" int demsph_dem_hook_global_damping(demsph_dem* d, double c)
+{
+    DEM_GUARD({
+        demsph_dem::HookOp h{};
+        h.kind = 1; h.c = c;
+        d->hooks.push_back(h);
+    });
+}
+extern "C" int demsph_dem_hook_buoyancy_drag(demsph_dem* d, double rho, double cd, const double* g, const double* vf,
+                                             double level)
+{
+    DEM_GUARD({
+        if (!g || !vf) throw InvalidArg("null vector");
+        demsph_dem::HookOp h{};
+        h.kind = 2; h.rho = rho; h.cd = cd; h.level = level;
+        for (int k = 0; k < 3; k++) { h.g[k] = g[k]; h.vf[k] = vf[k]; }
+        d->hooks.push_back(h);
+    });
+}
+
+extern "C" int demsph_dem_sizes(demsph_dem* d, size_t* nb, size_t* nc, size_t* nw, size_t* nt, size_t* np, size_t* nbd)
+{
+    DEM_GUARD({
+        if (nb) *nb = d->initialized ? d->n : d->hb.size();
+        if (nc) *nc = d->initialized ? d->nc : d->hc.size();
+        if (nw) *nw = d->initialized ? d->nw : d->hw.mat.size();
+        if (nt) *nt = d->initialized ? d->nt : d->hw.t0.size();
+        if (np) *np = d->npairs;
+        if (nbd) *nbd = d->bond_a.size();
+    });
+}
+
+extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity, size_t* count)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!field) throw InvalidArg("null field");
+        const std::string f = field;
+        hipStream_t st = d->stream;
+        const int cb = d->cb;
+        const size_t n = d->n;
+        auto orig = fetch(d->borig[cb].p, n, st);
+        auto vec = [&](const DBuf<double>& x, const DBuf<double>& y, const DBuf<double>& z) {
+            auto hx = fetch(x.p, n, st), hy = fetch(y.p, n, st), hz = fetch(z.p, n, st);
+            std::vector<double> v(3 * n);
+            for (size_t i = 0; i < n; i++) {
+                v[3 * (size_t)orig[i]] = hx[i]; v[3 * (size_t)orig[i] + 1] = hy[i]; v[3 * (size_t)orig[i] + 2] = hz[i];
+            }
+            return v;
+        };
+        auto scal = [&](const DBuf<double>& x) {
+            auto hx = fetch(x.p, n, st);
+            std::vector<double> v(n);
+            for (size_t i = 0; i < n; i++) v[orig[i]] = hx[i];
+            return v;
+        };
+        auto plain3 = [&](const DBuf<double>& x, const DBuf<double>& y, const DBuf<double>& z, size_t m) {
+            auto hx = fetch(x.p, m, st), hy = fetch(y.p, m, st), hz = fetch(z.p, m, st);
+            std::vector<double> v(3 * m);
+            for (size_t i = 0; i < m; i++) { v[3 * i] = hx[i]; v[3 * i + 1] = hy[i]; v[3 * i + 2] = hz[i]; }
+            return v;
+        };
+        auto plain4 = [&](const DBuf<double>& a, const DBuf<double>& b, const DBuf<double>& c, const DBuf<double>& e,
+                          size_t m) {
+            auto ha = fetch(a.p, m, st), hb = fetch(b.p, m, st), hc = fetch(c.p, m, st), he = fetch(e.p, m, st);
+            std::vector<double> v(4 * m);
+            for (size_t i = 0; i < m; i++) { v[4 * i] = ha[i]; v[4 * i + 1] = hb[i]; v[4 * i + 2] = hc[i]; v[4 * i + 3] = he[i]; }
+            return v;
+        };
+        if (f == "position") return emit(vec(d->bpx[cb], d->bpy[cb], d->bpz[cb]), out, capacity, count);
+        if (f == "velocity") return emit(vec(d->bvx[cb], d->bvy[cb], d->bvz[cb]), out, capacity, count);
+        if (f == "angularVelocity") return emit(vec(d->bwx[cb], d->bwy[cb], d->bwz[cb]), out, capacity, count);
+        if (f == "force") return emit(vec(d->fx, d->fy, d->fz), out, capacity, count);
+        if (f == "torque") return emit(vec(d->tx, d->ty, d->tz), out, capacity, count);
+        if (f == "radius") return emit(scal(d->br[cb]), out, capacity, count);
+        if (f == "inverseMass") return emit(scal(d->bim[cb]), out, capacity, count);
+        if (f == "hashValue") {
+            auto k = fetch(d->skey.p, n, st);
+            return emit(std::vector<int>(k.begin(), k.end()), out, capacity, count);
+        }
+        if (f == "hashIndex") return emit(orig, out, capacity, count);
+        if (f == "cellHashStart") return emit(fetch(d->cellStart.p, (size_t)d->bng, st), out, capacity, count);
+        if (f == "cellHashEnd") return emit(fetch(d->cellEnd.p, (size_t)d->bng, st), out, capacity, count);
+        const bool pairf = f == "objectPointed" || f == "objectPointing" || f == "slidingSpring" ||
+                           f == "rollingSpring" || f == "torsionSpring" || f == "overlap";
+        if (pairf) {
+            if (!d->has_list) throw InvalidArg("no pair list yet");
+            const int c = d->cp;
+            auto start = fetch(d->pstart[c].p, n + 1, st);
+            auto part = fetch(d->ppartner[c].p, (size_t)d->npairs, st);
+            std::vector<int> rankh(n);
+            for (size_t i = 0; i < n; i++) rankh[orig[i]] = (int)i;
+            std::vector<double> s0, s1, s2;
+            int comps = 0;
+            if (f == "slidingSpring") { s0 = fetch(d->psx[c].p, d->npairs, st); s1 = fetch(d->psy[c].p, d->npairs, st); s2 = fetch(d->psz[c].p, d->npairs, st); comps = 3; }
+            if (f == "rollingSpring") { s0 = fetch(d->prx[c].p, d->npairs, st); s1 = fetch(d->pry[c].p, d->npairs, st); s2 = fetch(d->prz[c].p, d->npairs, st); comps = 3; }
+            if (f == "torsionSpring") { s0 = fetch(d->pox[c].p, d->npairs, st); s1 = fetch(d->poy[c].p, d->npairs, st); s2 = fetch(d->poz[c].p, d->npairs, st); comps = 3; }
+            if (f == "overlap") { s0 = fetch(d->pov[c].p, d->npairs, st); comps = 1; }
+            std::vector<int> vi;
+            std::vector<double> vd;
+            for (size_t o = 0; o < n; o++) {
+                const int i = rankh[o];
+                for (int p = start[i]; p < start[i + 1]; p++) {
+                    if (f == "objectPointed") vi.push_back((int)o);
+                    else if (f == "objectPointing") vi.push_back(orig[part[p]]);
+                    else {
+                        vd.push_back(s0[p]);
+                        if (comps == 3) { vd.push_back(s1[p]); vd.push_back(s2[p]); }
+                    }
+                }
+            }
+            if (f == "objectPointed" || f == "objectPointing") return emit(vi, out, capacity, count);
+            return emit(vd, out, capacity, count);
+        }
+        const bool trif = f == "triPointed" || f == "triPointing" || f == "triCancel" || f == "triSlidingSpring" ||
+                          f == "triOverlap";
+        if (trif) {
+            std::vector<int> vi;
+            std::vector<double> vd;
+            if (d->nt > 0) {
+                const size_t hn = (size_t)d->KW * n;
+                auto hc = fetch(d->hcnt.p, n, st);
+                auto ht = fetch(d->htri.p, hn, st);
+                auto hcan = fetch(d->hcancel.p, hn, st);
+                std::vector<double> sx, sy, sz, ov;
+                if (f == "triSlidingSpring") { sx = fetch(d->hsx.p, hn, st); sy = fetch(d->hsy.p, hn, st); sz = fetch(d->hsz.p, hn, st); }
+                if (f == "triOverlap") ov = fetch(d->hov.p, hn, st);
+                for (size_t o = 0; o < n; o++)
+                    for (int c = 0; c < hc[o]; c++) {
+                        const size_t h = (size_t)c * n + o;
+                        if (f == "triPointed") vi.push_back((int)o);
+                        else if (f == "triPointing") vi.push_back(ht[h]);
+                        else if (f == "triCancel") vi.push_back(hcan[h]);
+                        else if (f == "triOverlap") vd.push_back(ov[h]);
+                        else { vd.push_back(sx[h]); vd.push_back(sy[h]); vd.push_back(sz[h]); }
+                    }
+            }
+            if (f == "triSlidingSpring" || f == "triOverlap") return emit(vd, out, capacity, count);
+            return emit(vi, out, capacity, count);
+        }
+        const size_t nb = d->nbond;
+        if (f == "bondPointed") return emit(d->bond_a, out, capacity, count);
+        if (f == "bondPointing") return emit(d->bond_b, out, capacity, count);
+        if (f == "bondIsBonded") return emit(fetch(d->dbonded.p, nb, st), out, capacity, count);
+        if (f == "bondNormalForce") return emit(fetch(d->dbFn.p, nb, st), out, capacity, count);
+        if (f == "bondTorsionTorque") return emit(fetch(d->dbTt.p, nb, st), out, capacity, count);
+        if (f == "bondShearForce") return emit(plain3(d->dbsx, d->dbsy, d->dbsz, nb), out, capacity, count);
+        if (f == "bondBendingTorque") return emit(plain3(d->dbtx, d->dbty, d->dbtz, nb), out, capacity, count);
+        if (f == "bondPoint") return emit(plain3(d->dbpx, d->dbpy, d->dbpz, nb), out, capacity, count);
+        if (f == "bondNormal") return emit(plain3(d->dbnx, d->dbny, d->dbnz, nb), out, capacity, count);
+        const size_t nc = d->nc;
+        if (f == "clumpPosition") return emit(plain3(d->cpx, d->cpy, d->cpz, nc), out, capacity, count);
+        if (f == "clumpVelocity") return emit(plain3(d->cvx, d->cvy, d->cvz, nc), out, capacity, count);
+        if (f == "clumpAngularVelocity") return emit(plain3(d->cwx, d->cwy, d->cwz, nc), out, capacity, count);
+        if (f == "clumpOrientation") return emit(plain4(d->cq0, d->cq1, d->cq2, d->cq3, nc), out, capacity, count);
+        if (f == "wallPosition") return emit(plain3(d->wpx, d->wpy, d->wpz, d->nw), out, capacity, count);
+        if (f == "wallOrientation") return emit(plain4(d->wq0, d->wq1, d->wq2, d->wq3, d->nw), out, capacity, count);
+        if (f == "vertexGlobalPosition") return emit(plain3(d->vgx, d->vgy, d->vgz, d->nv), out, capacity, count);
+        throw InvalidArg("unknown field '" + f + "'");
+    });
+}
+
+extern "C" int demsph_dem_set(demsph_dem* d, const char* field, const void* in, size_t count)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!field || !in) throw InvalidArg("null argument");
+        const std::string f = field;
+        const double* src = static_cast<const double*>(in);
+        hipStream_t st = d->stream;
+        const int cb = d->cb;
+        if (f == "position" || f == "velocity" || f == "angularVelocity") {
+            const size_t n = d->n;
+            if (count != 3 * n) throw InvalidArg("expected 3*numBalls values");
+            auto orig = fetch(d->borig[cb].p, n, st);
+            std::vector<double> hx(n), hy(n), hz(n);
+            for (size_t i = 0; i < n; i++) {
+                hx[i] = src[3 * (size_t)orig[i]]; hy[i] = src[3 * (size_t)orig[i] + 1]; hz[i] = src[3 * (size_t)orig[i] + 2];
+            }
+            DBuf<double>* t[3];
+            if (f == "position") { t[0] = &d->bpx[cb]; t[1] = &d->bpy[cb]; t[2] = &d->bpz[cb]; }
+            else if (f == "velocity") { t[0] = &d->bvx[cb]; t[1] = &d->bvy[cb]; t[2] = &d->bvz[cb]; }
+            else { t[0] = &d->bwx[cb]; t[1] = &d->bwy[cb]; t[2] = &d->bwz[cb]; }
+            DEMSPH_HIP(hipMemcpy(t[0]->p, hx.data(), n * 8, hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(t[1]->p, hy.data(), n * 8, hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(t[2]->p, hz.data(), n * 8, hipMemcpyHostToDevice));
+            return;
+        }
+        if (f == "wallVelocity" || f == "wallAngularVelocity") {
+            const size_t m = d->nw;
+            if (count != 3 * m) throw InvalidArg("expected 3*numWalls values");
+            if (m == 0) return;
+            std::vector<double> hx(m), hy(m), hz(m);
+            for (size_t i = 0; i < m; i++) { hx[i] = src[3 * i]; hy[i] = src[3 * i + 1]; hz[i] = src[3 * i + 2]; }
+            DEMSPH_HIP(hipStreamSynchronize(st));
+            if (f == "wallVelocity") {
+                DEMSPH_HIP(hipMemcpy(d->wvx.p, hx.data(), m * 8, hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpy(d->wvy.p, hy.data(), m * 8, hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpy(d->wvz.p, hz.data(), m * 8, hipMemcpyHostToDevice));
+            } else {
+                DEMSPH_HIP(hipMemcpy(d->wwx.p, hx.data(), m * 8, hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpy(d->wwy.p, hy.data(), m * 8, hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpy(d->wwz.p, hz.data(), m * 8, hipMemcpyHostToDevice));
+            }
+            return;
+        }
+        throw InvalidArg("unknown field '" + f + "'");
+    });
+}
+
+extern "C" int demsph_dem_enable_timing(demsph_dem* d, int enable)
+{
+    DEM_GUARD({
+        DEMSPH_HIP(hipStreamSynchronize(d->stream));
+        d->timer.harvest();
+        d->timer.enabled = enable != 0;
+        d->timer.reset();
+    });
+}
+
+extern "C" int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long* launches, double* totalMs)
+{
+    DEM_GUARD({
+        DEMSPH_HIP(hipStreamSynchronize(d->stream));
+        d->timer.harvest();
+        const auto& t = d->timer.totals();
+        if (index < 0 || (size_t)index >= t.size()) throw InvalidArg("timing index out of range");
+        auto it = t.begin();
+        std::advance(it, index);
+        if (name && nameCap) {
+            std::strncpy(name, it->first.c_str(), nameCap - 1);
+            name[nameCap - 1] = 0;
+        }
+        if (launches) *launches = it->second.first;
+        if (totalMs) *totalMs = it->second.second;
+    });
+}

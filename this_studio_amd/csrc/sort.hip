@@ -196,6 +196,28 @@ __global__ void k_fill_i32(int* p, long long n, int v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_iota_i32(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_hash_soa_by_orig(int n, GridGeom g, const double* __restrict__ px, const double* __restrict__ py,
+                                   const double* __restrict__ pz, const int* __restrict__ orig,
+                                   uint32_t* __restrict__ key)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    key[orig ? orig[s] : s] = (uint32_t)cell_hash(g, px[s], py[s], pz[s]);
+}
+
+__global__ void k_hash_aos_by_index(const double* __restrict__ pos, int n, GridGeom g, uint32_t* __restrict__ key)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = (uint32_t)cell_hash(g, pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]);
+}
+
 // Radix histogram offsets are uint32; scan them as int (n < 2^31).
 }  // namespace
 
@@ -255,6 +277,28 @@ void radix_sort_pairs(const uint32_t* keys_in, const int* vals_in, uint32_t* key
     }
 }
 
+void iota_i32(int* p, int n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+void hash_soa_by_orig(int n, const GridGeom& g, const double* px, const double* py, const double* pz, const int* orig,
+                      uint32_t* key, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_hash_soa_by_orig, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, g, px, py, pz, orig, key);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+void hash_aos(const double* pos, int n, const GridGeom& g, uint32_t* key, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_hash_aos_by_index, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, n, g, key);
+    DEMSPH_HIP(hipGetLastError());
+}
+
 void fill_i32(int* p, long long n, int v, hipStream_t s)
 {
     if (n <= 0) return;
@@ -293,13 +337,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 // ----------------------------------------------------------------- raw cells
 namespace {
-__global__ void k_hash_aos(const double* __restrict__ pos, int n, GridGeom g, uint32_t* __restrict__ hash)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    hash[i] = (uint32_t)cell_hash(g, pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]);
-}
-
 // findStartAndEnd (buildHashStartEnd.cu:14-34) over sorted hashes.
 __global__ void k_cell_ranges(const uint32_t* __restrict__ sh, int n, long long numGrids, int* __restrict__ start,
                               int* __restrict__ end)
@@ -312,6 +349,16 @@ __global__ void k_cell_ranges(const uint32_t* __restrict__ sh, int n, long long 
     if (i == n - 1 || sh[i + 1] != sh[i]) end[h] = i + 1;
 }
 
+}  // namespace
+
+void cell_ranges(const uint32_t* sorted_keys, int n, long long numGrids, int* start, int* end, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, sorted_keys, n, numGrids, start, end);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+namespace {
 struct RawCellCache {
     std::mutex mu;
     int device = -1;
@@ -380,12 +427,9 @@ extern "C" int demsph_build_cells(const double* position, int* hashIndex, int* h
             DEMSPH_HIP(hipMalloc(&g_raw.temp, g_raw.temp_bytes));
             g_raw.device = dev;
         }
-        hipLaunchKernelGGL(k_hash_aos, dim3(blocks_for(n, 256)), dim3(256), 0, s, position, n, g, g_raw.keys);
+        hash_aos(position, n, g, g_raw.keys, s);
         radix_sort_pairs(g_raw.keys, nullptr, reinterpret_cast<uint32_t*>(hashValue), hashIndex, g_raw.keys_tmp,
                          g_raw.vals_tmp, g_raw.temp, n, key_bits((long long)numGrids), s);
-        hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s,
-                           reinterpret_cast<const uint32_t*>(hashValue), n, (long long)numGrids, cellHashStart,
-                           cellHashEnd);
-        DEMSPH_HIP(hipGetLastError());
+        cell_ranges(reinterpret_cast<const uint32_t*>(hashValue), n, (long long)numGrids, cellHashStart, cellHashEnd, s);
     });
 }
