@@ -14,6 +14,7 @@
 // contact force model runs in FP32 (default) or FP64, selected per scene.
 #include "common.hpp"
 #include "dem_kernels.hpp"
+#include "hook_math.hpp"
 #include "timing.hpp"
 
 #include <algorithm>
@@ -43,7 +44,7 @@ struct Pairs {
     int* start;    // n + 1 (owner CSR)
     int* partner;  // cap
     double *sx, *sy, *sz, *rx, *ry, *rz, *ox, *oy, *oz;  // sliding, rolling, torsion springs
-    float *Fx, *Fy, *Fz, *Rx, *Ry, *Rz;                  // pair force, partner-side torque
+    void *Fx, *Fy, *Fz, *Rx, *Ry, *Rz;                   // pair force, partner-side torque (precision T)
     double* ov;                                           // overlap
 };
 
@@ -206,7 +207,7 @@ struct PairForceArgs {
     Pairs p;
     Tables tb;
     double dt;
-    float *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner-side sums per slot
+    void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner-side sums per slot (precision T)
 };
 
 // updateBallContactKernel + calBallContactForceTorqueKernel
@@ -234,30 +235,33 @@ __global__ __launch_bounds__(kT) void k_pair_forces(PairForceArgs a)
         const D3 rc = rj + n * (aj - 0.5 * delta);
         a.p.ov[p] = delta;
         D3 es{a.p.sx[p], a.p.sy[p], a.p.sz[p]}, er{a.p.rx[p], a.p.ry[p], a.p.rz[p]}, et{a.p.ox[p], a.p.oy[p], a.p.oz[p]};
-        V3<T> F{0, 0, 0}, Tq{0, 0, 0};
+        V3<T> F{0, 0, 0}, Tq{0, 0, 0}, Fs{0, 0, 0};
         if (delta > 0) {
             const double R = ai * aj / (ai + aj);
             const double m = 1. / (imi + a.b.im[j]);
             const D3 vj = ld3(a.b.vx, a.b.vy, a.b.vz, j), wj = ld3(a.b.wx, a.b.wy, a.b.wz, j);
             const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
             const int ip = pair_index(mi, a.b.mat[j], a.tb.nMat, a.tb.cap);
-            contact_model<T>(a.tb, ip, m, R, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es, er, et, F, Tq);
+            contact_model<T>(a.tb, ip, m, R, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es, er, et, F, Tq,
+                             Fs);
         } else {
             es = er = et = D3{0, 0, 0};
         }
         a.p.sx[p] = es.x; a.p.sy[p] = es.y; a.p.sz[p] = es.z;
         a.p.rx[p] = er.x; a.p.ry[p] = er.y; a.p.rz[p] = er.z;
         a.p.ox[p] = et.x; a.p.oy[p] = et.y; a.p.oz[p] = et.z;
+        // T_c + (r_c - r) x F_c (contactKernel.cu:676, :707-708), with the
+        // normal part of F_c (parallel to the lever arm) left out
         const V3<T> li = cvt<T>(rc - ri), lj = cvt<T>(rc - rj);
-        const V3<T> Ti = Tq + cross(li, F);
-        const V3<T> Tj = Tq + cross(lj, F);
-        a.p.Fx[p] = (float)F.x; a.p.Fy[p] = (float)F.y; a.p.Fz[p] = (float)F.z;
-        a.p.Rx[p] = (float)Tj.x; a.p.Ry[p] = (float)Tj.y; a.p.Rz[p] = (float)Tj.z;
+        const V3<T> Ti = Tq + cross(li, Fs);
+        const V3<T> Tj = Tq + cross(lj, Fs);
+        ((T*)a.p.Fx)[p] = F.x; ((T*)a.p.Fy)[p] = F.y; ((T*)a.p.Fz)[p] = F.z;
+        ((T*)a.p.Rx)[p] = Tj.x; ((T*)a.p.Ry)[p] = Tj.y; ((T*)a.p.Rz)[p] = Tj.z;
         Fo = Fo + F;
         To = To + Ti;
     }
-    a.ofx[i] = (float)Fo.x; a.ofy[i] = (float)Fo.y; a.ofz[i] = (float)Fo.z;
-    a.otx[i] = (float)To.x; a.oty[i] = (float)To.y; a.otz[i] = (float)To.z;
+    ((T*)a.ofx)[i] = Fo.x; ((T*)a.ofy)[i] = Fo.y; ((T*)a.ofz)[i] = Fo.z;
+    ((T*)a.otx)[i] = To.x; ((T*)a.oty)[i] = To.y; ((T*)a.otz)[i] = To.z;
 }
 
 struct BondArgs {
@@ -267,7 +271,7 @@ struct BondArgs {
     const int *bi, *bj;  // insertion indices
     int* bonded;
     double *bpx, *bpy, *bpz, *bnx, *bny, *bnz, *bsx, *bsy, *bsz, *btx, *bty, *btz, *bFn, *bTt, *bsig, *btau;
-    float *Fx, *Fy, *Fz, *Tix, *Tiy, *Tiz, *Tjx, *Tjy, *Tjz;  // per-bond contributions to i and j
+    void *Fx, *Fy, *Fz, *Tix, *Tiy, *Tiz, *Tjx, *Tjy, *Tjz;  // per-bond contributions to i and j (T)
     Tables tb;
     double dt;
 };
@@ -275,6 +279,7 @@ struct BondArgs {
 // calBondedForceTorqueKernel (contactKernel.cu:535-655).  The bond force
 // goes to both balls through the per-ball gather (no atomics); the lever
 // arm of the shear force is taken about each ball centre.
+template <class T>
 __global__ __launch_bounds__(kT) void k_bond_forces(BondArgs a)
 {
     const int k = blockIdx.x * kT + threadIdx.x;
@@ -283,9 +288,9 @@ __global__ __launch_bounds__(kT) void k_bond_forces(BondArgs a)
         a.bFn[k] = 0; a.bTt[k] = 0;
         a.bsx[k] = a.bsy[k] = a.bsz[k] = 0;
         a.btx[k] = a.bty[k] = a.btz[k] = 0;
-        a.Fx[k] = a.Fy[k] = a.Fz[k] = 0.f;
-        a.Tix[k] = a.Tiy[k] = a.Tiz[k] = 0.f;
-        a.Tjx[k] = a.Tjy[k] = a.Tjz[k] = 0.f;
+        ((T*)a.Fx)[k] = ((T*)a.Fy)[k] = ((T*)a.Fz)[k] = 0;
+        ((T*)a.Tix)[k] = ((T*)a.Tiy)[k] = ((T*)a.Tiz)[k] = 0;
+        ((T*)a.Tjx)[k] = ((T*)a.Tjy)[k] = ((T*)a.Tjz)[k] = 0;
         return;
     }
     const int i = a.rank[a.bi[k]], j = a.rank[a.bj[k]];
@@ -313,9 +318,9 @@ __global__ __launch_bounds__(kT) void k_bond_forces(BondArgs a)
     const D3 Tn = n * Tt + Tb;
     const D3 Ti = Tn + cross(rc - ri, Fs);
     const D3 Tj = -Tn + cross(rc - rj, -Fs);
-    a.Fx[k] = (float)Fb.x; a.Fy[k] = (float)Fb.y; a.Fz[k] = (float)Fb.z;
-    a.Tix[k] = (float)Ti.x; a.Tiy[k] = (float)Ti.y; a.Tiz[k] = (float)Ti.z;
-    a.Tjx[k] = (float)Tj.x; a.Tjy[k] = (float)Tj.y; a.Tjz[k] = (float)Tj.z;
+    ((T*)a.Fx)[k] = (T)Fb.x; ((T*)a.Fy)[k] = (T)Fb.y; ((T*)a.Fz)[k] = (T)Fb.z;
+    ((T*)a.Tix)[k] = (T)Ti.x; ((T*)a.Tiy)[k] = (T)Ti.y; ((T*)a.Tiz)[k] = (T)Ti.z;
+    ((T*)a.Tjx)[k] = (T)Tj.x; ((T*)a.Tjy)[k] = (T)Tj.y; ((T*)a.Tjz)[k] = (T)Tj.z;
 }
 
 // ----------------------------------------------------------------- walls
@@ -335,7 +340,7 @@ struct WallArgs {
     int *htri, *hcancel;
     double *hsx, *hsy, *hsz, *hrx, *hry, *hrz, *hox, *hoy, *hoz;
     double *hov;
-    float *ofx, *ofy, *ofz, *otx, *oty, *otz;  // wall force/torque per slot
+    void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // wall force/torque per slot (precision T)
     int* overflow;
 };
 
@@ -459,11 +464,11 @@ __global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
             const D3 rj = ld3(a.wpx, a.wpy, a.wpz, w), vj = ld3(a.wvx, a.wvy, a.wvz, w), wj = ld3(a.wwx, a.wwy, a.wwz, w);
             const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
             const int ip = pair_index(a.b.mat[i], a.wmat[w], a.tb.nMat, a.tb.cap);
-            V3<T> F{0, 0, 0}, Tq{0, 0, 0};
+            V3<T> F{0, 0, 0}, Tq{0, 0, 0}, Fs{0, 0, 0};
             contact_model<T>(a.tb, ip, m, rad, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es[c], er[c], et[c],
-                             F, Tq);
+                             F, Tq, Fs);
             Fw = Fw + F;
-            Tw = Tw + Tq + cross(cvt<T>(rc - ri), F);
+            Tw = Tw + Tq + cross(cvt<T>(rc - ri), Fs);
         }
     }
     // new history
@@ -477,34 +482,38 @@ __global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
         a.hox[h] = et[c].x; a.hoy[h] = et[c].y; a.hoz[h] = et[c].z;
         a.hov[h] = rad - len(ri - cp[c]);
     }
-    a.ofx[i] = (float)Fw.x; a.ofy[i] = (float)Fw.y; a.ofz[i] = (float)Fw.z;
-    a.otx[i] = (float)Tw.x; a.oty[i] = (float)Tw.y; a.otz[i] = (float)Tw.z;
+    ((T*)a.ofx)[i] = Fw.x; ((T*)a.ofy)[i] = Fw.y; ((T*)a.ofz)[i] = Fw.z;
+    ((T*)a.otx)[i] = Tw.x; ((T*)a.oty)[i] = Tw.y; ((T*)a.otz)[i] = Tw.z;
 }
 
 // ----------------------------------------------------------------- gather
 struct GatherArgs {
     Balls b;
-    const float *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner sums
-    const float *pFx, *pFy, *pFz, *pRx, *pRy, *pRz;  // pair results
+    const void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner sums
+    const void *pFx, *pFy, *pFz, *pRx, *pRy, *pRz;  // pair results
     const int *rstart, *rpair;
-    const int *istart, *ibond;                     // bond incidence by insertion index (id*2 + side)
-    const float *bFx, *bFy, *bFz, *bTix, *bTiy, *bTiz, *bTjx, *bTjy, *bTjz;
-    const float *wfx, *wfy, *wfz, *wtx, *wty, *wtz;  // walls (nullable)
+    const int *istart, *ibond;                    // bond incidence by insertion index (id*2 + side)
+    const void *bFx, *bFy, *bFz, *bTix, *bTiy, *bTiz, *bTjx, *bTjy, *bTjz;
+    const void *wfx, *wfy, *wfz, *wtx, *wty, *wtz;  // walls (nullable)
     Forces out;
 };
 
+template <class T>
+__device__ __forceinline__ T at(const void* p, int i) { return ((const T*)p)[i]; }
+
 // sumObjectPointingForceTorqueFromInteraction (contactKernel.cu:683-714) and
 // the per-ball totals: owner sums - partner-side pair terms + bonds + walls.
+template <class T>
 __global__ __launch_bounds__(kT) void k_gather(GatherArgs a)
 {
     const int j = xcd_block() * kT + threadIdx.x;
     if (j >= a.b.n) return;
-    float fx = a.ofx[j], fy = a.ofy[j], fz = a.ofz[j];
-    float tx = a.otx[j], ty = a.oty[j], tz = a.otz[j];
+    T fx = at<T>(a.ofx, j), fy = at<T>(a.ofy, j), fz = at<T>(a.ofz, j);
+    T tx = at<T>(a.otx, j), ty = at<T>(a.oty, j), tz = at<T>(a.otz, j);
     for (int q = a.rstart[j]; q < a.rstart[j + 1]; q++) {
         const int p = a.rpair[q];
-        fx -= a.pFx[p]; fy -= a.pFy[p]; fz -= a.pFz[p];
-        tx -= a.pRx[p]; ty -= a.pRy[p]; tz -= a.pRz[p];
+        fx -= at<T>(a.pFx, p); fy -= at<T>(a.pFy, p); fz -= at<T>(a.pFz, p);
+        tx -= at<T>(a.pRx, p); ty -= at<T>(a.pRy, p); tz -= at<T>(a.pRz, p);
     }
     if (a.istart) {
         const int o = a.b.orig[j];
@@ -512,17 +521,17 @@ __global__ __launch_bounds__(kT) void k_gather(GatherArgs a)
             const int e = a.ibond[q];
             const int k = e >> 1;
             if ((e & 1) == 0) {
-                fx += a.bFx[k]; fy += a.bFy[k]; fz += a.bFz[k];
-                tx += a.bTix[k]; ty += a.bTiy[k]; tz += a.bTiz[k];
+                fx += at<T>(a.bFx, k); fy += at<T>(a.bFy, k); fz += at<T>(a.bFz, k);
+                tx += at<T>(a.bTix, k); ty += at<T>(a.bTiy, k); tz += at<T>(a.bTiz, k);
             } else {
-                fx -= a.bFx[k]; fy -= a.bFy[k]; fz -= a.bFz[k];
-                tx += a.bTjx[k]; ty += a.bTjy[k]; tz += a.bTjz[k];
+                fx -= at<T>(a.bFx, k); fy -= at<T>(a.bFy, k); fz -= at<T>(a.bFz, k);
+                tx += at<T>(a.bTjx, k); ty += at<T>(a.bTjy, k); tz += at<T>(a.bTjz, k);
             }
         }
     }
     if (a.wfx) {
-        fx += a.wfx[j]; fy += a.wfy[j]; fz += a.wfz[j];
-        tx += a.wtx[j]; ty += a.wty[j]; tz += a.wtz[j];
+        fx += at<T>(a.wfx, j); fy += at<T>(a.wfy, j); fz += at<T>(a.wfz, j);
+        tx += at<T>(a.wtx, j); ty += at<T>(a.wty, j); tz += at<T>(a.wtz, j);
     }
     a.out.fx[j] = fx; a.out.fy[j] = fy; a.out.fz[j] = fz;
     a.out.tx[j] = tx; a.out.ty[j] = ty; a.out.tz[j] = tz;
@@ -670,25 +679,34 @@ __global__ void k_circumcenters(WallMoveArgs a)
 // addConstantForce (externalForceTorque.cu:4-12), per insertion index.
 __global__ void k_hook_const(Balls b, Forces f, const double* fo, double s)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.n) return;
     const int o = b.orig[i];
-    f.fx[i] += fo[3 * (size_t)o] * s;
-    f.fy[i] += fo[3 * (size_t)o + 1] * s;
-    f.fz[i] += fo[3 * (size_t)o + 2] * s;
+    const double ax = fo[3 * (size_t)o] * s, ay = fo[3 * (size_t)o + 1] * s, az = fo[3 * (size_t)o + 2] * s;
+    f.fx[i] += ax;
+    f.fy[i] += ay;
+    f.fz[i] += az;
 }
 // addGlobalDampingForceTorque (externalForceTorque.cu:26-37)
 __global__ void k_hook_damping(Balls b, Forces f, double c)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.n) return;
     const D3 F = ld3(f.fx, f.fy, f.fz, i), T = ld3(f.tx, f.ty, f.tz, i);
-    st3(f.fx, f.fy, f.fz, i, F - normalize(ld3(b.vx, b.vy, b.vz, i)) * (c * len(F)));
-    st3(f.tx, f.ty, f.tz, i, T - normalize(ld3(b.wx, b.wy, b.wz, i)) * (c * len(T)));
+    double nx, ny, nz;
+    const double cf = c * hk_len(F.x, F.y, F.z);
+    hk_unit(b.vx[i], b.vy[i], b.vz[i], nx, ny, nz);
+    st3(f.fx, f.fy, f.fz, i, D3{F.x - nx * cf, F.y - ny * cf, F.z - nz * cf});
+    const double ct = c * hk_len(T.x, T.y, T.z);
+    hk_unit(b.wx[i], b.wy[i], b.wz[i], nx, ny, nz);
+    st3(f.tx, f.ty, f.tz, i, D3{T.x - nx * ct, T.y - ny * ct, T.z - nz * ct});
 }
 // addBuoyancyDragKernel (externalForceTorque.cu:66-133)
 __global__ void k_hook_buoyancy(Balls b, Forces f, double rho, double cd, D3 g, D3 vf, double level)
 {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.n) return;
     if (b.im[i] <= 0.0) return;
@@ -844,7 +862,7 @@ struct demsph_dem {
     int pcap[2] = {0, 0};
     DBuf<int> pstart[2], ppartner[2];
     DBuf<double> psx[2], psy[2], psz[2], prx[2], pry[2], prz[2], pox[2], poy[2], poz[2], pov[2];
-    DBuf<float> pFx[2], pFy[2], pFz[2], pRx[2], pRy[2], pRz[2];
+    DBuf<double> pFx[2], pFy[2], pFz[2], pRx[2], pRy[2], pRz[2];  // hold float or double
     int cp = 0;
     bool has_list = false;
     int npairs = 0;
@@ -852,13 +870,13 @@ struct demsph_dem {
     void* scan_temp = nullptr;
     DBuf<int> total;  // device int
     int* total_h = nullptr;
-    DBuf<float> ofx, ofy, ofz, otx, oty, otz;
+    DBuf<double> ofx, ofy, ofz, otx, oty, otz;  // hold float or double
     // material tables
     DBuf<double> tH, tL, tB;
     // bonds
     DBuf<int> dbi, dbj, dbonded, dinc_start, dinc_bond;
     DBuf<double> dbpx, dbpy, dbpz, dbnx, dbny, dbnz, dbsx, dbsy, dbsz, dbtx, dbty, dbtz, dbFn, dbTt, dbsig, dbtau;
-    DBuf<float> dbFx, dbFy, dbFz, dbTix, dbTiy, dbTiz, dbTjx, dbTjy, dbTjz;
+    DBuf<double> dbFx, dbFy, dbFz, dbTix, dbTiy, dbTiz, dbTjx, dbTjy, dbTjz;  // hold float or double
     // clumps
     DBuf<double> cpx, cpy, cpz, cvx, cvy, cvz, cwx, cwy, cwz, cq0, cq1, cq2, cq3, cixx, ciyy, cizz, cixy, cixz, ciyz,
         cim, cFx, cFy, cFz, cTx, cTy, cTz;
@@ -875,7 +893,7 @@ struct demsph_dem {
     int KW = 16;
     DBuf<int> hcnt, htri, hcancel;
     DBuf<double> hsx, hsy, hsz, hrx, hry, hrz, hox, hoy, hoz, hov;
-    DBuf<float> wfx, wfy, wfz, wtx, wty, wtz;
+    DBuf<double> wfx, wfy, wfz, wtx, wty, wtz;  // hold float or double
     DBuf<int> overflow;
     int* overflow_h = nullptr;
     bool walls_valid = false;  // wall forces of the last triangle evaluation are held
@@ -1255,7 +1273,8 @@ struct demsph_dem {
             ba.Fx = dbFx.p; ba.Fy = dbFy.p; ba.Fz = dbFz.p;
             ba.Tix = dbTix.p; ba.Tiy = dbTiy.p; ba.Tiz = dbTiz.p; ba.Tjx = dbTjx.p; ba.Tjy = dbTjy.p; ba.Tjz = dbTjz.p;
             ba.tb = tables(); ba.dt = springDt;
-            DEMSPH_LAUNCH(timer, "dem_bond_forces", k_bond_forces, dim3(blocks_for(nbond, kT)), tb, 0, stream, ba);
+            if (fp64) DEMSPH_LAUNCH(timer, "dem_bond_forces", k_bond_forces<double>, dim3(blocks_for(nbond, kT)), tb, 0, stream, ba);
+            else DEMSPH_LAUNCH(timer, "dem_bond_forces", k_bond_forces<float>, dim3(blocks_for(nbond, kT)), tb, 0, stream, ba);
         }
     }
 
@@ -1306,7 +1325,8 @@ struct demsph_dem {
             ga.wfx = wfx.p; ga.wfy = wfy.p; ga.wfz = wfz.p; ga.wtx = wtx.p; ga.wty = wty.p; ga.wtz = wtz.p;
         }
         ga.out = forces();
-        DEMSPH_LAUNCH(timer, "dem_gather", k_gather, gb, tb, 0, stream, ga);
+        if (fp64) DEMSPH_LAUNCH(timer, "dem_gather", k_gather<double>, gb, tb, 0, stream, ga);
+        else DEMSPH_LAUNCH(timer, "dem_gather", k_gather<float>, gb, tb, 0, stream, ga);
     }
 
     void move_walls(double dt)

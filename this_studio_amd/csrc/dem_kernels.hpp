@@ -212,11 +212,13 @@ __device__ __forceinline__ D3 spring_torsion(D3 prev, V3<T> tv, V3<T> n, V3<T> F
 
 // LinearContact / HertzianMindlinContact (contactKernel.cuh:128-290) with
 // the model choice of contactKernel.cu:218-252 (linear if k_n > 1e-20).
-// delta <= 0 zeroes the springs and leaves F, Tq untouched.
+// delta <= 0 zeroes the springs and leaves F, Tq, Fs untouched.  Fs is the
+// tangential (sliding) part of F: the lever arm of a contact is parallel to
+// n, so only Fs carries a moment about a ball centre.
 template <class T>
 __device__ __forceinline__ void contact_model(const Tables& tb, int ip, double m, double R, double delta, V3<T> vc,
                                               V3<T> wij, V3<T> n, double dt, D3& es, D3& er, D3& et, V3<T>& F,
-                                              V3<T>& Tq)
+                                              V3<T>& Tq, V3<T>& Ftan)
 {
     if (!(delta > 0)) {
         es = er = et = D3{0.0, 0.0, 0.0};
@@ -261,6 +263,7 @@ __device__ __forceinline__ void contact_model(const Tables& tb, int ip, double m
     et = spring_torsion<T>(et, vt, n, Fn, mut, kt, ct, dt);
     const V3<T> Ft = cvt<T>(et) * (-kt) - vt * ct;
     F = Fn + Fs;
+    Ftan = Fs;
     Tq = Tr + Ft * D2;
 }
 

@@ -3,20 +3,14 @@
 // DEMSPH/externalForceTorque.cu:4-161 (constant force, global damping,
 // spherical-cap buoyancy with quadratic drag).
 #include "common.hpp"
+#include "hook_math.hpp"
 #include "../../include/externalForceTorque.h"
 
 namespace {
 
-__device__ __forceinline__ void norm3(double x, double y, double z, double& nx, double& ny, double& nz)
-{
-    const double l2 = x * x + y * y + z * z;
-    if (l2 < 1.0e-20) { nx = ny = nz = 0.0; return; }
-    const double inv = 1.0 / sqrt(l2);
-    nx = x * inv; ny = y * inv; nz = z * inv;
-}
-
 __global__ void k_add_constant(double3* force, const double3* fe, size_t num)
 {
+#pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
     force[i].x += fe[i].x;
@@ -26,17 +20,18 @@ __global__ void k_add_constant(double3* force, const double3* fe, size_t num)
 
 __global__ void k_add_damping(double3* force, double3* torque, const double3* v, const double3* w, double c, size_t num)
 {
+#pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
     const double3 f = force[i], t = torque[i];
     double nx, ny, nz;
-    const double lf = sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
-    norm3(v[i].x, v[i].y, v[i].z, nx, ny, nz);
+    const double lf = hk_len(f.x, f.y, f.z);
+    hk_unit(v[i].x, v[i].y, v[i].z, nx, ny, nz);
     force[i].x = f.x - nx * (c * lf);
     force[i].y = f.y - ny * (c * lf);
     force[i].z = f.z - nz * (c * lf);
-    const double lt = sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
-    norm3(w[i].x, w[i].y, w[i].z, nx, ny, nz);
+    const double lt = hk_len(t.x, t.y, t.z);
+    hk_unit(w[i].x, w[i].y, w[i].z, nx, ny, nz);
     torque[i].x = t.x - nx * (c * lt);
     torque[i].y = t.y - ny * (c * lt);
     torque[i].z = t.z - nz * (c * lt);
@@ -46,6 +41,7 @@ __global__ void k_add_buoyancy(double3* force, const double3* pos, const double3
                                const double* invMass, double rho, double cd, double3 g, double3 vf, double level,
                                size_t num)
 {
+#pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num) return;
     if (invMass && invMass[i] <= 0.0) return;
