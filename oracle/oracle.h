@@ -16,8 +16,39 @@
  *   D2  pointing ranges are reset every rebuild (intended).
  *   D3/D11/D16/D6/D7/D9/D10 are reproduced.
  *
- * Parity status of each function is recorded in the header comment of the
- * function ("pinned by" / "parity unpinned").
+ * What pins each function (tests/test_oracle_*.py unless noted; a pin is a
+ * check the restatement could fail: a golden vector of the reference's own
+ * run, a closed form, an invariant or brute force):
+ *   o_grid_set            test_grid_set_closed_form (incl. a binary64 truncation case)
+ *   o_hash                test_hash_known_cells_and_clamps (hand-computed cells, all clamps)
+ *   o_sort_by_hash        test_sort_is_stable_lexicographic (numpy lexsort of (hash, index))
+ *   o_cell_ranges         test_cell_ranges_brute_force
+ *   o_stencil, o_sph_count/write
+ *                         test_sph_pairs_match_brute_force (O(N^2) within the same stencil),
+ *                         test_sph_pair_set_independent_of_cell_size (invariant)
+ *   o_wendland            test_wendland_unit_integral (integral of W = 1)
+ *   o_grad_wendland       test_grad_wendland_is_derivative_of_w, test_lattice_gradient_moment_is_identity
+ *   o_sph dummy normals   test_dummy_normals_point_into_the_fluid_on_flat_walls (symmetry)
+ *   o_sph step (acceleration, density change, EOS, kick/drift)
+ *                         test_rest_lattice_has_no_density_change_in_interior,
+ *                         test_hydrostatic_column_balances_gravity_in_interior,
+ *                         test_tait_hydrostatic_column (Tait inversion + balance),
+ *                         test_oracle_reproduces_reference_dam_break_frame1 (reference frame, step 670)
+ *   o_dem neighbour search + history carry
+ *                         test_ball_pairs_match_brute_force_stencil, test_spring_history_survives_rebuild
+ *   linear contact, ball-wall contact, edge/vertex cancel
+ *                         test_elastic_bounce_matches_reference_frames (reference frames, 10 decimals),
+ *                         test_linear_collision_restitution_closed_form
+ *   Hertz-Mindlin         test_hertz_collision_matches_closed_form
+ *   Coulomb cap, rolling  test_sliding_ball_coulomb_cap_and_rolling_transition
+ *   parallel bond         test_bonded_beam_tip_deflection_is_euler_bernoulli
+ *   o_sphere_triangle     test_closest_point_on_triangle_brute_force
+ *   mesh walls            test_cylinder_mesh_matches_reference_wall_frame (reference wall frame)
+ *   clumps                test_rigid_clump_torque_free_rotation
+ *   hooks                 test_buoyancy_hook_is_archimedes, test_global_damping_hook_closed_form,
+ *                         constant force through the beam's Euler-Bernoulli deflection
+ * Parity unpinned: none of the functions above.  Clump-on-wall contact
+ * reuses the pinned ball-wall and clump code paths and has no pin of its own.
  */
 #ifndef DEMSPH_ORACLE_H
 #define DEMSPH_ORACLE_H

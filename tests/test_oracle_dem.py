@@ -265,3 +265,26 @@ def test_cylinder_mesh_matches_reference_wall_frame():
     np.testing.assert_allclose(v + pos, fx["wall_vertices"], atol=1e-9)
     conn = np.stack([t0, t1, t2], 1).reshape(-1)
     assert np.array_equal(conn, fx["wall_connectivity"])
+
+
+def test_global_damping_hook_closed_form():
+    """addGlobalDampingForceTorque (externalForceTorque.cu:26-57): F -= c|F| v_hat.
+    A free ball pushed by a constant external force f in zero gravity feels f
+    at the first evaluation of a step sequence (v = 0, so v_hat = 0) and
+    f (1 - c) at every later one, since v stays parallel to f.  After N
+    kick-drift-kick steps (two half kicks per step):
+    v = f/m dt/2 + (2N - 1) f (1 - c)/m dt/2."""
+    r, rho, c, fz, dt, N = 0.01, 2000.0, 0.2, 3.0, 1e-4, 50
+    s = scenes.DEMScene("damped")
+    s.add_balls([0.0, 0.0, 0.0], [0, 0, 0], [0, 0, 0], r, rho, 0)
+    s.set_linear_pair(0, 0, 1e3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    s.hooks = [("const", np.array([[0.0, 0.0, fz]]), 0.0), ("damping", c)]
+    s.min_bound, s.max_bound = -np.ones(3), np.ones(3)
+    s.gravity = np.zeros(3)
+    s.dt = dt
+    o = oracle.OracleDEM(s)
+    o.step(N)
+    m = 4.0 / 3.0 * math.pi * r ** 3 * rho
+    v = fz / m * dt / 2 + (2 * N - 1) * fz * (1 - c) / m * dt / 2
+    assert o.get("vel")[0, 2] == pytest.approx(v, rel=1e-12)
+    assert o.get("force")[0, 2] == pytest.approx(fz * (1 - c), rel=1e-12)

@@ -115,3 +115,58 @@ def test_oracle_reproduces_reference_dam_break_frame1():
     assert err < 5e-7, err  # fixture is float32 text of 10 decimals
     assert np.abs(o.get("vel") - ref["vel1"]).max() < 5e-5
     assert np.abs(o.get("rho") - ref["rho1"]).max() < 5e-4
+
+
+def test_dummy_normals_point_into_the_fluid_on_flat_walls():
+    """calDummyParticleNormalKernel (WCSPHIntegrationKernel.cu:4-40):
+    n = normalize(-sum m/rho grad W) over the dummy neighbours.  On the inner
+    layer of a flat wall, away from edges, the sum is the wall's inward normal
+    by symmetry: +z on the floor, +x on the wall at x = min."""
+    spec = scenes.small_dam_break(nx=12, ny=16, nz=10)
+    o = oracle.OracleSPH(spec)
+    n = o.get("dnormal")
+    dp = spec.dummy_pos
+    dx = spec.dx
+    lo, hi = dp.min(axis=0), dp.max(axis=0)
+    margin = 6 * dx  # 3 wall layers plus the 2h = 2.6 dx reach of the sum
+
+    def interior(axes):
+        m = np.ones(len(dp), dtype=bool)
+        for a in axes:
+            m &= (dp[:, a] > lo[a] + margin) & (dp[:, a] < hi[a] - margin)
+        return m
+
+    fz0 = spec.fluid_pos[:, 2].min()
+    floor = interior((0, 1)) & (dp[:, 2] < fz0)
+    floor &= np.isclose(dp[:, 2], dp[floor, 2].max())
+    assert floor.sum() > 10
+    np.testing.assert_allclose(n[floor], np.tile([0.0, 0.0, 1.0], (floor.sum(), 1)), atol=1e-9)
+    fx0 = spec.fluid_pos[:, 0].min()
+    wall = interior((1, 2)) & (dp[:, 0] < fx0)
+    wall &= np.isclose(dp[:, 0], dp[wall, 0].max())
+    assert wall.sum() > 4
+    np.testing.assert_allclose(n[wall], np.tile([1.0, 0.0, 0.0], (wall.sum(), 1)), atol=1e-9)
+
+
+def test_tait_hydrostatic_column():
+    """Tait option of the EOS, p = B ((rho/rho0)^gamma - 1) with
+    B = rho0 c^2 / gamma: a density field built by inverting it for the
+    hydrostatic pressure rho0 g (H - z) reads back that pressure, and the
+    Riemann momentum balances gravity in the interior."""
+    dx, g, gamma, c0 = 0.01, 9.81, 7.0, 20.0
+    pos = _block(10, 10, 14, dx)
+    spec = scenes.SPHSceneSpec(fluid_pos=pos, dummy_pos=np.zeros((0, 3)), dx=dx, rho0=1000.0, c0=c0, nu=0.0,
+                               min_bound=np.array([-0.05] * 3), max_bound=np.array([0.2, 0.2, 0.3]),
+                               gravity=np.array([0, 0, -g]), dt=1e-5, eos=1, gamma=gamma)
+    o = oracle.OracleSPH(spec)
+    H = pos[:, 2].max() + dx
+    p = 1000.0 * g * (H - pos[:, 2])
+    B = 1000.0 * c0 ** 2 / gamma
+    o.set("drho", 1000.0 * (1.0 + p / B) ** (1.0 / gamma) - 1000.0)
+    np.testing.assert_allclose(o.get("p"), p, rtol=1e-9, atol=1e-9)
+    o.neighbor_search()
+    o.first_half(spec.dt)
+    interior = np.all((pos > 3 * dx - 1e-9) & (pos < np.array([6, 6, 10]) * dx + 1e-9), axis=1)
+    a = o.get("acc")[interior]
+    assert np.abs(a[:, 2]).max() < 0.05 * g
+    assert np.abs(a[:, :2]).max() < 0.05 * g

@@ -1,5 +1,5 @@
-// WCSPH scene for MI355X: cell-sorted SoA state, per-step hash/sort/reorder,
-// ELL neighbour lists and the three Riemann pair passes of one step.
+// WCSPH scene for MI355X: cell-sorted state, per-step hash/sort/reorder,
+// chunked ELL neighbour lists and the three Riemann pair passes of one step.
 //
 // Reference: DEMSPH/kernel/WCSPHSolver.h (step loop :607-621, neighbour
 // search :150-285), SPHNeighborSearchKernel.cu (lists), and
@@ -8,16 +8,21 @@
 //
 // Layout in HBM (slot order = lexicographic (cell hash, insertion index),
 // rebuilt every step; "orig" maps slot -> insertion index, "rank" the
-// inverse):
-//   position   3 x double[n]   (FP64: integration)
-//   velocity   3 x double[n]   (FP64: integration)
-//   drho, rho, p   float[n]    (density perturbation and its EOS images)
-//   phase      uint8[n]        (index into a table of h, m, c, nu, rho0)
-// Neighbour lists are ELL, column-major: entry k of slot i at k*n + i, so
-// the lanes of a wavefront read consecutive words.  Each slot's list holds
-// neighbour slots in the reference's exact walk order (z, y, x cells, then
-// ascending insertion index), so the pair set and its per-particle order are
-// identical to the reference's.
+// inverse).  Everything a neighbour contributes to a pair is packed into two
+// records so a pair costs three wide gathers:
+//   FluidPos  {x, y, z: f64; rho, p: f32}     32 B  (pair record 1)
+//   float4    {vx, vy, vz: f32; phase bits}  16 B  (pair record 2)
+//   velocity  3 x f64[n]                     own-slot only (integration)
+//   drho      f32[n]                         own-slot only
+// Dummies (sorted once): DummyPos {x, y, z: f64; nx, ny: f32} and float4
+// {vx, vy, vz, nz}.
+// Neighbour lists are ELL in 32-byte chunks: entries 8c..8c+7 of slot i sit
+// at int offset (c*n + i)*8.  The list kernel fills a chunk in registers and
+// stores it whole, so every store covers full 32-byte sectors; the lanes of
+// a wavefront read one chunk of consecutive slots as consecutive 16-byte
+// words.  Each slot's list holds neighbour slots in the reference's exact
+// walk order (z, y, x cells, then ascending insertion index), so the pair
+// set and its per-particle order are identical to the reference's.
 //
 // Pair arithmetic is FP32 (positions differenced in FP64 first); state
 // integration is FP64.
@@ -37,6 +42,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxPhases = 256;
+constexpr int kChunk = 8;  // list entries per 32-byte chunk
 
 struct Phase {
     double h;       // smoothing length, FP64 for the exact neighbour cut
@@ -50,23 +56,40 @@ struct Phase {
     float pad;
 };
 
+struct alignas(32) FluidPos {
+    double x, y, z;
+    float rho, p;
+};
+static_assert(sizeof(FluidPos) == 32, "FluidPos is one 32-byte record");
+
+struct alignas(32) DummyPos {
+    double x, y, z;
+    float nx, ny;
+};
+static_assert(sizeof(DummyPos) == 32, "DummyPos is one 32-byte record");
+
+__device__ __forceinline__ unsigned phase_bits(float w) { return __float_as_uint(w); }
+
 struct SphArgs {
     int nf, nd;
-    GridGeom grid;
     const Phase* phases;
+    Phase phf, phd;  // the single fluid / dummy phase of uniform scenes
     // fluid, current buffers
-    const double *px, *py, *pz, *vx, *vy, *vz;
-    const float *drho, *rho, *p;
-    const uint8_t* phase;
+    const FluidPos* P;
+    const float4* V;
+    const double *vx, *vy, *vz;
+    const float* drho;
     // fluid, destination buffers
-    double *opx, *opy, *opz, *ovx, *ovy, *ovz;
-    float *odrho, *orho, *op;
+    FluidPos* oP;
+    float4* oV;
+    double *ovx, *ovy, *ovz;
+    float* odrho;
     // lists
     const int *nbr_ff, *cnt_ff, *nbr_fd, *cnt_fd;
     int K_ff, K_fd;
     // dummies (sorted once)
-    const double *dpx, *dpy, *dpz, *dvx, *dvy, *dvz;
-    const float *dnx, *dny, *dnz;
+    const DummyPos* DP;
+    const float4* DV;
     const uint8_t* dphase;
     // physics
     float gx, gy, gz;
@@ -75,17 +98,21 @@ struct SphArgs {
     float gamma, inv_gamma;
 };
 
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // Wendland C2 gradient factor f with gradW = f * r_ij
 // (WCSPHIntegrationKernel.cuh:15-32).  The reference's
 // sigma/h * [-2 t^3 (1+2q) + 2 t^4] / r equals -5 sigma t^3 / h^2 with
-// t = 1 - q/2, q = r/h, sigma = 21 / (16 pi h^3).
+// t = 1 - q/2, q = r/h, sigma = 21 / (16 pi h^3).  With a uniform h every
+// factor but t is loop-invariant and hoisted by the compiler.
 __device__ __forceinline__ float grad_factor(float r, float h)
 {
     if (r < 1.e-20f || r >= 2.0f * h) return 0.0f;
     const float invh = 1.0f / h;
     const float t = 1.0f - 0.5f * r * invh;
     const float sigma = (float)(21.0 / (16.0 * M_PI)) * invh * invh * invh;
-    return -5.0f * sigma * t * t * t * invh * invh;
+    return -5.0f * sigma * invh * invh * (t * t * t);
 }
 
 // Inverse EOS for the dummy density (WCSPHIntegrationKernel.cu:132,249):
@@ -102,28 +129,40 @@ __device__ __forceinline__ float dummy_rho(const SphArgs& a, const Phase& pj, fl
 
 // EOS of SPHDensityPressureIntegrationKernel (WCSPHIntegrationKernel.cu:266-282)
 // with the Tait option.
-__device__ __forceinline__ void eos_eval(const SphArgs& a, const Phase& ph, float drho, float& rho, float& p)
+__device__ __forceinline__ void eos_eval(int eos, float gamma, const Phase& ph, float drho, float& rho, float& p)
 {
     float ch = drho;
     if (ch < -0.1f * ph.rho0) ch = -0.1f * ph.rho0;
     rho = ch + ph.rho0;
-    if (a.eos == 1) {
+    if (eos == 1) {
         const double ratio = (double)rho / (double)ph.rho0;
-        p = (float)((double)ph.taitB * (pow(ratio, (double)a.gamma) - 1.0));
+        p = (float)((double)ph.taitB * (pow(ratio, (double)gamma) - 1.0));
     } else {
         p = ph.c2 * ch;
     }
 }
 
 // ----------------------------------------------------------- hash & reorder
-__global__ __launch_bounds__(kThreads) void k_hash_by_orig(int n, GridGeom g, const double* __restrict__ px,
-                                                           const double* __restrict__ py,
-                                                           const double* __restrict__ pz,
-                                                           const int* __restrict__ orig, uint32_t* __restrict__ key)
+// Cell hash of every slot, scattered by insertion index for the stable sort.
+// When prev_skey is given (the sorted keys that produced the current cell
+// ranges), the first slot of each previously occupied cell also clears that
+// cell's range, which replaces a fill of the whole grid.
+__global__ __launch_bounds__(kThreads) void k_hash(int n, GridGeom g, long long numGrids,
+                                                   const FluidPos* __restrict__ P, const int* __restrict__ orig,
+                                                   uint32_t* __restrict__ key, const uint32_t* __restrict__ prev_skey,
+                                                   int* __restrict__ cs, int* __restrict__ ce)
 {
     const int s = blockIdx.x * kThreads + threadIdx.x;
     if (s >= n) return;
-    key[orig[s]] = (uint32_t)cell_hash(g, px[s], py[s], pz[s]);
+    if (prev_skey) {
+        const uint32_t h = prev_skey[s];
+        if ((long long)h < numGrids && (s == 0 || prev_skey[s - 1] != h)) {
+            cs[h] = -1;
+            ce[h] = -1;
+        }
+    }
+    const FluidPos p = P[s];
+    key[orig[s]] = (uint32_t)cell_hash(g, p.x, p.y, p.z);
 }
 
 struct ReorderArgs {
@@ -133,16 +172,15 @@ struct ReorderArgs {
     const int* perm;       // sorted insertion indices
     int* rank;             // insertion index -> slot (updated)
     int* orig_out;
-    const double *px, *py, *pz, *vx, *vy, *vz;
+    const FluidPos* P;
+    const float4* V;
+    const double *vx, *vy, *vz;
     const float* drho;
-    const uint8_t* phase;
-    double *opx, *opy, *opz, *ovx, *ovy, *ovz;
-    float *odrho, *orho, *op;
-    uint8_t* ophase;
+    FluidPos* oP;
+    float4* oV;
+    double *ovx, *ovy, *ovz;
+    float* odrho;
     int *cellStart, *cellEnd;
-    const Phase* phases;
-    int eos;
-    float gamma;
 };
 
 __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
@@ -153,21 +191,10 @@ __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
     const int s = a.rank[o];
     a.rank[o] = k;
     a.orig_out[k] = o;
-    a.opx[k] = a.px[s]; a.opy[k] = a.py[s]; a.opz[k] = a.pz[s];
+    a.oP[k] = a.P[s];
+    a.oV[k] = a.V[s];
     a.ovx[k] = a.vx[s]; a.ovy[k] = a.vy[s]; a.ovz[k] = a.vz[s];
-    const uint8_t ph = a.phase[s];
-    a.ophase[k] = ph;
-    if (a.odrho) {
-        const float d = a.drho[s];
-        a.odrho[k] = d;
-        SphArgs e;
-        e.eos = a.eos;
-        e.gamma = a.gamma;
-        float rho, p;
-        eos_eval(e, a.phases[ph], d, rho, p);
-        a.orho[k] = rho;
-        a.op[k] = p;
-    }
+    a.odrho[k] = a.drho[s];
     // findStartAndEnd (buildHashStartEnd.cu:14-34)
     const uint32_t h = a.skey[k];
     if ((long long)h < a.numGrids) {
@@ -177,138 +204,213 @@ __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
 }
 
 // ----------------------------------------------------------- neighbour lists
-// Walks the 27-cell stencil of one point over one sorted set and appends
-// every member within 2*max(hA,hB) to an ELL list (SPHNeighborSearchKernel.cu
-// :57-115 with excludeSelf, :169-228 without).
-__device__ __forceinline__ int walk_list(const GridGeom& g, double x, double y, double z, double hA, int self,
-                                         const int* __restrict__ cs, const int* __restrict__ ce,
-                                         const double* __restrict__ bx, const double* __restrict__ by,
-                                         const double* __restrict__ bz, const uint8_t* __restrict__ bph,
-                                         const Phase* __restrict__ phases, int* __restrict__ out, int stride, int K)
-{
-    const Stencil st = make_stencil(g, x, y, z);
-    int count = 0;
-    for (int zz = st.loz; zz <= st.hiz; zz++) {
-        for (int yy = st.loy; yy <= st.hiy; yy++) {
-            const int row = (st.cz + zz) * g.ny * g.nx + (st.cy + yy) * g.nx + st.cx;
-            for (int xx = st.lox; xx <= st.hix; xx++) {
-                const int hb = row + xx;
-                const int b0 = cs[hb];
-                if (b0 == -1) continue;
-                const int b1 = ce[hb];
-                for (int j = b0; j < b1; j++) {
-                    if (j == self) continue;
-                    const double cut = 2.0 * fmax(hA, phases[bph[j]].h);
-                    if (within_cut(cut, x - bx[j], y - by[j], z - bz[j])) {
-                        if (count < K) out[(size_t)count * stride] = j;
-                        count++;
-                    }
-                }
-            }
-        }
-    }
-    return count;
-}
-
 struct ListArgs {
     int nf, nd;
     GridGeom grid;
     const Phase* phases;
-    const double *px, *py, *pz;
-    const uint8_t* phase;
-    const int *cs, *ce;    // fluid cells
-    const double *dpx, *dpy, *dpz;
+    double hf, hd;  // uniform fluid / dummy smoothing lengths
+    const FluidPos* P;
+    const float4* V;
+    const int *cs, *ce;  // fluid cells
+    const DummyPos* DP;
     const uint8_t* dphase;
     const int *dcs, *dce;  // dummy cells
     int *nbr_ff, *cnt_ff, *nbr_fd, *cnt_fd;
-    int K_ff, K_fd;
-    int* maxcount;  // [0] fluid, [1] dummy
+    int K_ff, K_fd;  // multiples of kChunk
+    int* maxcount;   // [0] fluid, [1] dummy
 };
 
+// One 32-byte list chunk held in registers while it fills.
+struct Chunk {
+    int e0, e1, e2, e3, e4, e5, e6, e7;
+    __device__ __forceinline__ void put(int q, int j)
+    {
+        e0 = q == 0 ? j : e0; e1 = q == 1 ? j : e1; e2 = q == 2 ? j : e2; e3 = q == 3 ? j : e3;
+        e4 = q == 4 ? j : e4; e5 = q == 5 ? j : e5; e6 = q == 6 ? j : e6; e7 = q == 7 ? j : e7;
+    }
+    __device__ __forceinline__ void store(int* dst) const
+    {
+        reinterpret_cast<int4*>(dst)[0] = make_int4(e0, e1, e2, e3);
+        reinterpret_cast<int4*>(dst)[1] = make_int4(e4, e5, e6, e7);
+    }
+};
+
+// Walks the 27-cell stencil of one point over one sorted set and appends
+// every member within 2*max(hA,hB) to the point's chunked list
+// (SPHNeighborSearchKernel.cu:57-115 with excludeSelf, :169-228 without).
+// The three x-cells of a stencil row are consecutive hashes, so their
+// members form one contiguous run of slots, walked in ascending order.
+template <bool UNIF, bool DUMMY>
+__device__ __forceinline__ int walk_list(const ListArgs& a, double x, double y, double z, double hA, int self,
+                                         int* __restrict__ out, int K)
+{
+    const GridGeom& g = a.grid;
+    const int* __restrict__ cs = DUMMY ? a.dcs : a.cs;
+    const int* __restrict__ ce = DUMMY ? a.dce : a.ce;
+    const double cut_u = 2.0 * fmax(hA, DUMMY ? a.hd : a.hf);
+    const size_t cstride = (size_t)a.nf * kChunk;
+    const int nchunks = K / kChunk;
+    const Stencil st = make_stencil(g, x, y, z);
+    Chunk buf{0, 0, 0, 0, 0, 0, 0, 0};
+    int count = 0;
+    for (int zz = st.loz; zz <= st.hiz; zz++) {
+        for (int yy = st.loy; yy <= st.hiy; yy++) {
+            const int row = (st.cz + zz) * g.ny * g.nx + (st.cy + yy) * g.nx + st.cx;
+            int s = -1, e = -1;
+            for (int xx = st.lox; xx <= st.hix; xx++) {
+                const int c0 = cs[row + xx], c1 = ce[row + xx];
+                if (c0 != -1) {
+                    if (s < 0) s = c0;
+                    e = c1;
+                }
+            }
+            for (int j = s; j < e; j++) {
+                if (!DUMMY && j == self) continue;
+                double bx, by, bz, hB;
+                if (DUMMY) {
+                    const DummyPos q = a.DP[j];
+                    bx = q.x; by = q.y; bz = q.z;
+                    hB = UNIF ? a.hd : a.phases[a.dphase[j]].h;
+                } else {
+                    const FluidPos q = a.P[j];
+                    bx = q.x; by = q.y; bz = q.z;
+                    hB = UNIF ? a.hf : a.phases[phase_bits(a.V[j].w)].h;
+                }
+                const double cut = UNIF ? cut_u : 2.0 * fmax(hA, hB);
+                if (!within_cut(cut, x - bx, y - by, z - bz)) continue;
+                const int q = count & (kChunk - 1);
+                buf.put(q, j);
+                count++;
+                if (q == kChunk - 1) {
+                    const int c = (count >> 3) - 1;
+                    if (c < nchunks) buf.store(out + (size_t)c * cstride);
+                }
+            }
+        }
+    }
+    if ((count & (kChunk - 1)) && (count >> 3) < nchunks) buf.store(out + (size_t)(count >> 3) * cstride);
+    return count;
+}
+
+template <bool UNIF>
 __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
 {
     const int i = xcd_block() * kThreads + threadIdx.x;
     if (i >= a.nf) return;
-    const double x = a.px[i], y = a.py[i], z = a.pz[i];
-    const double h = a.phases[a.phase[i]].h;
-    const int c1 = walk_list(a.grid, x, y, z, h, i, a.cs, a.ce, a.px, a.py, a.pz, a.phase, a.phases, a.nbr_ff + i,
-                             a.nf, a.K_ff);
+    const FluidPos p = a.P[i];
+    const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
+    const int c1 = walk_list<UNIF, false>(a, p.x, p.y, p.z, h, i, a.nbr_ff + (size_t)i * kChunk, a.K_ff);
     a.cnt_ff[i] = c1;
     int c2 = 0;
-    if (a.nd > 0)
-        c2 = walk_list(a.grid, x, y, z, h, -1, a.dcs, a.dce, a.dpx, a.dpy, a.dpz, a.dphase, a.phases, a.nbr_fd + i,
-                       a.nf, a.K_fd);
+    if (a.nd > 0) c2 = walk_list<UNIF, true>(a, p.x, p.y, p.z, h, -1, a.nbr_fd + (size_t)i * kChunk, a.K_fd);
     a.cnt_fd[i] = c2;
     if (c1 > a.K_ff) atomicMax(&a.maxcount[0], c1);
     if (c2 > a.K_fd) atomicMax(&a.maxcount[1], c2);
 }
 
 // ----------------------------------------------------------- pair passes
+// Pair loops consume one list chunk at a time: the chunk's eight neighbour
+// records are gathered independently.  Entries past the list's end are
+// replaced by a valid slot with a zero kernel factor, which adds exact zeros
+// and keeps the loop free of branches.
+#ifndef DEMSPH_PAIR_BATCH
+#define DEMSPH_PAIR_BATCH 4
+#endif
+// Neighbour records gathered together per batch: 4 keeps the pair kernels
+// under 128 VGPRs; 8 doubles the gathers in flight per wavefront at lower
+// occupancy.
+constexpr int kBatch = DEMSPH_PAIR_BATCH;
+static_assert(kBatch == 4 || kBatch == 8, "pair batch is half or all of a chunk");
+
+template <class F>
+__device__ __forceinline__ void for_chunks(const int* __restrict__ list, int cnt, int K, int nf, int i, int safe,
+                                           F&& body)
+{
+    const int n = min(cnt, K);
+    const int* L = list + (size_t)i * kChunk;
+    const size_t cs = (size_t)nf * kChunk;
+    for (int base = 0; base < n; base += kBatch) {
+        const int* src = L + (size_t)(base / kChunk) * cs + (base % kChunk);
+        int js[kBatch];
+        const int4 lo = reinterpret_cast<const int4*>(src)[0];
+        js[0] = lo.x; js[1] = lo.y; js[2] = lo.z; js[3] = lo.w;
+        if (kBatch == 8) {
+            const int4 hi = reinterpret_cast<const int4*>(src)[1];
+            js[4 % kBatch] = hi.x; js[5 % kBatch] = hi.y; js[6 % kBatch] = hi.z; js[7 % kBatch] = hi.w;
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            const bool valid = base + q < n;
+            body(valid ? js[q] : safe, valid);
+        }
+    }
+}
+
 // updateWCSPHAccelerationKernel (WCSPHIntegrationKernel.cu:42-148) fused
 // with the half kick (:150-159) and, in the first half, the drift (:161-170).
-template <bool FIRST>
+template <bool FIRST, bool UNIF>
 __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
 {
     const int i = xcd_block() * kThreads + threadIdx.x;
     if (i >= a.nf) return;
-    const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
-    const float vix = (float)a.vx[i], viy = (float)a.vy[i], viz = (float)a.vz[i];
-    const Phase pi = a.phases[a.phase[i]];
-    const float rhoi = a.rho[i], PL = a.p[i];
+    const FluidPos Pi = a.P[i];
+    const float4 Vi = a.V[i];
+    const double xi = Pi.x, yi = Pi.y, zi = Pi.z;
+    const float vix = Vi.x, viy = Vi.y, viz = Vi.z;
+    const Phase pi = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
+    const float rhoi = Pi.rho, PL = Pi.p;
+    const float inv_rhoi = 1.0f / rhoi;
     float ax = 0.f, ay = 0.f, az = 0.f;
 
-    const int nff = min(a.cnt_ff[i], a.K_ff);
-    for (int k = 0; k < nff; k++) {
-        const int j = a.nbr_ff[(size_t)k * a.nf + i];
-        const float rx = (float)(xi - a.px[j]), ry = (float)(yi - a.py[j]), rz = (float)(zi - a.pz[j]);
-        const Phase pj = a.phases[a.phase[j]];
+    for_chunks(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, i, [&](int j, bool valid) {
+        const FluidPos Pj = a.P[j];
+        const float4 Vj = a.V[j];
+        const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
+        const float rx = (float)(xi - Pj.x), ry = (float)(yi - Pj.y), rz = (float)(zi - Pj.z);
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = sqrtf(r2);
-        const float f = grad_factor(r, 0.5f * (pi.hf + pj.hf));
-        if (f == 0.0f) continue;
-        const float vjx = (float)a.vx[j], vjy = (float)a.vy[j], vjz = (float)a.vz[j];
-        const float rhoj = a.rho[j], PR = a.p[j];
         // e = -r_ij/|r_ij| (zero below |r|^2 = 1e-20, myVec.h:112-128)
-        const float inv_r = (r2 < 1.e-20f) ? 0.0f : 1.0f / r;
+        const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+        const float r = r2 * inv_r;
+        const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
         const float UL = -(vix * rx + viy * ry + viz * rz) * inv_r;
-        const float UR = -(vjx * rx + vjy * ry + vjz * rz) * inv_r;
-        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
+        const float UR = -(Vj.x * rx + Vj.y * ry + Vj.z * rz) * inv_r;
+        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + Pj.rho);
         const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
-        const float Ps = 0.5f * (PL + PR) + 0.5f * beta * rbar * (UL - UR);
-        const float mr = pj.m / (rhoi * rhoj);
+        const float Ps = 0.5f * (PL + Pj.p) + 0.5f * beta * rbar * (UL - UR);
+        const float mr = pj.m * inv_rhoi * rcp(Pj.rho);
         const float kp = -2.0f * mr * Ps * f;
         // r_hat . gradW = f * r
-        const float kv = 2.0f * (0.5f * (pi.nu + pj.nu)) * (r2 < 1.e-20f ? 0.0f : f * r) * mr;
-        ax += kp * rx + kv * (vix - vjx);
-        ay += kp * ry + kv * (viy - vjy);
-        az += kp * rz + kv * (viz - vjz);
-    }
-    const int nfd = min(a.cnt_fd[i], a.K_fd);
-    for (int k = 0; k < nfd; k++) {
-        const int j = a.nbr_fd[(size_t)k * a.nf + i];
-        const float rx = (float)(xi - a.dpx[j]), ry = (float)(yi - a.dpy[j]), rz = (float)(zi - a.dpz[j]);
-        const Phase pj = a.phases[a.dphase[j]];
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = sqrtf(r2);
-        const float f = grad_factor(r, 0.5f * (pi.hf + pj.hf));
-        if (f == 0.0f) continue;
-        const float nx = a.dnx[j], ny = a.dny[j], nz = a.dnz[j];
-        const float vjx = (float)a.dvx[j], vjy = (float)a.dvy[j], vjz = (float)a.dvz[j];
-        const float UL = -(nx * vix + ny * viy + nz * viz);
-        const float UR = -UL - 2.0f * (nx * vjx + ny * vjy + nz * vjz);
-        float grj = -(a.gx * rx + a.gy * ry + a.gz * rz);
-        if (grj < 0.f) grj = 0.f;
-        const float PR = PL + rhoi * grj;
-        const float rhoj = dummy_rho(a, pj, PR);
-        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
-        const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
-        const float Ps = 0.5f * (PL + PR) + 0.5f * beta * rbar * (UL - UR);
-        const float mr = pj.m / (rhoi * rhoj);
-        const float kp = -2.0f * mr * Ps * f;
-        const float kv = 2.0f * (0.5f * (pi.nu + pj.nu)) * (r2 < 1.e-20f ? 0.0f : f * r) * mr;
-        ax += kp * rx + kv * (vix - vjx);
-        ay += kp * ry + kv * (viy - vjy);
-        az += kp * rz + kv * (viz - vjz);
+        const float kv = 2.0f * (0.5f * (pi.nu + pj.nu)) * (f * r) * mr;
+        ax += kp * rx + kv * (vix - Vj.x);
+        ay += kp * ry + kv * (viy - Vj.y);
+        az += kp * rz + kv * (viz - Vj.z);
+    });
+    if (a.nd > 0) {
+        for_chunks(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, 0, [&](int j, bool valid) {
+            const DummyPos Dj = a.DP[j];
+            const float4 DVj = a.DV[j];
+            const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
+            const float rx = (float)(xi - Dj.x), ry = (float)(yi - Dj.y), rz = (float)(zi - Dj.z);
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+            const float r = r2 * inv_r;
+            const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
+            const float nx = Dj.nx, ny = Dj.ny, nz = DVj.w;
+            const float UL = -(nx * vix + ny * viy + nz * viz);
+            const float UR = -UL - 2.0f * (nx * DVj.x + ny * DVj.y + nz * DVj.z);
+            const float grj = fmaxf(-(a.gx * rx + a.gy * ry + a.gz * rz), 0.0f);
+            const float PR = PL + rhoi * grj;
+            const float rhoj = dummy_rho(a, pj, PR);
+            const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
+            const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
+            const float Ps = 0.5f * (PL + PR) + 0.5f * beta * rbar * (UL - UR);
+            const float mr = pj.m * inv_rhoi * rcp(rhoj);
+            const float kp = -2.0f * mr * Ps * f;
+            const float kv = 2.0f * (0.5f * (pi.nu + pj.nu)) * (f * r) * mr;
+            ax += kp * rx + kv * (vix - DVj.x);
+            ay += kp * ry + kv * (viy - DVj.y);
+            az += kp * rz + kv * (viz - DVj.z);
+        });
     }
     ax += a.gx; ay += a.gy; az += a.gz;
     const double hdt = 0.5 * a.dt;
@@ -316,68 +418,72 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
     const double nvy = a.vy[i] + hdt * (double)ay;
     const double nvz = a.vz[i] + hdt * (double)az;
     a.ovx[i] = nvx; a.ovy[i] = nvy; a.ovz[i] = nvz;
+    a.oV[i] = make_float4((float)nvx, (float)nvy, (float)nvz, Vi.w);
     if (FIRST) {
-        a.opx[i] = xi + a.dt * nvx;
-        a.opy[i] = yi + a.dt * nvy;
-        a.opz[i] = zi + a.dt * nvz;
+        FluidPos o;
+        o.x = xi + a.dt * nvx;
+        o.y = yi + a.dt * nvy;
+        o.z = zi + a.dt * nvz;
+        o.rho = rhoi;
+        o.p = PL;
+        a.oP[i] = o;
     }
 }
 
 // updateWCSPHDensityChangeKernel (WCSPHIntegrationKernel.cu:172-264) fused
 // with the EOS update (:266-282).  With gradW = f r_ij and e = -r_ij/r:
 // (v_i - v*) . gradW = f [ (v_i - v_j) . r_ij / 2 + (P_L - P_R) r / (2 rho_bar c_bar) ].
+template <bool UNIF>
 __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
 {
     const int i = xcd_block() * kThreads + threadIdx.x;
     if (i >= a.nf) return;
-    const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
-    const float vix = (float)a.vx[i], viy = (float)a.vy[i], viz = (float)a.vz[i];
-    const Phase pi = a.phases[a.phase[i]];
-    const float rhoi = a.rho[i], PL = a.p[i];
+    const FluidPos Pi = a.P[i];
+    const float4 Vi = a.V[i];
+    const double xi = Pi.x, yi = Pi.y, zi = Pi.z;
+    const float vix = Vi.x, viy = Vi.y, viz = Vi.z;
+    const Phase pi = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
+    const float rhoi = Pi.rho, PL = Pi.p;
     float d = 0.f;
-    const int nff = min(a.cnt_ff[i], a.K_ff);
-    for (int k = 0; k < nff; k++) {
-        const int j = a.nbr_ff[(size_t)k * a.nf + i];
-        const float rx = (float)(xi - a.px[j]), ry = (float)(yi - a.py[j]), rz = (float)(zi - a.pz[j]);
-        const Phase pj = a.phases[a.phase[j]];
+    for_chunks(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, i, [&](int j, bool valid) {
+        const FluidPos Pj = a.P[j];
+        const float4 Vj = a.V[j];
+        const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
+        const float rx = (float)(xi - Pj.x), ry = (float)(yi - Pj.y), rz = (float)(zi - Pj.z);
         const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = sqrtf(r2);
-        const float f = grad_factor(r, 0.5f * (pi.hf + pj.hf));
-        if (f == 0.0f) continue;
-        const float vjx = (float)a.vx[j], vjy = (float)a.vy[j], vjz = (float)a.vz[j];
-        const float rhoj = a.rho[j], PR = a.p[j];
-        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
-        const float er = (r2 < 1.e-20f) ? 0.0f : r;
-        const float t = 0.5f * ((vix - vjx) * rx + (viy - vjy) * ry + (viz - vjz) * rz) +
-                        0.5f * (PL - PR) * er / (rbar * cbar);
-        d += pj.m / rhoj * f * t;
-    }
-    const int nfd = min(a.cnt_fd[i], a.K_fd);
-    for (int k = 0; k < nfd; k++) {
-        const int j = a.nbr_fd[(size_t)k * a.nf + i];
-        const float rx = (float)(xi - a.dpx[j]), ry = (float)(yi - a.dpy[j]), rz = (float)(zi - a.dpz[j]);
-        const Phase pj = a.phases[a.dphase[j]];
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float r = sqrtf(r2);
-        const float f = grad_factor(r, 0.5f * (pi.hf + pj.hf));
-        if (f == 0.0f) continue;
-        const float vjx = (float)a.dvx[j], vjy = (float)a.dvy[j], vjz = (float)a.dvz[j];
-        float grj = -(a.gx * rx + a.gy * ry + a.gz * rz);
-        if (grj < 0.f) grj = 0.f;
-        const float PR = PL + rhoi * grj;
-        const float rhoj = dummy_rho(a, pj, PR);
-        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
-        const float er = (r2 < 1.e-20f) ? 0.0f : r;
-        const float t = 0.5f * ((vix - vjx) * rx + (viy - vjy) * ry + (viz - vjz) * rz) +
-                        0.5f * (PL - PR) * er / (rbar * cbar);
-        d += pj.m / rhoj * f * t;
+        const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+        const float r = r2 * inv_r;
+        const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
+        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + Pj.rho);
+        const float t = 0.5f * ((vix - Vj.x) * rx + (viy - Vj.y) * ry + (viz - Vj.z) * rz) +
+                        0.5f * (PL - Pj.p) * r * rcp(rbar * cbar);
+        d += pj.m * rcp(Pj.rho) * f * t;
+    });
+    if (a.nd > 0) {
+        for_chunks(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, 0, [&](int j, bool valid) {
+            const DummyPos Dj = a.DP[j];
+            const float4 DVj = a.DV[j];
+            const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
+            const float rx = (float)(xi - Dj.x), ry = (float)(yi - Dj.y), rz = (float)(zi - Dj.z);
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+            const float r = r2 * inv_r;
+            const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
+            const float grj = fmaxf(-(a.gx * rx + a.gy * ry + a.gz * rz), 0.0f);
+            const float PR = PL + rhoi * grj;
+            const float rhoj = dummy_rho(a, pj, PR);
+            const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
+            const float t = 0.5f * ((vix - DVj.x) * rx + (viy - DVj.y) * ry + (viz - DVj.z) * rz) +
+                            0.5f * (PL - PR) * r * rcp(rbar * cbar);
+            d += pj.m * rcp(rhoj) * f * t;
+        });
     }
     const float nd = a.drho[i] + (float)(a.dt * (double)(2.0f * rhoi * d));
     a.odrho[i] = nd;
-    float rho, p;
-    eos_eval(a, pi, nd, rho, p);
-    a.orho[i] = rho;
-    a.op[i] = p;
+    FluidPos o;
+    o.x = xi; o.y = yi; o.z = zi;
+    eos_eval(a.eos, a.gamma, pi, nd, o.rho, o.p);
+    a.oP[i] = o;
 }
 
 // ----------------------------------------------------------- dummy normals
@@ -387,10 +493,10 @@ struct NormalArgs {
     int nd;
     GridGeom grid;
     const Phase* phases;
-    const double *dpx, *dpy, *dpz;
+    DummyPos* DP;
+    float4* DV;
     const uint8_t* dphase;
     const int *dcs, *dce;
-    float *dnx, *dny, *dnz;
 };
 
 __device__ __forceinline__ void grad_w64(double rx, double ry, double rz, double h, double& gx, double& gy, double& gz)
@@ -410,7 +516,7 @@ __global__ __launch_bounds__(kThreads) void k_dummy_normals(NormalArgs a)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= a.nd) return;
-    const double x = a.dpx[i], y = a.dpy[i], z = a.dpz[i];
+    const double x = a.DP[i].x, y = a.DP[i].y, z = a.DP[i].z;
     const Phase pi = a.phases[a.dphase[i]];
     const Stencil st = make_stencil(a.grid, x, y, z);
     double tx = 0, ty = 0, tz = 0;
@@ -425,7 +531,7 @@ __global__ __launch_bounds__(kThreads) void k_dummy_normals(NormalArgs a)
                     if (j == i) continue;
                     const Phase pj = a.phases[a.dphase[j]];
                     const double cut = 2.0 * fmax(pi.h, pj.h);
-                    const double rx = x - a.dpx[j], ry = y - a.dpy[j], rz = z - a.dpz[j];
+                    const double rx = x - a.DP[j].x, ry = y - a.DP[j].y, rz = z - a.DP[j].z;
                     if (!within_cut(cut, rx, ry, rz)) continue;
                     double gx, gy, gz;
                     grad_w64(rx, ry, rz, 0.5 * (pi.h + pj.h), gx, gy, gz);
@@ -441,7 +547,11 @@ __global__ __launch_bounds__(kThreads) void k_dummy_normals(NormalArgs a)
         const double inv = 1.0 / sqrt(l2);
         nx = (float)(tx * inv); ny = (float)(ty * inv); nz = (float)(tz * inv);
     }
-    a.dnx[i] = nx; a.dny[i] = ny; a.dnz[i] = nz;
+    // Only this thread touches slot i's normal words; other threads read
+    // the position words of the same records.
+    a.DP[i].nx = nx;
+    a.DP[i].ny = ny;
+    a.DV[i].w = nz;
 }
 
 // Dummies: hash by insertion index, then gather into sorted order once.
@@ -452,7 +562,8 @@ struct DummyReorderArgs {
     const int* perm;
     const double *px, *py, *pz, *vx, *vy, *vz;
     const uint8_t* phase;
-    double *opx, *opy, *opz, *ovx, *ovy, *ovz;
+    DummyPos* DP;
+    float4* DV;
     uint8_t* ophase;
     int* orig;
     int *cellStart, *cellEnd;
@@ -464,8 +575,11 @@ __global__ __launch_bounds__(kThreads) void k_dummy_reorder(DummyReorderArgs a)
     if (k >= a.n) return;
     const int o = a.perm[k];
     a.orig[k] = o;
-    a.opx[k] = a.px[o]; a.opy[k] = a.py[o]; a.opz[k] = a.pz[o];
-    a.ovx[k] = a.vx[o]; a.ovy[k] = a.vy[o]; a.ovz[k] = a.vz[o];
+    DummyPos d;
+    d.x = a.px[o]; d.y = a.py[o]; d.z = a.pz[o];
+    d.nx = 0.f; d.ny = 0.f;
+    a.DP[k] = d;
+    a.DV[k] = make_float4((float)a.vx[o], (float)a.vy[o], (float)a.vz[o], 0.f);
     a.ophase[k] = a.phase[o];
     const uint32_t h = a.skey[k];
     if ((long long)h < a.numGrids) {
@@ -474,21 +588,47 @@ __global__ __launch_bounds__(kThreads) void k_dummy_reorder(DummyReorderArgs a)
     }
 }
 
+__global__ void k_dummy_hash(int n, GridGeom g, const double* __restrict__ px, const double* __restrict__ py,
+                             const double* __restrict__ pz, uint32_t* __restrict__ key)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    key[s] = (uint32_t)cell_hash(g, px[s], py[s], pz[s]);
+}
+
 __global__ void k_iota(int* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = i;
 }
 
-__global__ void k_eos_all(int n, const Phase* phases, const uint8_t* phase, const float* drho, float* rho, float* p,
-                          int eos, float gamma)
+// Fluid records from insertion-ordered host uploads (slot == index).
+__global__ void k_fluid_init(int n, const double* __restrict__ pos, const double* __restrict__ vel,
+                             const uint8_t* __restrict__ phase, FluidPos* P, float4* V, double* vx, double* vy,
+                             double* vz, float* drho)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    SphArgs e;
-    e.eos = eos;
-    e.gamma = gamma;
-    eos_eval(e, phases[phase[i]], drho[i], rho[i], p[i]);
+    FluidPos p;
+    p.x = pos[3 * (size_t)i]; p.y = pos[3 * (size_t)i + 1]; p.z = pos[3 * (size_t)i + 2];
+    p.rho = 0.f; p.p = 0.f;
+    P[i] = p;
+    const double ux = vel[3 * (size_t)i], uy = vel[3 * (size_t)i + 1], uz = vel[3 * (size_t)i + 2];
+    vx[i] = ux; vy[i] = uy; vz[i] = uz;
+    V[i] = make_float4((float)ux, (float)uy, (float)uz, __uint_as_float((unsigned)phase[i]));
+    drho[i] = 0.f;
+}
+
+// rho, p of every slot from its drho (after initialisation or a host write).
+__global__ void k_eos_all(int n, const Phase* phases, const float4* V, const float* drho, FluidPos* P, int eos,
+                          float gamma)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float rho, p;
+    eos_eval(eos, gamma, phases[phase_bits(V[i].w)], drho[i], rho, p);
+    P[i].rho = rho;
+    P[i].p = p;
 }
 
 }  // namespace
@@ -534,35 +674,39 @@ struct demsph_sph {
     std::vector<Phase> phases;
     std::vector<uint8_t> fphase_h, dphase_h;
     Phase* d_phases = nullptr;
+    bool uniform = false;  // one fluid phase and at most one dummy phase
+    Phase phf{}, phd{};
 
-    // fluid state, double-buffered per field group
-    double *px[2] = {}, *py[2] = {}, *pz[2] = {};
+    // fluid state, double-buffered per record group
+    FluidPos* P[2] = {};
+    float4* V[2] = {};
     double *vx[2] = {}, *vy[2] = {}, *vz[2] = {};
-    float *drho[2] = {}, *rho[2] = {}, *p[2] = {};
-    uint8_t* phase[2] = {};
+    float* drho[2] = {};
     int* orig[2] = {};
-    int cx = 0, cv = 0, cd = 0, cp = 0;  // current buffer of position, velocity, density, phase/orig
+    int cP = 0, cV = 0, cD = 0, cO = 0;  // current buffer of P, V (with FP64 velocity), drho, orig
     int* rank = nullptr;
     // sort
     uint32_t *key = nullptr, *skey = nullptr, *ktmp = nullptr;
     int *perm = nullptr, *vtmp = nullptr;
     void* sort_temp = nullptr;
     int *cellStart = nullptr, *cellEnd = nullptr;
+    bool cells_track_skey = false;  // cell ranges were last written from the fluid keys in skey
     // lists
     int *nbr_ff = nullptr, *cnt_ff = nullptr, *nbr_fd = nullptr, *cnt_fd = nullptr;
     int* maxcount = nullptr;      // device [2]
     int* maxcount_h = nullptr;    // pinned [2]
     bool lists_valid = false;
     // dummies
-    double *dpx = nullptr, *dpy = nullptr, *dpz = nullptr, *dvx = nullptr, *dvy = nullptr, *dvz = nullptr;
-    float *dnx = nullptr, *dny = nullptr, *dnz = nullptr;
+    DummyPos* DP = nullptr;
+    float4* DV = nullptr;
     uint8_t* dphase = nullptr;
     int* dorig = nullptr;
     int *dcellStart = nullptr, *dcellEnd = nullptr;
     // checkpoint for replay on list overflow
-    double *ck_px = nullptr, *ck_py = nullptr, *ck_pz = nullptr, *ck_vx = nullptr, *ck_vy = nullptr, *ck_vz = nullptr;
+    FluidPos* ck_P = nullptr;
+    float4* ck_V = nullptr;
+    double *ck_vx = nullptr, *ck_vy = nullptr, *ck_vz = nullptr;
     float* ck_drho = nullptr;
-    uint8_t* ck_phase = nullptr;
     int *ck_orig = nullptr, *ck_rank = nullptr;
     KernelTimer timer;
 
@@ -572,8 +716,7 @@ struct demsph_sph {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         for (int b = 0; b < 2; b++) {
-            dfree(px[b]); dfree(py[b]); dfree(pz[b]); dfree(vx[b]); dfree(vy[b]); dfree(vz[b]);
-            dfree(drho[b]); dfree(rho[b]); dfree(p[b]); dfree(phase[b]); dfree(orig[b]);
+            dfree(P[b]); dfree(V[b]); dfree(vx[b]); dfree(vy[b]); dfree(vz[b]); dfree(drho[b]); dfree(orig[b]);
         }
         dfree(rank); dfree(key); dfree(skey); dfree(ktmp); dfree(perm); dfree(vtmp);
         if (sort_temp) (void)hipFree(sort_temp);
@@ -582,10 +725,9 @@ struct demsph_sph {
         dfree(maxcount);
         if (maxcount_h) (void)hipHostFree(maxcount_h);
         maxcount_h = nullptr;
-        dfree(dpx); dfree(dpy); dfree(dpz); dfree(dvx); dfree(dvy); dfree(dvz); dfree(dnx); dfree(dny); dfree(dnz);
-        dfree(dphase); dfree(dorig); dfree(dcellStart); dfree(dcellEnd); dfree(d_phases);
-        dfree(ck_px); dfree(ck_py); dfree(ck_pz); dfree(ck_vx); dfree(ck_vy); dfree(ck_vz); dfree(ck_drho);
-        dfree(ck_phase); dfree(ck_orig); dfree(ck_rank);
+        dfree(DP); dfree(DV); dfree(dphase); dfree(dorig); dfree(dcellStart); dfree(dcellEnd); dfree(d_phases);
+        dfree(ck_P); dfree(ck_V); dfree(ck_vx); dfree(ck_vy); dfree(ck_vz); dfree(ck_drho); dfree(ck_orig);
+        dfree(ck_rank);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -616,18 +758,13 @@ struct demsph_sph {
     SphArgs args() const
     {
         SphArgs a{};
-        a.nf = nf; a.nd = nd; a.grid = grid; a.phases = d_phases;
-        a.px = px[cx]; a.py = py[cx]; a.pz = pz[cx];
-        a.vx = vx[cv]; a.vy = vy[cv]; a.vz = vz[cv];
-        a.drho = drho[cd]; a.rho = rho[cd]; a.p = p[cd];
-        a.phase = phase[cp];
-        a.opx = px[1 - cx]; a.opy = py[1 - cx]; a.opz = pz[1 - cx];
-        a.ovx = vx[1 - cv]; a.ovy = vy[1 - cv]; a.ovz = vz[1 - cv];
-        a.odrho = drho[1 - cd]; a.orho = rho[1 - cd]; a.op = p[1 - cd];
+        a.nf = nf; a.nd = nd; a.phases = d_phases; a.phf = phf; a.phd = phd;
+        a.P = P[cP]; a.V = V[cV]; a.vx = vx[cV]; a.vy = vy[cV]; a.vz = vz[cV]; a.drho = drho[cD];
+        a.oP = P[1 - cP]; a.oV = V[1 - cV]; a.ovx = vx[1 - cV]; a.ovy = vy[1 - cV]; a.ovz = vz[1 - cV];
+        a.odrho = drho[1 - cD];
         a.nbr_ff = nbr_ff; a.cnt_ff = cnt_ff; a.nbr_fd = nbr_fd; a.cnt_fd = cnt_fd;
         a.K_ff = K_ff; a.K_fd = K_fd;
-        a.dpx = dpx; a.dpy = dpy; a.dpz = dpz; a.dvx = dvx; a.dvy = dvy; a.dvz = dvz;
-        a.dnx = dnx; a.dny = dny; a.dnz = dnz; a.dphase = dphase;
+        a.DP = DP; a.DV = DV; a.dphase = dphase;
         a.gx = (float)g[0]; a.gy = (float)g[1]; a.gz = (float)g[2];
         a.eos = eos;
         a.gamma = (float)gamma;
@@ -635,9 +772,13 @@ struct demsph_sph {
         return a;
     }
 
+    static int round_chunk(int k) { return std::max(kChunk, (k + kChunk - 1) / kChunk * kChunk); }
+
     void alloc_lists()
     {
         dfree(nbr_ff); dfree(nbr_fd);
+        K_ff = round_chunk(K_ff);
+        K_fd = round_chunk(K_fd);
         nbr_ff = dmalloc<int>((size_t)K_ff * (nf > 0 ? nf : 1));
         nbr_fd = dmalloc<int>((size_t)K_fd * (nf > 0 ? nf : 1));
         lists_valid = false;
@@ -667,7 +808,6 @@ struct demsph_sph {
         phases.clear();
         fphase_h.resize(nf);
         dphase_h.resize(nd);
-        std::vector<double> fm(nf), dm(nd);
         for (int i = 0; i < nf; i++) {
             const double dx = hf.spacing[i];
             fphase_h[i] = phase_of(1.3 * dx, dx * dx * dx * hf.rho0[i], hf.c[i], hf.nu[i], hf.rho0[i]);
@@ -676,15 +816,20 @@ struct demsph_sph {
             const double dx = hd.spacing[i];
             dphase_h[i] = phase_of(1.3 * dx, dx * dx * dx * hd.rho0[i], hd.c[i], hd.nu[i], hd.rho0[i]);
         }
+        bool fu = true, du = true;
+        for (int i = 1; i < nf; i++) fu = fu && fphase_h[i] == fphase_h[0];
+        for (int i = 1; i < nd; i++) du = du && dphase_h[i] == dphase_h[0];
+        uniform = fu && du;
+        phf = phases[fphase_h[0]];
+        phd = nd > 0 ? phases[dphase_h[0]] : phf;
         d_phases = dmalloc<Phase>(phases.size());
         DEMSPH_HIP(hipMemcpyAsync(d_phases, phases.data(), sizeof(Phase) * phases.size(), hipMemcpyHostToDevice, stream));
 
         const size_t n = nf;
         for (int b = 0; b < 2; b++) {
-            px[b] = dmalloc<double>(n); py[b] = dmalloc<double>(n); pz[b] = dmalloc<double>(n);
+            P[b] = dmalloc<FluidPos>(n); V[b] = dmalloc<float4>(n);
             vx[b] = dmalloc<double>(n); vy[b] = dmalloc<double>(n); vz[b] = dmalloc<double>(n);
-            drho[b] = dmalloc<float>(n); rho[b] = dmalloc<float>(n); p[b] = dmalloc<float>(n);
-            phase[b] = dmalloc<uint8_t>(n); orig[b] = dmalloc<int>(n);
+            drho[b] = dmalloc<float>(n); orig[b] = dmalloc<int>(n);
         }
         rank = dmalloc<int>(n);
         const int nsort = std::max(nf, nd);
@@ -696,63 +841,71 @@ struct demsph_sph {
         maxcount = dmalloc<int>(2);
         DEMSPH_HIP(hipHostMalloc((void**)&maxcount_h, 2 * sizeof(int), hipHostMallocDefault));
         alloc_lists();
-        cx = cv = cd = cp = 0;
+        cP = cV = cD = cO = 0;
 
         // upload fluid in insertion order (slot == index)
-        std::vector<double> tx(n), ty(n), tz(n);
-        auto up3 = [&](const std::vector<double>& src, double* dx_, double* dy_, double* dz_, size_t cnt) {
-            tx.resize(cnt); ty.resize(cnt); tz.resize(cnt);
-            for (size_t i = 0; i < cnt; i++) { tx[i] = src[3 * i]; ty[i] = src[3 * i + 1]; tz[i] = src[3 * i + 2]; }
-            DEMSPH_HIP(hipMemcpy(dx_, tx.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(dy_, ty.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(dz_, tz.data(), cnt * sizeof(double), hipMemcpyHostToDevice));
-        };
         DEMSPH_HIP(hipStreamSynchronize(stream));
-        up3(hf.pos, px[0], py[0], pz[0], n);
-        up3(hf.vel, vx[0], vy[0], vz[0], n);
-        DEMSPH_HIP(hipMemcpy(phase[0], fphase_h.data(), n, hipMemcpyHostToDevice));
-        DEMSPH_HIP(hipMemsetAsync(drho[0], 0, n * sizeof(float), stream));
-        hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, stream, orig[0], (int)n);
-        hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, stream, rank, (int)n);
-        hipLaunchKernelGGL(k_eos_all, dim3(blocks_for(n, 256)), dim3(256), 0, stream, (int)n, d_phases, phase[0],
-                           drho[0], rho[0], p[0], eos, (float)gamma);
-        DEMSPH_HIP(hipGetLastError());
+        {
+            double* upos = dmalloc<double>(3 * n);
+            double* uvel = dmalloc<double>(3 * n);
+            uint8_t* uph = dmalloc<uint8_t>(n);
+            DEMSPH_HIP(hipMemcpy(upos, hf.pos.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(uvel, hf.vel.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(uph, fphase_h.data(), n, hipMemcpyHostToDevice));
+            const dim3 gb(blocks_for(n, 256)), tb(256);
+            hipLaunchKernelGGL(k_fluid_init, gb, tb, 0, stream, (int)n, upos, uvel, uph, P[0], V[0], vx[0], vy[0],
+                               vz[0], drho[0]);
+            hipLaunchKernelGGL(k_iota, gb, tb, 0, stream, orig[0], (int)n);
+            hipLaunchKernelGGL(k_iota, gb, tb, 0, stream, rank, (int)n);
+            hipLaunchKernelGGL(k_eos_all, gb, tb, 0, stream, (int)n, d_phases, V[0], drho[0], P[0], eos,
+                               (float)gamma);
+            DEMSPH_HIP(hipGetLastError());
+            DEMSPH_HIP(hipStreamSynchronize(stream));
+            dfree(upos); dfree(uvel); dfree(uph);
+        }
 
         // dummies: sort once (they are static), cell ranges, normals
         dcellStart = dmalloc<int>(numGrids); dcellEnd = dmalloc<int>(numGrids);
         fill_i32(dcellStart, numGrids, -1, stream);
         fill_i32(dcellEnd, numGrids, -1, stream);
         const size_t m = nd;
-        dpx = dmalloc<double>(m); dpy = dmalloc<double>(m); dpz = dmalloc<double>(m);
-        dvx = dmalloc<double>(m); dvy = dmalloc<double>(m); dvz = dmalloc<double>(m);
-        dnx = dmalloc<float>(m); dny = dmalloc<float>(m); dnz = dmalloc<float>(m);
+        DP = dmalloc<DummyPos>(m); DV = dmalloc<float4>(m);
         dphase = dmalloc<uint8_t>(m); dorig = dmalloc<int>(m);
         if (nd > 0) {
+            std::vector<double> tx(m), ty(m), tz(m);
             double *ux = dmalloc<double>(m), *uy = dmalloc<double>(m), *uz = dmalloc<double>(m);
             double *wx = dmalloc<double>(m), *wy = dmalloc<double>(m), *wz = dmalloc<double>(m);
             uint8_t* uph = dmalloc<uint8_t>(m);
-            int* uorig = dmalloc<int>(m);
-            up3(hd.pos, ux, uy, uz, m);
-            up3(hd.vel, wx, wy, wz, m);
+            auto up3 = [&](const std::vector<double>& src, double* dx_, double* dy_, double* dz_) {
+                for (size_t i = 0; i < m; i++) { tx[i] = src[3 * i]; ty[i] = src[3 * i + 1]; tz[i] = src[3 * i + 2]; }
+                DEMSPH_HIP(hipMemcpy(dx_, tx.data(), m * sizeof(double), hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpy(dy_, ty.data(), m * sizeof(double), hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpy(dz_, tz.data(), m * sizeof(double), hipMemcpyHostToDevice));
+            };
+            up3(hd.pos, ux, uy, uz);
+            up3(hd.vel, wx, wy, wz);
             DEMSPH_HIP(hipMemcpy(uph, dphase_h.data(), m, hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_iota, dim3(blocks_for(m, 256)), dim3(256), 0, stream, uorig, (int)m);
-            hipLaunchKernelGGL(k_hash_by_orig, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, (int)m, grid,
-                               ux, uy, uz, uorig, key);
+            hipLaunchKernelGGL(k_dummy_hash, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, (int)m, grid,
+                               ux, uy, uz, key);
             radix_sort_pairs(key, nullptr, skey, perm, ktmp, vtmp, sort_temp, (int)m, key_bits(numGrids), stream);
             DummyReorderArgs ra{};
             ra.n = (int)m; ra.numGrids = numGrids; ra.skey = skey; ra.perm = perm;
             ra.px = ux; ra.py = uy; ra.pz = uz; ra.vx = wx; ra.vy = wy; ra.vz = wz; ra.phase = uph;
-            ra.opx = dpx; ra.opy = dpy; ra.opz = dpz; ra.ovx = dvx; ra.ovy = dvy; ra.ovz = dvz; ra.ophase = dphase;
+            ra.DP = DP; ra.DV = DV; ra.ophase = dphase;
             ra.orig = dorig; ra.cellStart = dcellStart; ra.cellEnd = dcellEnd;
             hipLaunchKernelGGL(k_dummy_reorder, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, ra);
             NormalArgs na{};
-            na.nd = (int)m; na.grid = grid; na.phases = d_phases; na.dpx = dpx; na.dpy = dpy; na.dpz = dpz;
-            na.dphase = dphase; na.dcs = dcellStart; na.dce = dcellEnd; na.dnx = dnx; na.dny = dny; na.dnz = dnz;
+            na.nd = (int)m; na.grid = grid; na.phases = d_phases; na.DP = DP; na.DV = DV;
+            na.dphase = dphase; na.dcs = dcellStart; na.dce = dcellEnd;
             hipLaunchKernelGGL(k_dummy_normals, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, na);
             DEMSPH_HIP(hipGetLastError());
             DEMSPH_HIP(hipStreamSynchronize(stream));
-            dfree(ux); dfree(uy); dfree(uz); dfree(wx); dfree(wy); dfree(wz); dfree(uph); dfree(uorig);
+            dfree(ux); dfree(uy); dfree(uz); dfree(wx); dfree(wy); dfree(wz); dfree(uph);
         }
+        // the fluid cell ranges start empty; skey now holds dummy keys
+        fill_i32(cellStart, numGrids, -1, stream);
+        fill_i32(cellEnd, numGrids, -1, stream);
+        cells_track_skey = false;
         DEMSPH_HIP(hipStreamSynchronize(stream));
         initialized = true;
         lists_valid = false;
@@ -782,32 +935,34 @@ struct demsph_sph {
     {
         const int n = nf;
         const dim3 gb(blocks_for(n, kThreads)), tb(kThreads);
-        DEMSPH_LAUNCH(timer, "sph_hash", k_hash_by_orig, gb, tb, 0, stream, n, grid, px[cx], py[cx], pz[cx],
-                      orig[cp], key);
+        if (!cells_track_skey) {
+            timer.begin("sph_fill_cells", stream);
+            fill_i32(cellStart, numGrids, -1, stream);
+            fill_i32(cellEnd, numGrids, -1, stream);
+            timer.end(stream);
+        }
+        DEMSPH_LAUNCH(timer, "sph_hash", k_hash, gb, tb, 0, stream, n, grid, numGrids, P[cP], orig[cO], key,
+                      cells_track_skey ? skey : nullptr, cellStart, cellEnd);
         timer.begin("sph_sort", stream);
         radix_sort_pairs(key, nullptr, skey, perm, ktmp, vtmp, sort_temp, n, key_bits(numGrids), stream);
         timer.end(stream);
-        timer.begin("sph_fill_cells", stream);
-        fill_i32(cellStart, numGrids, -1, stream);
-        fill_i32(cellEnd, numGrids, -1, stream);
-        timer.end(stream);
         ReorderArgs ra{};
-        ra.n = n; ra.numGrids = numGrids; ra.skey = skey; ra.perm = perm; ra.rank = rank; ra.orig_out = orig[1 - cp];
-        ra.px = px[cx]; ra.py = py[cx]; ra.pz = pz[cx]; ra.vx = vx[cv]; ra.vy = vy[cv]; ra.vz = vz[cv];
-        ra.drho = drho[cd]; ra.phase = phase[cp];
-        ra.opx = px[1 - cx]; ra.opy = py[1 - cx]; ra.opz = pz[1 - cx];
-        ra.ovx = vx[1 - cv]; ra.ovy = vy[1 - cv]; ra.ovz = vz[1 - cv];
-        ra.odrho = drho[1 - cd]; ra.orho = rho[1 - cd]; ra.op = p[1 - cd]; ra.ophase = phase[1 - cp];
-        ra.cellStart = cellStart; ra.cellEnd = cellEnd; ra.phases = d_phases; ra.eos = eos; ra.gamma = (float)gamma;
+        ra.n = n; ra.numGrids = numGrids; ra.skey = skey; ra.perm = perm; ra.rank = rank; ra.orig_out = orig[1 - cO];
+        ra.P = P[cP]; ra.V = V[cV]; ra.vx = vx[cV]; ra.vy = vy[cV]; ra.vz = vz[cV]; ra.drho = drho[cD];
+        ra.oP = P[1 - cP]; ra.oV = V[1 - cV]; ra.ovx = vx[1 - cV]; ra.ovy = vy[1 - cV]; ra.ovz = vz[1 - cV];
+        ra.odrho = drho[1 - cD];
+        ra.cellStart = cellStart; ra.cellEnd = cellEnd;
         DEMSPH_LAUNCH(timer, "sph_reorder", k_reorder, gb, tb, 0, stream, ra);
-        cx ^= 1; cv ^= 1; cd ^= 1; cp ^= 1;
+        cP ^= 1; cV ^= 1; cD ^= 1; cO ^= 1;
+        cells_track_skey = true;
         ListArgs la{};
-        la.nf = nf; la.nd = nd; la.grid = grid; la.phases = d_phases;
-        la.px = px[cx]; la.py = py[cx]; la.pz = pz[cx]; la.phase = phase[cp]; la.cs = cellStart; la.ce = cellEnd;
-        la.dpx = dpx; la.dpy = dpy; la.dpz = dpz; la.dphase = dphase; la.dcs = dcellStart; la.dce = dcellEnd;
+        la.nf = nf; la.nd = nd; la.grid = grid; la.phases = d_phases; la.hf = phf.h; la.hd = phd.h;
+        la.P = P[cP]; la.V = V[cV]; la.cs = cellStart; la.ce = cellEnd;
+        la.DP = DP; la.dphase = dphase; la.dcs = dcellStart; la.dce = dcellEnd;
         la.nbr_ff = nbr_ff; la.cnt_ff = cnt_ff; la.nbr_fd = nbr_fd; la.cnt_fd = cnt_fd;
         la.K_ff = K_ff; la.K_fd = K_fd; la.maxcount = maxcount;
-        DEMSPH_LAUNCH(timer, "sph_lists", k_build_lists, gb, tb, 0, stream, la);
+        if (uniform) DEMSPH_LAUNCH(timer, "sph_lists", k_build_lists<true>, gb, tb, 0, stream, la);
+        else DEMSPH_LAUNCH(timer, "sph_lists", k_build_lists<false>, gb, tb, 0, stream, la);
         lists_valid = true;
     }
 
@@ -816,8 +971,9 @@ struct demsph_sph {
         SphArgs a = args();
         a.dt = dt;
         const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
-        DEMSPH_LAUNCH(timer, "sph_momentum_first", k_momentum<true>, gb, tb, 0, stream, a);
-        cx ^= 1; cv ^= 1;
+        if (uniform) DEMSPH_LAUNCH(timer, "sph_momentum_first", (k_momentum<true, true>), gb, tb, 0, stream, a);
+        else DEMSPH_LAUNCH(timer, "sph_momentum_first", (k_momentum<true, false>), gb, tb, 0, stream, a);
+        cP ^= 1; cV ^= 1;
     }
 
     void second_half_enqueue(double dt)
@@ -825,12 +981,14 @@ struct demsph_sph {
         const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
         SphArgs a = args();
         a.dt = dt;
-        DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity, gb, tb, 0, stream, a);
-        cd ^= 1;
+        if (uniform) DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity<true>, gb, tb, 0, stream, a);
+        else DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity<false>, gb, tb, 0, stream, a);
+        cP ^= 1; cD ^= 1;
         a = args();
         a.dt = dt;
-        DEMSPH_LAUNCH(timer, "sph_momentum_second", k_momentum<false>, gb, tb, 0, stream, a);
-        cv ^= 1;
+        if (uniform) DEMSPH_LAUNCH(timer, "sph_momentum_second", (k_momentum<false, true>), gb, tb, 0, stream, a);
+        else DEMSPH_LAUNCH(timer, "sph_momentum_second", (k_momentum<false, false>), gb, tb, 0, stream, a);
+        cV ^= 1;
     }
 
     void reset_overflow() { DEMSPH_HIP(hipMemsetAsync(maxcount, 0, 2 * sizeof(int), stream)); }
@@ -851,35 +1009,26 @@ struct demsph_sph {
 
     void ensure_checkpoint()
     {
-        if (ck_px) return;
+        if (ck_P) return;
         const size_t n = nf;
-        ck_px = dmalloc<double>(n); ck_py = dmalloc<double>(n); ck_pz = dmalloc<double>(n);
+        ck_P = dmalloc<FluidPos>(n); ck_V = dmalloc<float4>(n);
         ck_vx = dmalloc<double>(n); ck_vy = dmalloc<double>(n); ck_vz = dmalloc<double>(n);
-        ck_drho = dmalloc<float>(n); ck_phase = dmalloc<uint8_t>(n); ck_orig = dmalloc<int>(n); ck_rank = dmalloc<int>(n);
+        ck_drho = dmalloc<float>(n); ck_orig = dmalloc<int>(n); ck_rank = dmalloc<int>(n);
     }
 
     void checkpoint(bool save)
     {
         const size_t n = nf;
-        auto cp8 = [&](double* ck, double* cur) {
-            if (save) DEMSPH_HIP(hipMemcpyAsync(ck, cur, n * 8, hipMemcpyDeviceToDevice, stream));
-            else DEMSPH_HIP(hipMemcpyAsync(cur, ck, n * 8, hipMemcpyDeviceToDevice, stream));
-        };
-        cp8(ck_px, px[cx]); cp8(ck_py, py[cx]); cp8(ck_pz, pz[cx]);
-        cp8(ck_vx, vx[cv]); cp8(ck_vy, vy[cv]); cp8(ck_vz, vz[cv]);
         auto cpb = [&](void* ck, void* cur, size_t bytes) {
             if (save) DEMSPH_HIP(hipMemcpyAsync(ck, cur, bytes, hipMemcpyDeviceToDevice, stream));
             else DEMSPH_HIP(hipMemcpyAsync(cur, ck, bytes, hipMemcpyDeviceToDevice, stream));
         };
-        cpb(ck_drho, drho[cd], n * 4);
-        cpb(ck_phase, phase[cp], n);
-        cpb(ck_orig, orig[cp], n * 4);
+        cpb(ck_P, P[cP], n * sizeof(FluidPos));
+        cpb(ck_V, V[cV], n * sizeof(float4));
+        cpb(ck_vx, vx[cV], n * 8); cpb(ck_vy, vy[cV], n * 8); cpb(ck_vz, vz[cV], n * 8);
+        cpb(ck_drho, drho[cD], n * 4);
+        cpb(ck_orig, orig[cO], n * 4);
         cpb(ck_rank, rank, n * 4);
-        if (!save) {
-            hipLaunchKernelGGL(k_eos_all, dim3(blocks_for(n, 256)), dim3(256), 0, stream, (int)n, d_phases,
-                               phase[cp], drho[cd], rho[cd], p[cd], eos, (float)gamma);
-            DEMSPH_HIP(hipGetLastError());
-        }
     }
 
     void neighbor_search()
@@ -902,7 +1051,7 @@ struct demsph_sph {
         ensure_checkpoint();
         for (int attempt = 0; attempt < 8; attempt++) {
             checkpoint(true);
-            const int scx = cx, scv = cv, scd = cd, scp = cp;
+            const int sP = cP, sV = cV, sD = cD, sO = cO;
             reset_overflow();
             for (int k = 0; k < nsteps; k++) {
                 search_enqueue();
@@ -911,23 +1060,21 @@ struct demsph_sph {
             }
             if (!check_overflow()) return;
             // Replay the block from the checkpoint with larger lists.
-            cx = scx; cv = scv; cd = scd; cp = scp;
+            cP = sP; cV = sV; cD = sD; cO = sO;
             checkpoint(false);
         }
         throw std::runtime_error("neighbour list capacity did not converge");
     }
+
+    void refresh_eos()
+    {
+        hipLaunchKernelGGL(k_eos_all, dim3(blocks_for(nf, 256)), dim3(256), 0, stream, nf, d_phases, V[cV], drho[cD],
+                           P[cP], eos, (float)gamma);
+        DEMSPH_HIP(hipGetLastError());
+    }
 };
 
 namespace {
-
-template <class T>
-void scatter_by_orig(const std::vector<T>& slot, const std::vector<int>& orig, std::vector<T>& out, int comps)
-{
-    out.resize(slot.size());
-    const size_t n = orig.size();
-    for (size_t s = 0; s < n; s++)
-        for (int c = 0; c < comps; c++) out[(size_t)orig[s] * comps + c] = slot[s * comps + c];
-}
 
 template <class T>
 std::vector<T> fetch(const T* d, size_t n, hipStream_t s)
@@ -1082,34 +1229,44 @@ extern "C" int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_
         const std::string f = field;
         hipStream_t st = s->stream;
         const size_t n = s->nf;
-        auto orig = fetch(s->orig[s->cp], n, st);
-        auto vec3 = [&](double* const* xs, double* const* ys, double* const* zs, int b) {
-            auto hx = fetch(xs[b], n, st), hy = fetch(ys[b], n, st), hz = fetch(zs[b], n, st);
+        auto orig = fetch(s->orig[s->cO], n, st);
+        if (f == "position" || f == "density" || f == "pressure") {
+            auto h = fetch(s->P[s->cP], n, st);
+            const bool pos = f == "position";
+            std::vector<double> v(pos ? 3 * n : n);
+            for (size_t i = 0; i < n; i++) {
+                const size_t o = orig[i];
+                if (pos) {
+                    v[3 * o] = h[i].x; v[3 * o + 1] = h[i].y; v[3 * o + 2] = h[i].z;
+                } else {
+                    v[o] = f == "density" ? h[i].rho : h[i].p;
+                }
+            }
+            return emit(v, out, capacity, count);
+        }
+        if (f == "velocity") {
+            auto hx = fetch(s->vx[s->cV], n, st), hy = fetch(s->vy[s->cV], n, st), hz = fetch(s->vz[s->cV], n, st);
             std::vector<double> v(3 * n);
             for (size_t i = 0; i < n; i++) {
                 const size_t o = orig[i];
                 v[3 * o] = hx[i]; v[3 * o + 1] = hy[i]; v[3 * o + 2] = hz[i];
             }
-            return v;
-        };
-        auto scal = [&](const float* d) {
-            auto h = fetch(d, n, st);
+            return emit(v, out, capacity, count);
+        }
+        if (f == "densityChange") {
+            auto h = fetch(s->drho[s->cD], n, st);
             std::vector<double> v(n);
             for (size_t i = 0; i < n; i++) v[orig[i]] = h[i];
-            return v;
-        };
-        if (f == "position") return emit(vec3(s->px, s->py, s->pz, s->cx), out, capacity, count);
-        if (f == "velocity") return emit(vec3(s->vx, s->vy, s->vz, s->cv), out, capacity, count);
-        if (f == "densityChange") return emit(scal(s->drho[s->cd]), out, capacity, count);
-        if (f == "density") return emit(scal(s->rho[s->cd]), out, capacity, count);
-        if (f == "pressure") return emit(scal(s->p[s->cd]), out, capacity, count);
+            return emit(v, out, capacity, count);
+        }
         if (f == "dummyNormal") {
             const size_t m = s->nd;
             auto dor = fetch(s->dorig, m, st);
-            auto hx = fetch(s->dnx, m, st), hy = fetch(s->dny, m, st), hz = fetch(s->dnz, m, st);
+            auto hp = fetch(s->DP, m, st);
+            auto hv = fetch(s->DV, m, st);
             std::vector<double> v(3 * m);
             for (size_t i = 0; i < m; i++) {
-                v[3 * dor[i]] = hx[i]; v[3 * dor[i] + 1] = hy[i]; v[3 * dor[i] + 2] = hz[i];
+                v[3 * dor[i]] = hp[i].nx; v[3 * dor[i] + 1] = hp[i].ny; v[3 * dor[i] + 2] = hv[i].w;
             }
             return emit(v, out, capacity, count);
         }
@@ -1140,7 +1297,7 @@ extern "C" int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_
                 const int i = rankh[o];
                 const int c = std::min(cnt[i], K);
                 for (int k = 0; k < c; k++) {
-                    const int j = nbr[(size_t)k * n + i];
+                    const int j = nbr[(((size_t)(k / kChunk) * n + i) * kChunk) + (k % kChunk)];
                     v.push_back(ff ? orig[j] : dor[j]);
                 }
             }
@@ -1158,33 +1315,41 @@ extern "C" int demsph_sph_set(demsph_sph* s, const char* field, const void* in, 
         const std::string f = field;
         const size_t n = s->nf;
         hipStream_t st = s->stream;
-        auto rankv = fetch(s->orig[s->cp], n, st);  // slot -> orig
+        auto orig = fetch(s->orig[s->cO], n, st);  // slot -> insertion index
         const double* src = static_cast<const double*>(in);
-        if (f == "position" || f == "velocity") {
+        if (f == "position") {
             if (count != 3 * n) throw InvalidArg("expected 3*numFluid values");
+            auto h = fetch(s->P[s->cP], n, st);
+            for (size_t i = 0; i < n; i++) {
+                const size_t o = orig[i];
+                h[i].x = src[3 * o]; h[i].y = src[3 * o + 1]; h[i].z = src[3 * o + 2];
+            }
+            DEMSPH_HIP(hipMemcpy(s->P[s->cP], h.data(), n * sizeof(FluidPos), hipMemcpyHostToDevice));
+            s->lists_valid = false;
+            return;
+        }
+        if (f == "velocity") {
+            if (count != 3 * n) throw InvalidArg("expected 3*numFluid values");
+            auto hv = fetch(s->V[s->cV], n, st);
             std::vector<double> hx(n), hy(n), hz(n);
             for (size_t i = 0; i < n; i++) {
-                const size_t o = rankv[i];
+                const size_t o = orig[i];
                 hx[i] = src[3 * o]; hy[i] = src[3 * o + 1]; hz[i] = src[3 * o + 2];
+                hv[i].x = (float)hx[i]; hv[i].y = (float)hy[i]; hv[i].z = (float)hz[i];
             }
-            const int b = (f == "position") ? s->cx : s->cv;
-            double* dx = (f == "position") ? s->px[b] : s->vx[b];
-            double* dy = (f == "position") ? s->py[b] : s->vy[b];
-            double* dz = (f == "position") ? s->pz[b] : s->vz[b];
-            DEMSPH_HIP(hipMemcpy(dx, hx.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(dy, hy.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(dz, hz.data(), n * 8, hipMemcpyHostToDevice));
-            s->lists_valid = false;
+            const int b = s->cV;
+            DEMSPH_HIP(hipMemcpy(s->vx[b], hx.data(), n * 8, hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(s->vy[b], hy.data(), n * 8, hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(s->vz[b], hz.data(), n * 8, hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpy(s->V[b], hv.data(), n * sizeof(float4), hipMemcpyHostToDevice));
             return;
         }
         if (f == "densityChange") {
             if (count != n) throw InvalidArg("expected numFluid values");
             std::vector<float> h(n);
-            for (size_t i = 0; i < n; i++) h[i] = (float)src[rankv[i]];
-            DEMSPH_HIP(hipMemcpy(s->drho[s->cd], h.data(), n * 4, hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_eos_all, dim3(blocks_for(n, 256)), dim3(256), 0, st, (int)n, s->d_phases,
-                               s->phase[s->cp], s->drho[s->cd], s->rho[s->cd], s->p[s->cd], s->eos, (float)s->gamma);
-            DEMSPH_HIP(hipGetLastError());
+            for (size_t i = 0; i < n; i++) h[i] = (float)src[orig[i]];
+            DEMSPH_HIP(hipMemcpy(s->drho[s->cD], h.data(), n * 4, hipMemcpyHostToDevice));
+            s->refresh_eos();
             DEMSPH_HIP(hipStreamSynchronize(st));
             return;
         }
