@@ -245,13 +245,12 @@ int demsph_dem_hook_buoyancy_drag(demsph_dem* d, double rhoFluid, double Cd, con
 /* Sizes: balls, clumps, walls, triangles, ball pairs, bonds. */
 int demsph_dem_sizes(demsph_dem* d, size_t* numBalls, size_t* numClumps, size_t* numWalls, size_t* numTriangles,
                      size_t* numPairs, size_t* numBonds);
-/* Copies a field in insertion order (see the DEM field table in
- * INTEGRATION.md): "position" "velocity" "angularVelocity" "force" "torque"
+/* Copies a field in insertion order: "position" "velocity" "angularVelocity" "force" "torque"
  * (double 3n), "radius" "inverseMass" (double n), "objectPointed"
  * "objectPointing" (int, ball pairs as the reference lists them),
  * "slidingSpring" "rollingSpring" "torsionSpring" (double 3/pair),
  * "overlap" (double/pair), "triPointed" "triPointing" "triCancel" (int),
- * "triSlidingSpring" (double 3/pair), "bondPointed" "bondPointing"
+ * "triSlidingSpring" "triRollingSpring" "triTorsionSpring" (double 3/pair), "bondPointed" "bondPointing"
  * "bondIsBonded" (int), "bondNormalForce" "bondTorsionTorque" (double),
  * "bondShearForce" "bondBendingTorque" "bondPoint" "bondNormal" (double 3),
  * "clumpPosition" "clumpVelocity" "clumpAngularVelocity" (double 3/clump),

@@ -275,7 +275,7 @@ def _declare_dem(L):
 DEM_INT_FIELDS = {"objectPointed", "objectPointing", "triPointed", "triPointing", "triCancel", "bondPointed",
                   "bondPointing", "bondIsBonded", "hashValue", "hashIndex", "cellHashStart", "cellHashEnd"}
 DEM_VEC_FIELDS = {"position", "velocity", "angularVelocity", "force", "torque", "slidingSpring", "rollingSpring",
-                  "torsionSpring", "triSlidingSpring", "bondShearForce", "bondBendingTorque", "bondPoint",
+                  "torsionSpring", "triSlidingSpring", "triRollingSpring", "triTorsionSpring", "bondShearForce", "bondBendingTorque", "bondPoint",
                   "bondNormal", "clumpPosition", "clumpVelocity", "clumpAngularVelocity", "wallPosition",
                   "vertexGlobalPosition"}
 

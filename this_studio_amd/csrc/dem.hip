@@ -1905,8 +1905,8 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
             if (f == "objectPointed" || f == "objectPointing") return emit(vi, out, capacity, count);
             return emit(vd, out, capacity, count);
         }
-        const bool trif = f == "triPointed" || f == "triPointing" || f == "triCancel" || f == "triSlidingSpring" ||
-                          f == "triOverlap";
+        const bool tris = f == "triSlidingSpring" || f == "triRollingSpring" || f == "triTorsionSpring";
+        const bool trif = f == "triPointed" || f == "triPointing" || f == "triCancel" || tris || f == "triOverlap";
         if (trif) {
             std::vector<int> vi;
             std::vector<double> vd;
@@ -1917,6 +1917,8 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
                 auto hcan = fetch(d->hcancel.p, hn, st);
                 std::vector<double> sx, sy, sz, ov;
                 if (f == "triSlidingSpring") { sx = fetch(d->hsx.p, hn, st); sy = fetch(d->hsy.p, hn, st); sz = fetch(d->hsz.p, hn, st); }
+                if (f == "triRollingSpring") { sx = fetch(d->hrx.p, hn, st); sy = fetch(d->hry.p, hn, st); sz = fetch(d->hrz.p, hn, st); }
+                if (f == "triTorsionSpring") { sx = fetch(d->hox.p, hn, st); sy = fetch(d->hoy.p, hn, st); sz = fetch(d->hoz.p, hn, st); }
                 if (f == "triOverlap") ov = fetch(d->hov.p, hn, st);
                 for (size_t o = 0; o < n; o++)
                     for (int c = 0; c < hc[o]; c++) {
@@ -1928,7 +1930,7 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
                         else { vd.push_back(sx[h]); vd.push_back(sy[h]); vd.push_back(sz[h]); }
                     }
             }
-            if (f == "triSlidingSpring" || f == "triOverlap") return emit(vd, out, capacity, count);
+            if (tris || f == "triOverlap") return emit(vd, out, capacity, count);
             return emit(vi, out, capacity, count);
         }
         const size_t nb = d->nbond;

@@ -150,16 +150,13 @@ __device__ __forceinline__ void eos_eval(int eos, float gamma, const Phase& ph, 
 __global__ __launch_bounds__(kThreads) void k_hash(int n, GridGeom g, long long numGrids,
                                                    const FluidPos* __restrict__ P, const int* __restrict__ orig,
                                                    uint32_t* __restrict__ key, const uint32_t* __restrict__ prev_skey,
-                                                   int* __restrict__ cs, int* __restrict__ ce)
+                                                   int2* __restrict__ cells)
 {
     const int s = blockIdx.x * kThreads + threadIdx.x;
     if (s >= n) return;
     if (prev_skey) {
         const uint32_t h = prev_skey[s];
-        if ((long long)h < numGrids && (s == 0 || prev_skey[s - 1] != h)) {
-            cs[h] = -1;
-            ce[h] = -1;
-        }
+        if ((long long)h < numGrids && (s == 0 || prev_skey[s - 1] != h)) cells[h] = make_int2(-1, -1);
     }
     const FluidPos p = P[s];
     key[orig[s]] = (uint32_t)cell_hash(g, p.x, p.y, p.z);
@@ -180,7 +177,7 @@ struct ReorderArgs {
     float4* oV;
     double *ovx, *ovy, *ovz;
     float* odrho;
-    int *cellStart, *cellEnd;
+    int2* cells;  // (start, end) per cell
 };
 
 __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
@@ -198,8 +195,8 @@ __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
     // findStartAndEnd (buildHashStartEnd.cu:14-34)
     const uint32_t h = a.skey[k];
     if ((long long)h < a.numGrids) {
-        if (k == 0 || a.skey[k - 1] != h) a.cellStart[h] = k;
-        if (k == a.n - 1 || a.skey[k + 1] != h) a.cellEnd[h] = k + 1;
+        if (k == 0 || a.skey[k - 1] != h) a.cells[h].x = k;
+        if (k == a.n - 1 || a.skey[k + 1] != h) a.cells[h].y = k + 1;
     }
 }
 
@@ -211,10 +208,11 @@ struct ListArgs {
     double hf, hd;  // uniform fluid / dummy smoothing lengths
     const FluidPos* P;
     const float4* V;
-    const int *cs, *ce;  // fluid cells
+    const int2* cells;  // fluid cell ranges
     const DummyPos* DP;
     const uint8_t* dphase;
-    const int *dcs, *dce;  // dummy cells
+    const int2* dcells;          // dummy cell ranges
+    const uint8_t* dummy_near;   // per cell: a dummy cell lies in its 27-cell stencil
     int *nbr_ff, *cnt_ff, *nbr_fd, *cnt_fd;
     int K_ff, K_fd;  // multiples of kChunk
     int* maxcount;   // [0] fluid, [1] dummy
@@ -239,51 +237,73 @@ struct Chunk {
 // every member within 2*max(hA,hB) to the point's chunked list
 // (SPHNeighborSearchKernel.cu:57-115 with excludeSelf, :169-228 without).
 // The three x-cells of a stencil row are consecutive hashes, so their
-// members form one contiguous run of slots, walked in ascending order.
-template <bool UNIF, bool DUMMY>
-__device__ __forceinline__ int walk_list(const ListArgs& a, double x, double y, double z, double hA, int self,
-                                         int* __restrict__ out, int K)
+// members form one contiguous run of slots, walked in ascending order: the
+// nine row runs are resolved first (all cell-range loads in flight at
+// once), then each run is walked kWalkAhead candidates at a time with
+// their position loads issued before any test.
+#ifndef DEMSPH_WALK_AHEAD
+#define DEMSPH_WALK_AHEAD 4
+#endif
+constexpr int kWalkAhead = DEMSPH_WALK_AHEAD;
+
+template <bool UNIF, bool DUMMY, class PosT>
+__device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restrict__ B,
+                                         const int2* __restrict__ cells, double x, double y, double z, double hA,
+                                         int self, int* __restrict__ out, int K)
 {
     const GridGeom& g = a.grid;
-    const int* __restrict__ cs = DUMMY ? a.dcs : a.cs;
-    const int* __restrict__ ce = DUMMY ? a.dce : a.ce;
     const double cut_u = 2.0 * fmax(hA, DUMMY ? a.hd : a.hf);
     const size_t cstride = (size_t)a.nf * kChunk;
     const int nchunks = K / kChunk;
     const Stencil st = make_stencil(g, x, y, z);
+    int rs[9], re[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int dz = k / 3 - 1, dy = k % 3 - 1;
+        const bool row_ok = dz >= st.loz && dz <= st.hiz && dy >= st.loy && dy <= st.hiy;
+        const int row = (st.cz + dz) * g.ny * g.nx + (st.cy + dy) * g.nx + st.cx;
+        int s = -1, e = -1;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ok = row_ok && dx >= st.lox && dx <= st.hix;
+            const int2 c = cells[ok ? row + dx : 0];
+            const int2 r = ok ? c : make_int2(-1, -1);
+            if (r.x != -1) {
+                if (s < 0) s = r.x;
+                e = r.y;
+            }
+        }
+        rs[k] = s < 0 ? 0 : s;
+        re[k] = s < 0 ? 0 : e;
+    }
     Chunk buf{0, 0, 0, 0, 0, 0, 0, 0};
     int count = 0;
-    for (int zz = st.loz; zz <= st.hiz; zz++) {
-        for (int yy = st.loy; yy <= st.hiy; yy++) {
-            const int row = (st.cz + zz) * g.ny * g.nx + (st.cy + yy) * g.nx + st.cx;
-            int s = -1, e = -1;
-            for (int xx = st.lox; xx <= st.hix; xx++) {
-                const int c0 = cs[row + xx], c1 = ce[row + xx];
-                if (c0 != -1) {
-                    if (s < 0) s = c0;
-                    e = c1;
-                }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int e = re[k];
+        for (int j0 = rs[k]; j0 < e; j0 += kWalkAhead) {
+            double qx[kWalkAhead], qy[kWalkAhead], qz[kWalkAhead];
+#pragma unroll
+            for (int u = 0; u < kWalkAhead; u++) {
+                const PosT* b = B + min(j0 + u, e - 1);
+                qx[u] = b->x; qy[u] = b->y; qz[u] = b->z;
             }
-            for (int j = s; j < e; j++) {
-                if (!DUMMY && j == self) continue;
-                double bx, by, bz, hB;
-                if (DUMMY) {
-                    const DummyPos q = a.DP[j];
-                    bx = q.x; by = q.y; bz = q.z;
-                    hB = UNIF ? a.hd : a.phases[a.dphase[j]].h;
-                } else {
-                    const FluidPos q = a.P[j];
-                    bx = q.x; by = q.y; bz = q.z;
-                    hB = UNIF ? a.hf : a.phases[phase_bits(a.V[j].w)].h;
+#pragma unroll
+            for (int u = 0; u < kWalkAhead; u++) {
+                const int j = j0 + u;
+                if (j >= e || (!DUMMY && j == self)) continue;
+                double cut = cut_u;
+                if (!UNIF) {
+                    const double hB = DUMMY ? a.phases[a.dphase[j]].h : a.phases[phase_bits(a.V[j].w)].h;
+                    cut = 2.0 * fmax(hA, hB);
                 }
-                const double cut = UNIF ? cut_u : 2.0 * fmax(hA, hB);
-                if (!within_cut(cut, x - bx, y - by, z - bz)) continue;
-                const int q = count & (kChunk - 1);
-                buf.put(q, j);
+                if (!within_cut(cut, x - qx[u], y - qy[u], z - qz[u])) continue;
+                const int c = count & (kChunk - 1);
+                buf.put(c, j);
                 count++;
-                if (q == kChunk - 1) {
-                    const int c = (count >> 3) - 1;
-                    if (c < nchunks) buf.store(out + (size_t)c * cstride);
+                if (c == kChunk - 1) {
+                    const int cc = (count >> 3) - 1;
+                    if (cc < nchunks) buf.store(out + (size_t)cc * cstride);
                 }
             }
         }
@@ -299,10 +319,12 @@ __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
     if (i >= a.nf) return;
     const FluidPos p = a.P[i];
     const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
-    const int c1 = walk_list<UNIF, false>(a, p.x, p.y, p.z, h, i, a.nbr_ff + (size_t)i * kChunk, a.K_ff);
+    const int c1 =
+        walk_list<UNIF, false>(a, a.P, a.cells, p.x, p.y, p.z, h, i, a.nbr_ff + (size_t)i * kChunk, a.K_ff);
     a.cnt_ff[i] = c1;
     int c2 = 0;
-    if (a.nd > 0) c2 = walk_list<UNIF, true>(a, p.x, p.y, p.z, h, -1, a.nbr_fd + (size_t)i * kChunk, a.K_fd);
+    if (a.nd > 0 && a.dummy_near[cell_hash(a.grid, p.x, p.y, p.z)])
+        c2 = walk_list<UNIF, true>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, a.nbr_fd + (size_t)i * kChunk, a.K_fd);
     a.cnt_fd[i] = c2;
     if (c1 > a.K_ff) atomicMax(&a.maxcount[0], c1);
     if (c2 > a.K_fd) atomicMax(&a.maxcount[1], c2);
@@ -496,7 +518,7 @@ struct NormalArgs {
     DummyPos* DP;
     float4* DV;
     const uint8_t* dphase;
-    const int *dcs, *dce;
+    const int2* dcells;
 };
 
 __device__ __forceinline__ void grad_w64(double rx, double ry, double rz, double h, double& gx, double& gy, double& gz)
@@ -524,9 +546,9 @@ __global__ __launch_bounds__(kThreads) void k_dummy_normals(NormalArgs a)
         for (int yy = st.loy; yy <= st.hiy; yy++)
             for (int xx = st.lox; xx <= st.hix; xx++) {
                 const int hb = (st.cz + zz) * a.grid.ny * a.grid.nx + (st.cy + yy) * a.grid.nx + st.cx + xx;
-                const int b0 = a.dcs[hb];
+                const int2 rg = a.dcells[hb];
+                const int b0 = rg.x, b1 = rg.y;
                 if (b0 == -1) continue;
-                const int b1 = a.dce[hb];
                 for (int j = b0; j < b1; j++) {
                     if (j == i) continue;
                     const Phase pj = a.phases[a.dphase[j]];
@@ -566,7 +588,7 @@ struct DummyReorderArgs {
     float4* DV;
     uint8_t* ophase;
     int* orig;
-    int *cellStart, *cellEnd;
+    int2* cells;
 };
 
 __global__ __launch_bounds__(kThreads) void k_dummy_reorder(DummyReorderArgs a)
@@ -583,9 +605,27 @@ __global__ __launch_bounds__(kThreads) void k_dummy_reorder(DummyReorderArgs a)
     a.ophase[k] = a.phase[o];
     const uint32_t h = a.skey[k];
     if ((long long)h < a.numGrids) {
-        if (k == 0 || a.skey[k - 1] != h) a.cellStart[h] = k;
-        if (k == a.n - 1 || a.skey[k + 1] != h) a.cellEnd[h] = k + 1;
+        if (k == 0 || a.skey[k - 1] != h) a.cells[h].x = k;
+        if (k == a.n - 1 || a.skey[k + 1] != h) a.cells[h].y = k + 1;
     }
+}
+
+// Per cell: 1 when any cell of its 27-cell stencil (clamped at the grid
+// faces, as the list walk clamps it) holds a dummy.
+__global__ void k_dummy_near(GridGeom g, long long numGrids, const int2* __restrict__ dcells, uint8_t* __restrict__ near)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= numGrids) return;
+    const int cx = (int)(c % g.nx), cy = (int)((c / g.nx) % g.ny), cz = (int)(c / ((long long)g.nx * g.ny));
+    uint8_t any = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int x = cx + dx, y = cy + dy, z = cz + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                if (dcells[((long long)z * g.ny + y) * g.nx + x].x != -1) any = 1;
+            }
+    near[c] = any;
 }
 
 __global__ void k_dummy_hash(int n, GridGeom g, const double* __restrict__ px, const double* __restrict__ py,
@@ -689,7 +729,7 @@ struct demsph_sph {
     uint32_t *key = nullptr, *skey = nullptr, *ktmp = nullptr;
     int *perm = nullptr, *vtmp = nullptr;
     void* sort_temp = nullptr;
-    int *cellStart = nullptr, *cellEnd = nullptr;
+    int2* cells = nullptr;  // (start, end) per cell
     bool cells_track_skey = false;  // cell ranges were last written from the fluid keys in skey
     // lists
     int *nbr_ff = nullptr, *cnt_ff = nullptr, *nbr_fd = nullptr, *cnt_fd = nullptr;
@@ -701,7 +741,8 @@ struct demsph_sph {
     float4* DV = nullptr;
     uint8_t* dphase = nullptr;
     int* dorig = nullptr;
-    int *dcellStart = nullptr, *dcellEnd = nullptr;
+    int2* dcells = nullptr;
+    uint8_t* dummy_near = nullptr;
     // checkpoint for replay on list overflow
     FluidPos* ck_P = nullptr;
     float4* ck_V = nullptr;
@@ -721,11 +762,11 @@ struct demsph_sph {
         dfree(rank); dfree(key); dfree(skey); dfree(ktmp); dfree(perm); dfree(vtmp);
         if (sort_temp) (void)hipFree(sort_temp);
         sort_temp = nullptr;
-        dfree(cellStart); dfree(cellEnd); dfree(nbr_ff); dfree(cnt_ff); dfree(nbr_fd); dfree(cnt_fd);
+        dfree(cells); dfree(nbr_ff); dfree(cnt_ff); dfree(nbr_fd); dfree(cnt_fd);
         dfree(maxcount);
         if (maxcount_h) (void)hipHostFree(maxcount_h);
         maxcount_h = nullptr;
-        dfree(DP); dfree(DV); dfree(dphase); dfree(dorig); dfree(dcellStart); dfree(dcellEnd); dfree(d_phases);
+        dfree(DP); dfree(DV); dfree(dphase); dfree(dorig); dfree(dcells); dfree(dummy_near); dfree(d_phases);
         dfree(ck_P); dfree(ck_V); dfree(ck_vx); dfree(ck_vy); dfree(ck_vz); dfree(ck_drho); dfree(ck_orig);
         dfree(ck_rank);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -836,7 +877,7 @@ struct demsph_sph {
         key = dmalloc<uint32_t>(nsort); skey = dmalloc<uint32_t>(nsort); ktmp = dmalloc<uint32_t>(nsort);
         perm = dmalloc<int>(nsort); vtmp = dmalloc<int>(nsort);
         DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(nsort)));
-        cellStart = dmalloc<int>(numGrids); cellEnd = dmalloc<int>(numGrids);
+        cells = dmalloc<int2>(numGrids);
         cnt_ff = dmalloc<int>(n); cnt_fd = dmalloc<int>(n);
         maxcount = dmalloc<int>(2);
         DEMSPH_HIP(hipHostMalloc((void**)&maxcount_h, 2 * sizeof(int), hipHostMallocDefault));
@@ -865,9 +906,9 @@ struct demsph_sph {
         }
 
         // dummies: sort once (they are static), cell ranges, normals
-        dcellStart = dmalloc<int>(numGrids); dcellEnd = dmalloc<int>(numGrids);
-        fill_i32(dcellStart, numGrids, -1, stream);
-        fill_i32(dcellEnd, numGrids, -1, stream);
+        dcells = dmalloc<int2>(numGrids);
+        fill_i32(reinterpret_cast<int*>(dcells), 2 * numGrids, -1, stream);
+        dummy_near = dmalloc<uint8_t>(numGrids);
         const size_t m = nd;
         DP = dmalloc<DummyPos>(m); DV = dmalloc<float4>(m);
         dphase = dmalloc<uint8_t>(m); dorig = dmalloc<int>(m);
@@ -892,19 +933,21 @@ struct demsph_sph {
             ra.n = (int)m; ra.numGrids = numGrids; ra.skey = skey; ra.perm = perm;
             ra.px = ux; ra.py = uy; ra.pz = uz; ra.vx = wx; ra.vy = wy; ra.vz = wz; ra.phase = uph;
             ra.DP = DP; ra.DV = DV; ra.ophase = dphase;
-            ra.orig = dorig; ra.cellStart = dcellStart; ra.cellEnd = dcellEnd;
+            ra.orig = dorig; ra.cells = dcells;
             hipLaunchKernelGGL(k_dummy_reorder, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, ra);
             NormalArgs na{};
             na.nd = (int)m; na.grid = grid; na.phases = d_phases; na.DP = DP; na.DV = DV;
-            na.dphase = dphase; na.dcs = dcellStart; na.dce = dcellEnd;
+            na.dphase = dphase; na.dcells = dcells;
             hipLaunchKernelGGL(k_dummy_normals, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, na);
             DEMSPH_HIP(hipGetLastError());
             DEMSPH_HIP(hipStreamSynchronize(stream));
             dfree(ux); dfree(uy); dfree(uz); dfree(wx); dfree(wy); dfree(wz); dfree(uph);
         }
+        hipLaunchKernelGGL(k_dummy_near, dim3(blocks_for(numGrids, 256)), dim3(256), 0, stream, grid, numGrids, dcells,
+                           dummy_near);
+        DEMSPH_HIP(hipGetLastError());
         // the fluid cell ranges start empty; skey now holds dummy keys
-        fill_i32(cellStart, numGrids, -1, stream);
-        fill_i32(cellEnd, numGrids, -1, stream);
+        fill_i32(reinterpret_cast<int*>(cells), 2 * numGrids, -1, stream);
         cells_track_skey = false;
         DEMSPH_HIP(hipStreamSynchronize(stream));
         initialized = true;
@@ -937,12 +980,11 @@ struct demsph_sph {
         const dim3 gb(blocks_for(n, kThreads)), tb(kThreads);
         if (!cells_track_skey) {
             timer.begin("sph_fill_cells", stream);
-            fill_i32(cellStart, numGrids, -1, stream);
-            fill_i32(cellEnd, numGrids, -1, stream);
+            fill_i32(reinterpret_cast<int*>(cells), 2 * numGrids, -1, stream);
             timer.end(stream);
         }
         DEMSPH_LAUNCH(timer, "sph_hash", k_hash, gb, tb, 0, stream, n, grid, numGrids, P[cP], orig[cO], key,
-                      cells_track_skey ? skey : nullptr, cellStart, cellEnd);
+                      cells_track_skey ? skey : nullptr, cells);
         timer.begin("sph_sort", stream);
         radix_sort_pairs(key, nullptr, skey, perm, ktmp, vtmp, sort_temp, n, key_bits(numGrids), stream);
         timer.end(stream);
@@ -951,14 +993,14 @@ struct demsph_sph {
         ra.P = P[cP]; ra.V = V[cV]; ra.vx = vx[cV]; ra.vy = vy[cV]; ra.vz = vz[cV]; ra.drho = drho[cD];
         ra.oP = P[1 - cP]; ra.oV = V[1 - cV]; ra.ovx = vx[1 - cV]; ra.ovy = vy[1 - cV]; ra.ovz = vz[1 - cV];
         ra.odrho = drho[1 - cD];
-        ra.cellStart = cellStart; ra.cellEnd = cellEnd;
+        ra.cells = cells;
         DEMSPH_LAUNCH(timer, "sph_reorder", k_reorder, gb, tb, 0, stream, ra);
         cP ^= 1; cV ^= 1; cD ^= 1; cO ^= 1;
         cells_track_skey = true;
         ListArgs la{};
         la.nf = nf; la.nd = nd; la.grid = grid; la.phases = d_phases; la.hf = phf.h; la.hd = phd.h;
-        la.P = P[cP]; la.V = V[cV]; la.cs = cellStart; la.ce = cellEnd;
-        la.DP = DP; la.dphase = dphase; la.dcs = dcellStart; la.dce = dcellEnd;
+        la.P = P[cP]; la.V = V[cV]; la.cells = cells;
+        la.DP = DP; la.dphase = dphase; la.dcells = dcells; la.dummy_near = dummy_near;
         la.nbr_ff = nbr_ff; la.cnt_ff = cnt_ff; la.nbr_fd = nbr_fd; la.cnt_fd = cnt_fd;
         la.K_ff = K_ff; la.K_fd = K_fd; la.maxcount = maxcount;
         if (uniform) DEMSPH_LAUNCH(timer, "sph_lists", k_build_lists<true>, gb, tb, 0, stream, la);
@@ -1276,8 +1318,12 @@ extern "C" int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_
             return emit(v, out, capacity, count);
         }
         if (f == "hashIndex") return emit(orig, out, capacity, count);
-        if (f == "cellHashStart") return emit(fetch(s->cellStart, (size_t)s->numGrids, st), out, capacity, count);
-        if (f == "cellHashEnd") return emit(fetch(s->cellEnd, (size_t)s->numGrids, st), out, capacity, count);
+        if (f == "cellHashStart" || f == "cellHashEnd") {
+            auto c = fetch(s->cells, (size_t)s->numGrids, st);
+            std::vector<int> v(c.size());
+            for (size_t k = 0; k < c.size(); k++) v[k] = f == "cellHashStart" ? c[k].x : c[k].y;
+            return emit(v, out, capacity, count);
+        }
         const bool ff = (f == "neighborCount" || f == "objectPointing");
         const bool fd = (f == "dummyNeighborCount" || f == "dummyObjectPointing");
         if (ff || fd) {

@@ -272,27 +272,86 @@ def cylinder_mesh(center, axis, radius, height, n_around, n_height, caps=True):
     return np.array(verts), np.array(t0), np.array(t1), np.array(t2)
 
 
-def hex_sheet(min_bound, max_bound, spacing):
-    """generateHexPointCloud without periodic axes (pointCloudGeneration.h:
-    92-195): rows dy = a sqrt(3)/2 apart, odd rows shifted by a/2."""
+def hex_sheet(min_bound, max_bound, spacing, periodic_x=False, periodic_y=False, with_cell=False):
+    """generateHexPointCloud (pointCloudGeneration.h:92-195): rows
+    dy = a sqrt(3)/2 apart, odd rows shifted by a/2.  A periodic axis is
+    snapped to a whole number of steps (ceil), a non-periodic one floors and
+    stops before max_bound.  with_cell=True also returns the periodic cell
+    (min, max) of makeHexPBC2D."""
     dx = spacing
     dy = spacing * math.sqrt(3.0) * 0.5
     lx = max(0.0, max_bound[0] - min_bound[0])
     ly = max(0.0, max_bound[1] - min_bound[1])
-    nx = max(1, int(math.floor(lx / dx + 1e-12)))
-    ny = max(1, int(math.floor(ly / dy + 1e-12)))
+    nx = max(1, int(math.ceil(lx / dx - 1e-12))) if periodic_x else max(1, int(math.floor(lx / dx + 1e-12)))
+    ny = max(1, int(math.ceil(ly / dy - 1e-12))) if periodic_y else max(1, int(math.floor(ly / dy + 1e-12)))
     pts = []
     for j in range(ny):
         y = min_bound[1] + j * dy
-        if y >= max_bound[1]:
+        if not periodic_y and y >= max_bound[1]:
             break
         shift = 0.5 * dx if (j & 1) else 0.0
         for i in range(nx):
             x = min_bound[0] + i * dx + shift
-            if x >= max_bound[0]:
+            if not periodic_x and x >= max_bound[0]:
                 break
             pts.append((x, y, min_bound[2]))
-    return np.array(pts)
+    pts = np.array(pts)
+    if not with_cell:
+        return pts
+    cmax = np.array([min_bound[0] + nx * dx if periodic_x else max_bound[0],
+                     min_bound[1] + ny * dy if periodic_y else max_bound[1], max_bound[2]], dtype=np.float64)
+    return pts, np.asarray(min_bound, dtype=np.float64), cmax
+
+
+def static_sphere_water_level(R: float, rho_ball: float, rho_water: float) -> float:
+    """Height of the water plane, relative to the centre of a floating sphere,
+    at which the submerged cap volume fraction (R+h)^2 (2R-h) / (4R^3)
+    equals rho_ball/rho_water: 80 bisection steps on [-R, R]
+    (iceCylinderInteraction.cpp:5-35)."""
+    if not (R > 0.0) or not (rho_water > 0.0) or not (rho_ball >= 0.0):
+        return R
+    f = rho_ball / rho_water
+    if f <= 0.0:
+        return -R
+    if f >= 1.0:
+        return R
+    lo, hi = -R, R
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        b = R + mid
+        if (b * b) * (2.0 * R - mid) / (4.0 * R * R * R) < f:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def ice_cylinder() -> DEMScene:
+    """iceCylinderInteraction.cpp: a 100 x 116 hex sheet of bonded ice balls
+    (r = 0.1 m, 900 kg/m^3, periodic snap in y) drifting at 0.4 m/s onto a
+    36 x 36 triangulated cylinder (R = 1 m, H = 6 m), buoyancy + drag hook,
+    linear contacts with Coulomb friction, bonds breaking at 0.5 MPa,
+    dt = t_c / 50 with t_c = 2e-4 s."""
+    tc, r, den = 0.0002, 0.1, 900.0
+    m = r ** 3 * math.pi * 4.0 / 3.0 * den
+    kn = 0.5 * m * math.pi ** 2 / tc / tc
+    E = kn * 2.0 * r / (r * r * math.pi)
+    s = DEMScene("ice_cylinder")
+    s.set_bonded_pair(0, 0, 1.0, E, 2.6, 0.5e6, 0.5e6, 0.1)
+    s.set_linear_pair(0, 0, kn, kn / 2.6, 0.0, 0.0, 0.1, 0.1, 0.0, 0.0, 0.1, 0.0, 0.0)
+    s.set_linear_pair(0, 1, kn, kn / 2.6, 0.0, 0.0, 0.1, 0.1, 0.0, 0.0, 0.1, 0.0, 0.0)
+    pts, cmin, cmax = hex_sheet([0.0, 0.0, 0.0], [200 * r, 200 * r, 0.0], 2 * r, False, True, with_cell=True)
+    s.add_balls(pts, [-0.4, 0.0, 0.0], [0.0, 0.0, 0.0], r, den, 0)
+    v, t0, t1, t2 = cylinder_mesh([0.0, 0.0, 0.0], [0.0, 0.0, 1.0], 1.0, 6.0, 36, 36)
+    s.add_mesh_wall(v, t0, t1, t2, [-1.1, 100 * r, 0.0], [0, 0, 0], [0, 0, 0], 1)
+    level = static_sphere_water_level(r, den, 1000.0)
+    s.hooks = [("buoyancy", 1000.0, 0.1, [0.0, 0.0, -9.81], [-0.4, 0.0, 0.0], level)]
+    s.bonds_from_initial_pairs = True
+    s.min_bound = np.array([-cmax[0], cmin[1], -3.0])
+    s.max_bound = np.array([cmax[0], cmax[1], 3.0])
+    s.gravity = np.array([0.0, 0.0, -9.81])
+    s.dt = tc / 50.0
+    return s
 
 
 def elastic_bounce() -> DEMScene:
