@@ -205,13 +205,14 @@ public:
     // true return re-initialises the scene (DEMSolver.h:1536-1539, 1994-1997).
     virtual bool addInitialCondition() { return false; }
     // Called at every output frame of solve() (the reference writes VTU
-    // files there).
+    // files there).  requestStop() inside it ends solve() after the frame.
     virtual void onFrame(const size_t iFrame, const size_t iStep, const double time)
     {
         (void)iFrame;
         (void)iStep;
         (void)time;
     }
+    void requestStop() { stop_ = true; }
 
     // ------------------------------------------------------------ solve
     void solve(const double3 minBoundary, const double3 maxBoundary, const double3 gravity, const double timeStep,
@@ -241,6 +242,7 @@ public:
 
         defaultHook_ = false;
         demsph::check(demsph_dem_set_hook(h_, &DEMSolver::hookTrampoline, this));
+        stop_ = false;
         size_t iStep = 0, iFrame = 0;
         while (iStep <= numStep) {
             const size_t next = (iStep / frameInterval + 1) * frameInterval;
@@ -256,6 +258,7 @@ public:
                 iFrame++;
                 std::printf("Frame %zu at Time %g\n", iFrame, time);
                 onFrame(iFrame, iStep, time);
+                if (stop_) break;
             }
         }
         demsph::check(demsph_dem_set_hook(h_, nullptr, nullptr));
@@ -269,6 +272,7 @@ private:
     demsph_dem* h_ = nullptr;
     DEMBallAccess ball_;
     bool defaultHook_ = false;
+    bool stop_ = false;
     std::exception_ptr hookError_;
 
     static void hookTrampoline(void* user, size_t iStep, double time)

@@ -122,6 +122,7 @@ public:
         (void)iStep;
         (void)time;
     }
+    void requestStop() { stop_ = true; }
 
     void solve(const double3 minBoundary, const double3 maxBoundary, const double3 gravity, const double timeStep,
                const double maximumTime, const size_t numFrame, const std::string dir, const size_t deviceID = 0,
@@ -147,6 +148,7 @@ public:
         demsph::check(demsph_sph_initialize(h_, &minBoundary.x, &maxBoundary.x, &gravity.x));
         std::printf("Initialization Completed.\n");
 
+        stop_ = false;
         size_t iStep = 0, iFrame = 0;
         double time = 0.0;
         while (iStep <= numStep) {
@@ -159,6 +161,7 @@ public:
                 iFrame++;
                 std::printf("Frame %zu at Time %g\n", iFrame, time);
                 onFrame(iFrame, iStep, time);
+                if (stop_) break;
             }
         }
         demsph::check(demsph_sph_synchronize(h_));
@@ -170,6 +173,7 @@ private:
     hipStream_t stream_;
     demsph_sph* h_ = nullptr;
     WCSPHAccess fluid_, dummy_;
+    bool stop_ = false;
 
     void upload(const std::vector<WCSPHAccess::Row>& rows, bool dummy)
     {

@@ -2,7 +2,9 @@
 
 Usage: python -m this_studio_amd.build  (or build() from Python).
 Every .hip file under csrc/ is compiled to an object in _build/ and linked
-into this_studio_amd/libdemsph.so.
+into this_studio_amd/libdemsph.so; the C++ example programs under
+examples/ (host code over the DEMSolver / WCSPHSolver facade) are built
+into examples/bin/ against that library.
 """
 from __future__ import annotations
 
@@ -18,6 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libdemsph.so"
+EXAMPLES = PKG / "examples"
+EXAMPLES_BIN = EXAMPLES / "bin"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DEMSPH_ARCH", "gfx950")
@@ -68,5 +72,27 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+def build_examples(verbose: bool = True) -> list[Path]:
+    """Compiles examples/*.cpp (host-only C++) against libdemsph.so."""
+    EXAMPLES_BIN.mkdir(parents=True, exist_ok=True)
+    srcs = sorted(EXAMPLES.glob("*.cpp"))
+
+    def one(src: Path) -> Path:
+        exe = EXAMPLES_BIN / src.stem
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I", str(ROOT / "include"), "-I", str(EXAMPLES), str(src), "-o", str(exe),
+               "-L", str(PKG), "-ldemsph", "-Wl,-rpath,$ORIGIN/../.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, max(1, len(srcs)))) as ex:
+        exes = list(ex.map(one, srcs))
+    if verbose:
+        print(f"built {len(exes)} example programs in {EXAMPLES_BIN}")
+    return exes
+
+
 if __name__ == "__main__":
     build()
+    build_examples()

@@ -324,12 +324,109 @@ __global__ __launch_bounds__(kT) void k_bond_forces(BondArgs a)
 }
 
 // ----------------------------------------------------------------- walls
+// Squared distance from a point to an axis-aligned box against r^2
+// (ballNeighborSearchKernel.cu:202-222), unfused like the FP64 oracle.
+__device__ __forceinline__ bool box_within(const D3& p, double r2, double2 bx, double2 by, double2 bz)
+{
+#pragma clang fp contract(off)
+    const double cx = fmin(fmax(p.x, bx.x), bx.y), cy = fmin(fmax(p.y, by.x), by.y), cz = fmin(fmax(p.z, bz.x), bz.y);
+    const double dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+    return dx * dx + dy * dy + dz * dz <= r2;
+}
+
+// Bounding box of each triangle in sorted-slot order (fmin/fmax are exact,
+// so the boxes equal the ones built from the vertices during the walk).
+__global__ void k_tri_boxes(int nt, const int* __restrict__ tsorted, const int* __restrict__ t0,
+                            const int* __restrict__ t1, const int* __restrict__ t2, const double* __restrict__ gx,
+                            const double* __restrict__ gy, const double* __restrict__ gz, double2* __restrict__ box)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nt) return;
+    const int t = tsorted[s];
+    const int i0 = t0[t], i1 = t1[t], i2 = t2[t];
+    box[3 * (size_t)s] = make_double2(fmin(gx[i0], fmin(gx[i1], gx[i2])), fmax(gx[i0], fmax(gx[i1], gx[i2])));
+    box[3 * (size_t)s + 1] = make_double2(fmin(gy[i0], fmin(gy[i1], gy[i2])), fmax(gy[i0], fmax(gy[i1], gy[i2])));
+    box[3 * (size_t)s + 2] = make_double2(fmin(gz[i0], fmin(gz[i1], gz[i2])), fmax(gz[i0], fmax(gz[i1], gz[i2])));
+}
+
+// Per triangle-grid cell, the triangles a ball hashed into that cell can
+// touch: the members of the cell's 27-cell stencil (in the walk order of
+// ballNeighborSearchKernel.cu:153-339) whose bounding box comes within
+// `reach` (the largest ball radius plus a rounding margin) of the cell's
+// box.  Cells on the grid faces are unbounded outward, because positions
+// beyond the domain are clamped into them.  Any triangle that passes a
+// ball's exact box test is in its cell's list, in the same relative order,
+// so the ball walk over this list yields the reference's candidate list.
+struct CellCandArgs {
+    GridGeom tg;
+    int ntc;
+    double reach;
+    const int *tcs, *tce, *tsorted;
+    const double2* tbox;
+    const int* start;  // write pass
+    int* count;        // count pass
+    int* out_tri;      // write pass: triangle ids
+    double2* out_box;  // write pass: their boxes
+};
+
+template <bool WRITE>
+__global__ __launch_bounds__(kT) void k_cell_candidates(CellCandArgs a)
+{
+    const int c = blockIdx.x * kT + threadIdx.x;
+    if (c >= a.ntc) return;
+    const GridGeom& g = a.tg;
+    const int cx = c % g.nx, cy = (c / g.nx) % g.ny, cz = c / (g.nx * g.ny);
+    const int lox = cx > 0 ? -1 : 0, hix = cx < g.nx - 1 ? 1 : 0;
+    const int loy = cy > 0 ? -1 : 0, hiy = cy < g.ny - 1 ? 1 : 0;
+    const int loz = cz > 0 ? -1 : 0, hiz = cz < g.nz - 1 ? 1 : 0;
+    const double inf = HUGE_VAL;
+    const double blx = cx > 0 ? g.minx + cx * g.cx - a.reach : -inf;
+    const double bhx = cx < g.nx - 1 ? g.minx + (cx + 1) * g.cx + a.reach : inf;
+    const double bly = cy > 0 ? g.miny + cy * g.cy - a.reach : -inf;
+    const double bhy = cy < g.ny - 1 ? g.miny + (cy + 1) * g.cy + a.reach : inf;
+    const double blz = cz > 0 ? g.minz + cz * g.cz - a.reach : -inf;
+    const double bhz = cz < g.nz - 1 ? g.minz + (cz + 1) * g.cz + a.reach : inf;
+    const int base = WRITE ? a.start[c] : 0;
+    int k = 0;
+    for (int zz = loz; zz <= hiz; zz++)
+        for (int yy = loy; yy <= hiy; yy++) {
+            const int row = (cz + zz) * g.ny * g.nx + (cy + yy) * g.nx + cx;
+            int s0 = -1, s1 = -1;
+            for (int xx = lox; xx <= hix; xx++) {
+                const int c0 = a.tcs[row + xx], c1 = a.tce[row + xx];
+                if (c0 != -1) {
+                    if (s0 < 0) s0 = c0;
+                    s1 = c1;
+                }
+            }
+            for (int sl = s0; sl < s1; sl++) {
+                const double2 bx = a.tbox[3 * (size_t)sl], by = a.tbox[3 * (size_t)sl + 1],
+                              bz = a.tbox[3 * (size_t)sl + 2];
+                if (bx.x <= bhx && bx.y >= blx && by.x <= bhy && by.y >= bly && bz.x <= bhz && bz.y >= blz) {
+                    if (WRITE) {
+                        const size_t o = (size_t)base + k;
+                        a.out_tri[o] = a.tsorted[sl];
+                        a.out_box[3 * o] = bx;
+                        a.out_box[3 * o + 1] = by;
+                        a.out_box[3 * o + 2] = bz;
+                    }
+                    k++;
+                }
+            }
+        }
+    if (!WRITE) a.count[c] = k;
+}
+
 struct WallArgs {
     Balls b;
     Tables tb;
     double dt;
     GridGeom tg;
     const int *tcs, *tce, *tsorted;  // triangle cells over circumcentres
+    const double2* tbox;             // per sorted slot: (min,max) x, y, z of the triangle
+    const int* cstart;               // per triangle cell: candidate range [cstart[c], cstart[c+1])
+    const int* ctri;                 // candidate triangle ids
+    const double2* cbox;             // their boxes
     const int *t0, *t1, *t2, *twall;
     const double *vgx, *vgy, *vgz;     // vertex global positions
     const double *wpx, *wpy, *wpz, *wvx, *wvy, *wvz, *wwx, *wwy, *wwz;
@@ -361,33 +458,29 @@ __global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
     const double rad = a.b.r[i];
     int tri[KMAX];
     int cnt = 0;
-    // candidate walk over the triangle grid, AABB distance test
+    // Candidate walk: the triangles of the ball's cell list (the stencil
+    // members near the cell, in walk order) tested against the ball with
+    // the distance to each triangle's bounding box (no skin), four ahead.
     const Stencil st = make_stencil(a.tg, ri.x, ri.y, ri.z);
-    for (int zz = st.loz; zz <= st.hiz; zz++)
-        for (int yy = st.loy; yy <= st.hiy; yy++)
-            for (int xx = st.lox; xx <= st.hix; xx++) {
-                const int hb = (st.cz + zz) * a.tg.ny * a.tg.nx + (st.cy + yy) * a.tg.nx + st.cx + xx;
-                const int s0 = a.tcs[hb];
-                if (s0 == -1) continue;
-                const int s1 = a.tce[hb];
-                for (int s = s0; s < s1; s++) {
-                    const int t = a.tsorted[s];
-                    const int i0 = a.t0[t], i1 = a.t1[t], i2 = a.t2[t];
-                    const double minx = fmin(a.vgx[i0], fmin(a.vgx[i1], a.vgx[i2]));
-                    const double miny = fmin(a.vgy[i0], fmin(a.vgy[i1], a.vgy[i2]));
-                    const double minz = fmin(a.vgz[i0], fmin(a.vgz[i1], a.vgz[i2]));
-                    const double maxx = fmax(a.vgx[i0], fmax(a.vgx[i1], a.vgx[i2]));
-                    const double maxy = fmax(a.vgy[i0], fmax(a.vgy[i1], a.vgy[i2]));
-                    const double maxz = fmax(a.vgz[i0], fmax(a.vgz[i1], a.vgz[i2]));
-                    const double cx = fmin(fmax(ri.x, minx), maxx), cy = fmin(fmax(ri.y, miny), maxy),
-                                 cz = fmin(fmax(ri.z, minz), maxz);
-                    const double dx = ri.x - cx, dy = ri.y - cy, dz = ri.z - cz;
-                    if (dx * dx + dy * dy + dz * dz <= rad * rad) {
-                        if (cnt < KMAX && cnt < a.KW) tri[cnt] = t;
-                        cnt++;
-                    }
-                }
+    const int cell = (st.cz * a.tg.ny + st.cy) * a.tg.nx + st.cx;
+    const double r2 = rad * rad;
+    const int c0 = a.cstart[cell], c1 = a.cstart[cell + 1];
+    for (int j0 = c0; j0 < c1; j0 += 4) {
+        double2 bx[4], by[4], bz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double2* q = a.cbox + 3 * (size_t)min(j0 + u, c1 - 1);
+            bx[u] = q[0]; by[u] = q[1]; bz[u] = q[2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (j0 + u >= c1) break;
+            if (box_within(ri, r2, bx[u], by[u], bz[u])) {
+                if (cnt < KMAX && cnt < a.KW) tri[cnt] = a.ctri[j0 + u];
+                cnt++;
             }
+        }
+    }
     if (cnt > KMAX || cnt > a.KW) {
         atomicMax(a.overflow, cnt);
         cnt = min(KMAX, a.KW);
@@ -889,6 +982,16 @@ struct demsph_dem {
     DBuf<double> tcc;
     DBuf<uint32_t> tkey, tskey, tktmp;
     DBuf<int> tperm, tvtmp, tcs, tce;
+    DBuf<double> tbox;            // 6 per triangle, sorted-slot order
+    DBuf<int> tcand_cnt, tcand_start, tcand_tri;  // per-cell candidate lists
+    DBuf<double> tcand_box;       // 6 per candidate
+    void* tscan_temp = nullptr;
+    double rmax = 0.0;            // largest ball radius
+    bool tri_cells_valid = false;  // triangle cells and boxes match the current vertices
+    // Walls start at rest with identity orientation (addMeshWall stores no
+    // velocities, DEMSolver.h:1901-1911); until a wall velocity is set,
+    // their motion is an exact no-op and the triangle cells stay valid.
+    bool walls_static = true;
     void* tsort_temp = nullptr;
     int KW = 16;
     DBuf<int> hcnt, htri, hcancel;
@@ -918,6 +1021,8 @@ struct demsph_dem {
         (void)hipStreamSynchronize(stream);
         if (sort_temp) (void)hipFree(sort_temp);
         if (tsort_temp) (void)hipFree(tsort_temp);
+        if (tscan_temp) (void)hipFree(tscan_temp);
+        tscan_temp = nullptr;
         if (scan_temp) (void)hipFree(scan_temp);
         if (total_h) (void)hipHostFree(total_h);
         if (overflow_h) (void)hipHostFree(overflow_h);
@@ -1155,8 +1260,15 @@ struct demsph_dem {
             up(dt0, hw.t0, nt); up(dt1, hw.t1, nt); up(dt2, hw.t2, nt); up(dtwall, hw.twall, nt);
             up(tcc, hw.tcc, 3 * (size_t)nt);
             tkey.alloc(nt); tskey.alloc(nt); tktmp.alloc(nt); tperm.alloc(nt); tvtmp.alloc(nt);
+            tbox.alloc(6 * (size_t)nt);
+            tri_cells_valid = false;
+            walls_static = true;
             DEMSPH_HIP(hipMalloc(&tsort_temp, radix_sort_temp_bytes(nt)));
             tcs.alloc(tng); tce.alloc(tng);
+            tcand_cnt.alloc(tng + 1); tcand_start.alloc(tng + 1);
+            DEMSPH_HIP(hipMalloc(&tscan_temp, scan_temp_bytes((int)tng + 1)));
+            rmax = 0.0;
+            for (double rr : hb.r) rmax = std::max(rmax, rr);
             const size_t hn = (size_t)KW * n;
             hcnt.alloc(n);
             DEMSPH_HIP(hipMemset(hcnt.p, 0, n * sizeof(int)));
@@ -1251,7 +1363,7 @@ struct demsph_dem {
         gather(nt > 0 && (walls || walls_valid));
         if (walls) {
             walls_valid = true;
-            move_walls(wallDt);
+            if (!walls_static) move_walls(wallDt);
         }
     }
 
@@ -1283,15 +1395,26 @@ struct demsph_dem {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         Balls b = cur();
         {
-            timer.begin("dem_triangle_cells", stream);
-            hash_aos(tcc.p, nt, tg, tkey.p, stream);
-            radix_sort_pairs(tkey.p, nullptr, tskey.p, tperm.p, tktmp.p, tvtmp.p, tsort_temp, nt, key_bits(tng), stream);
-            fill_i32(tcs.p, tng, -1, stream);
-            fill_i32(tce.p, tng, -1, stream);
-            cell_ranges(tskey.p, nt, tng, tcs.p, tce.p, stream);
-            timer.end(stream);
+            if (!tri_cells_valid) {
+                timer.begin("dem_triangle_cells", stream);
+                hash_aos(tcc.p, nt, tg, tkey.p, stream);
+                radix_sort_pairs(tkey.p, nullptr, tskey.p, tperm.p, tktmp.p, tvtmp.p, tsort_temp, nt, key_bits(tng),
+                                 stream);
+                fill_i32(tcs.p, tng, -1, stream);
+                fill_i32(tce.p, tng, -1, stream);
+                cell_ranges(tskey.p, nt, tng, tcs.p, tce.p, stream);
+                hipLaunchKernelGGL(k_tri_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, stream, nt, tperm.p, dt0.p,
+                                   dt1.p, dt2.p, vgx.p, vgy.p, vgz.p, reinterpret_cast<double2*>(tbox.p));
+                DEMSPH_HIP(hipGetLastError());
+                build_tri_candidates();
+                timer.end(stream);
+                tri_cells_valid = walls_static;
+            }
             WallArgs wa{};
             wa.b = b; wa.tb = tables(); wa.dt = wallDt; wa.tg = tg; wa.tcs = tcs.p; wa.tce = tce.p; wa.tsorted = tperm.p;
+            wa.tbox = reinterpret_cast<const double2*>(tbox.p);
+            wa.cstart = tcand_start.p; wa.ctri = tcand_tri.p;
+            wa.cbox = reinterpret_cast<const double2*>(tcand_box.p);
             wa.t0 = dt0.p; wa.t1 = dt1.p; wa.t2 = dt2.p; wa.twall = dtwall.p;
             wa.vgx = vgx.p; wa.vgy = vgy.p; wa.vgz = vgz.p;
             wa.wpx = wpx.p; wa.wpy = wpy.p; wa.wpz = wpz.p; wa.wvx = wvx.p; wa.wvy = wvy.p; wa.wvz = wvz.p;
@@ -1305,6 +1428,33 @@ struct demsph_dem {
             if (fp64) DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<double>, gb, tb, 0, stream, wa);
             else DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<float>, gb, tb, 0, stream, wa);
         }
+    }
+
+    // Per-cell triangle candidate lists (count, scan, one host read of the
+    // total for the capacity, write).
+    void build_tri_candidates()
+    {
+        CellCandArgs ca{};
+        ca.tg = tg; ca.ntc = (int)tng;
+        const double cmax = std::max(tg.cx, std::max(tg.cy, tg.cz));
+        ca.reach = rmax * (1.0 + 1e-9) + 1e-9 * cmax;
+        ca.tcs = tcs.p; ca.tce = tce.p; ca.tsorted = tperm.p; ca.tbox = reinterpret_cast<const double2*>(tbox.p);
+        ca.count = tcand_cnt.p;
+        const dim3 gb(blocks_for(tng, kT)), tb(kT);
+        hipLaunchKernelGGL(k_cell_candidates<false>, gb, tb, 0, stream, ca);
+        DEMSPH_HIP(hipGetLastError());
+        exclusive_scan(tcand_cnt.p, tcand_start.p, (int)tng, tscan_temp, total.p + 1, stream);
+        DEMSPH_HIP(hipMemcpyAsync(tcand_start.p + tng, total.p + 1, sizeof(int), hipMemcpyDeviceToDevice, stream));
+        DEMSPH_HIP(hipMemcpyAsync(total_h + 1, total.p + 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+        DEMSPH_HIP(hipStreamSynchronize(stream));
+        const size_t tot = (size_t)total_h[1];
+        if (tot > tcand_tri.n || tcand_tri.p == nullptr) {
+            tcand_tri.alloc(std::max<size_t>(tot, 1));
+            tcand_box.alloc(6 * std::max<size_t>(tot, 1));
+        }
+        ca.start = tcand_start.p; ca.out_tri = tcand_tri.p; ca.out_box = reinterpret_cast<double2*>(tcand_box.p);
+        hipLaunchKernelGGL(k_cell_candidates<true>, gb, tb, 0, stream, ca);
+        DEMSPH_HIP(hipGetLastError());
     }
 
     void gather(bool with_walls)
@@ -1987,6 +2137,11 @@ extern "C" int demsph_dem_set(demsph_dem* d, const char* field, const void* in, 
             if (m == 0) return;
             std::vector<double> hx(m), hy(m), hz(m);
             for (size_t i = 0; i < m; i++) { hx[i] = src[3 * i]; hy[i] = src[3 * i + 1]; hz[i] = src[3 * i + 2]; }
+            for (size_t i = 0; i < 3 * m; i++)
+                if (src[i] != 0.0) {
+                    d->walls_static = false;
+                    d->tri_cells_valid = false;
+                }
             DEMSPH_HIP(hipStreamSynchronize(st));
             if (f == "wallVelocity") {
                 DEMSPH_HIP(hipMemcpy(d->wvx.p, hx.data(), m * 8, hipMemcpyHostToDevice));
