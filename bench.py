@@ -12,8 +12,11 @@ All state is resident in HBM before the timed region starts.
 Prints one JSON line:
   value      = fluid particle-steps per second over the timed steps
   roofline   = the dominant kernel's algorithmic bytes / its mean launch time
-               (HIP events on the scene stream, inside the timed region)
+               (HIP events on the scene stream, inside the timed region);
+               traffic = its rocprofv3-measured HBM bytes per launch
   cpu_baseline = the FP64 oracle on the host cores over a bounded sample
+  dem_collapse = the 4,096-ball DEM granular collapse, ball-steps per second,
+               with its own oracle baseline
 """
 from __future__ import annotations
 
@@ -29,28 +32,40 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+METRIC = "particle-steps/s + HBM GB/s; batched-scene p50/p99 step latency vs plain"
 HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
 def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> float:
-    """Compulsory HBM bytes of one launch (each array once per launch;
-    neighbour re-reads served by caches are not counted).  Layout: position
-    3 x f64, velocity 3 x f64, drho/rho/p f32, phase u8, ELL list 4 B/entry,
-    counts 4 B."""
-    state_in = 24 + 24 + 8 + 1          # x, v, rho+p, phase
+    """Compulsory HBM bytes of one launch: each array the launch needs is read
+    or written once; neighbour re-reads served by caches are not counted.
+    Layout: FluidPos {x,y,z f64; rho,p f32} 32 B, V {v f32 x3; phase} 16 B,
+    FP64 velocity 24 B, drho 4 B, list entries 4 B, per-slot counts 8 B."""
     lists = 4 * (pairs_ff + pairs_fd) + 8 * nf
     if kernel == "sph_momentum_first":
-        return nf * (state_in + 24 + 24) + lists       # + write v, x
+        return nf * (80 + 72) + lists      # read P, V, v64, counts; write P', V', v64'
     if kernel == "sph_momentum_second":
-        return nf * (state_in + 24) + lists            # + write v
+        return nf * (80 + 40) + lists      # read P, V, v64, counts; write V', v64'
     if kernel == "sph_continuity":
-        return nf * (state_in + 4 + 12) + lists        # + read drho, write drho/rho/p
+        return nf * (60 + 36) + lists      # read P, V, drho, counts; write P', drho'
     if kernel == "sph_lists":
-        return nf * (24 + 1) + lists                   # read x, phase; write lists + counts
+        return nf * 24 + lists             # read own x; write the lists and counts
     if kernel == "sph_reorder":
-        return nf * 2 * (24 + 24 + 4 + 1 + 4) + nf * (4 + 4 + 8 + 12)
+        return nf * 168                    # gather P, V, v64, drho, ids; write them
     return 0.0
+
+
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+# this workload (HBM bytes = 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE, from a
+# plain 256 MiB copy), as recorded in profiles/wcsph_dam_break_256k_kernels.md.
+MEASURED_HBM_BYTES = {
+    "sph_lists": (24.3 + 102.1) * 1e6,
+    "sph_momentum_first": (120.7 + 18.4) * 1e6,
+    "sph_momentum_second": (120.4 + 10.2) * 1e6,
+    "sph_continuity": (115.4 + 9.2) * 1e6,
+    "sph_reorder": (25.3 + 22.9) * 1e6,
+}
 
 
 def algorithmic_flops(kernel: str, pairs_ff: int, pairs_fd: int) -> float:
@@ -84,6 +99,44 @@ def cpu_baseline(spec, budget_s: float = 15.0) -> dict:
                       f"1 warm-up step ({el:.1f} s; setup {t1 - t0:.1f} s excluded)"}
 
 
+def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
+    """The DEM granular collapse (4,096 Hertz-Mindlin spheres in a 0.3 m box
+    of 12 triangles, dt = 1e-5 s): ball-steps per second over `steps` steps
+    after solve() setup and 50 warm-up steps, state resident; the same scene
+    through the FP64 oracle beside it."""
+    import this_studio_amd as ts
+    from this_studio_amd import scenes
+
+    s = scenes.granular_collapse()
+    g = ts.DEMSceneGPU(s)
+    g.step(50)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    n = g.sizes()[0]
+    out = {"workload": "dem_granular_collapse", "balls": n, "triangles": g.sizes()[3], "steps": steps,
+           "value": round(n * steps / el, 1), "unit": "ball-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
+           "precision": "fp32 contact forces, fp64 positions/velocities/springs"}
+    if with_cpu:
+        import oracle
+
+        o = oracle.OracleDEM(s)
+        o.step(5)
+        k, t1 = 0, time.perf_counter()
+        while True:
+            o.step(10)
+            k += 10
+            ce = time.perf_counter() - t1
+            if ce >= cpu_budget or k >= 2000:
+                break
+        out["cpu_baseline"] = {"value": round(n * k / ce, 1), "unit": "ball-steps/s",
+                               "cores": int(oracle.lib().o_num_threads()), "kind": "port",
+                               "sample": f"FP64 oracle (C, OpenMP) on the same collapse, {k} steps ({ce:.1f} s)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +144,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dx", type=float, default=0.005)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dem-steps", type=int, default=2000, help="steps of the DEM collapse leg (0: skip it)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate HBM counters")
@@ -162,7 +216,8 @@ def main() -> int:
     d_bytes = algorithmic_bytes(dom, nf, pairs_ff, pairs_fd)
     achieved = d_bytes / d_avg_s / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                "traffic": MEASURED_HBM_BYTES.get(dom),
                 "avg_launch_us": round(d_avg_s * 1e6, 2), "algorithmic_bytes": int(d_bytes)}
     fl = algorithmic_flops(dom, pairs_ff, pairs_fd)
     if fl:
@@ -175,8 +230,13 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(spec, args.cpu_budget)
+    dem = None
+    if args.dem_steps > 0:
+        print("[bench] dem granular collapse ...", file=sys.stderr, flush=True)
+        dem = dem_collapse(args.dem_steps, min(args.cpu_budget, 10.0), not args.no_cpu_baseline)
+        print(f"[bench] dem collapse: {dem['ms_per_step'] * 1e3:.1f} us/step", file=sys.stderr, flush=True)
     out = {
-        "metric": "particle-steps/s",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -195,6 +255,7 @@ def main() -> int:
                    "precision": "fp32 pair arithmetic, fp64 position/velocity, fp32 drho"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dem_collapse": dem,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
