@@ -266,6 +266,34 @@ int demsph_dem_set(demsph_dem* d, const char* field, const void* in, size_t coun
 int demsph_dem_enable_timing(demsph_dem* d, int enable);
 int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long* launches, double* totalMs);
 
+/* ------------------------------------------------------------------ coupling
+ * Two-way DEM-SPH coupling.  The reference has none (its DEM and WCSPH
+ * solvers run separately; buoyancy and drag enter the DEM only through the
+ * analytic hook of externalForceTorque.cu:66-161), so this model is the
+ * project's own: every coupled ball is a solid particle of the fluid phase
+ * (mass rho0 * 4/3 pi r^3, pressure interpolated from the fluid at its
+ * centre, moving with its surface velocity) for the fluid particles within
+ * their support 2h, with the Riemann interface pressure of
+ * WCSPHIntegrationKernel.cu:42-148.  The ball receives exactly the opposite
+ * of every fluid particle's pair force (and the torque of its viscous
+ * part), so momentum is exchanged pairwise.  Both scenes must be
+ * initialised (the DEM scene through solve_begin) and are advanced with one
+ * time step on the fluid scene's stream. */
+typedef struct demsph_couple demsph_couple;
+int demsph_couple_create(demsph_couple** out, demsph_sph* fluid, demsph_dem* dem);
+int demsph_couple_destroy(demsph_couple* c);
+/* Per fluid particle capacity of the fluid -> ball list (default 32). */
+int demsph_couple_set_capacity(demsph_couple* c, int ballsPerFluid);
+/* nsteps coupled kick-drift-kick steps of both scenes (DEM pair rebuild
+ * every 5th step as in DEMSolver.h:1384).  A list overflow inside the block
+ * returns DEMSPH_ERR_CAPACITY (capacities are grown; the block is lost). */
+int demsph_couple_step(demsph_couple* c, const double* gravity, double timeStep, int nsteps);
+/* "couplingForce" "couplingTorque" (double 3 per ball, DEM insertion order,
+ * last evaluation), "ballPressure" (double per ball), and
+ * "fluidCouplingAcceleration" (double 3 per fluid particle, insertion order,
+ * the coupling part of the last fluid momentum pass). */
+int demsph_couple_get(demsph_couple* c, const char* field, void* out, size_t capacity, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

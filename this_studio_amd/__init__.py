@@ -446,3 +446,46 @@ class DEMSceneGPU:
             out[buf.value.decode()] = (n.value, ms.value)
             i += 1
         return out
+
+
+class CoupledScene:
+    """Two-way DEM-SPH coupling of one fluid scene and one DEM scene on the
+    GPU (demsph_couple_*): both advance together with one time step."""
+
+    def __init__(self, fluid: "SPHScene", dem: "DEMSceneGPU", capacity: int | None = None):
+        L = lib()
+        L.demsph_couple_create.argtypes = [C.POINTER(_vp), _vp, _vp]
+        L.demsph_couple_destroy.argtypes = [_vp]
+        L.demsph_couple_set_capacity.argtypes = [_vp, C.c_int]
+        L.demsph_couple_step.argtypes = [_vp, _dp, C.c_double, C.c_int]
+        L.demsph_couple_get.argtypes = [_vp, C.c_char_p, _vp, _sz, C.POINTER(_sz)]
+        self.L, self.fluid, self.dem = L, fluid, dem
+        self._h = _vp()
+        check(L.demsph_couple_create(C.byref(self._h), fluid._h, dem._h))
+        if capacity:
+            check(L.demsph_couple_set_capacity(self._h, int(capacity)))
+
+    def close(self) -> None:
+        if self._h:
+            check(self.L.demsph_couple_destroy(self._h))
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, dt: float, n: int = 1, gravity=(0.0, 0.0, -9.81)):
+        g = _f64(gravity, 3)
+        check(self.L.demsph_couple_step(self._h, _d(g), float(dt), int(n)))
+
+    def get(self, field: str) -> np.ndarray:
+        cnt = _sz()
+        check(self.L.demsph_couple_get(self._h, field.encode(), None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value, dtype=np.float64)
+        check(self.L.demsph_couple_get(self._h, field.encode(), out.ctypes.data_as(_vp), cnt.value, C.byref(cnt)))
+        if field != "ballPressure":
+            out = out.reshape(-1, 3)
+        return out
+

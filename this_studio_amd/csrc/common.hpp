@@ -28,6 +28,11 @@ struct HipFailure : std::runtime_error {
 struct InvalidArg : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// A per-particle list or candidate capacity was exceeded where the work
+// cannot be replayed with a larger one.
+struct CapacityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 // Runs f, translating exceptions to C-ABI status codes.
 template <class F>
@@ -39,6 +44,9 @@ int guarded(F&& f)
     } catch (const InvalidArg& e) {
         set_error(e.what());
         return DEMSPH_ERR_INVALID;
+    } catch (const CapacityError& e) {
+        set_error(e.what());
+        return DEMSPH_ERR_CAPACITY;
     } catch (const HipFailure& e) {
         set_error(e.what());
         return DEMSPH_ERR_HIP;

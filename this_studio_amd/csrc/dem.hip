@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "dem_kernels.hpp"
 #include "hook_math.hpp"
+#include "internal.hpp"
 #include "timing.hpp"
 
 #include <algorithm>
@@ -1572,8 +1573,8 @@ struct demsph_dem {
         if (overflow_h[0] > 0) {
             const int c = overflow_h[0];
             DEMSPH_HIP(hipMemset(overflow.p, 0, sizeof(int)));
-            throw std::length_error("a ball had " + std::to_string(c) +
-                                    " wall-triangle candidates, above the per-ball capacity of 16");
+            throw CapacityError("a ball had " + std::to_string(c) +
+                                " wall-triangle candidates, above the per-ball capacity of 16");
         }
     }
 
@@ -1608,6 +1609,40 @@ struct demsph_dem {
         check_overflow();
     }
 };
+
+// ------------------------------------------------------------ internal API
+namespace demsph {
+
+DemBallRefs dem_ball_refs(demsph_dem* d)
+{
+    const Balls b = d->cur();
+    const Forces f = d->forces();
+    return DemBallRefs{b.n, b.px, b.py, b.pz, b.vx, b.vy, b.vz, b.wx, b.wy, b.wz, b.r, b.im, b.orig,
+                       f.fx, f.fy, f.fz, f.tx, f.ty, f.tz};
+}
+hipStream_t dem_stream(demsph_dem* d) { return d->stream; }
+void dem_set_stream(demsph_dem* d, hipStream_t s) { d->stream = s; }
+void dem_require_ready(demsph_dem* d)
+{
+    d->require_init();
+    if (!d->has_list) throw InvalidArg("solve_begin() (or neighbor_search) must precede coupled stepping");
+}
+double dem_begin_step(demsph_dem* d, double dt)
+{
+    d->iStep++;
+    d->time += dt;
+    if (d->iStep % 5 == 0) d->neighbor_search();
+    return d->time;
+}
+void dem_evaluate(demsph_dem* d, double h, double dt) { d->evaluate(h, dt); }
+void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, double hookTime)
+{
+    d->run_hooks(hookTime);
+    d->half(first, g, h);
+}
+void dem_end_block(demsph_dem* d) { d->check_overflow(); }
+
+}  // namespace demsph
 
 namespace {
 

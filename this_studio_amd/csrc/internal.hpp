@@ -1,0 +1,39 @@
+// Internal C++ interfaces between the scene implementations of the library
+// (used by the DEM-SPH coupling); not part of the C ABI.
+#pragma once
+
+#include "common.hpp"
+
+struct demsph_dem;
+struct demsph_sph;
+
+namespace demsph {
+
+// The DEM scene's current ball buffers (cell-sorted slot order) and its
+// force / torque accumulators.
+struct DemBallRefs {
+    int n;
+    const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *r, *im;
+    const int* orig;
+    double *fx, *fy, *fz, *tx, *ty, *tz;
+};
+
+DemBallRefs dem_ball_refs(demsph_dem* d);
+hipStream_t dem_stream(demsph_dem* d);
+void dem_set_stream(demsph_dem* d, hipStream_t s);
+// Throws unless the scene is initialised and solve_begin() has run.
+void dem_require_ready(demsph_dem* d);
+
+// One DEM step split at the points where external forces enter
+// (loop_ball / loop_ball_wall, DEMSolver.h:1376-1405, 1452-1485):
+//   begin_step: iStep++, time += dt, pair rebuild every 5th step; returns time
+//   evaluate:   contact, bond and wall forces (overwrite the accumulators)
+//   hooks_and_half: the external-force hook at hookTime, then a half kick
+//                   (first: kick + drift)
+//   end_block:  deferred capacity checks
+double dem_begin_step(demsph_dem* d, double dt);
+void dem_evaluate(demsph_dem* d, double h, double dt);
+void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, double hookTime);
+void dem_end_block(demsph_dem* d);
+
+}  // namespace demsph

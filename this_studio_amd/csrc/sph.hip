@@ -27,6 +27,7 @@
 // Pair arithmetic is FP32 (positions differenced in FP64 first); state
 // integration is FP64.
 #include "common.hpp"
+#include "internal.hpp"
 #include "timing.hpp"
 
 #include <algorithm>
@@ -68,6 +69,15 @@ struct alignas(32) DummyPos {
 };
 static_assert(sizeof(DummyPos) == 32, "DummyPos is one 32-byte record");
 
+// A coupled DEM ball seen from the fluid: centre (FP64), radius and the
+// fluid pressure interpolated at its centre.  Velocity and spin travel in
+// two float4 records.
+struct alignas(32) SolidPos {
+    double x, y, z;
+    float r, P;
+};
+static_assert(sizeof(SolidPos) == 32, "SolidPos is one 32-byte record");
+
 __device__ __forceinline__ unsigned phase_bits(float w) { return __float_as_uint(w); }
 
 struct SphArgs {
@@ -91,6 +101,13 @@ struct SphArgs {
     const DummyPos* DP;
     const float4* DV;
     const uint8_t* dphase;
+    // coupled DEM balls (solids), sorted into the fluid grid each step
+    int ns;
+    const SolidPos* SP;
+    const float4 *SV, *SW;  // velocity, angular velocity
+    const int *nbr_fs, *cnt_fs;
+    int K_fs;
+    float* acs;             // per fluid slot: coupling acceleration of the last pass (3 floats), or null
     // physics
     float gx, gy, gz;
     double dt;
@@ -140,6 +157,89 @@ __device__ __forceinline__ void eos_eval(int eos, float gamma, const Phase& ph, 
     } else {
         p = ph.c2 * ch;
     }
+}
+
+// ----------------------------------------------------------- DEM-SPH coupling
+// A coupled DEM ball j is a solid particle of the fluid phase for the fluid
+// particles within its support: mass rho0 * 4/3 pi r_j^3, pressure P_j
+// interpolated from the fluid at its centre, density from the inverse EOS
+// of P_j, moving with its surface velocity v_j + w_j x (r_j n) where n points
+// from the ball centre to the fluid particle.  The interface pressure is the
+// Riemann one of the fluid-fluid branch (WCSPHIntegrationKernel.cu:42-112)
+// with P_R = P_j.  The reference has no coupling; this is the project's
+// model.  The fluid particle's acceleration contribution is returned split
+// into its central pressure part `ap` and viscous part `av`; the ball
+// receives -m_i (ap + av) and the torque (r_j n) x (-m_i av).  Contraction
+// is off so that the fluid pass and the ball pass, which both call this,
+// compute bit-identical values (exact pairwise momentum exchange).
+struct FsGeom {
+    float rx, ry, rz, r, inv_r, f;   // r_ij = x_i - x_j, |r_ij|, 1/|r_ij|, kernel factor
+    float vsx, vsy, vsz;             // ball surface velocity at the pair
+    float lx, ly, lz;                // lever r_j n
+    float rhoR, mj;                  // ball density and fluid-equivalent mass
+};
+
+__device__ __forceinline__ FsGeom fs_geom(const SphArgs& a, const Phase& pi, double xi, double yi, double zi,
+                                          const SolidPos& S, float4 SV, float4 SW)
+{
+#pragma clang fp contract(off)
+    FsGeom g;
+    g.rx = (float)(xi - S.x); g.ry = (float)(yi - S.y); g.rz = (float)(zi - S.z);
+    const float r2 = g.rx * g.rx + g.ry * g.ry + g.rz * g.rz;
+    g.inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    g.r = r2 * g.inv_r;
+    const float h = pi.hf;
+    if (g.r < 1.e-20f || g.r >= 2.0f * h) {
+        g.f = 0.0f;
+    } else {
+        const float invh = 1.0f / h;
+        const float t = 1.0f - 0.5f * g.r * invh;
+        const float sigma = (float)(21.0 / (16.0 * M_PI)) * invh * invh * invh;
+        g.f = -5.0f * sigma * invh * invh * (t * t * t);
+    }
+    const float s = S.r * g.inv_r;
+    g.lx = s * g.rx; g.ly = s * g.ry; g.lz = s * g.rz;
+    g.vsx = SV.x + (SW.y * g.lz - SW.z * g.ly);
+    g.vsy = SV.y + (SW.z * g.lx - SW.x * g.lz);
+    g.vsz = SV.z + (SW.x * g.ly - SW.y * g.lx);
+    if (a.eos == 1) {
+        const float base = fmaxf(1.0f + S.P / pi.taitB, 1.e-12f);
+        g.rhoR = pi.rho0 * __powf(base, a.inv_gamma);
+    } else {
+        g.rhoR = S.P / pi.c2 + pi.rho0;
+    }
+    g.mj = pi.rho0 * ((float)(4.0 / 3.0 * M_PI) * (S.r * S.r * S.r));
+    return g;
+}
+
+__device__ __forceinline__ void fs_momentum(const SphArgs& a, const Phase& pi, double xi, double yi, double zi,
+                                            float vix, float viy, float viz, float rhoi, float PL, const SolidPos& S,
+                                            float4 SV, float4 SW, float3& ap, float3& av, float3& lever)
+{
+#pragma clang fp contract(off)
+    const FsGeom g = fs_geom(a, pi, xi, yi, zi, S, SV, SW);
+    const float UL = -(vix * g.rx + viy * g.ry + viz * g.rz) * g.inv_r;
+    const float UR = -(g.vsx * g.rx + g.vsy * g.ry + g.vsz * g.rz) * g.inv_r;
+    const float cbar = pi.c, rbar = 0.5f * (rhoi + g.rhoR);
+    const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
+    const float Ps = 0.5f * (PL + S.P) + 0.5f * beta * rbar * (UL - UR);
+    const float mr = g.mj / (rhoi * g.rhoR);
+    const float kp = -2.0f * mr * Ps * g.f;
+    const float kv = 2.0f * pi.nu * (g.f * g.r) * mr;
+    ap = make_float3(kp * g.rx, kp * g.ry, kp * g.rz);
+    av = make_float3(kv * (vix - g.vsx), kv * (viy - g.vsy), kv * (viz - g.vsz));
+    lever = make_float3(g.lx, g.ly, g.lz);
+}
+
+__device__ __forceinline__ float fs_density(const SphArgs& a, const Phase& pi, double xi, double yi, double zi,
+                                            float vix, float viy, float viz, float rhoi, float PL, const SolidPos& S,
+                                            float4 SV, float4 SW)
+{
+    const FsGeom g = fs_geom(a, pi, xi, yi, zi, S, SV, SW);
+    const float cbar = pi.c, rbar = 0.5f * (rhoi + g.rhoR);
+    const float t = 0.5f * ((vix - g.vsx) * g.rx + (viy - g.vsy) * g.ry + (viz - g.vsz) * g.rz) +
+                    0.5f * (PL - S.P) * g.r * rcp(rbar * cbar);
+    return g.mj * rcp(g.rhoR) * g.f * t;
 }
 
 // ----------------------------------------------------------- hash & reorder
@@ -215,8 +315,15 @@ struct ListArgs {
     const uint8_t* dummy_near;   // per cell: a dummy cell lies in its 27-cell stencil
     int *nbr_ff, *cnt_ff, *nbr_fd, *cnt_fd;
     int K_ff, K_fd;  // multiples of kChunk
-    int* maxcount;   // [0] fluid, [1] dummy
+    int* maxcount;   // [0] fluid, [1] dummy, [2] coupled balls
+    // coupled balls: cell ranges over the fluid grid, records, lists
+    const int2* scells;
+    const SolidPos* SP;
+    int *nbr_fs, *cnt_fs;
+    int K_fs;
 };
+
+enum WalkKind { kFluid = 0, kDummy = 1, kSolid = 2 };
 
 // One 32-byte list chunk held in registers while it fills.
 struct Chunk {
@@ -246,13 +353,14 @@ struct Chunk {
 #endif
 constexpr int kWalkAhead = DEMSPH_WALK_AHEAD;
 
-template <bool UNIF, bool DUMMY, class PosT>
+template <bool UNIF, int KIND, class PosT>
 __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restrict__ B,
                                          const int2* __restrict__ cells, double x, double y, double z, double hA,
                                          int self, int* __restrict__ out, int K)
 {
     const GridGeom& g = a.grid;
-    const double cut_u = 2.0 * fmax(hA, DUMMY ? a.hd : a.hf);
+    // balls couple to the fluid particles within the fluid's support 2 h_A
+    const double cut_u = KIND == kSolid ? 2.0 * hA : 2.0 * fmax(hA, KIND == kDummy ? a.hd : a.hf);
     const size_t cstride = (size_t)a.nf * kChunk;
     const int nchunks = K / kChunk;
     const Stencil st = make_stencil(g, x, y, z);
@@ -291,10 +399,10 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
 #pragma unroll
             for (int u = 0; u < kWalkAhead; u++) {
                 const int j = j0 + u;
-                if (j >= e || (!DUMMY && j == self)) continue;
+                if (j >= e || (KIND == kFluid && j == self)) continue;
                 double cut = cut_u;
-                if (!UNIF) {
-                    const double hB = DUMMY ? a.phases[a.dphase[j]].h : a.phases[phase_bits(a.V[j].w)].h;
+                if (!UNIF && KIND != kSolid) {
+                    const double hB = KIND == kDummy ? a.phases[a.dphase[j]].h : a.phases[phase_bits(a.V[j].w)].h;
                     cut = 2.0 * fmax(hA, hB);
                 }
                 if (!within_cut(cut, x - qx[u], y - qy[u], z - qz[u])) continue;
@@ -320,14 +428,27 @@ __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
     const FluidPos p = a.P[i];
     const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
     const int c1 =
-        walk_list<UNIF, false>(a, a.P, a.cells, p.x, p.y, p.z, h, i, a.nbr_ff + (size_t)i * kChunk, a.K_ff);
+        walk_list<UNIF, kFluid>(a, a.P, a.cells, p.x, p.y, p.z, h, i, a.nbr_ff + (size_t)i * kChunk, a.K_ff);
     a.cnt_ff[i] = c1;
     int c2 = 0;
     if (a.nd > 0 && a.dummy_near[cell_hash(a.grid, p.x, p.y, p.z)])
-        c2 = walk_list<UNIF, true>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, a.nbr_fd + (size_t)i * kChunk, a.K_fd);
+        c2 = walk_list<UNIF, kDummy>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, a.nbr_fd + (size_t)i * kChunk, a.K_fd);
     a.cnt_fd[i] = c2;
     if (c1 > a.K_ff) atomicMax(&a.maxcount[0], c1);
     if (c2 > a.K_fd) atomicMax(&a.maxcount[1], c2);
+}
+
+// Fluid -> coupled-ball lists over the balls sorted into the fluid grid.
+__global__ __launch_bounds__(kThreads) void k_build_fs_lists(ListArgs a)
+{
+    const int i = xcd_block() * kThreads + threadIdx.x;
+    if (i >= a.nf) return;
+    const FluidPos p = a.P[i];
+    const double h = a.phases[phase_bits(a.V[i].w)].h;
+    const int c = walk_list<true, kSolid>(a, a.SP, a.scells, p.x, p.y, p.z, h, -1, a.nbr_fs + (size_t)i * kChunk,
+                                          a.K_fs);
+    a.cnt_fs[i] = c;
+    if (c > a.K_fs) atomicMax(&a.maxcount[2], c);
 }
 
 // ----------------------------------------------------------- pair passes
@@ -434,6 +555,22 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
             az += kp * rz + kv * (viz - DVj.z);
         });
     }
+    if (a.ns > 0) {
+        float cx = 0.f, cy = 0.f, cz = 0.f;
+        for_chunks(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, 0, [&](int j, bool valid) {
+            float3 ap, av, lv;
+            fs_momentum(a, pi, xi, yi, zi, vix, viy, viz, rhoi, PL, a.SP[j], a.SV[j], a.SW[j], ap, av, lv);
+            cx += valid ? ap.x + av.x : 0.0f;
+            cy += valid ? ap.y + av.y : 0.0f;
+            cz += valid ? ap.z + av.z : 0.0f;
+        });
+        ax += cx; ay += cy; az += cz;
+        if (a.acs) {
+            a.acs[3 * (size_t)i] = cx;
+            a.acs[3 * (size_t)i + 1] = cy;
+            a.acs[3 * (size_t)i + 2] = cz;
+        }
+    }
     ax += a.gx; ay += a.gy; az += a.gz;
     const double hdt = 0.5 * a.dt;
     const double nvx = a.vx[i] + hdt * (double)ax;
@@ -500,12 +637,169 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
             d += pj.m * rcp(rhoj) * f * t;
         });
     }
+    if (a.ns > 0) {
+        for_chunks(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, 0, [&](int j, bool valid) {
+            const float t = fs_density(a, pi, xi, yi, zi, vix, viy, viz, rhoi, PL, a.SP[j], a.SV[j], a.SW[j]);
+            d += valid ? t : 0.0f;
+        });
+    }
     const float nd = a.drho[i] + (float)(a.dt * (double)(2.0f * rhoi * d));
     a.odrho[i] = nd;
     FluidPos o;
     o.x = xi; o.y = yi; o.z = zi;
     eos_eval(a.eos, a.gamma, pi, nd, o.rho, o.p);
     a.oP[i] = o;
+}
+
+// ----------------------------------------------------------- coupled balls
+// Cell hash of each DEM ball (DEM slot order) on the fluid grid.
+__global__ void k_solid_hash(int ns, GridGeom g, const double* __restrict__ px, const double* __restrict__ py,
+                             const double* __restrict__ pz, uint32_t* __restrict__ key)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    key[s] = (uint32_t)cell_hash(g, px[s], py[s], pz[s]);
+}
+
+struct SolidGatherArgs {
+    int ns;
+    long long numGrids;
+    const uint32_t* skey;  // sorted keys (null: refresh kinematics only)
+    const int* s2d;        // solid slot -> DEM slot
+    const double *px, *py, *pz, *vx, *vy, *vz, *wx, *wy, *wz, *r;
+    SolidPos* SP;
+    float4 *SV, *SW;
+    int2* cells;  // null when refreshing
+};
+
+// Ball records in fluid-grid order (the pressure P is left as it is) and,
+// after a sort, the balls' cell ranges on the fluid grid.
+__global__ void k_solid_gather(SolidGatherArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.ns) return;
+    const int s = a.s2d[k];
+    SolidPos& P = a.SP[k];
+    P.x = a.px[s]; P.y = a.py[s]; P.z = a.pz[s];
+    P.r = (float)a.r[s];
+    a.SV[k] = make_float4((float)a.vx[s], (float)a.vy[s], (float)a.vz[s], 0.f);
+    a.SW[k] = make_float4((float)a.wx[s], (float)a.wy[s], (float)a.wz[s], 0.f);
+    if (a.cells) {
+        const uint32_t h = a.skey[k];
+        if ((long long)h < a.numGrids) {
+            if (k == 0 || a.skey[k - 1] != h) a.cells[h].x = k;
+            if (k == a.ns - 1 || a.skey[k + 1] != h) a.cells[h].y = k + 1;
+        }
+    }
+}
+
+// Reverse index of the fluid -> ball lists: for each ball, the fluid slots
+// that list it, ascending (the same pair set seen from the ball).
+__global__ void k_fs_rev_count(int nf, const int* __restrict__ nbr, const int* __restrict__ cnt, int K,
+                               int* __restrict__ rcnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const int n = min(cnt[i], K);
+    for (int k = 0; k < n; k++) atomicAdd(&rcnt[nbr[((size_t)(k / kChunk) * nf + i) * kChunk + (k % kChunk)]], 1);
+}
+__global__ void k_fs_rev_fill(int nf, const int* __restrict__ nbr, const int* __restrict__ cnt, int K,
+                              const int* __restrict__ rstart, int* __restrict__ rcursor, int* __restrict__ rfluid)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const int n = min(cnt[i], K);
+    for (int k = 0; k < n; k++) {
+        const int j = nbr[((size_t)(k / kChunk) * nf + i) * kChunk + (k % kChunk)];
+        rfluid[rstart[j] + atomicAdd(&rcursor[j], 1)] = i;
+    }
+}
+__global__ void k_fs_rev_sort(int ns, const int* __restrict__ rstart, int* __restrict__ rfluid)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    const int b = rstart[j], e = rstart[j + 1];
+    for (int q = b + 1; q < e; q++) {
+        const int v = rfluid[q];
+        int t = q - 1;
+        while (t >= b && rfluid[t] > v) {
+            rfluid[t + 1] = rfluid[t];
+            t--;
+        }
+        rfluid[t + 1] = v;
+    }
+}
+
+// Wendland C2 value (WCSPHIntegrationKernel.cuh:5-13), FP32.
+__device__ __forceinline__ float wendland(float r, float h)
+{
+    const float q = r / h;
+    if (q >= 2.0f) return 0.0f;
+    const float t = 1.0f - 0.5f * q;
+    return (float)(21.0 / (16.0 * M_PI)) / (h * h * h) * (t * t * t * t) * (2.0f * q + 1.0f);
+}
+
+// P_j = sum V_i p_i W_ij / sum V_i W_ij over the ball's fluid neighbours
+// (Shepard interpolation of the fluid pressure at the ball centre).
+__global__ void k_solid_pressure(int ns, const int* __restrict__ rstart, const int* __restrict__ rfluid,
+                                 const FluidPos* __restrict__ P, const float4* __restrict__ V,
+                                 const Phase* __restrict__ phases, SolidPos* __restrict__ SP)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    const double sx = SP[j].x, sy = SP[j].y, sz = SP[j].z;
+    float num = 0.f, den = 0.f;
+    for (int q = rstart[j]; q < rstart[j + 1]; q++) {
+        const int i = rfluid[q];
+        const FluidPos f = P[i];
+        const Phase& ph = phases[phase_bits(V[i].w)];
+        const float rx = (float)(f.x - sx), ry = (float)(f.y - sy), rz = (float)(f.z - sz);
+        const float w = wendland(sqrtf(rx * rx + ry * ry + rz * rz), ph.hf) * (ph.m / f.rho);
+        num += w * f.p;
+        den += w;
+    }
+    SP[j].P = den > 0.f ? num / den : 0.f;
+}
+
+struct CoupleForceArgs {
+    SphArgs a;  // fluid state and ball records, as the fluid pass sees them
+    const int *rstart, *rfluid, *s2d;
+    double *fx, *fy, *fz, *tx, *ty, *tz;  // DEM force / torque accumulators (DEM slot order)
+    double* Fc;                           // per ball slot: coupling force and torque (6 doubles)
+};
+
+// The reaction of every fluid-ball pair of the fluid pass on the ball:
+// -m_i (ap + av) and (r_j n) x (-m_i av), summed in ascending fluid slot
+// order and added to the DEM accumulators.
+__global__ void k_couple_forces(CoupleForceArgs c)
+{
+    const SphArgs& a = c.a;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.ns) return;
+    const SolidPos S = a.SP[j];
+    const float4 SV = a.SV[j], SW = a.SW[j];
+    double Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
+    for (int q = c.rstart[j]; q < c.rstart[j + 1]; q++) {
+        const int i = c.rfluid[q];
+        const FluidPos Pi = a.P[i];
+        const float4 Vi = a.V[i];
+        const Phase pi = a.phases[phase_bits(Vi.w)];
+        float3 ap, av, lv;
+        fs_momentum(a, pi, Pi.x, Pi.y, Pi.z, Vi.x, Vi.y, Vi.z, Pi.rho, Pi.p, S, SV, SW, ap, av, lv);
+        const double m = (double)pi.m;
+        Fx -= m * (double)(ap.x + av.x);
+        Fy -= m * (double)(ap.y + av.y);
+        Fz -= m * (double)(ap.z + av.z);
+        const double vx = -m * (double)av.x, vy = -m * (double)av.y, vz = -m * (double)av.z;
+        Tx += (double)lv.y * vz - (double)lv.z * vy;
+        Ty += (double)lv.z * vx - (double)lv.x * vz;
+        Tz += (double)lv.x * vy - (double)lv.y * vx;
+    }
+    const int s = c.s2d[j];
+    c.fx[s] += Fx; c.fy[s] += Fy; c.fz[s] += Fz;
+    c.tx[s] += Tx; c.ty[s] += Ty; c.tz[s] += Tz;
+    double* o = c.Fc + 6 * (size_t)j;
+    o[0] = Fx; o[1] = Fy; o[2] = Fz; o[3] = Tx; o[4] = Ty; o[5] = Tz;
 }
 
 // ----------------------------------------------------------- dummy normals
@@ -699,6 +993,18 @@ struct SphHost {
 
 using namespace demsph;
 
+namespace demsph {
+// What the fluid passes see of the coupled balls during coupled stepping.
+struct SolidView {
+    int ns = 0;
+    const SolidPos* SP = nullptr;
+    const float4 *SV = nullptr, *SW = nullptr;
+    const int *nbr_fs = nullptr, *cnt_fs = nullptr;
+    int K_fs = 0;
+    float* acs = nullptr;
+};
+}  // namespace demsph
+
 struct demsph_sph {
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -715,6 +1021,7 @@ struct demsph_sph {
     std::vector<uint8_t> fphase_h, dphase_h;
     Phase* d_phases = nullptr;
     bool uniform = false;  // one fluid phase and at most one dummy phase
+    SolidView solids;      // coupled balls (ns == 0 outside coupled stepping)
     Phase phf{}, phd{};
 
     // fluid state, double-buffered per record group
@@ -806,6 +1113,8 @@ struct demsph_sph {
         a.nbr_ff = nbr_ff; a.cnt_ff = cnt_ff; a.nbr_fd = nbr_fd; a.cnt_fd = cnt_fd;
         a.K_ff = K_ff; a.K_fd = K_fd;
         a.DP = DP; a.DV = DV; a.dphase = dphase;
+        a.ns = solids.ns; a.SP = solids.SP; a.SV = solids.SV; a.SW = solids.SW;
+        a.nbr_fs = solids.nbr_fs; a.cnt_fs = solids.cnt_fs; a.K_fs = solids.K_fs; a.acs = solids.acs;
         a.gx = (float)g[0]; a.gy = (float)g[1]; a.gz = (float)g[2];
         a.eos = eos;
         a.gamma = (float)gamma;
@@ -879,8 +1188,8 @@ struct demsph_sph {
         DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(nsort)));
         cells = dmalloc<int2>(numGrids);
         cnt_ff = dmalloc<int>(n); cnt_fd = dmalloc<int>(n);
-        maxcount = dmalloc<int>(2);
-        DEMSPH_HIP(hipHostMalloc((void**)&maxcount_h, 2 * sizeof(int), hipHostMallocDefault));
+        maxcount = dmalloc<int>(3);
+        DEMSPH_HIP(hipHostMalloc((void**)&maxcount_h, 3 * sizeof(int), hipHostMallocDefault));
         alloc_lists();
         cP = cV = cD = cO = 0;
 
@@ -1018,7 +1327,7 @@ struct demsph_sph {
         cP ^= 1; cV ^= 1;
     }
 
-    void second_half_enqueue(double dt)
+    void continuity_enqueue(double dt)
     {
         const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
         SphArgs a = args();
@@ -1026,20 +1335,31 @@ struct demsph_sph {
         if (uniform) DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity<true>, gb, tb, 0, stream, a);
         else DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity<false>, gb, tb, 0, stream, a);
         cP ^= 1; cD ^= 1;
-        a = args();
+    }
+
+    void momentum2_enqueue(double dt)
+    {
+        const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
+        SphArgs a = args();
         a.dt = dt;
         if (uniform) DEMSPH_LAUNCH(timer, "sph_momentum_second", (k_momentum<false, true>), gb, tb, 0, stream, a);
         else DEMSPH_LAUNCH(timer, "sph_momentum_second", (k_momentum<false, false>), gb, tb, 0, stream, a);
         cV ^= 1;
     }
 
-    void reset_overflow() { DEMSPH_HIP(hipMemsetAsync(maxcount, 0, 2 * sizeof(int), stream)); }
+    void second_half_enqueue(double dt)
+    {
+        continuity_enqueue(dt);
+        momentum2_enqueue(dt);
+    }
+
+    void reset_overflow() { DEMSPH_HIP(hipMemsetAsync(maxcount, 0, 3 * sizeof(int), stream)); }
 
     // Returns true when a list overflowed since reset_overflow(); grows the
     // capacities in that case.  Synchronises the stream.
     bool check_overflow()
     {
-        DEMSPH_HIP(hipMemcpyAsync(maxcount_h, maxcount, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        DEMSPH_HIP(hipMemcpyAsync(maxcount_h, maxcount, 3 * sizeof(int), hipMemcpyDeviceToHost, stream));
         DEMSPH_HIP(hipStreamSynchronize(stream));
         timer.harvest();
         bool grew = false;
@@ -1430,5 +1750,271 @@ extern "C" int demsph_sph_timing(demsph_sph* s, int index, char* name, size_t na
         }
         if (launches) *launches = it->second.first;
         if (totalMs) *totalMs = it->second.second;
+    });
+}
+
+// ============================================================= coupling
+// DEM-SPH coupling: one fluid scene and one DEM scene advanced together on
+// the fluid scene's stream.  Each coupled step interleaves the two KDK
+// sequences so that both force evaluations see the same state:
+//   DEM step begin (pair rebuild every 5th) | fluid search | balls onto the
+//   fluid grid, fluid->ball lists, reverse index, ball pressures
+//   DEM forces + coupling reaction (start state) | DEM hook, kick, drift
+//   fluid momentum + kick + drift (start state)
+//   ball kinematics refreshed | fluid continuity | ball pressures
+//   DEM forces + coupling reaction (end state) | DEM hook, kick
+//   fluid momentum + kick (end state)
+struct demsph_couple {
+    demsph_sph* f = nullptr;
+    demsph_dem* d = nullptr;
+    bool ready = false;
+    int ns = 0;
+    int K_fs = 32;
+    uint32_t *key = nullptr, *skey = nullptr, *ktmp = nullptr;
+    int *perm = nullptr, *vtmp = nullptr;  // perm: ball slot -> DEM slot
+    void* sort_temp = nullptr;
+    int2* cells = nullptr;
+    SolidPos* SP = nullptr;
+    float4 *SV = nullptr, *SW = nullptr;
+    int *nbr_fs = nullptr, *cnt_fs = nullptr;
+    int *rcnt = nullptr, *rstart = nullptr, *rcursor = nullptr, *rfluid = nullptr;
+    void* scan_temp = nullptr;
+    double* Fc = nullptr;  // 6 per ball slot (force, torque) of the last evaluation
+    float* acs = nullptr;  // 3 per fluid slot, coupling acceleration of the last fluid pass
+
+    ~demsph_couple() { release(); }
+
+    void release()
+    {
+        if (f && f->stream) (void)hipStreamSynchronize(f->stream);
+        dfree(key); dfree(skey); dfree(ktmp); dfree(perm); dfree(vtmp);
+        if (sort_temp) (void)hipFree(sort_temp);
+        sort_temp = nullptr;
+        dfree(cells); dfree(SP); dfree(SV); dfree(SW); dfree(nbr_fs); dfree(cnt_fs);
+        dfree(rcnt); dfree(rstart); dfree(rcursor); dfree(rfluid);
+        if (scan_temp) (void)hipFree(scan_temp);
+        scan_temp = nullptr;
+        dfree(Fc); dfree(acs);
+        ready = false;
+    }
+
+    void setup()
+    {
+        f->require_init();
+        dem_require_ready(d);
+        dem_set_stream(d, f->stream);
+        const DemBallRefs b = dem_ball_refs(d);
+        if (b.n == 0) throw InvalidArg("the DEM scene has no balls");
+        if (ready && b.n == ns) return;
+        release();
+        ns = b.n;
+        K_fs = demsph_sph::round_chunk(K_fs);
+        const size_t nf = f->nf;
+        key = dmalloc<uint32_t>(ns); skey = dmalloc<uint32_t>(ns); ktmp = dmalloc<uint32_t>(ns);
+        perm = dmalloc<int>(ns); vtmp = dmalloc<int>(ns);
+        DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(ns)));
+        cells = dmalloc<int2>(f->numGrids);
+        SP = dmalloc<SolidPos>(ns); SV = dmalloc<float4>(ns); SW = dmalloc<float4>(ns);
+        nbr_fs = dmalloc<int>((size_t)K_fs * nf); cnt_fs = dmalloc<int>(nf);
+        rcnt = dmalloc<int>(ns + 1); rstart = dmalloc<int>(ns + 1); rcursor = dmalloc<int>(ns);
+        rfluid = dmalloc<int>((size_t)K_fs * nf);
+        DEMSPH_HIP(hipMalloc(&scan_temp, scan_temp_bytes(ns + 1)));
+        Fc = dmalloc<double>(6 * (size_t)ns);
+        acs = dmalloc<float>(3 * nf);
+        DEMSPH_HIP(hipMemsetAsync(Fc, 0, 6 * (size_t)ns * sizeof(double), f->stream));
+        DEMSPH_HIP(hipMemsetAsync(acs, 0, 3 * nf * sizeof(float), f->stream));
+        DEMSPH_HIP(hipMemsetAsync(SP, 0, ns * sizeof(SolidPos), f->stream));
+        // size the fluid lists before the first block (a coupled block cannot be replayed)
+        f->neighbor_search();
+        ready = true;
+    }
+
+    SolidView view() const
+    {
+        SolidView v;
+        v.ns = ns; v.SP = SP; v.SV = SV; v.SW = SW; v.nbr_fs = nbr_fs; v.cnt_fs = cnt_fs; v.K_fs = K_fs; v.acs = acs;
+        return v;
+    }
+
+    void gather(hipStream_t st, bool with_cells)
+    {
+        const DemBallRefs b = dem_ball_refs(d);
+        SolidGatherArgs g{};
+        g.ns = ns; g.numGrids = f->numGrids; g.skey = skey; g.s2d = perm;
+        g.px = b.px; g.py = b.py; g.pz = b.pz; g.vx = b.vx; g.vy = b.vy; g.vz = b.vz;
+        g.wx = b.wx; g.wy = b.wy; g.wz = b.wz; g.r = b.r;
+        g.SP = SP; g.SV = SV; g.SW = SW; g.cells = with_cells ? cells : nullptr;
+        DEMSPH_LAUNCH(f->timer, "couple_gather", k_solid_gather, dim3(blocks_for(ns, 256)), dim3(256), 0, st, g);
+    }
+
+    void pressure(hipStream_t st)
+    {
+        DEMSPH_LAUNCH(f->timer, "couple_pressure", k_solid_pressure, dim3(blocks_for(ns, 256)), dim3(256), 0, st, ns,
+                      rstart, rfluid, f->P[f->cP], f->V[f->cV], f->d_phases, SP);
+    }
+
+    // Balls onto the fluid grid, fluid->ball lists and their reverse index,
+    // ball pressures; at the start of a step, after the fluid search.
+    void prepare(hipStream_t st)
+    {
+        const DemBallRefs b = dem_ball_refs(d);
+        const int nf = f->nf;
+        timer_begin(st, "couple_sort");
+        hipLaunchKernelGGL(k_solid_hash, dim3(blocks_for(ns, 256)), dim3(256), 0, st, ns, f->grid, b.px, b.py, b.pz,
+                           key);
+        DEMSPH_HIP(hipGetLastError());
+        radix_sort_pairs(key, nullptr, skey, perm, ktmp, vtmp, sort_temp, ns, key_bits(f->numGrids), st);
+        fill_i32(reinterpret_cast<int*>(cells), 2 * f->numGrids, -1, st);
+        f->timer.end(st);
+        gather(st, true);
+        ListArgs la{};
+        la.nf = nf; la.nd = f->nd; la.grid = f->grid; la.phases = f->d_phases; la.hf = f->phf.h; la.hd = f->phd.h;
+        la.P = f->P[f->cP]; la.V = f->V[f->cV];
+        la.scells = cells; la.SP = SP; la.nbr_fs = nbr_fs; la.cnt_fs = cnt_fs; la.K_fs = K_fs; la.maxcount = f->maxcount;
+        DEMSPH_LAUNCH(f->timer, "couple_lists", k_build_fs_lists, dim3(blocks_for(nf, kThreads)), dim3(kThreads), 0,
+                      st, la);
+        timer_begin(st, "couple_reverse");
+        fill_i32(rcnt, ns + 1, 0, st);
+        hipLaunchKernelGGL(k_fs_rev_count, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, nbr_fs, cnt_fs, K_fs, rcnt);
+        DEMSPH_HIP(hipGetLastError());
+        exclusive_scan(rcnt, rstart, ns, scan_temp, rstart + ns, st);
+        fill_i32(rcursor, ns, 0, st);
+        hipLaunchKernelGGL(k_fs_rev_fill, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, nbr_fs, cnt_fs, K_fs, rstart,
+                           rcursor, rfluid);
+        hipLaunchKernelGGL(k_fs_rev_sort, dim3(blocks_for(ns, 256)), dim3(256), 0, st, ns, rstart, rfluid);
+        DEMSPH_HIP(hipGetLastError());
+        f->timer.end(st);
+        pressure(st);
+    }
+
+    void timer_begin(hipStream_t st, const char* name) { f->timer.begin(name, st); }
+
+    void forces(hipStream_t st)
+    {
+        const DemBallRefs b = dem_ball_refs(d);
+        CoupleForceArgs c{};
+        c.a = f->args();
+        c.rstart = rstart; c.rfluid = rfluid; c.s2d = perm;
+        c.fx = b.fx; c.fy = b.fy; c.fz = b.fz; c.tx = b.tx; c.ty = b.ty; c.tz = b.tz;
+        c.Fc = Fc;
+        DEMSPH_LAUNCH(f->timer, "couple_forces", k_couple_forces, dim3(blocks_for(ns, 256)), dim3(256), 0, st, c);
+    }
+
+    void step(const double* g, double dt, int nsteps)
+    {
+        if (!(dt > 0.0)) throw InvalidArg("time step must be positive");
+        setup();
+        if (nsteps <= 0) return;
+        const double h = 0.5 * dt;
+        hipStream_t st = f->stream;
+        f->reset_overflow();
+        f->solids = view();
+        try {
+            for (int k = 0; k < nsteps; k++) {
+                const double t = dem_begin_step(d, dt);
+                f->search_enqueue();
+                prepare(st);
+                dem_evaluate(d, h, dt);
+                forces(st);
+                dem_hooks_and_half(d, true, g, h, t - h);
+                f->first_half_enqueue(dt);
+                gather(st, false);
+                f->continuity_enqueue(dt);
+                pressure(st);
+                dem_evaluate(d, h, dt);
+                forces(st);
+                dem_hooks_and_half(d, false, g, h, t);
+                f->momentum2_enqueue(dt);
+            }
+        } catch (...) {
+            f->solids = SolidView();
+            throw;
+        }
+        f->solids = SolidView();
+        const int K_ff = f->K_ff, K_fd = f->K_fd;
+        const bool grew = f->check_overflow();  // reads all three list maxima, grows the fluid lists
+        const int mfs = f->maxcount_h[2];
+        if (grew)
+            throw CapacityError("fluid neighbour lists overflowed during a coupled block (capacities " +
+                                std::to_string(K_ff) + "/" + std::to_string(K_fd) +
+                                " grown for the next scene; this block's state is invalid)");
+        if (mfs > K_fs) {
+            K_fs = mfs + mfs / 4 + 8;
+            ready = false;
+            throw CapacityError("fluid-ball lists overflowed (" + std::to_string(mfs) +
+                                " balls near one fluid particle); the capacity was grown for the next scene");
+        }
+        dem_end_block(d);
+    }
+};
+
+extern "C" int demsph_couple_create(demsph_couple** out, demsph_sph* fluid, demsph_dem* dem)
+{
+    return guarded([&] {
+        if (!out || !fluid || !dem) throw InvalidArg("null argument");
+        auto* c = new demsph_couple();
+        c->f = fluid;
+        c->d = dem;
+        *out = c;
+    });
+}
+
+extern "C" int demsph_couple_destroy(demsph_couple* c)
+{
+    return guarded([&] { delete c; });
+}
+
+extern "C" int demsph_couple_set_capacity(demsph_couple* c, int ballsPerFluid)
+{
+    return guarded([&] {
+        if (!c) throw InvalidArg("null coupling");
+        if (ballsPerFluid < 1) throw InvalidArg("capacity must be positive");
+        c->K_fs = ballsPerFluid;
+        c->ready = false;
+    });
+}
+
+extern "C" int demsph_couple_step(demsph_couple* c, const double* gravity, double timeStep, int nsteps)
+{
+    return guarded([&] {
+        if (!c || !gravity) throw InvalidArg("null argument");
+        c->step(gravity, timeStep, nsteps);
+    });
+}
+
+extern "C" int demsph_couple_get(demsph_couple* c, const char* field, void* out, size_t capacity, size_t* count)
+{
+    return guarded([&] {
+        if (!c || !field) throw InvalidArg("null argument");
+        if (!c->ready) throw InvalidArg("no coupled step has run");
+        const std::string fld = field;
+        hipStream_t st = c->f->stream;
+        if (fld == "couplingForce" || fld == "couplingTorque" || fld == "ballPressure") {
+            const DemBallRefs b = dem_ball_refs(c->d);
+            auto perm = fetch(c->perm, (size_t)c->ns, st);
+            auto orig = fetch(b.orig, (size_t)b.n, st);
+            const bool p = fld == "ballPressure";
+            std::vector<double> v(p ? c->ns : 3 * (size_t)c->ns);
+            if (p) {
+                auto sp = fetch(c->SP, (size_t)c->ns, st);
+                for (int k = 0; k < c->ns; k++) v[orig[perm[k]]] = sp[k].P;
+            } else {
+                auto fc = fetch(c->Fc, 6 * (size_t)c->ns, st);
+                const int off = fld == "couplingForce" ? 0 : 3;
+                for (int k = 0; k < c->ns; k++)
+                    for (int q = 0; q < 3; q++) v[3 * (size_t)orig[perm[k]] + q] = fc[6 * (size_t)k + off + q];
+            }
+            return emit(v, out, capacity, count);
+        }
+        if (fld == "fluidCouplingAcceleration") {
+            const size_t n = c->f->nf;
+            auto a = fetch(c->acs, 3 * n, st);
+            auto orig = fetch(c->f->orig[c->f->cO], n, st);
+            std::vector<double> v(3 * n);
+            for (size_t i = 0; i < n; i++)
+                for (int q = 0; q < 3; q++) v[3 * (size_t)orig[i] + q] = a[3 * i + q];
+            return emit(v, out, capacity, count);
+        }
+        throw InvalidArg("unknown field '" + fld + "'");
     });
 }
