@@ -367,3 +367,27 @@ class OracleDEM:
     @property
     def step_count(self) -> int:
         return int(self.L.o_dem_step_count(self.h))
+
+
+class OracleCoupled:
+    """The oracle's DEM-SPH coupling (oracle_sph.c o_couple_step): one
+    OracleSPH and one OracleDEM advanced together in FP64.  The reference
+    has no coupling; this states the project's model."""
+
+    def __init__(self, fluid: "OracleSPH", dem: "OracleDEM"):
+        L = lib()
+        L.o_couple_step.argtypes = [_vp, _vp, _dp, C.c_double, C.c_int]
+        L.o_couple_get.argtypes = [_vp, C.c_char_p, _dp]
+        L.o_couple_get.restype = C.c_long
+        self.L, self.fluid, self.dem = L, fluid, dem
+
+    def step(self, dt: float, n: int = 1, gravity=(0.0, 0.0, -9.81)):
+        self.L.o_couple_step(self.fluid.h, self.dem.h, d(_a(gravity)), float(dt), int(n))
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.L.o_couple_get(self.fluid.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.float64)
+        self.L.o_couple_get(self.fluid.h, name.encode(), d(out))
+        return out if name == "ballPressure" else out.reshape(-1, 3)

@@ -1226,6 +1226,35 @@ void o_dem_step(o_dem* d, const double* g, double dt, int nsteps)
     }
 }
 
+/* ------------------------------------------------------------- coupling */
+void o_dem_couple_view(o_dem* d, o_dem_balls* v)
+{
+    v->n = (int)d->radius.n;
+    v->pos = d->pos.p; v->vel = d->vel.p; v->w = d->w.p; v->radius = d->radius.p; v->invMass = d->invMass.p;
+    v->force = d->force.p; v->torque = d->torque.p;
+}
+
+double o_dem_couple_begin(o_dem* d, double dt)
+{
+    d->iStep++;
+    d->time += dt;
+    if (d->iStep % 5 == 0) o_dem_neighbor_search(d);
+    return d->time;
+}
+
+void o_dem_couple_evaluate(o_dem* d, double h, double dt)
+{
+    o_dem_contact(d, h);
+    if (d->t0.n > 0) o_dem_triangle_contact(d, dt);
+}
+
+void o_dem_couple_kick(o_dem* d, int first, const double* g, double h, double hookTime)
+{
+    run_hooks(d, hookTime);
+    if (first) o_dem_first_half(d, g, h);
+    else o_dem_second_half(d, g, h);
+}
+
 size_t o_dem_step_count(o_dem* d) { return d->iStep; }
 
 void o_dem_grids(o_dem* d, o_grid* balls, o_grid* tris)

@@ -81,6 +81,24 @@ size_t o_dem_step_count(o_dem* d);
 /* Closed-form helpers for tests. */
 int o_sphere_triangle(const double* c, double r, const double* v0, const double* v1, const double* v2, double* cp);
 
+/* ------------------------------------------------------------ coupling
+ * Access for the DEM-SPH coupling (oracle_sph.c): the ball arrays
+ * (interleaved x3 for vectors, insertion order) and one step of loop_ball /
+ * loop_ball_wall (DEMSolver.h:1376-1405, 1452-1485) split where external
+ * forces enter. */
+typedef struct {
+    int n;
+    double *pos, *vel, *w, *radius, *invMass, *force, *torque;
+} o_dem_balls;
+void o_dem_couple_view(o_dem* d, o_dem_balls* v);
+double o_dem_couple_begin(o_dem* d, double dt);
+void o_dem_couple_evaluate(o_dem* d, double h, double dt);
+void o_dem_couple_kick(o_dem* d, int first, const double* g, double h, double hookTime);
+/* nsteps coupled steps of a fluid scene and a DEM scene (oracle_sph.c). */
+void o_couple_step(o_sph* s, o_dem* d, const double* g, double dt, int nsteps);
+/* Coupling results of the last evaluation; returns the element count. */
+long o_couple_get(o_sph* s, const char* name, double* out);
+
 #ifdef __cplusplus
 }
 #endif

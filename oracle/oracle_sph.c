@@ -16,6 +16,7 @@
 #include <string.h>
 #include <math.h>
 #include "oracle_common.h"
+#include "oracle_dem.h"
 
 int o_grid_set(o_grid* g, const double* minb, const double* maxb, double c)
 {
@@ -231,6 +232,12 @@ struct o_sph {
     long cap_fd, n_fd;
     int *count_dd, *prefix_dd, *pointed_dd, *pointing_dd;
     long cap_dd, n_dd;
+    /* coupled DEM balls (solids), insertion order, copied from the DEM scene */
+    int ns;
+    double *spos, *svel, *sw, *sr, *sP, *szero, *sF, *acs;
+    int *bhash, *bindex, *bshash;
+    int *count_fs, *prefix_fs, *pointed_fs, *pointing_fs;
+    long cap_fs, n_fs;
 };
 
 static double* dup_d(const double* src, long n, int stride)
@@ -291,7 +298,9 @@ void o_sph_destroy(o_sph* s)
                     s->hash, s->index, s->shash, s->cellStart, s->cellEnd, s->dhash, s->dindex, s->dshash,
                     s->count_ff, s->prefix_ff, s->pointed_ff, s->pointing_ff,
                     s->count_fd, s->prefix_fd, s->pointed_fd, s->pointing_fd,
-                    s->count_dd, s->prefix_dd, s->pointed_dd, s->pointing_dd};
+                    s->count_dd, s->prefix_dd, s->pointed_dd, s->pointing_dd,
+                    s->spos, s->svel, s->sw, s->sr, s->sP, s->szero, s->sF, s->acs, s->bhash, s->bindex, s->bshash,
+                    s->count_fs, s->prefix_fs, s->pointed_fs, s->pointing_fs};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(s);
 }
@@ -393,6 +402,42 @@ static double dummy_density(const o_sph* s, double PR, double cj, double rho0j)
     return PR / (cj * cj) + rho0j;
 }
 
+/* DEM-SPH coupling (no reference counterpart; the project's model, the
+ * FP64 statement of the GPU library's): coupled ball j is a solid particle of
+ * fluid i's phase with mass rho0_i 4/3 pi r_j^3, pressure P_j interpolated
+ * from the fluid, density from the inverse EOS of P_j, moving with its
+ * surface velocity v_j + w_j x (r_j n), n = unit(x_i - x_j); kernel support
+ * 2 h_i; the Riemann interface pressure of the fluid-fluid branch with
+ * P_R = P_j.  Returns fluid i's acceleration contribution split into the
+ * central pressure part *ap and the viscous part *av, and the lever r_j n. */
+static void fs_pair(const o_sph* s, int i, int j, o_vec3* ap, o_vec3* av, o_vec3* lever, double* drho_rate)
+{
+    o_vec3 ri = vld(s->pos, i), vi = vld(s->vel, i);
+    double ci = s->c[i], hi = s->h[i], nui = s->nu[i], rhoi = s->rho[i], PL = s->p[i], r0 = s->rho0[i];
+    o_vec3 rj = vld(s->spos, j), vj = vld(s->svel, j), wj = vld(s->sw, j);
+    double rad = s->sr[j], PR = s->sP[j];
+    o_vec3 rij = vsub(ri, rj);
+    o_vec3 n = vnormalize(rij);
+    o_vec3 l = vmul(n, rad);
+    o_vec3 vs = vadd(vj, vcross(wj, l));
+    o_vec3 e = vneg(n);
+    double UL = vdot(vi, e), UR = vdot(vs, e);
+    double rhoR = dummy_density(s, PR, ci, r0);
+    double mj = r0 * (4.0 / 3.0 * O_PI) * rad * rad * rad;
+    o_vec3 dw = grad_w(rij, hi);
+    double cbar = ci, rbar = 0.5 * (rhoi + rhoR);
+    double beta = fmin(3.0 * fmax(UL - UR, 0.0), cbar);
+    double Ps = 0.5 * (PL + PR) + 0.5 * beta * rbar * (UL - UR);
+    *ap = vneg(vmul(dw, 2.0 * mj * (Ps / (rhoi * rhoR))));
+    *av = vmul(vsub(vi, vs), 2.0 * nui * vdot(n, dw) * mj / (rhoi * rhoR));
+    *lever = l;
+    if (drho_rate) {
+        o_vec3 vstar = vmul(vadd(vi, vs), 0.5);
+        vstar = vadd(vstar, vmul(e, 0.5 * ((PL - PR) / (rbar * cbar))));
+        *drho_rate = mj / rhoR * vdot(vsub(vi, vstar), dw);
+    }
+}
+
 /* updateWCSPHAccelerationKernel (WCSPHIntegrationKernel.cu:42-148) */
 static void acceleration(o_sph* s)
 {
@@ -437,6 +482,17 @@ static void acceleration(o_sph* s)
             double nuij = 0.5 * (nui + nuj);
             a = vadd(a, vmul(vsub(vi, vj), 2.0 * nuij * vdot(vnormalize(rij), dw) * mj / (rhoi * rhoj)));
         }
+        if (s->ns > 0) {
+            o_vec3 ac = v3(0, 0, 0);
+            st = i > 0 ? s->prefix_fs[i - 1] : 0; en = s->prefix_fs[i];
+            for (int k = st; k < en; k++) {
+                o_vec3 ap, av, l;
+                fs_pair(s, i, s->pointing_fs[k], &ap, &av, &l, NULL);
+                ac = vadd(ac, vadd(ap, av));
+            }
+            vst(s->acs, i, ac);
+            a = vadd(a, ac);
+        }
         vst(s->acc, i, vadd(a, s->g));
     }
 }
@@ -478,6 +534,15 @@ static void density_change(o_sph* s, double dt)
             o_vec3 vs = vmul(vadd(vi, vj), 0.5);
             vs = vadd(vs, vmul(e, 0.5 * ((PL - PR) / (rbar * cbar))));
             d += mj / rhoj * vdot(vsub(vi, vs), dw);
+        }
+        if (s->ns > 0) {
+            st = i > 0 ? s->prefix_fs[i - 1] : 0; en = s->prefix_fs[i];
+            for (int k = st; k < en; k++) {
+                o_vec3 ap, av, l;
+                double rate;
+                fs_pair(s, i, s->pointing_fs[k], &ap, &av, &l, &rate);
+                d += rate;
+            }
         }
         d *= 2.0 * rhoi;
         s->drho[i] += d * dt;
@@ -597,3 +662,143 @@ int o_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------- coupling */
+static void couple_load(o_sph* s, o_dem* d)
+{
+    o_dem_balls b;
+    o_dem_couple_view(d, &b);
+    if (s->ns != b.n) {
+        void* old[] = {s->spos, s->svel, s->sw, s->sr, s->sP, s->szero, s->sF, s->acs, s->bhash, s->bindex,
+                       s->bshash};
+        for (size_t q = 0; q < sizeof(old) / sizeof(old[0]); q++) free(old[q]);
+        s->ns = b.n;
+        int na = b.n > 0 ? b.n : 1;
+        s->spos = dup_d(NULL, na, 3); s->svel = dup_d(NULL, na, 3); s->sw = dup_d(NULL, na, 3);
+        s->sr = dup_d(NULL, na, 1); s->sP = dup_d(NULL, na, 1); s->szero = dup_d(NULL, na, 1);
+        s->sF = dup_d(NULL, na, 6); s->acs = dup_d(NULL, s->nf > 0 ? s->nf : 1, 3);
+        s->bhash = (int*)malloc(sizeof(int) * na); s->bindex = (int*)malloc(sizeof(int) * na);
+        s->bshash = (int*)malloc(sizeof(int) * na);
+        if (!s->count_fs) {
+            int nfa = s->nf > 0 ? s->nf : 1;
+            s->count_fs = (int*)calloc(nfa, sizeof(int)); s->prefix_fs = (int*)calloc(nfa, sizeof(int));
+        }
+    }
+    memcpy(s->spos, b.pos, sizeof(double) * 3 * (size_t)b.n);
+    memcpy(s->svel, b.vel, sizeof(double) * 3 * (size_t)b.n);
+    memcpy(s->sw, b.w, sizeof(double) * 3 * (size_t)b.n);
+    memcpy(s->sr, b.radius, sizeof(double) * (size_t)b.n);
+}
+
+/* Fluid -> ball lists on the fluid grid: every ball within 2 h_i of fluid
+ * particle i (hB = 0 makes the cut 2 max(h_i, 0)). */
+static void couple_lists(o_sph* s)
+{
+    build_cells(s, s->spos, s->ns, s->bhash, s->bshash, s->bindex);
+    build_list(s, s->pos, s->h, s->nf, s->spos, s->szero, s->bindex, 0, s->count_fs, s->prefix_fs,
+               &s->pointed_fs, &s->pointing_fs, &s->cap_fs, &s->n_fs);
+}
+
+/* Shepard interpolation of the fluid pressure at each ball centre over the
+ * fluid particles that list the ball. */
+static void couple_pressure(o_sph* s)
+{
+    double* num = (double*)calloc((size_t)s->ns, sizeof(double));
+    double* den = (double*)calloc((size_t)s->ns, sizeof(double));
+    for (int i = 0; i < s->nf; i++) {
+        int st = i > 0 ? s->prefix_fs[i - 1] : 0, en = s->prefix_fs[i];
+        for (int k = st; k < en; k++) {
+            int j = s->pointing_fs[k];
+            o_vec3 rij = vsub(vld(s->pos, i), vld(s->spos, j));
+            double w = o_wendland(sqrt(vdot(rij, rij)), s->h[i]) * (s->m[i] / s->rho[i]);
+            num[j] += w * s->p[i];
+            den[j] += w;
+        }
+    }
+    for (int j = 0; j < s->ns; j++) s->sP[j] = den[j] > 0.0 ? num[j] / den[j] : 0.0;
+    free(num);
+    free(den);
+}
+
+/* The reaction -m_i (ap + av) and torque (r_j n) x (-m_i av) of every
+ * fluid-ball pair on the ball, added to the DEM accumulators. */
+static void couple_forces(o_sph* s, o_dem* d)
+{
+    o_dem_balls b;
+    o_dem_couple_view(d, &b);
+    memset(s->sF, 0, sizeof(double) * 6 * (size_t)s->ns);
+    for (int i = 0; i < s->nf; i++) {
+        int st = i > 0 ? s->prefix_fs[i - 1] : 0, en = s->prefix_fs[i];
+        for (int k = st; k < en; k++) {
+            int j = s->pointing_fs[k];
+            o_vec3 ap, av, l;
+            fs_pair(s, i, j, &ap, &av, &l, NULL);
+            o_vec3 F = vmul(vadd(ap, av), -s->m[i]);
+            o_vec3 T = vcross(l, vmul(av, -s->m[i]));
+            for (int q = 0; q < 3; q++) {
+                s->sF[6 * j + q] += (&F.x)[q];
+                s->sF[6 * j + 3 + q] += (&T.x)[q];
+            }
+        }
+    }
+    for (int j = 0; j < s->ns; j++)
+        for (int q = 0; q < 3; q++) {
+            b.force[3 * j + q] += s->sF[6 * j + q];
+            b.torque[3 * j + q] += s->sF[6 * j + 3 + q];
+        }
+}
+
+/* One coupled step per iteration, the sequence of the GPU library's
+ * demsph_couple_step: DEM step begin | fluid search | balls, lists,
+ * pressures | DEM forces + reaction | DEM hook, kick, drift | fluid
+ * momentum, kick, drift | ball kinematics | continuity + EOS | pressures |
+ * DEM forces + reaction | DEM hook, kick | fluid momentum, kick. */
+void o_couple_step(o_sph* s, o_dem* d, const double* g, double dt, int nsteps)
+{
+    double h = 0.5 * dt;
+    for (int k = 0; k < nsteps; k++) {
+        double t = o_dem_couple_begin(d, dt);
+        o_sph_neighbor_search(s);
+        couple_load(s, d);
+        couple_lists(s);
+        couple_pressure(s);
+        o_dem_couple_evaluate(d, h, dt);
+        couple_forces(s, d);
+        o_dem_couple_kick(d, 1, g, h, t - h);
+        acceleration(s);
+        kick(s, h);
+        drift(s, dt);
+        couple_load(s, d);
+        density_change(s, dt);
+        eos(s);
+        couple_pressure(s);
+        o_dem_couple_evaluate(d, h, dt);
+        couple_forces(s, d);
+        o_dem_couple_kick(d, 0, g, h, t);
+        acceleration(s);
+        kick(s, h);
+    }
+}
+
+/* "couplingForce" "couplingTorque" (3 per ball), "ballPressure" (1 per
+ * ball), "fluidCouplingAcceleration" (3 per fluid particle). */
+long o_couple_get(o_sph* s, const char* name, double* out)
+{
+    if (!strcmp(name, "couplingForce") || !strcmp(name, "couplingTorque")) {
+        int off = name[8] == 'F' ? 0 : 3;
+        if (out)
+            for (int j = 0; j < s->ns; j++)
+                for (int q = 0; q < 3; q++) out[3 * j + q] = s->sF[6 * j + off + q];
+        return 3L * s->ns;
+    }
+    if (!strcmp(name, "ballPressure")) {
+        if (out) memcpy(out, s->sP, sizeof(double) * (size_t)s->ns);
+        return s->ns;
+    }
+    if (!strcmp(name, "fluidCouplingAcceleration")) {
+        if (out) memcpy(out, s->acs, sizeof(double) * 3 * (size_t)s->nf);
+        return 3L * s->nf;
+    }
+    return -1;
+}
+

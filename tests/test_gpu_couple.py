@@ -14,11 +14,6 @@ from this_studio_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def lattice(nx, ny, nz, dx, origin=(0.0, 0.0, 0.0)):
-    z, y, x = np.meshgrid(np.arange(nz) * dx, np.arange(ny) * dx, np.arange(nx) * dx, indexing="ij")
-    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1) + np.asarray(origin)
-
-
 def fluid_scene(spec, drho=None):
     s = ts.SPHScene()
     s.set_eos(spec.eos, spec.gamma)
@@ -32,29 +27,8 @@ def fluid_scene(spec, drho=None):
 
 
 def column_with_ball(velocity=None, gravity=(0.0, 0.0, -9.81)):
-    """A 17 x 17 x 25 lattice at dx = 1 cm with the centre particle replaced
-    by a fixed ball of the same volume (r = dx (3/(4 pi))^(1/3)); at rest the
-    column carries the hydrostatic pressure rho0 |g| (H - z)."""
-    dx, rho0, c0 = 0.01, 1000.0, 20.0
-    pos = lattice(17, 17, 25, dx)
-    centre = np.array([8, 8, 12]) * dx
-    keep = np.linalg.norm(pos - centre, axis=1) > 1e-9
-    pos = pos[keep]
-    g = np.asarray(gravity, float)
-    H = pos[:, 2].max() + 0.5 * dx
-    vel = None if velocity is None else np.tile(np.asarray(velocity, float), (pos.shape[0], 1))
-    spec = scenes.SPHSceneSpec(fluid_pos=pos, dummy_pos=np.zeros((0, 3)), dx=dx, rho0=rho0, c0=c0, nu=1e-6,
-                               min_bound=np.array([-0.05, -0.05, -0.05]), max_bound=np.array([0.22, 0.22, 0.3]),
-                               gravity=g, dt=1e-6, fluid_vel=vel)
-    p = rho0 * abs(g[2]) * (H - pos[:, 2])
-    fluid = fluid_scene(spec, p / c0 ** 2)
-    r = dx * (3.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
-    d = scenes.DEMScene("ball")
-    d.add_fixed_balls(centre, r, 0)
-    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
-    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, g
-    dem = ts.DEMSceneGPU(d)
-    return spec, fluid, dem, r
+    spec, drho, d, r = scenes.column_with_ball(velocity, gravity)
+    return spec, fluid_scene(spec, drho), ts.DEMSceneGPU(d), r
 
 
 def test_ball_at_rest_in_hydrostatic_column_feels_archimedes():
@@ -81,18 +55,8 @@ def test_fixed_ball_in_uniform_stream_is_pushed_downstream():
 
 
 def falling_balls():
-    spec = scenes.small_dam_break()
-    fluid = fluid_scene(spec)
-    d = scenes.DEMScene("balls in water")
-    lo = spec.fluid_pos.min(axis=0)
-    hi = spec.fluid_pos.max(axis=0)
-    centres = lattice(3, 2, 2, 0.025, origin=lo + np.array([0.02, 0.015, 0.03]))
-    assert np.all(centres < hi)
-    d.add_balls(centres, [0.0, 0.0, -0.1], [0.0, 0.0, 0.0], 0.006, 2500.0, 0)
-    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
-    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
-    dem = ts.DEMSceneGPU(d)
-    return spec, fluid, dem
+    spec, d = scenes.balls_in_dam_break()
+    return spec, fluid_scene(spec), ts.DEMSceneGPU(d)
 
 
 def test_momentum_exchange_is_pairwise():
@@ -122,3 +86,24 @@ def test_coupled_dam_break_with_balls_stays_finite_and_drags_balls():
     t = 50 * spec.dt
     free = v0 - 9.81 * t
     assert np.all(vel[:, 2] > free)
+
+
+def test_coupled_steps_match_oracle():
+    """Twelve balls sinking in the small dam break, 10 coupled steps: the
+    GPU (FP32 pair arithmetic) against the FP64 oracle of the same model."""
+    import oracle
+
+    spec, d = scenes.balls_in_dam_break()
+    of = oracle.OracleSPH(spec)
+    od = oracle.OracleDEM(d)
+    oc = oracle.OracleCoupled(of, od)
+    gf, gd = fluid_scene(spec), ts.DEMSceneGPU(d)
+    gc = ts.CoupledScene(gf, gd)
+    oc.step(spec.dt, 10, spec.gravity)
+    gc.step(spec.dt, 10, spec.gravity)
+    Fo, Fg = oc.get("couplingForce"), gc.get("couplingForce")
+    assert np.abs(Fg - Fo).max() < 1e-4 * np.abs(Fo).max()
+    assert np.abs(gd.get("position") - od.get("pos")).max() < 1e-9
+    assert np.abs(gd.get("velocity") - od.get("vel")).max() < 1e-5 * 0.1
+    assert np.abs(gf.get("position") - of.get("pos")).max() < 1e-9
+    assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5

@@ -432,3 +432,50 @@ def granular_collapse(n_side: int = 16, seed: int = 20261202) -> DEMScene:
     s.max_bound = np.array([L, L, L])
     s.dt = 1e-5
     return s
+
+
+# ---------------------------------------------------------------------------
+# Coupled DEM-SPH scenes
+# ---------------------------------------------------------------------------
+def lattice(nx: int, ny: int, nz: int, dx: float, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Simple cubic lattice, x fastest."""
+    z, y, x = np.meshgrid(np.arange(nz) * dx, np.arange(ny) * dx, np.arange(nx) * dx, indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1) + np.asarray(origin, dtype=np.float64)
+
+
+def column_with_ball(velocity=None, gravity=(0.0, 0.0, -9.81)):
+    """A 17 x 17 x 25 fluid lattice at dx = 1 cm (rho0 = 1000, c0 = 20,
+    linear EOS) whose centre particle is replaced by a fixed ball of the same
+    volume, r = dx (3/(4 pi))^(1/3).  Returns (spec, drho, ball scene, r):
+    drho is the hydrostatic density perturbation rho0 |g| (H - z) / c0^2 of
+    a column at rest."""
+    dx, rho0, c0 = 0.01, 1000.0, 20.0
+    pos = lattice(17, 17, 25, dx)
+    centre = np.array([8, 8, 12]) * dx
+    pos = pos[np.linalg.norm(pos - centre, axis=1) > 1e-9]
+    g = np.asarray(gravity, dtype=np.float64)
+    H = pos[:, 2].max() + 0.5 * dx
+    vel = None if velocity is None else np.tile(np.asarray(velocity, float), (pos.shape[0], 1))
+    spec = SPHSceneSpec(fluid_pos=pos, dummy_pos=np.zeros((0, 3)), dx=dx, rho0=rho0, c0=c0, nu=1e-6,
+                        min_bound=np.array([-0.05, -0.05, -0.05]), max_bound=np.array([0.22, 0.22, 0.3]),
+                        gravity=g, dt=1e-6, fluid_vel=vel)
+    drho = rho0 * abs(g[2]) * (H - pos[:, 2]) / c0 ** 2
+    r = dx * (3.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
+    d = DEMScene("ball in a column")
+    d.add_fixed_balls(centre, r, 0)
+    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, g
+    return spec, drho, d, r
+
+
+def balls_in_dam_break():
+    """small_dam_break() with twelve 6 mm balls (2500 kg/m^3) sinking at
+    0.1 m/s inside the water column."""
+    spec = small_dam_break()
+    lo = spec.fluid_pos.min(axis=0)
+    d = DEMScene("balls in water")
+    centres = lattice(3, 2, 2, 0.025, origin=lo + np.array([0.02, 0.015, 0.03]))
+    d.add_balls(centres, [0.0, 0.0, -0.1], [0.0, 0.0, 0.0], 0.006, 2500.0, 0)
+    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
+    return spec, d
