@@ -132,6 +132,16 @@ int demsph_sph_set(demsph_sph* s, const char* field, const void* in, size_t coun
 int demsph_sph_enable_timing(demsph_sph* s, int enable);
 int demsph_sph_timing(demsph_sph* s, int index, char* name, size_t nameCap, long* launches, double* totalMs);
 
+/* Step graphs (no counterpart in the reference, whose solve() loop launches
+ * every kernel from the host): demsph_sph_step runs pairs of steps as
+ * replays of a captured two-step hipGraph, the first step after
+ * initialisation and an odd remainder eagerly.  On by default; off while
+ * kernel timing is enabled.  demsph_sph_graph_captures counts the captures
+ * made so far (one per time step size and buffer parity until the lists are
+ * reallocated). */
+int demsph_sph_set_graphs(demsph_sph* s, int enable);
+int demsph_sph_graph_captures(demsph_sph* s, long* count);
+
 /* ------------------------------------------------------------------ DEM
  * Opaque DEMSolver scene (kernel/DEMSolver.h:144-2079). */
 typedef struct demsph_dem demsph_dem;

@@ -132,6 +132,24 @@ def test_200_steps_match_oracle(small):
     assert dp / small.dx < 1e-5
 
 
+def test_graph_replays_equal_eager_steps(small):
+    """Two-step graph replays against eager launches, bit for bit, over
+    several blocks with host work and extra neighbour searches (which flip
+    the buffer parity and so exercise a second captured graph) in between."""
+    a = make_gpu(small)
+    b = make_gpu(small)
+    b.set_graphs(False)
+    for nsteps in (6, 5, 8, 3, 4):
+        a.step(small.dt, nsteps)
+        b.step(small.dt, nsteps)
+        for f in ("position", "velocity", "densityChange"):
+            assert np.array_equal(a.get(f), b.get(f)), (nsteps, f)
+        a.neighbor_search()
+        b.neighbor_search()
+    assert a.graph_captures() >= 2
+    assert b.graph_captures() == 0
+
+
 def test_tait_eos_matches_oracle():
     spec = scenes.small_dam_break(eos=1)
     o = oracle.OracleSPH(spec)

@@ -67,6 +67,8 @@ def _declare(L: C.CDLL) -> None:
         "demsph_sph_set": [_vp, C.c_char_p, _vp, _sz],
         "demsph_sph_enable_timing": [_vp, C.c_int],
         "demsph_sph_timing": [_vp, C.c_int, C.c_char_p, _sz, C.POINTER(C.c_long), C.POINTER(C.c_double)],
+        "demsph_sph_set_graphs": [_vp, C.c_int],
+        "demsph_sph_graph_captures": [_vp, C.POINTER(C.c_long)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -178,6 +180,15 @@ class SPHScene:
 
     def enable_timing(self, on: bool = True):
         check(lib().demsph_sph_enable_timing(self._h, 1 if on else 0))
+
+    def set_graphs(self, on: bool = True):
+        """Replay captured two-step graphs in step() (on by default)."""
+        check(lib().demsph_sph_set_graphs(self._h, 1 if on else 0))
+
+    def graph_captures(self) -> int:
+        n = C.c_long()
+        check(lib().demsph_sph_graph_captures(self._h, C.byref(n)))
+        return n.value
 
     def timing(self) -> dict:
         out = {}

@@ -230,7 +230,8 @@ size_t scan_temp_bytes(int n)
 void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, hipStream_t s)
 {
     if (n <= 0) {
-        if (total_dev) DEMSPH_HIP(hipMemsetAsync(total_dev, 0, sizeof(int), s));
+        // A fill kernel rather than a memset node keeps the scan capturable.
+        if (total_dev) fill_i32(total_dev, 1, 0, s);
         return;
     }
     const int m = (n + kScanTile - 1) / kScanTile;

@@ -1057,12 +1057,43 @@ struct demsph_sph {
     float* ck_drho = nullptr;
     int *ck_orig = nullptr, *ck_rank = nullptr;
     KernelTimer timer;
+    // Captured two-step graphs.  A step flips the P, V and orig buffers once
+    // and drho twice, so two steps return every ping-pong index to where it
+    // started and a replay sees the same buffers as the capture.  Everything
+    // else a capture bakes in (pointers, sizes, grid, phases, capacities) is
+    // fixed between alloc_lists() / release(), which drop the cache.
+    struct GraphKey {
+        double dt;
+        int parity;  // cP | cV << 1 | cD << 2 | cO << 3
+        int ns, K_fs;
+        const void *SP, *nbr_fs;
+        bool operator==(const GraphKey& o) const
+        {
+            return dt == o.dt && parity == o.parity && ns == o.ns && K_fs == o.K_fs && SP == o.SP &&
+                   nbr_fs == o.nbr_fs;
+        }
+    };
+    struct GraphEntry {
+        GraphKey key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    bool use_graphs = true;
+    long graph_captures = 0;
 
     ~demsph_sph() { release(); }
+
+    void drop_graphs()
+    {
+        if (!graphs.empty() && stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
+        graphs.clear();
+    }
 
     void release()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (int b = 0; b < 2; b++) {
             dfree(P[b]); dfree(V[b]); dfree(vx[b]); dfree(vy[b]); dfree(vz[b]); dfree(drho[b]); dfree(orig[b]);
         }
@@ -1126,6 +1157,7 @@ struct demsph_sph {
 
     void alloc_lists()
     {
+        drop_graphs();
         dfree(nbr_ff); dfree(nbr_fd);
         K_ff = round_chunk(K_ff);
         K_fd = round_chunk(K_fd);
@@ -1353,7 +1385,9 @@ struct demsph_sph {
         momentum2_enqueue(dt);
     }
 
-    void reset_overflow() { DEMSPH_HIP(hipMemsetAsync(maxcount, 0, 3 * sizeof(int), stream)); }
+    // A fill kernel rather than a memset node, so that the reset may sit in a
+    // captured graph.
+    void reset_overflow() { fill_i32(maxcount, 3, 0, stream); }
 
     // Returns true when a list overflowed since reset_overflow(); grows the
     // capacities in that case.  Synchronises the stream.
@@ -1405,6 +1439,76 @@ struct demsph_sph {
         }
     }
 
+    void one_step(double dt)
+    {
+        search_enqueue();
+        first_half_enqueue(dt);
+        second_half_enqueue(dt);
+    }
+
+    GraphKey graph_key(double dt) const
+    {
+        GraphKey k{};
+        k.dt = dt;
+        k.parity = cP | cV << 1 | cD << 2 | cO << 3;
+        k.ns = solids.ns;
+        k.K_fs = solids.K_fs;
+        k.SP = solids.SP;
+        k.nbr_fs = solids.nbr_fs;
+        return k;
+    }
+
+    // The executable graph of two steps from the current state, captured on
+    // first use.  Capturing enqueues nothing; the caller launches the graph.
+    hipGraphExec_t pair_graph(double dt)
+    {
+        const GraphKey key = graph_key(dt);
+        for (auto& e : graphs)
+            if (e.key == key) return e.exec;
+        const int sP = cP, sV = cV, sD = cD, sO = cO;
+        hipGraph_t g = nullptr;
+        DEMSPH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            one_step(dt);
+            one_step(dt);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            cP = sP; cV = sV; cD = sD; cO = sO;
+            throw;
+        }
+        DEMSPH_HIP(hipStreamEndCapture(stream, &g));
+        if (cP != sP || cV != sV || cD != sD || cO != sO) {
+            (void)hipGraphDestroy(g);
+            throw std::logic_error("two steps did not restore the buffer parity");
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DEMSPH_HIP(e);
+        if (graphs.size() >= 4) {
+            DEMSPH_HIP(hipStreamSynchronize(stream));
+            (void)hipGraphExecDestroy(graphs.front().exec);
+            graphs.erase(graphs.begin());
+        }
+        graphs.push_back({key, ex});
+        graph_captures++;
+        return ex;
+    }
+
+    // nsteps steps: eagerly while the cell ranges are not yet tied to the
+    // sorted keys (the first step after initialisation) and for an odd
+    // remainder, as replays of the two-step graph otherwise.
+    void enqueue_steps(double dt, int nsteps)
+    {
+        int k = 0;
+        if (use_graphs && !timer.enabled) {
+            if (!cells_track_skey && k < nsteps) { one_step(dt); k++; }
+            for (; k + 2 <= nsteps; k += 2) DEMSPH_HIP(hipGraphLaunch(pair_graph(dt), stream));
+        }
+        for (; k < nsteps; k++) one_step(dt);
+    }
+
     void step(double dt, int nsteps)
     {
         require_init();
@@ -1415,11 +1519,7 @@ struct demsph_sph {
             checkpoint(true);
             const int sP = cP, sV = cV, sD = cD, sO = cO;
             reset_overflow();
-            for (int k = 0; k < nsteps; k++) {
-                search_enqueue();
-                first_half_enqueue(dt);
-                second_half_enqueue(dt);
-            }
+            enqueue_steps(dt, nsteps);
             if (!check_overflow()) return;
             // Replay the block from the checkpoint with larger lists.
             cP = sP; cV = sV; cD = sD; cO = sO;
@@ -1731,6 +1831,23 @@ extern "C" int demsph_sph_enable_timing(demsph_sph* s, int enable)
         s->timer.harvest();
         s->timer.enabled = enable != 0;
         s->timer.reset();
+    });
+}
+
+extern "C" int demsph_sph_set_graphs(demsph_sph* s, int enable)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArg("null scene");
+        s->use_graphs = enable != 0;
+        if (!s->use_graphs) s->drop_graphs();
+    });
+}
+
+extern "C" int demsph_sph_graph_captures(demsph_sph* s, long* count)
+{
+    return guarded([&] {
+        if (!s || !count) throw InvalidArg("null argument");
+        *count = s->graph_captures;
     });
 }
 
