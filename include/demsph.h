@@ -304,6 +304,64 @@ int demsph_couple_step(demsph_couple* c, const double* gravity, double timeStep,
  * the coupling part of the last fluid momentum pass). */
 int demsph_couple_get(demsph_couple* c, const char* field, void* out, size_t capacity, size_t* count);
 
+/* ------------------------------------------------------------ serving
+ * Batched serving of independent scenes on one GPU.  No counterpart in the
+ * reference, which runs one scene per process with host synchronisation
+ * inside its step loop (interaction.h:164-175).
+ *
+ * A job is a staged, not yet initialised WCSPH scene, DEM scene, or both
+ * (coupled), with its bounds, gravity, time step, step count and arrival
+ * time (seconds after demsph_server_run starts).  Submitting hands the scene
+ * handles to the server, which destroys them when the job ends.  Jobs are
+ * admitted in arrival order (FIFO) once they have arrived, one of `slots`
+ * slots is free and their estimated device memory fits `memoryBudgetBytes`
+ * beside the admitted jobs; a job is initialised on admission, advanced on
+ * its own stream in blocks of steps (WCSPH by two-step graph replays when
+ * `graphs` is non-zero), its final state is copied to the host and its
+ * memory released.  Each step ends with a device timestamp; a job's step
+ * latencies are the intervals between consecutive step completions.
+ * slots = 1 with graphs = 0 serves the same jobs one at a time with one
+ * launch per stage from the host (the plain path). */
+typedef struct demsph_server demsph_server;
+typedef struct {
+    int kind;           /* 0 WCSPH, 1 DEM, 2 coupled */
+    int status;         /* DEMSPH_OK or the job's error code */
+    long particles;     /* fluid particles + balls */
+    long steps, stepsDone;
+    size_t bytes;       /* admission estimate of the job's device memory */
+    /* seconds after run() started (-1: never): arrival, admission, first
+     * step enqueued (set-up done), last step completed, memory released */
+    double arrival, admitted, started, stepped, finished;
+} demsph_job_info;
+typedef struct {
+    double wall;                    /* run() duration, s */
+    double particleSteps;           /* sum over completed jobs of particles * steps */
+    double particleStepsPerSecond;  /* particleSteps / wall */
+    double p50, p99;                /* step latency percentiles over all completed jobs, s */
+    int jobsDone, jobsFailed;
+    size_t peakBytes;               /* peak admitted memory estimate */
+    int peakActive;                 /* peak number of jobs on the device together */
+} demsph_server_summary;
+int demsph_server_create(demsph_server** out, int slots, double memoryBudgetBytes, int graphs);
+int demsph_server_destroy(demsph_server* v);
+/* Steps per step() call inside a job (default 100). */
+int demsph_server_set_block(demsph_server* v, long steps);
+/* fluid and/or dem non-NULL; fluidMin/Max with a fluid, demMin/Max with a
+ * DEM scene (a coupled DEM scene usually shares the fluid bounds). */
+int demsph_server_submit(demsph_server* v, demsph_sph* fluid, demsph_dem* dem, const double* fluidMin,
+                         const double* fluidMax, const double* demMin, const double* demMax, const double* gravity,
+                         double timeStep, long steps, double arrival, int* jobId);
+/* Serves every submitted job; returns when all have ended.  Once per server. */
+int demsph_server_run(demsph_server* v);
+int demsph_server_job_info(demsph_server* v, int jobId, demsph_job_info* info, char* error, size_t errorCap);
+/* stepsDone - 1 intervals, s. */
+int demsph_server_job_latencies(demsph_server* v, int jobId, double* out, size_t capacity, size_t* count);
+/* Final state in insertion order: "position" "velocity" (fluid), "ballPosition"
+ * "ballVelocity" (DEM); double, 3 per particle. */
+int demsph_server_job_result(demsph_server* v, int jobId, const char* field, double* out, size_t capacity,
+                             size_t* count);
+int demsph_server_stats(demsph_server* v, demsph_server_summary* s);
+
 #ifdef __cplusplus
 }
 #endif

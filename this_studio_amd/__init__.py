@@ -300,7 +300,11 @@ class DEMSceneGPU:
     DEMSPH/kernel/DEMSolver.h:144-2079) through the C ABI, built from a
     this_studio_amd.scenes.DEMScene recording."""
 
-    def __init__(self, scene=None, fp64: bool = False, solve_begin: bool = True, stream: int | None = None):
+    def __init__(self, scene=None, fp64: bool = False, solve_begin: bool = True, stream: int | None = None,
+                 initialize: bool = True):
+        """initialize=False stages the scene's materials, balls and walls on
+        the host only (a job for Server.submit, which initialises it on
+        admission); hooks and bonds need an initialised scene."""
         L = lib()
         _declare_dem(L)
         self.L = L
@@ -310,11 +314,13 @@ class DEMSceneGPU:
         check(L.demsph_dem_set_precision(self._h, 1 if fp64 else 0))
         self.scene = scene
         if scene is not None:
-            self._replay(scene)
-            if solve_begin:
+            if not initialize and (scene.hooks or scene.bonds_from_initial_pairs or scene.explicit_bonds is not None):
+                raise ValueError("hooks and bonds need an initialised scene")
+            self._replay(scene, initialize)
+            if solve_begin and initialize:
                 self.solve_begin()
 
-    def _replay(self, s):
+    def _replay(self, s, initialize: bool = True):
         L, h = self.L, self._h
         for kind, a, b, p in s.materials:
             fn = {"linear": L.demsph_dem_set_linear_pair, "hertz": L.demsph_dem_set_hertzian_pair,
@@ -347,6 +353,8 @@ class DEMSceneGPU:
                                                  t0.ctypes.data_as(_ip), t1.ctypes.data_as(_ip),
                                                  t2.ctypes.data_as(_ip), _d(_f64(o["pos"], 3)), _d(_f64(o["vel"], 3)),
                                                  _d(_f64(o["w"], 3)), int(o["mat"])))
+        if not initialize:
+            return
         check(L.demsph_dem_initialize(h, _d(_f64(s.min_bound, 3)), _d(_f64(s.max_bound, 3))))
         n = self.sizes()[0]
         for hk in s.hooks:
@@ -500,3 +508,119 @@ class CoupledScene:
             out = out.reshape(-1, 3)
         return out
 
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("status", C.c_int), ("particles", C.c_long), ("steps", C.c_long),
+                ("stepsDone", C.c_long), ("bytes", C.c_size_t), ("arrival", C.c_double), ("admitted", C.c_double),
+                ("started", C.c_double), ("stepped", C.c_double), ("finished", C.c_double)]
+
+
+class ServerSummary(C.Structure):
+    _fields_ = [("wall", C.c_double), ("particleSteps", C.c_double), ("particleStepsPerSecond", C.c_double),
+                ("p50", C.c_double), ("p99", C.c_double), ("jobsDone", C.c_int), ("jobsFailed", C.c_int),
+                ("peakBytes", C.c_size_t), ("peakActive", C.c_int)]
+
+
+_SERVER_DECLARED = False
+
+
+def _declare_server(L):
+    global _SERVER_DECLARED
+    if _SERVER_DECLARED:
+        return
+    sig = {
+        "demsph_server_create": [C.POINTER(_vp), C.c_int, C.c_double, C.c_int],
+        "demsph_server_destroy": [_vp],
+        "demsph_server_set_block": [_vp, C.c_long],
+        "demsph_server_submit": [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_long, C.c_double,
+                                 C.POINTER(C.c_int)],
+        "demsph_server_run": [_vp],
+        "demsph_server_job_info": [_vp, C.c_int, C.POINTER(JobInfo), C.c_char_p, _sz],
+        "demsph_server_job_latencies": [_vp, C.c_int, _dp, _sz, C.POINTER(_sz)],
+        "demsph_server_job_result": [_vp, C.c_int, C.c_char_p, _dp, _sz, C.POINTER(_sz)],
+        "demsph_server_stats": [_vp, C.POINTER(ServerSummary)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _SERVER_DECLARED = True
+
+
+class Server:
+    """Batched serving of independent scenes on one GPU (demsph_server_*):
+    FIFO admission by arrival time, free slot and device-memory budget, each
+    admitted job on its own stream, per-step device timestamps.  slots=1,
+    graphs=False is the plain one-scene-at-a-time path."""
+
+    KINDS = ("wcsph", "dem", "coupled")
+
+    def __init__(self, slots: int = 4, memory_budget: float = 288e9, graphs: bool = True, block: int = 100):
+        L = lib()
+        _declare_server(L)
+        self.L = L
+        self._h = _vp()
+        check(L.demsph_server_create(C.byref(self._h), int(slots), float(memory_budget), 1 if graphs else 0))
+        check(L.demsph_server_set_block(self._h, int(block)))
+
+    def close(self) -> None:
+        if self._h:
+            check(self.L.demsph_server_destroy(self._h))
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, fluid: "SPHScene | None", dem: "DEMSceneGPU | None", gravity, dt: float, steps: int,
+               arrival: float = 0.0, fluid_bounds=None, dem_bounds=None) -> int:
+        """Hands the staged (not initialised) scenes to the server; bounds are
+        (min, max) pairs.  Returns the job id."""
+        fb = [_f64(x, 3) for x in fluid_bounds] if fluid is not None else [None, None]
+        db = [_f64(x, 3) for x in dem_bounds] if dem is not None else [None, None]
+        g = _f64(gravity, 3)
+        jid = C.c_int()
+        p = lambda a: _d(a) if a is not None else None
+        check(self.L.demsph_server_submit(self._h, fluid._h if fluid is not None else None,
+                                          dem._h if dem is not None else None, p(fb[0]), p(fb[1]), p(db[0]),
+                                          p(db[1]), _d(g), float(dt), int(steps), float(arrival), C.byref(jid)))
+        # the server owns the handles now
+        if fluid is not None:
+            fluid._h = _vp()
+        if dem is not None:
+            dem._h = _vp()
+        return jid.value
+
+    def run(self) -> None:
+        check(self.L.demsph_server_run(self._h))
+
+    def info(self, job: int) -> dict:
+        i = JobInfo()
+        err = C.create_string_buffer(512)
+        check(self.L.demsph_server_job_info(self._h, int(job), C.byref(i), err, 512))
+        return {"kind": self.KINDS[i.kind], "status": i.status, "error": err.value.decode(),
+                "particles": i.particles, "steps": i.steps, "steps_done": i.stepsDone, "bytes": i.bytes,
+                "arrival": i.arrival, "admitted": i.admitted, "started": i.started, "stepped": i.stepped,
+                "finished": i.finished}
+
+    def latencies(self, job: int) -> np.ndarray:
+        n = _sz()
+        check(self.L.demsph_server_job_latencies(self._h, int(job), None, 0, C.byref(n)))
+        out = np.empty(n.value)
+        check(self.L.demsph_server_job_latencies(self._h, int(job), _d(out), n.value, C.byref(n)))
+        return out
+
+    def result(self, job: int, field: str) -> np.ndarray:
+        n = _sz()
+        check(self.L.demsph_server_job_result(self._h, int(job), field.encode(), None, 0, C.byref(n)))
+        out = np.empty(n.value)
+        check(self.L.demsph_server_job_result(self._h, int(job), field.encode(), _d(out), n.value, C.byref(n)))
+        return out.reshape(-1, 3)
+
+    def stats(self) -> dict:
+        s = ServerSummary()
+        check(self.L.demsph_server_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerSummary._fields_}

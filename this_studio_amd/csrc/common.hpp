@@ -187,4 +187,14 @@ void hash_aos(const double* pos, int n, const GridGeom& g, uint32_t* key, hipStr
 // findStartAndEnd over sorted keys; start/end must hold -1 beforehand.
 void cell_ranges(const uint32_t* sorted_keys, int n, long long numGrids, int* start, int* end, hipStream_t s);
 
+// Step-end timestamps: a one-thread kernel appends the device's constant-rate
+// wall clock to buf[counter % cap] and advances the device counter, so that
+// replays of a captured graph keep appending.  buf == nullptr: no stamp.
+struct StepStamps {
+    unsigned long long* buf = nullptr;
+    int* counter = nullptr;
+    int cap = 0;
+};
+void step_stamp(const StepStamps& st, hipStream_t s);
+
 }  // namespace demsph

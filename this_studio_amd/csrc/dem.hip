@@ -940,6 +940,7 @@ struct demsph_dem {
     bool bgValid = false, tgValid = false;
     int nMat = 0, cap = 0;
     KernelTimer timer;
+    StepStamps stamps;  // step-end timestamps (batched serving)
 
     // balls: two buffers, cur is the live one
     DBuf<double> bpx[2], bpy[2], bpz[2], bvx[2], bvy[2], bvz[2], bwx[2], bwy[2], bwz[2], br[2], bim[2];
@@ -1096,22 +1097,28 @@ struct demsph_dem {
             H[H_D * cap + i] = -logR / std::sqrt(logR * logR + M_PI * M_PI);
         }
         tH.alloc(H.size()); tL.alloc(L.size()); tB.alloc(B.size());
-        DEMSPH_HIP(hipMemcpy(tH.p, H.data(), H.size() * 8, hipMemcpyHostToDevice));
-        DEMSPH_HIP(hipMemcpy(tL.p, L.data(), L.size() * 8, hipMemcpyHostToDevice));
-        DEMSPH_HIP(hipMemcpy(tB.p, B.data(), B.size() * 8, hipMemcpyHostToDevice));
+        // Uploads are ordered on the scene's stream (whatever the runtime
+        // does with pageable sources); waiting keeps the host arrays alive.
+        DEMSPH_HIP(hipMemcpyAsync(tH.p, H.data(), H.size() * 8, hipMemcpyHostToDevice, stream));
+        DEMSPH_HIP(hipMemcpyAsync(tL.p, L.data(), L.size() * 8, hipMemcpyHostToDevice, stream));
+        DEMSPH_HIP(hipMemcpyAsync(tB.p, B.data(), B.size() * 8, hipMemcpyHostToDevice, stream));
+        DEMSPH_HIP(hipStreamSynchronize(stream));
     }
 
+    // Upload on the scene's stream; h must outlive the copy (the host
+    // staging members do, until initialize() synchronises).
     template <class T>
     void up(DBuf<T>& d, const std::vector<T>& h, size_t count)
     {
         d.alloc(count);
-        if (count) DEMSPH_HIP(hipMemcpy(d.p, h.data(), count * sizeof(T), hipMemcpyHostToDevice));
+        if (count) DEMSPH_HIP(hipMemcpyAsync(d.p, h.data(), count * sizeof(T), hipMemcpyHostToDevice, stream));
     }
     void up3(DBuf<double>& x, DBuf<double>& y, DBuf<double>& z, const std::vector<double>& h, size_t count)
     {
         std::vector<double> a(count), b(count), c(count);
         for (size_t i = 0; i < count; i++) { a[i] = h[3 * i]; b[i] = h[3 * i + 1]; c[i] = h[3 * i + 2]; }
         up(x, a, count); up(y, b, count); up(z, c, count);
+        DEMSPH_HIP(hipStreamSynchronize(stream));  // a, b, c are local
     }
 
     void upload_bonds()
@@ -1153,6 +1160,26 @@ struct demsph_dem {
         up(dinc_bond, ib, ib.size());
         nbond_dev = nb_new;
         nbond = nb_new;
+    }
+
+    // Device bytes initialize() allocates for the staged scene: double-
+    // buffered ball state, sort buffers, cell ranges and the two CSR pair
+    // lists with their history at the initial capacity of 6 entries per ball
+    // (DEMSolver.h:174), plus a margin for walls and clumps.  Used to admit
+    // scenes under a memory budget.
+    size_t bytes_estimate(const double* minb, const double* maxb) const
+    {
+        const size_t nb = hb.size();
+        double c = 0.0;
+        for (double r : hb.r) c = std::max(c, r * 2.0);
+        GridGeom gg{};
+        const size_t cells = grid_set(gg, minb, maxb, c) ? (size_t)gg.numGrids() : 0;
+        const size_t per_ball = 2 * (11 * 8 + 3 * 4)          // state, two buffers
+                                + 2 * 4 + 12 * 8              // ranks, force and torque accumulators
+                                + 5 * 4 + 4 * 4 + 2 * 4       // sort keys, counts and scans, pair starts
+                                + 6 * (2 * (4 + 16 * 8) + 4); // pairs with history (two lists), reverse index
+        return nb * per_ball + 2 * cells * sizeof(int) + radix_sort_temp_bytes((int)std::max<size_t>(nb, 1)) +
+               hw.t0.size() * 256 + hc.size() * 512 + (size_t(1) << 20);
     }
 
     void initialize(const double* minb, const double* maxb)
@@ -1272,13 +1299,13 @@ struct demsph_dem {
             for (double rr : hb.r) rmax = std::max(rmax, rr);
             const size_t hn = (size_t)KW * n;
             hcnt.alloc(n);
-            DEMSPH_HIP(hipMemset(hcnt.p, 0, n * sizeof(int)));
+            DEMSPH_HIP(hipMemsetAsync(hcnt.p, 0, n * sizeof(int), stream));
             htri.alloc(hn); hcancel.alloc(hn);
             for (auto* d : {&hsx, &hsy, &hsz, &hrx, &hry, &hrz, &hox, &hoy, &hoz, &hov}) d->alloc(hn);
             for (auto* d : {&wfx, &wfy, &wfz, &wtx, &wty, &wtz}) d->alloc(n);
         }
         overflow.alloc(1);
-        DEMSPH_HIP(hipMemset(overflow.p, 0, sizeof(int)));
+        DEMSPH_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), stream));
         DEMSPH_HIP(hipHostMalloc((void**)&overflow_h, sizeof(int), hipHostMallocDefault));
         nbond_dev = 0;
         upload_bonds();
@@ -1572,7 +1599,7 @@ struct demsph_dem {
         timer.harvest();
         if (overflow_h[0] > 0) {
             const int c = overflow_h[0];
-            DEMSPH_HIP(hipMemset(overflow.p, 0, sizeof(int)));
+            DEMSPH_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), stream));
             throw CapacityError("a ball had " + std::to_string(c) +
                                 " wall-triangle candidates, above the per-ball capacity of 16");
         }
@@ -1605,6 +1632,7 @@ struct demsph_dem {
             evaluate(h, dt);
             run_hooks(time);
             half(false, g, h);
+            step_stamp(stamps, stream);
         }
         check_overflow();
     }
@@ -1641,6 +1669,12 @@ void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, do
     d->half(first, g, h);
 }
 void dem_end_block(demsph_dem* d) { d->check_overflow(); }
+size_t dem_bytes_estimate(demsph_dem* d, const double* minb, const double* maxb)
+{
+    return d->bytes_estimate(minb, maxb);
+}
+int dem_num_balls(demsph_dem* d) { return d->initialized ? d->n : (int)d->hb.size(); }
+void dem_set_stamps(demsph_dem* d, const StepStamps& st) { d->stamps = st; }
 
 }  // namespace demsph
 

@@ -36,4 +36,15 @@ void dem_evaluate(demsph_dem* d, double h, double dt);
 void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, double hookTime);
 void dem_end_block(demsph_dem* d);
 
+// Batched serving: device bytes a staged scene will allocate (admission
+// under a memory budget), sizes, step-end timestamps and streams.
+size_t sph_bytes_estimate(demsph_sph* s, const double* minb, const double* maxb);
+size_t dem_bytes_estimate(demsph_dem* d, const double* minb, const double* maxb);
+size_t couple_bytes_estimate(demsph_sph* f, demsph_dem* d, const double* fluidMin, const double* fluidMax);
+int sph_num_fluid(demsph_sph* s);
+int dem_num_balls(demsph_dem* d);
+void sph_set_stamps(demsph_sph* s, const StepStamps& st);
+void dem_set_stamps(demsph_dem* d, const StepStamps& st);
+hipStream_t sph_stream(demsph_sph* s);
+
 }  // namespace demsph

@@ -196,6 +196,14 @@ __global__ void k_fill_i32(int* p, long long n, int v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_step_stamp(unsigned long long* buf, int* counter, int cap)
+{
+    if (threadIdx.x != 0) return;
+    const int i = *counter;
+    buf[i % cap] = wall_clock64();
+    *counter = i + 1;
+}
+
 __global__ void k_iota_i32(int* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,6 +305,13 @@ void hash_aos(const double* pos, int n, const GridGeom& g, uint32_t* key, hipStr
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_hash_aos_by_index, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, n, g, key);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+void step_stamp(const StepStamps& st, hipStream_t s)
+{
+    if (!st.buf) return;
+    hipLaunchKernelGGL(k_step_stamp, dim3(1), dim3(64), 0, s, st.buf, st.counter, st.cap);
     DEMSPH_HIP(hipGetLastError());
 }
 

@@ -1080,6 +1080,7 @@ struct demsph_sph {
     std::vector<GraphEntry> graphs;
     bool use_graphs = true;
     long graph_captures = 0;
+    StepStamps stamps;  // step-end timestamps (batched serving)
 
     ~demsph_sph() { release(); }
 
@@ -1166,6 +1167,41 @@ struct demsph_sph {
         lists_valid = false;
     }
 
+    // grid cell = 2 * max smoothing length of fluid and dummies (WCSPHSolver.h:127-135)
+    double staged_cell_size() const
+    {
+        double cmax = 0.0;
+        for (double dx : hf.spacing) cmax = std::max(cmax, 1.3 * dx * 2.0);
+        double cd2 = 0.0;
+        for (double dx : hd.spacing) cd2 = std::max(cd2, 1.3 * dx * 2.0);
+        if (!hd.spacing.empty()) cmax = std::max(cmax, cd2);
+        return cmax;
+    }
+
+    // Device bytes initialize() and the first steps allocate for the staged
+    // particles: double-buffered fluid records, sort buffers, the chunked
+    // lists at the current capacities, the replay checkpoint, the dummy
+    // records, the per-cell ranges and the upload staging.  Used to admit scenes under a memory
+    // budget.
+    size_t bytes_estimate(const double* minb, const double* maxb) const
+    {
+        const size_t n = hf.size(), m = hd.size();
+        GridGeom gg{};
+        const size_t cells = grid_set(gg, minb, maxb, staged_cell_size()) ? (size_t)gg.numGrids() : 0;
+        const size_t rec = sizeof(FluidPos) + sizeof(float4) + 3 * sizeof(double) + sizeof(float) + sizeof(int);
+        const size_t per_fluid = 2 * rec + rec + sizeof(int)             // two buffers, checkpoint, rank
+                                 + 2 * sizeof(int)                        // list counts
+                                 + sizeof(int) * (size_t)(round_chunk(K_ff) + round_chunk(K_fd));
+        const size_t per_sort = 3 * sizeof(uint32_t) + 2 * sizeof(int);
+        const size_t per_dummy = sizeof(DummyPos) + sizeof(float4) + 1 + sizeof(int);
+        // initialisation uploads positions, velocities and phases through
+        // temporary buffers, which the runtime's allocator keeps cached
+        const size_t upload = 6 * sizeof(double) + 1;
+        return n * (per_fluid + upload) + m * (per_dummy + upload) + std::max(n, m) * per_sort +
+               cells * (2 * sizeof(int2) + 1) + radix_sort_temp_bytes((int)std::max<size_t>(std::max(n, m), 1)) +
+               (size_t(4) << 20);
+    }
+
     void initialize(const double* minb, const double* maxb, const double* gravity)
     {
         if (initialized) release_state_keep_host();
@@ -1177,13 +1213,7 @@ struct demsph_sph {
         nf = (int)hf.size();
         nd = (int)hd.size();
         if (nf == 0) throw InvalidArg("no fluid particles");
-        // grid cell = 2 * max smoothing length of fluid and dummies (WCSPHSolver.h:127-135)
-        double cmax = 0.0;
-        for (double dx : hf.spacing) cmax = std::max(cmax, 1.3 * dx * 2.0);
-        double cd2 = 0.0;
-        for (double dx : hd.spacing) cd2 = std::max(cd2, 1.3 * dx * 2.0);
-        if (nd > 0) cmax = std::max(cmax, cd2);
-        if (!grid_set(grid, minb, maxb, cmax)) throw InvalidArg("degenerate domain or smoothing length");
+        if (!grid_set(grid, minb, maxb, staged_cell_size())) throw InvalidArg("degenerate domain or smoothing length");
         numGrids = grid.numGrids();
         if (numGrids > INT32_MAX / 2) throw InvalidArg("grid too large");
 
@@ -1231,9 +1261,10 @@ struct demsph_sph {
             double* upos = dmalloc<double>(3 * n);
             double* uvel = dmalloc<double>(3 * n);
             uint8_t* uph = dmalloc<uint8_t>(n);
-            DEMSPH_HIP(hipMemcpy(upos, hf.pos.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(uvel, hf.vel.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(uph, fphase_h.data(), n, hipMemcpyHostToDevice));
+            // uploads ordered on the scene's stream ahead of the kernels that read them
+            DEMSPH_HIP(hipMemcpyAsync(upos, hf.pos.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, stream));
+            DEMSPH_HIP(hipMemcpyAsync(uvel, hf.vel.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, stream));
+            DEMSPH_HIP(hipMemcpyAsync(uph, fphase_h.data(), n, hipMemcpyHostToDevice, stream));
             const dim3 gb(blocks_for(n, 256)), tb(256);
             hipLaunchKernelGGL(k_fluid_init, gb, tb, 0, stream, (int)n, upos, uvel, uph, P[0], V[0], vx[0], vy[0],
                                vz[0], drho[0]);
@@ -1260,13 +1291,14 @@ struct demsph_sph {
             uint8_t* uph = dmalloc<uint8_t>(m);
             auto up3 = [&](const std::vector<double>& src, double* dx_, double* dy_, double* dz_) {
                 for (size_t i = 0; i < m; i++) { tx[i] = src[3 * i]; ty[i] = src[3 * i + 1]; tz[i] = src[3 * i + 2]; }
-                DEMSPH_HIP(hipMemcpy(dx_, tx.data(), m * sizeof(double), hipMemcpyHostToDevice));
-                DEMSPH_HIP(hipMemcpy(dy_, ty.data(), m * sizeof(double), hipMemcpyHostToDevice));
-                DEMSPH_HIP(hipMemcpy(dz_, tz.data(), m * sizeof(double), hipMemcpyHostToDevice));
+                DEMSPH_HIP(hipMemcpyAsync(dx_, tx.data(), m * sizeof(double), hipMemcpyHostToDevice, stream));
+                DEMSPH_HIP(hipMemcpyAsync(dy_, ty.data(), m * sizeof(double), hipMemcpyHostToDevice, stream));
+                DEMSPH_HIP(hipMemcpyAsync(dz_, tz.data(), m * sizeof(double), hipMemcpyHostToDevice, stream));
+                DEMSPH_HIP(hipStreamSynchronize(stream));  // tx, ty, tz are reused
             };
             up3(hd.pos, ux, uy, uz);
             up3(hd.vel, wx, wy, wz);
-            DEMSPH_HIP(hipMemcpy(uph, dphase_h.data(), m, hipMemcpyHostToDevice));
+            DEMSPH_HIP(hipMemcpyAsync(uph, dphase_h.data(), m, hipMemcpyHostToDevice, stream));
             hipLaunchKernelGGL(k_dummy_hash, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, (int)m, grid,
                                ux, uy, uz, key);
             radix_sort_pairs(key, nullptr, skey, perm, ktmp, vtmp, sort_temp, (int)m, key_bits(numGrids), stream);
@@ -1444,6 +1476,13 @@ struct demsph_sph {
         search_enqueue();
         first_half_enqueue(dt);
         second_half_enqueue(dt);
+        step_stamp(stamps, stream);
+    }
+
+    void set_stamps(const StepStamps& st)
+    {
+        drop_graphs();  // captured graphs hold the previous stamp buffer
+        stamps = st;
     }
 
     GraphKey graph_key(double dt) const
@@ -1886,7 +1925,8 @@ struct demsph_couple {
     demsph_dem* d = nullptr;
     bool ready = false;
     int ns = 0;
-    int K_fs = 32;
+    static constexpr int kDefaultKfs = 32;
+    int K_fs = kDefaultKfs;
     uint32_t *key = nullptr, *skey = nullptr, *ktmp = nullptr;
     int *perm = nullptr, *vtmp = nullptr;  // perm: ball slot -> DEM slot
     void* sort_temp = nullptr;
@@ -2042,6 +2082,7 @@ struct demsph_couple {
                 forces(st);
                 dem_hooks_and_half(d, false, g, h, t);
                 f->momentum2_enqueue(dt);
+                step_stamp(f->stamps, st);
             }
         } catch (...) {
             f->solids = SolidView();
@@ -2135,3 +2176,29 @@ extern "C" int demsph_couple_get(demsph_couple* c, const char* field, void* out,
         throw InvalidArg("unknown field '" + fld + "'");
     });
 }
+
+// ------------------------------------------------------------ internal API
+namespace demsph {
+
+size_t sph_bytes_estimate(demsph_sph* s, const double* minb, const double* maxb)
+{
+    return s->bytes_estimate(minb, maxb);
+}
+int sph_num_fluid(demsph_sph* s) { return s->initialized ? s->nf : (int)s->hf.size(); }
+void sph_set_stamps(demsph_sph* s, const StepStamps& st) { s->set_stamps(st); }
+hipStream_t sph_stream(demsph_sph* s) { return s->stream; }
+
+// Fluid-ball lists and their reverse index at the default capacity, ball
+// records and a fluid-grid cell array, as demsph_couple::setup allocates.
+size_t couple_bytes_estimate(demsph_sph* f, demsph_dem* d, const double* fluidMin, const double* fluidMax)
+{
+    const size_t nf = (size_t)sph_num_fluid(f), ns = (size_t)dem_num_balls(d);
+    GridGeom gg{};
+    const size_t cells = grid_set(gg, fluidMin, fluidMax, f->staged_cell_size()) ? (size_t)gg.numGrids() : 0;
+    const int K_fs = demsph_couple::kDefaultKfs;
+    return nf * (2 * sizeof(int) * (size_t)K_fs + sizeof(int) + 3 * sizeof(float)) +
+           ns * (5 * 4 + sizeof(SolidPos) + 2 * sizeof(float4) + 3 * sizeof(int) + 6 * sizeof(double)) +
+           cells * sizeof(int2) + radix_sort_temp_bytes((int)std::max<size_t>(ns, 1)) + (size_t(1) << 20);
+}
+
+}  // namespace demsph

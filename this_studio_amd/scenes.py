@@ -42,8 +42,15 @@ def splitmix64(seed: int):
 
 
 def uniform01(seed: int, n: int) -> np.ndarray:
-    g = splitmix64(seed)
-    return np.array([(next(g) >> 11) * (1.0 / 9007199254740992.0) for _ in range(n)])
+    """The first n values of splitmix64(seed) as doubles in [0, 1) (top 53
+    bits), vectorised: state k is seed + (k + 1) * golden gamma mod 2^64."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
 @dataclass
@@ -479,3 +486,102 @@ def balls_in_dam_break():
     d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
     d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
     return spec, d
+
+
+# ---------------------------------------------------------------------------
+# Batched serving mix: independent scenes of log-uniform size
+# ---------------------------------------------------------------------------
+def mix_wcsph(n: int, dx: float = 0.01, eos: int = 1) -> SPHSceneSpec:
+    """A dam break of about n fluid particles: a 2m x m x m column (2 m^3
+    ~ n) at spacing dx in a 4m x (m+1) x 1.5m open tank with its dummy
+    shell, Tait EOS, dt from the CFL rule of dam_break()."""
+    m = max(4, int(round((n / 2.0) ** (1.0 / 3.0))))
+    nx, ny, nz = 2 * m, m, m
+    spec = dam_break(dx=dx, column=(nx * dx, ny * dx, nz * dx), tank=(2 * nx * dx, (ny + 1) * dx, 1.5 * nz * dx),
+                     eos=eos, lattice_counts=(nx, ny, nz))
+    spec.name = f"mix_wcsph_{spec.nf}"
+    return spec
+
+
+def _hertz_box(name: str, L, H) -> DEMScene:
+    s = DEMScene(name)
+    E, nu = 1e7, 0.3
+    s.set_hertzian_pair(0, 0, E / (2 * (1 - nu * nu)), E / (4 * (2 - nu) * (1 + nu)), 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    v, t0, t1, t2 = box_mesh([0, 0, 0], [L[0], L[1], H])
+    s.add_mesh_wall(v, t0, t1, t2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    s.min_bound = np.zeros(3)
+    s.max_bound = np.array([L[0], L[1], H])
+    s.dt = 1e-5
+    return s
+
+
+def mix_dem(n: int, seed: int, r: float = 0.005) -> DEMScene:
+    """n Hertz-Mindlin spheres (r = 5 mm, rho = 2500, the granular_collapse
+    material) on a 2.2 r lattice, m x m wide and as tall as needed, with
+    seeded jitter of at most 0.09 r per axis (never an overlap: neighbours
+    stay at least 2.02 r apart along the lattice axis), settling in a closed
+    box twice the pile's height, dt = 1e-5."""
+    sp = 2.2 * r
+    m = max(2, int(round(n ** (1.0 / 3.0))))
+    mz = -(-n // (m * m))
+    base = lattice(m, m, mz, sp, origin=(1.5 * sp, 1.5 * sp, 1.5 * sp))[:n]
+    jit = (uniform01(seed, 3 * n).reshape(n, 3) * 2.0 - 1.0) * 0.09 * r
+    L = (m + 2) * sp
+    s = _hertz_box(f"mix_dem_{n}", (L, L), 2.0 * (mz + 2) * sp)
+    s.add_balls(base + jit, [0, 0, 0], [0, 0, 0], r, 2500.0, 0)
+    return s
+
+
+def mix_coupled(n: int, seed: int, dx: float = 0.01):
+    """A dam break of about 0.95 n fluid particles (mix_wcsph) with n/20
+    balls (r = 0.6 dx, rho = 2500, Hertz-Mindlin) on a lattice of 2.7 dx
+    inside the column, seeded jitter of 0.1 dx per axis, and the tank as a
+    closed DEM box.  Returns (fluid spec, DEM scene); both scenes share the
+    fluid's bounds and time step."""
+    spec = mix_wcsph(int(0.95 * n), dx)
+    nb = max(1, n // 20)
+    lo, hi = spec.fluid_pos.min(axis=0) + 1.5 * dx, spec.fluid_pos.max(axis=0) - 1.5 * dx
+    sp = 2.7 * dx
+    cnt = np.maximum(1, np.floor((hi - lo) / sp).astype(int) + 1)
+    pts = lattice(int(cnt[0]), int(cnt[1]), int(cnt[2]), sp, origin=lo)[:nb]
+    pts = pts + (uniform01(seed, pts.size).reshape(-1, 3) * 2.0 - 1.0) * 0.1 * dx
+    d = DEMScene(f"mix_coupled_{spec.nf}_{pts.shape[0]}")
+    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    d.add_balls(pts, [0, 0, 0], [0, 0, 0], 0.6 * dx, 2500.0, 0)
+    # the tank of mix_wcsph as a closed DEM box (the fluid's dummies do not act on balls)
+    top = spec.fluid_pos.max(axis=0) + dx
+    v, t0, t1, t2 = box_mesh([0, 0, 0], [2.0 * top[0], top[1] + dx, 1.5 * top[2]])
+    d.add_mesh_wall(v, t0, t1, t2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
+    d.dt = spec.dt
+    spec.name = d.name
+    return spec, d
+
+
+@dataclass
+class MixJob:
+    kind: str        # "wcsph" | "dem" | "coupled"
+    n: int           # requested particles (fluid + balls)
+    steps: int
+    arrival: float   # seconds after the start of serving
+    seed: int
+
+
+def serving_mix(count: int = 256, seed: int = 20261202, n_range=(2.0e3, 2.0e5), steps_range=(1000, 20000),
+                mean_interarrival: float = 0.05, fractions=(0.5, 0.3, 0.2)) -> list[MixJob]:
+    """The batched-serving job stream: sizes exp(U(ln a, ln b)), kind drawn
+    with the given DEM / WCSPH / coupled fractions, steps U{lo..hi},
+    Poisson arrivals (exponential gaps of the given mean)."""
+    u = uniform01(seed, 4 * count).reshape(count, 4)
+    jobs, t = [], 0.0
+    la, lb = math.log(n_range[0]), math.log(n_range[1])
+    for k in range(count):
+        n = int(round(math.exp(la + u[k, 0] * (lb - la))))
+        c = u[k, 1]
+        kind = "dem" if c < fractions[0] else ("wcsph" if c < fractions[0] + fractions[1] else "coupled")
+        steps = steps_range[0] + int(u[k, 2] * (steps_range[1] - steps_range[0] + 1))
+        steps = min(steps, steps_range[1])
+        if k > 0:
+            t += -mean_interarrival * math.log(max(1.0 - u[k, 3], 1e-300))
+        jobs.append(MixJob(kind, n, steps, t, seed + 7919 * (k + 1)))
+    return jobs
