@@ -288,7 +288,8 @@ int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long
  * of every fluid particle's pair force (and the torque of its viscous
  * part), so momentum is exchanged pairwise.  Both scenes must be
  * initialised (the DEM scene through solve_begin) and are advanced with one
- * time step on the fluid scene's stream. */
+ * time step on the fluid scene's stream.  Destroy the coupling before
+ * either scene; the DEM scene then returns to its own stream. */
 typedef struct demsph_couple demsph_couple;
 int demsph_couple_create(demsph_couple** out, demsph_sph* fluid, demsph_dem* dem);
 int demsph_couple_destroy(demsph_couple* c);

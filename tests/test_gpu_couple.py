@@ -88,6 +88,22 @@ def test_coupled_dam_break_with_balls_stays_finite_and_drags_balls():
     assert np.all(vel[:, 2] > free)
 
 
+def test_dem_scene_outlives_coupling_and_fluid():
+    """Ending the coupling hands the DEM scene its own stream back: after the
+    coupling and then the fluid scene are destroyed, the DEM scene keeps
+    stepping and reads back (it holds no handle to the fluid's stream)."""
+    spec, fluid, dem = falling_balls()
+    c = ts.CoupledScene(fluid, dem)
+    c.step(spec.dt, 5, spec.gravity)
+    c.close()
+    fluid.close()
+    z0 = dem.get("position")[:, 2].copy()
+    dem.step(20)
+    z1 = dem.get("position")[:, 2]
+    assert np.all(np.isfinite(z1)) and np.all(z1 < z0)  # still falling under gravity
+    dem.close()
+
+
 def test_coupled_steps_match_oracle():
     """Twelve balls sinking in the small dam break, 10 coupled steps: the
     GPU (FP32 pair arithmetic) against the FP64 oracle of the same model."""

@@ -106,12 +106,15 @@ class SPHScene:
 
     def __init__(self, stream: int | None = None):
         self._h = _vp()
-        check(lib().demsph_sph_create(C.byref(self._h), _vp(stream) if stream else None))
+        self.L = lib()  # kept so that close() works during interpreter teardown
+        check(self.L.demsph_sph_create(C.byref(self._h), _vp(stream) if stream else None))
 
     def close(self) -> None:
         if self._h:
-            check(lib().demsph_sph_destroy(self._h))
-            self._h = _vp()
+            h, self._h = self._h, _vp()
+            st = self.L.demsph_sph_destroy(h)
+            if st != 0:
+                raise DemsphError(f"demsph status {st}: {self.L.demsph_last_error().decode()}")
 
     def __del__(self):
         try:
@@ -386,8 +389,10 @@ class DEMSceneGPU:
 
     def close(self):
         if self._h:
-            check(self.L.demsph_dem_destroy(self._h))
-            self._h = _vp()
+            h, self._h = self._h, _vp()
+            st = self.L.demsph_dem_destroy(h)
+            if st != 0:
+                raise DemsphError(f"demsph status {st}: {self.L.demsph_last_error().decode()}")
 
     def __del__(self):
         try:
@@ -486,8 +491,10 @@ class CoupledScene:
 
     def close(self) -> None:
         if self._h:
-            check(self.L.demsph_couple_destroy(self._h))
-            self._h = _vp()
+            h, self._h = self._h, _vp()
+            st = self.L.demsph_couple_destroy(h)
+            if st != 0:
+                raise DemsphError(f"demsph status {st}: {self.L.demsph_last_error().decode()}")
 
     def __del__(self):
         try:
@@ -566,8 +573,10 @@ class Server:
 
     def close(self) -> None:
         if self._h:
-            check(self.L.demsph_server_destroy(self._h))
-            self._h = _vp()
+            h, self._h = self._h, _vp()
+            st = self.L.demsph_server_destroy(h)
+            if st != 0:
+                raise DemsphError(f"demsph status {st}: {self.L.demsph_last_error().decode()}")
 
     def __del__(self):
         try:

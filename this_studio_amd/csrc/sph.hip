@@ -693,41 +693,82 @@ __global__ void k_solid_gather(SolidGatherArgs a)
     }
 }
 
-// Reverse index of the fluid -> ball lists: for each ball, the fluid slots
-// that list it, ascending (the same pair set seen from the ball).
-__global__ void k_fs_rev_count(int nf, const int* __restrict__ nbr, const int* __restrict__ cnt, int K,
-                               int* __restrict__ rcnt)
+// One wavefront per coupled ball: lane l of the wave owns ball slot
+// blockIdx.x * kBallsPerBlock + wave, and the 64 lanes split that ball's
+// fluid candidates or list entries.
+constexpr int kBallsPerBlock = 4;
+__device__ __forceinline__ int wave_lane() { return threadIdx.x & 63; }
+__device__ __forceinline__ int wave_ball() { return blockIdx.x * kBallsPerBlock + (threadIdx.x >> 6); }
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nf) return;
-    const int n = min(cnt[i], K);
-    for (int k = 0; k < n; k++) atomicAdd(&rcnt[nbr[((size_t)(k / kChunk) * nf + i) * kChunk + (k % kChunk)]], 1);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
 }
-__global__ void k_fs_rev_fill(int nf, const int* __restrict__ nbr, const int* __restrict__ cnt, int K,
-                              const int* __restrict__ rstart, int* __restrict__ rcursor, int* __restrict__ rfluid)
+
+struct RevArgs {
+    int ns;
+    GridGeom grid;
+    const int2* cells;  // fluid cell ranges (fluid slots sorted by cell)
+    const FluidPos* P;
+    const float4* V;
+    const Phase* phases;
+    const SolidPos* SP;
+    int* rcnt;
+    const int* rstart;
+    int* rfluid;
+    int rcap;  // entries rfluid holds; a list overflow (reported by the fluid side) may exceed it
+};
+
+// Reverse index of the fluid -> ball lists: for ball j, the fluid slots i
+// whose support holds the ball (the predicate of k_build_fs_lists with the
+// same operands: within_cut(2 h_i, x_i - x_j)).  Cell adjacency is
+// symmetric, so the walk over the fluid cells of the ball's 27-cell stencil
+// finds every such i; rows are walked in cell order, so the slots come out
+// ascending.  COUNT: rcnt[j] = their number; else they fill
+// rfluid[rstart[j] ...] through a wave ballot (lane order = slot order).
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_fs_rev_walk(RevArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nf) return;
-    const int n = min(cnt[i], K);
-    for (int k = 0; k < n; k++) {
-        const int j = nbr[((size_t)(k / kChunk) * nf + i) * kChunk + (k % kChunk)];
-        rfluid[rstart[j] + atomicAdd(&rcursor[j], 1)] = i;
-    }
-}
-__global__ void k_fs_rev_sort(int ns, const int* __restrict__ rstart, int* __restrict__ rfluid)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    const int b = rstart[j], e = rstart[j + 1];
-    for (int q = b + 1; q < e; q++) {
-        const int v = rfluid[q];
-        int t = q - 1;
-        while (t >= b && rfluid[t] > v) {
-            rfluid[t + 1] = rfluid[t];
-            t--;
+    const int lane = wave_lane();
+    const int j = wave_ball();
+    if (j >= a.ns) return;  // uniform over the wave
+    const GridGeom& g = a.grid;
+    const double bx = a.SP[j].x, by = a.SP[j].y, bz = a.SP[j].z;
+    const Stencil st = make_stencil(g, bx, by, bz);
+    int off = COUNT ? 0 : a.rstart[j];
+    for (int k = 0; k < 9; k++) {
+        const int dz = k / 3 - 1, dy = k % 3 - 1;
+        if (dz < st.loz || dz > st.hiz || dy < st.loy || dy > st.hiy) continue;
+        const int row = (st.cz + dz) * g.ny * g.nx + (st.cy + dy) * g.nx + st.cx;
+        int s = -1, e = -1;
+        for (int dx = st.lox; dx <= st.hix; dx++) {
+            const int2 c = a.cells[row + dx];
+            if (c.x != -1) {
+                if (s < 0) s = c.x;
+                e = c.y;
+            }
         }
-        rfluid[t + 1] = v;
+        if (s < 0) continue;
+        for (int base = s; base < e; base += 64) {
+            const int i = base + lane;
+            bool hit = false;
+            if (i < e) {
+                const FluidPos p = a.P[i];
+                const double h = a.phases[phase_bits(a.V[i].w)].h;
+                hit = within_cut(2.0 * h, p.x - bx, p.y - by, p.z - bz);
+            }
+            const unsigned long long m = __ballot(hit);
+            if (!COUNT && hit) {
+                const int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+                if (pos < a.rcap) a.rfluid[pos] = i;
+            }
+            off += __popcll(m);
+        }
     }
+    if (COUNT && lane == 0) a.rcnt[j] = off;
 }
 
 // Wendland C2 value (WCSPHIntegrationKernel.cuh:5-13), FP32.
@@ -740,16 +781,22 @@ __device__ __forceinline__ float wendland(float r, float h)
 }
 
 // P_j = sum V_i p_i W_ij / sum V_i W_ij over the ball's fluid neighbours
-// (Shepard interpolation of the fluid pressure at the ball centre).
-__global__ void k_solid_pressure(int ns, const int* __restrict__ rstart, const int* __restrict__ rfluid,
-                                 const FluidPos* __restrict__ P, const float4* __restrict__ V,
-                                 const Phase* __restrict__ phases, SolidPos* __restrict__ SP)
+// (Shepard interpolation of the fluid pressure at the ball centre), one
+// wavefront per ball: lane l sums entries l, l + 64, ... of the ball's
+// reverse list, then the wave reduces (a fixed order, so the result is
+// deterministic).
+__global__ __launch_bounds__(256) void k_solid_pressure(int ns, const int* __restrict__ rstart,
+                                                        const int* __restrict__ rfluid, const FluidPos* __restrict__ P,
+                                                        const float4* __restrict__ V, const Phase* __restrict__ phases,
+                                                        SolidPos* __restrict__ SP, int rcap)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = wave_lane();
+    const int j = wave_ball();
     if (j >= ns) return;
     const double sx = SP[j].x, sy = SP[j].y, sz = SP[j].z;
     float num = 0.f, den = 0.f;
-    for (int q = rstart[j]; q < rstart[j + 1]; q++) {
+    const int qe = min(rstart[j + 1], rcap);
+    for (int q = rstart[j] + lane; q < qe; q += 64) {
         const int i = rfluid[q];
         const FluidPos f = P[i];
         const Phase& ph = phases[phase_bits(V[i].w)];
@@ -758,28 +805,34 @@ __global__ void k_solid_pressure(int ns, const int* __restrict__ rstart, const i
         num += w * f.p;
         den += w;
     }
-    SP[j].P = den > 0.f ? num / den : 0.f;
+    num = wave_sum(num);
+    den = wave_sum(den);
+    if (lane == 0) SP[j].P = den > 0.f ? num / den : 0.f;
 }
 
 struct CoupleForceArgs {
     SphArgs a;  // fluid state and ball records, as the fluid pass sees them
     const int *rstart, *rfluid, *s2d;
+    int rcap;                             // entries rfluid holds
     double *fx, *fy, *fz, *tx, *ty, *tz;  // DEM force / torque accumulators (DEM slot order)
     double* Fc;                           // per ball slot: coupling force and torque (6 doubles)
 };
 
 // The reaction of every fluid-ball pair of the fluid pass on the ball:
-// -m_i (ap + av) and (r_j n) x (-m_i av), summed in ascending fluid slot
-// order and added to the DEM accumulators.
-__global__ void k_couple_forces(CoupleForceArgs c)
+// -m_i (ap + av) and (r_j n) x (-m_i av), summed in FP64 by one wavefront
+// per ball (lane stripes of the ball's reverse list, then a fixed-order
+// wave reduction) and added to the DEM accumulators.
+__global__ __launch_bounds__(256) void k_couple_forces(CoupleForceArgs c)
 {
     const SphArgs& a = c.a;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = wave_lane();
+    const int j = wave_ball();
     if (j >= a.ns) return;
     const SolidPos S = a.SP[j];
     const float4 SV = a.SV[j], SW = a.SW[j];
     double Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
-    for (int q = c.rstart[j]; q < c.rstart[j + 1]; q++) {
+    const int qe = min(c.rstart[j + 1], c.rcap);
+    for (int q = c.rstart[j] + lane; q < qe; q += 64) {
         const int i = c.rfluid[q];
         const FluidPos Pi = a.P[i];
         const float4 Vi = a.V[i];
@@ -795,6 +848,9 @@ __global__ void k_couple_forces(CoupleForceArgs c)
         Ty += (double)lv.z * vx - (double)lv.x * vz;
         Tz += (double)lv.x * vy - (double)lv.y * vx;
     }
+    Fx = wave_sum(Fx); Fy = wave_sum(Fy); Fz = wave_sum(Fz);
+    Tx = wave_sum(Tx); Ty = wave_sum(Ty); Tz = wave_sum(Tz);
+    if (lane != 0) return;
     const int s = c.s2d[j];
     c.fx[s] += Fx; c.fy[s] += Fy; c.fz[s] += Fz;
     c.tx[s] += Tx; c.ty[s] += Ty; c.tz[s] += Tz;
@@ -1934,12 +1990,22 @@ struct demsph_couple {
     SolidPos* SP = nullptr;
     float4 *SV = nullptr, *SW = nullptr;
     int *nbr_fs = nullptr, *cnt_fs = nullptr;
-    int *rcnt = nullptr, *rstart = nullptr, *rcursor = nullptr, *rfluid = nullptr;
+    int *rcnt = nullptr, *rstart = nullptr, *rfluid = nullptr;
     void* scan_temp = nullptr;
     double* Fc = nullptr;  // 6 per ball slot (force, torque) of the last evaluation
     float* acs = nullptr;  // 3 per fluid slot, coupling acceleration of the last fluid pass
 
-    ~demsph_couple() { release(); }
+    // The DEM scene runs on the fluid's stream while coupled; it gets its
+    // own stream back when the coupling ends, so that it never keeps a
+    // handle to a stream the fluid scene destroys.
+    hipStream_t dem_own_stream = nullptr;
+    bool dem_switched = false;
+
+    ~demsph_couple()
+    {
+        release();
+        if (dem_switched) dem_set_stream(d, dem_own_stream);
+    }
 
     void release()
     {
@@ -1948,7 +2014,7 @@ struct demsph_couple {
         if (sort_temp) (void)hipFree(sort_temp);
         sort_temp = nullptr;
         dfree(cells); dfree(SP); dfree(SV); dfree(SW); dfree(nbr_fs); dfree(cnt_fs);
-        dfree(rcnt); dfree(rstart); dfree(rcursor); dfree(rfluid);
+        dfree(rcnt); dfree(rstart); dfree(rfluid);
         if (scan_temp) (void)hipFree(scan_temp);
         scan_temp = nullptr;
         dfree(Fc); dfree(acs);
@@ -1959,6 +2025,11 @@ struct demsph_couple {
     {
         f->require_init();
         dem_require_ready(d);
+        if (!dem_switched) {
+            dem_own_stream = dem_stream(d);
+            if (dem_own_stream != f->stream) DEMSPH_HIP(hipStreamSynchronize(dem_own_stream));
+            dem_switched = true;
+        }
         dem_set_stream(d, f->stream);
         const DemBallRefs b = dem_ball_refs(d);
         if (b.n == 0) throw InvalidArg("the DEM scene has no balls");
@@ -1973,8 +2044,8 @@ struct demsph_couple {
         cells = dmalloc<int2>(f->numGrids);
         SP = dmalloc<SolidPos>(ns); SV = dmalloc<float4>(ns); SW = dmalloc<float4>(ns);
         nbr_fs = dmalloc<int>((size_t)K_fs * nf); cnt_fs = dmalloc<int>(nf);
-        rcnt = dmalloc<int>(ns + 1); rstart = dmalloc<int>(ns + 1); rcursor = dmalloc<int>(ns);
-        rfluid = dmalloc<int>((size_t)K_fs * nf);
+        rcnt = dmalloc<int>(ns + 1); rstart = dmalloc<int>(ns + 1);
+        rfluid = dmalloc<int>(rcap());
         DEMSPH_HIP(hipMalloc(&scan_temp, scan_temp_bytes(ns + 1)));
         Fc = dmalloc<double>(6 * (size_t)ns);
         acs = dmalloc<float>(3 * nf);
@@ -2004,10 +2075,15 @@ struct demsph_couple {
         DEMSPH_LAUNCH(f->timer, "couple_gather", k_solid_gather, dim3(blocks_for(ns, 256)), dim3(256), 0, st, g);
     }
 
+    // The reverse lists hold the same pairs as the fluid lists, at most
+    // K_fs per fluid particle.
+    int rcap() const { return (int)std::min<size_t>((size_t)K_fs * (size_t)f->nf, (size_t)INT32_MAX); }
+
     void pressure(hipStream_t st)
     {
-        DEMSPH_LAUNCH(f->timer, "couple_pressure", k_solid_pressure, dim3(blocks_for(ns, 256)), dim3(256), 0, st, ns,
-                      rstart, rfluid, f->P[f->cP], f->V[f->cV], f->d_phases, SP);
+        DEMSPH_LAUNCH(f->timer, "couple_pressure", k_solid_pressure, dim3(blocks_for(ns, kBallsPerBlock)),
+                      dim3(64 * kBallsPerBlock), 0, st, ns, rstart, rfluid, f->P[f->cP], f->V[f->cV], f->d_phases, SP,
+                      rcap());
     }
 
     // Balls onto the fluid grid, fluid->ball lists and their reverse index,
@@ -2031,14 +2107,14 @@ struct demsph_couple {
         DEMSPH_LAUNCH(f->timer, "couple_lists", k_build_fs_lists, dim3(blocks_for(nf, kThreads)), dim3(kThreads), 0,
                       st, la);
         timer_begin(st, "couple_reverse");
-        fill_i32(rcnt, ns + 1, 0, st);
-        hipLaunchKernelGGL(k_fs_rev_count, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, nbr_fs, cnt_fs, K_fs, rcnt);
+        RevArgs ra{};
+        ra.ns = ns; ra.grid = f->grid; ra.cells = f->cells; ra.P = f->P[f->cP]; ra.V = f->V[f->cV];
+        ra.phases = f->d_phases; ra.SP = SP; ra.rcnt = rcnt; ra.rstart = rstart; ra.rfluid = rfluid; ra.rcap = rcap();
+        const dim3 wb(blocks_for(ns, kBallsPerBlock)), wt(64 * kBallsPerBlock);
+        hipLaunchKernelGGL(k_fs_rev_walk<true>, wb, wt, 0, st, ra);
         DEMSPH_HIP(hipGetLastError());
         exclusive_scan(rcnt, rstart, ns, scan_temp, rstart + ns, st);
-        fill_i32(rcursor, ns, 0, st);
-        hipLaunchKernelGGL(k_fs_rev_fill, dim3(blocks_for(nf, 256)), dim3(256), 0, st, nf, nbr_fs, cnt_fs, K_fs, rstart,
-                           rcursor, rfluid);
-        hipLaunchKernelGGL(k_fs_rev_sort, dim3(blocks_for(ns, 256)), dim3(256), 0, st, ns, rstart, rfluid);
+        hipLaunchKernelGGL(k_fs_rev_walk<false>, wb, wt, 0, st, ra);
         DEMSPH_HIP(hipGetLastError());
         f->timer.end(st);
         pressure(st);
@@ -2051,10 +2127,11 @@ struct demsph_couple {
         const DemBallRefs b = dem_ball_refs(d);
         CoupleForceArgs c{};
         c.a = f->args();
-        c.rstart = rstart; c.rfluid = rfluid; c.s2d = perm;
+        c.rstart = rstart; c.rfluid = rfluid; c.s2d = perm; c.rcap = rcap();
         c.fx = b.fx; c.fy = b.fy; c.fz = b.fz; c.tx = b.tx; c.ty = b.ty; c.tz = b.tz;
         c.Fc = Fc;
-        DEMSPH_LAUNCH(f->timer, "couple_forces", k_couple_forces, dim3(blocks_for(ns, 256)), dim3(256), 0, st, c);
+        DEMSPH_LAUNCH(f->timer, "couple_forces", k_couple_forces, dim3(blocks_for(ns, kBallsPerBlock)),
+                      dim3(64 * kBallsPerBlock), 0, st, c);
     }
 
     void step(const double* g, double dt, int nsteps)
