@@ -137,6 +137,44 @@ def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     return out
 
 
+def serving_mix(count: int, scale: float, slots: int) -> dict:
+    """The batched-serving job stream (256 scenes of 2k-200k particles,
+    log-uniform; 50/30/20 DEM/WCSPH/coupled; 1k-20k steps; Poisson arrivals
+    of mean 50 ms) scaled to finish in seconds: `count` scenes, step counts
+    and arrival gaps both times `scale` (so the stream stays as work-bound
+    as the full one).  Served twice from the same seed: plain (one slot, one
+    launch per stage from the host, every job alone) and batched (`slots`
+    concurrent jobs, WCSPH steps as graph replays).  Plain runs every job
+    alone, so equal results in the batched run show that the mix changes
+    nothing."""
+    import numpy as np
+
+    from this_studio_amd import scenes, serving
+
+    lo, hi = max(1, int(round(1000 * scale))), max(2, int(round(20000 * scale)))
+    jobs = scenes.serving_mix(count, steps_range=(lo, hi), mean_interarrival=0.05 * scale)
+    out = {"scenes": count, "steps_range": [lo, hi], "mean_interarrival_s": 0.05 * scale,
+           "kinds": {k: sum(j.kind == k for j in jobs) for k in ("dem", "wcsph", "coupled")},
+           "memory_budget_GB": 288}
+    results = {}
+    for name, sl, graphs in (("plain", 1, False), ("batched", slots, True)):
+        print(f"[bench] serving mix: {name} ({sl} slot(s), graphs={graphs}) ...", file=sys.stderr, flush=True)
+        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs)
+        st = srv.stats()
+        out[name] = {"slots": sl, "graphs": graphs, "wall_s": round(st["wall"], 4),
+                     "particle_steps_per_s": round(st["particleStepsPerSecond"], 1),
+                     "p50_step_ms": round(st["p50"] * 1e3, 4), "p99_step_ms": round(st["p99"] * 1e3, 4),
+                     "jobs_done": st["jobsDone"], "jobs_failed": st["jobsFailed"], "peak_active": st["peakActive"],
+                     "peak_admitted_GB": round(st["peakBytes"] / 1e9, 3)}
+        results[name] = [(srv.result(i, "position") if j.kind != "dem" else srv.result(i, "ballPosition"))
+                         for j, i in zip(jobs, ids)]
+        srv.close()
+    out["speedup"] = round(out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"], 3)
+    out["results_equal_alone_and_mixed"] = all(np.array_equal(a, b)
+                                               for a, b in zip(results["plain"], results["batched"]))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +184,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dem-steps", type=int, default=2000, help="steps of the DEM collapse leg (0: skip it)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--mix-scenes", type=int, default=48, help="scenes of the serving-mix leg (0: skip it)")
+    ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
+    ap.add_argument("--mix-slots", type=int, default=16, help="concurrent jobs of the batched server")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate HBM counters")
     args = ap.parse_args()
@@ -254,6 +295,14 @@ def main() -> int:
         print("[bench] dem granular collapse ...", file=sys.stderr, flush=True)
         dem = dem_collapse(args.dem_steps, min(args.cpu_budget, 10.0), not args.no_cpu_baseline)
         print(f"[bench] dem collapse: {dem['ms_per_step'] * 1e3:.1f} us/step", file=sys.stderr, flush=True)
+    mix = None
+    if args.mix_scenes > 0:
+        mix = serving_mix(args.mix_scenes, args.mix_scale, args.mix_slots)
+        print(f"[bench] serving mix: plain {mix['plain']['particle_steps_per_s']:.3e} particle-steps/s "
+              f"p50 {mix['plain']['p50_step_ms']:.3f} ms p99 {mix['plain']['p99_step_ms']:.3f} ms | batched "
+              f"{mix['batched']['particle_steps_per_s']:.3e} p50 {mix['batched']['p50_step_ms']:.3f} ms p99 "
+              f"{mix['batched']['p99_step_ms']:.3f} ms | x{mix['speedup']:.2f}, equal={mix['results_equal_alone_and_mixed']}",
+              file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -276,6 +325,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dem_collapse": dem,
+        "serving_mix": mix,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
