@@ -142,6 +142,18 @@ int demsph_sph_timing(demsph_sph* s, int index, char* name, size_t nameCap, long
 int demsph_sph_set_graphs(demsph_sph* s, int enable);
 int demsph_sph_graph_captures(demsph_sph* s, long* count);
 
+/* Kernel tuning knobs (no counterpart in the reference).  "pairLanes": lanes
+ * per fluid particle in the momentum and continuity passes, 1, 2, 4 or 8,
+ * or 0 to choose from the particle count (more lanes below a chip's worth
+ * of resident waves, where each particle's chain of neighbour gathers bounds
+ * the pass).  The neighbour sums are split over the lanes and combined in a
+ * fixed order, so a setting gives deterministic results; different settings
+ * differ by FP32 rounding.  "listLanes": lanes per fluid particle in the
+ * neighbour-list build (same values); every setting builds the same lists.
+ * demsph_sph_tune_get reads the value in effect. */
+int demsph_sph_tune(demsph_sph* s, const char* knob, int value);
+int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value);
+
 /* ------------------------------------------------------------------ DEM
  * Opaque DEMSolver scene (kernel/DEMSolver.h:144-2079). */
 typedef struct demsph_dem demsph_dem;

@@ -69,6 +69,8 @@ def _declare(L: C.CDLL) -> None:
         "demsph_sph_timing": [_vp, C.c_int, C.c_char_p, _sz, C.POINTER(C.c_long), C.POINTER(C.c_double)],
         "demsph_sph_set_graphs": [_vp, C.c_int],
         "demsph_sph_graph_captures": [_vp, C.POINTER(C.c_long)],
+        "demsph_sph_tune": [_vp, C.c_char_p, C.c_int],
+        "demsph_sph_tune_get": [_vp, C.c_char_p, C.POINTER(C.c_int)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -187,6 +189,15 @@ class SPHScene:
     def set_graphs(self, on: bool = True):
         """Replay captured two-step graphs in step() (on by default)."""
         check(lib().demsph_sph_set_graphs(self._h, 1 if on else 0))
+
+    def tune(self, knob: str, value: int) -> None:
+        """Kernel tuning knob ("pairLanes": 0 automatic, 1, 2, 4, 8)."""
+        check(lib().demsph_sph_tune(self._h, knob.encode(), int(value)))
+
+    def tuned(self, knob: str) -> int:
+        v = C.c_int()
+        check(lib().demsph_sph_tune_get(self._h, knob.encode(), C.byref(v)))
+        return v.value
 
     def graph_captures(self) -> int:
         n = C.c_long()

@@ -420,19 +420,98 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
     return count;
 }
 
-template <bool UNIF>
+// walk_list over L lanes of one point: in each run the lanes test
+// consecutive candidates (kWalkAhead per lane in flight), and a ballot over
+// the lane group ranks the hits, so the list holds the same entries in the
+// same order as walk_list's; entries are stored one int at a time.
+template <bool UNIF, int KIND, class PosT, int L>
+__device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __restrict__ B,
+                                               const int2* __restrict__ cells, double x, double y, double z,
+                                               double hA, int self, int* __restrict__ out, int K, int q)
+{
+    const GridGeom& g = a.grid;
+    const double cut_u = KIND == kSolid ? 2.0 * hA : 2.0 * fmax(hA, KIND == kDummy ? a.hd : a.hf);
+    const size_t cstride = (size_t)a.nf * kChunk;
+    const Stencil st = make_stencil(g, x, y, z);
+    int rs[9], re[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int dz = k / 3 - 1, dy = k % 3 - 1;
+        const bool row_ok = dz >= st.loz && dz <= st.hiz && dy >= st.loy && dy <= st.hiy;
+        const int row = (st.cz + dz) * g.ny * g.nx + (st.cy + dy) * g.nx + st.cx;
+        int s = -1, e = -1;
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const bool ok = row_ok && dx >= st.lox && dx <= st.hix;
+            const int2 c = cells[ok ? row + dx : 0];
+            const int2 r = ok ? c : make_int2(-1, -1);
+            if (r.x != -1) {
+                if (s < 0) s = r.x;
+                e = r.y;
+            }
+        }
+        rs[k] = s < 0 ? 0 : s;
+        re[k] = s < 0 ? 0 : e;
+    }
+    const int gshift = (threadIdx.x & 63) - q;  // first lane of the group in the wave
+    const unsigned below = (1u << q) - 1u;
+    int count = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int e = re[k];
+        for (int j0 = rs[k]; j0 < e; j0 += L * kWalkAhead) {
+            double qx[kWalkAhead], qy[kWalkAhead], qz[kWalkAhead];
+#pragma unroll
+            for (int u = 0; u < kWalkAhead; u++) {
+                const PosT* b = B + min(j0 + u * L + q, e - 1);
+                qx[u] = b->x; qy[u] = b->y; qz[u] = b->z;
+            }
+#pragma unroll
+            for (int u = 0; u < kWalkAhead; u++) {
+                const int j = j0 + u * L + q;
+                bool hit = j < e && !(KIND == kFluid && j == self);
+                if (hit) {
+                    double cut = cut_u;
+                    if (!UNIF && KIND != kSolid) {
+                        const double hB =
+                            KIND == kDummy ? a.phases[a.dphase[j]].h : a.phases[phase_bits(a.V[j].w)].h;
+                        cut = 2.0 * fmax(hA, hB);
+                    }
+                    hit = within_cut(cut, x - qx[u], y - qy[u], z - qz[u]);
+                }
+                const unsigned gm = (unsigned)(__ballot(hit) >> gshift) & ((1u << L) - 1u);
+                if (hit) {
+                    const int kk = count + __popc(gm & below);
+                    if (kk < K) out[(size_t)(kk >> 3) * cstride + (kk & (kChunk - 1))] = j;
+                }
+                count += __popc(gm);
+            }
+        }
+    }
+    return count;
+}
+
+template <bool UNIF, int LANES>
 __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
 {
-    const int i = xcd_block() * kThreads + threadIdx.x;
-    if (i >= a.nf) return;
+    const int gt = xcd_block() * kThreads + threadIdx.x;
+    const int i = gt / LANES, q = gt % LANES;
+    if (i >= a.nf) return;  // uniform over a lane group
     const FluidPos p = a.P[i];
     const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
-    const int c1 =
-        walk_list<UNIF, kFluid>(a, a.P, a.cells, p.x, p.y, p.z, h, i, a.nbr_ff + (size_t)i * kChunk, a.K_ff);
-    a.cnt_ff[i] = c1;
+    int* of = a.nbr_ff + (size_t)i * kChunk;
+    const int c1 = LANES == 1 ? walk_list<UNIF, kFluid>(a, a.P, a.cells, p.x, p.y, p.z, h, i, of, a.K_ff)
+                              : walk_list_lanes<UNIF, kFluid, FluidPos, LANES>(a, a.P, a.cells, p.x, p.y, p.z, h, i,
+                                                                               of, a.K_ff, q);
     int c2 = 0;
-    if (a.nd > 0 && a.dummy_near[cell_hash(a.grid, p.x, p.y, p.z)])
-        c2 = walk_list<UNIF, kDummy>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, a.nbr_fd + (size_t)i * kChunk, a.K_fd);
+    if (a.nd > 0 && a.dummy_near[cell_hash(a.grid, p.x, p.y, p.z)]) {
+        int* od = a.nbr_fd + (size_t)i * kChunk;
+        c2 = LANES == 1 ? walk_list<UNIF, kDummy>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, od, a.K_fd)
+                        : walk_list_lanes<UNIF, kDummy, DummyPos, LANES>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, od,
+                                                                         a.K_fd, q);
+    }
+    if (q != 0) return;
+    a.cnt_ff[i] = c1;
     a.cnt_fd[i] = c2;
     if (c1 > a.K_ff) atomicMax(&a.maxcount[0], c1);
     if (c2 > a.K_fd) atomicMax(&a.maxcount[1], c2);
@@ -465,37 +544,55 @@ __global__ __launch_bounds__(kThreads) void k_build_fs_lists(ListArgs a)
 constexpr int kBatch = DEMSPH_PAIR_BATCH;
 static_assert(kBatch == 4 || kBatch == 8, "pair batch is half or all of a chunk");
 
-template <class F>
-__device__ __forceinline__ void for_chunks(const int* __restrict__ list, int cnt, int K, int nf, int i, int safe,
-                                           F&& body)
+// LANES consecutive lanes share one particle: lane q takes the list chunks
+// q, q + LANES, ... (LANES = 1: every chunk in order), so a small scene
+// keeps more gathers in flight per particle.
+template <int LANES, class F>
+__device__ __forceinline__ void for_chunks(const int* __restrict__ list, int cnt, int K, int nf, int i, int q,
+                                           int safe, F&& body)
 {
     const int n = min(cnt, K);
     const int* L = list + (size_t)i * kChunk;
     const size_t cs = (size_t)nf * kChunk;
-    for (int base = 0; base < n; base += kBatch) {
-        const int* src = L + (size_t)(base / kChunk) * cs + (base % kChunk);
-        int js[kBatch];
-        const int4 lo = reinterpret_cast<const int4*>(src)[0];
-        js[0] = lo.x; js[1] = lo.y; js[2] = lo.z; js[3] = lo.w;
-        if (kBatch == 8) {
-            const int4 hi = reinterpret_cast<const int4*>(src)[1];
-            js[4 % kBatch] = hi.x; js[5 % kBatch] = hi.y; js[6 % kBatch] = hi.z; js[7 % kBatch] = hi.w;
-        }
+    for (int c = q; c * kChunk < n; c += LANES) {
 #pragma unroll
-        for (int q = 0; q < kBatch; q++) {
-            const bool valid = base + q < n;
-            body(valid ? js[q] : safe, valid);
+        for (int half = 0; half < kChunk; half += kBatch) {
+            const int base = c * kChunk + half;
+            if (base >= n) break;
+            const int* src = L + (size_t)c * cs + half;
+            int js[kBatch];
+            const int4 lo = reinterpret_cast<const int4*>(src)[0];
+            js[0] = lo.x; js[1] = lo.y; js[2] = lo.z; js[3] = lo.w;
+            if (kBatch == 8) {
+                const int4 hi = reinterpret_cast<const int4*>(src)[1];
+                js[4 % kBatch] = hi.x; js[5 % kBatch] = hi.y; js[6 % kBatch] = hi.z; js[7 % kBatch] = hi.w;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                const bool valid = base + u < n;
+                body(valid ? js[u] : safe, valid);
+            }
         }
     }
 }
 
+// Sum over the LANES lanes of a particle (a fixed butterfly order).
+template <int LANES, class T>
+__device__ __forceinline__ T lane_group_sum(T v)
+{
+#pragma unroll
+    for (int o = LANES / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // updateWCSPHAccelerationKernel (WCSPHIntegrationKernel.cu:42-148) fused
 // with the half kick (:150-159) and, in the first half, the drift (:161-170).
-template <bool FIRST, bool UNIF>
+template <bool FIRST, bool UNIF, int LANES>
 __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
 {
-    const int i = xcd_block() * kThreads + threadIdx.x;
-    if (i >= a.nf) return;
+    const int g = xcd_block() * kThreads + threadIdx.x;
+    const int i = g / LANES, q = g % LANES;
+    if (i >= a.nf) return;  // uniform over a lane group
     const FluidPos Pi = a.P[i];
     const float4 Vi = a.V[i];
     const double xi = Pi.x, yi = Pi.y, zi = Pi.z;
@@ -505,7 +602,7 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
     const float inv_rhoi = 1.0f / rhoi;
     float ax = 0.f, ay = 0.f, az = 0.f;
 
-    for_chunks(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, i, [&](int j, bool valid) {
+    for_chunks<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, [&](int j, bool valid) {
         const FluidPos Pj = a.P[j];
         const float4 Vj = a.V[j];
         const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
@@ -529,7 +626,7 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
         az += kp * rz + kv * (viz - Vj.z);
     });
     if (a.nd > 0) {
-        for_chunks(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, 0, [&](int j, bool valid) {
+        for_chunks<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, [&](int j, bool valid) {
             const DummyPos Dj = a.DP[j];
             const float4 DVj = a.DV[j];
             const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
@@ -555,22 +652,33 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
             az += kp * rz + kv * (viz - DVj.z);
         });
     }
+    if (LANES > 1) {
+        ax = lane_group_sum<LANES>(ax);
+        ay = lane_group_sum<LANES>(ay);
+        az = lane_group_sum<LANES>(az);
+    }
     if (a.ns > 0) {
         float cx = 0.f, cy = 0.f, cz = 0.f;
-        for_chunks(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, 0, [&](int j, bool valid) {
+        for_chunks<LANES>(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, q, 0, [&](int j, bool valid) {
             float3 ap, av, lv;
             fs_momentum(a, pi, xi, yi, zi, vix, viy, viz, rhoi, PL, a.SP[j], a.SV[j], a.SW[j], ap, av, lv);
             cx += valid ? ap.x + av.x : 0.0f;
             cy += valid ? ap.y + av.y : 0.0f;
             cz += valid ? ap.z + av.z : 0.0f;
         });
+        if (LANES > 1) {
+            cx = lane_group_sum<LANES>(cx);
+            cy = lane_group_sum<LANES>(cy);
+            cz = lane_group_sum<LANES>(cz);
+        }
         ax += cx; ay += cy; az += cz;
-        if (a.acs) {
+        if (a.acs && q == 0) {
             a.acs[3 * (size_t)i] = cx;
             a.acs[3 * (size_t)i + 1] = cy;
             a.acs[3 * (size_t)i + 2] = cz;
         }
     }
+    if (q != 0) return;
     ax += a.gx; ay += a.gy; az += a.gz;
     const double hdt = 0.5 * a.dt;
     const double nvx = a.vx[i] + hdt * (double)ax;
@@ -592,11 +700,12 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
 // updateWCSPHDensityChangeKernel (WCSPHIntegrationKernel.cu:172-264) fused
 // with the EOS update (:266-282).  With gradW = f r_ij and e = -r_ij/r:
 // (v_i - v*) . gradW = f [ (v_i - v_j) . r_ij / 2 + (P_L - P_R) r / (2 rho_bar c_bar) ].
-template <bool UNIF>
+template <bool UNIF, int LANES>
 __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
 {
-    const int i = xcd_block() * kThreads + threadIdx.x;
-    if (i >= a.nf) return;
+    const int g = xcd_block() * kThreads + threadIdx.x;
+    const int i = g / LANES, q = g % LANES;
+    if (i >= a.nf) return;  // uniform over a lane group
     const FluidPos Pi = a.P[i];
     const float4 Vi = a.V[i];
     const double xi = Pi.x, yi = Pi.y, zi = Pi.z;
@@ -604,7 +713,7 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
     const Phase pi = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
     const float rhoi = Pi.rho, PL = Pi.p;
     float d = 0.f;
-    for_chunks(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, i, [&](int j, bool valid) {
+    for_chunks<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, [&](int j, bool valid) {
         const FluidPos Pj = a.P[j];
         const float4 Vj = a.V[j];
         const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
@@ -619,7 +728,7 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
         d += pj.m * rcp(Pj.rho) * f * t;
     });
     if (a.nd > 0) {
-        for_chunks(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, 0, [&](int j, bool valid) {
+        for_chunks<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, [&](int j, bool valid) {
             const DummyPos Dj = a.DP[j];
             const float4 DVj = a.DV[j];
             const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
@@ -638,11 +747,13 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
         });
     }
     if (a.ns > 0) {
-        for_chunks(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, 0, [&](int j, bool valid) {
+        for_chunks<LANES>(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, q, 0, [&](int j, bool valid) {
             const float t = fs_density(a, pi, xi, yi, zi, vix, viy, viz, rhoi, PL, a.SP[j], a.SV[j], a.SW[j]);
             d += valid ? t : 0.0f;
         });
     }
+    if (LANES > 1) d = lane_group_sum<LANES>(d);
+    if (q != 0) return;
     const float nd = a.drho[i] + (float)(a.dt * (double)(2.0f * rhoi * d));
     a.odrho[i] = nd;
     FluidPos o;
@@ -1137,6 +1248,8 @@ struct demsph_sph {
     bool use_graphs = true;
     long graph_captures = 0;
     StepStamps stamps;  // step-end timestamps (batched serving)
+    int pair_lanes_set = 0;  // 0: chosen from the particle count
+    int list_lanes_set = 0;
 
     ~demsph_sph() { release(); }
 
@@ -1432,38 +1545,111 @@ struct demsph_sph {
         la.DP = DP; la.dphase = dphase; la.dcells = dcells; la.dummy_near = dummy_near;
         la.nbr_ff = nbr_ff; la.cnt_ff = cnt_ff; la.nbr_fd = nbr_fd; la.cnt_fd = cnt_fd;
         la.K_ff = K_ff; la.K_fd = K_fd; la.maxcount = maxcount;
-        if (uniform) DEMSPH_LAUNCH(timer, "sph_lists", k_build_lists<true>, gb, tb, 0, stream, la);
-        else DEMSPH_LAUNCH(timer, "sph_lists", k_build_lists<false>, gb, tb, 0, stream, la);
+        if (uniform) lists_lanes<true>(la);
+        else lists_lanes<false>(la);
         lists_valid = true;
+    }
+
+    // Lanes per fluid particle in the list build: the fewest of 2, 4, 8
+    // that put 600k threads in flight (measured on MI355X: 8 lanes at 2.7k
+    // and 35k particles, 4 at 256k; 2 beats 1 at every size measured).
+    int list_lanes() const
+    {
+        if (list_lanes_set > 0) return list_lanes_set;
+        return lanes_for(600000);
+    }
+    int lanes_for(long long fill) const
+    {
+        int L = 2;
+        while (L < 8 && (long long)nf * L < fill) L *= 2;
+        return L;
+    }
+    template <bool U, int L>
+    void lists_launch(const ListArgs& la)
+    {
+        const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
+        DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L>), gb, tb, 0, stream, la);
+    }
+    template <bool U>
+    void lists_lanes(const ListArgs& la)
+    {
+        switch (list_lanes()) {
+        case 2: lists_launch<U, 2>(la); break;
+        case 4: lists_launch<U, 4>(la); break;
+        case 8: lists_launch<U, 8>(la); break;
+        default: lists_launch<U, 1>(la); break;
+        }
+    }
+
+    // Lanes per fluid particle in the pair passes.  One lane per particle
+    // keeps every neighbour sum in list order, but each particle's chain of
+    // dependent gathers then bounds the pass; splitting the list over lanes
+    // shortens it.  The fewest of 2, 4, 8 that put 400k threads in flight
+    // (measured on MI355X: 8 lanes at 2.7k-35k particles, 4 at 110k, 2 at
+    // 256k and 741k, where 2 lanes are 20% faster than 1).
+    int pair_lanes() const
+    {
+        if (pair_lanes_set > 0) return pair_lanes_set;
+        return lanes_for(400000);
+    }
+
+    template <bool FIRST, bool U, int L>
+    void momentum_launch(const char* name, const SphArgs& a)
+    {
+        const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
+        DEMSPH_LAUNCH(timer, name, (k_momentum<FIRST, U, L>), gb, tb, 0, stream, a);
+    }
+    template <bool FIRST, bool U>
+    void momentum_lanes(const char* name, const SphArgs& a)
+    {
+        switch (pair_lanes()) {
+        case 2: momentum_launch<FIRST, U, 2>(name, a); break;
+        case 4: momentum_launch<FIRST, U, 4>(name, a); break;
+        case 8: momentum_launch<FIRST, U, 8>(name, a); break;
+        default: momentum_launch<FIRST, U, 1>(name, a); break;
+        }
+    }
+    template <bool U, int L>
+    void continuity_launch(const SphArgs& a)
+    {
+        const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
+        DEMSPH_LAUNCH(timer, "sph_continuity", (k_continuity<U, L>), gb, tb, 0, stream, a);
+    }
+    template <bool U>
+    void continuity_lanes(const SphArgs& a)
+    {
+        switch (pair_lanes()) {
+        case 2: continuity_launch<U, 2>(a); break;
+        case 4: continuity_launch<U, 4>(a); break;
+        case 8: continuity_launch<U, 8>(a); break;
+        default: continuity_launch<U, 1>(a); break;
+        }
     }
 
     void first_half_enqueue(double dt)
     {
         SphArgs a = args();
         a.dt = dt;
-        const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
-        if (uniform) DEMSPH_LAUNCH(timer, "sph_momentum_first", (k_momentum<true, true>), gb, tb, 0, stream, a);
-        else DEMSPH_LAUNCH(timer, "sph_momentum_first", (k_momentum<true, false>), gb, tb, 0, stream, a);
+        if (uniform) momentum_lanes<true, true>("sph_momentum_first", a);
+        else momentum_lanes<true, false>("sph_momentum_first", a);
         cP ^= 1; cV ^= 1;
     }
 
     void continuity_enqueue(double dt)
     {
-        const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
         SphArgs a = args();
         a.dt = dt;
-        if (uniform) DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity<true>, gb, tb, 0, stream, a);
-        else DEMSPH_LAUNCH(timer, "sph_continuity", k_continuity<false>, gb, tb, 0, stream, a);
+        if (uniform) continuity_lanes<true>(a);
+        else continuity_lanes<false>(a);
         cP ^= 1; cD ^= 1;
     }
 
     void momentum2_enqueue(double dt)
     {
-        const dim3 gb(blocks_for(nf, kThreads)), tb(kThreads);
         SphArgs a = args();
         a.dt = dt;
-        if (uniform) DEMSPH_LAUNCH(timer, "sph_momentum_second", (k_momentum<false, true>), gb, tb, 0, stream, a);
-        else DEMSPH_LAUNCH(timer, "sph_momentum_second", (k_momentum<false, false>), gb, tb, 0, stream, a);
+        if (uniform) momentum_lanes<false, true>("sph_momentum_second", a);
+        else momentum_lanes<false, false>("sph_momentum_second", a);
         cV ^= 1;
     }
 
@@ -1935,6 +2121,35 @@ extern "C" int demsph_sph_set_graphs(demsph_sph* s, int enable)
         if (!s) throw InvalidArg("null scene");
         s->use_graphs = enable != 0;
         if (!s->use_graphs) s->drop_graphs();
+    });
+}
+
+extern "C" int demsph_sph_tune(demsph_sph* s, const char* knob, int value)
+{
+    return guarded([&] {
+        if (!s || !knob) throw InvalidArg("null argument");
+        const std::string k = knob;
+        if (k == "pairLanes" || k == "listLanes") {
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+                throw InvalidArg(k + " must be 0 (automatic), 1, 2, 4 or 8");
+            s->drop_graphs();
+            (k == "pairLanes" ? s->pair_lanes_set : s->list_lanes_set) = value;
+            return;
+        }
+        throw InvalidArg("unknown tuning knob '" + k + "'");
+    });
+}
+
+extern "C" int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value)
+{
+    return guarded([&] {
+        if (!s || !knob || !value) throw InvalidArg("null argument");
+        const std::string k = knob;
+        if (k == "pairLanes" || k == "listLanes") {
+            *value = k == "pairLanes" ? s->pair_lanes() : s->list_lanes();
+            return;
+        }
+        throw InvalidArg("unknown tuning knob '" + k + "'");
     });
 }
 
