@@ -150,7 +150,10 @@ int demsph_sph_graph_captures(demsph_sph* s, long* count);
  * fixed order, so a setting gives deterministic results; different settings
  * differ by FP32 rounding.  "listLanes": lanes per fluid particle in the
  * neighbour-list build (same values); every setting builds the same lists.
- * demsph_sph_tune_get reads the value in effect. */
+ * "listPrefilter": 1 (default) tests list candidates first in FP32 on a
+ * 16-byte copy of the positions and only those within 1e-3 of the cut (or
+ * far from the grid origin) in exact FP64, 0 tests all in FP64; both give
+ * the same lists.  demsph_sph_tune_get reads the value in effect. */
 int demsph_sph_tune(demsph_sph* s, const char* knob, int value);
 int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value);
 

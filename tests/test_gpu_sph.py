@@ -180,6 +180,38 @@ def test_lane_settings_build_identical_lists_and_agree(small):
         assert np.abs(runs[0] - base).max() < 1e-6 * small.dx, L
 
 
+def test_prefilter_keeps_pairs_at_the_cut_exact():
+    """Shells of particles at |r| = cut (1 + u 1e-6), u in [-1, 1], around
+    seeded centres: every decision falls inside the FP32 prefilter's band and
+    is taken by the exact FP64 test, so the lists equal the oracle's with the
+    prefilter on and off, at every lane count."""
+    rng = np.random.default_rng(123)
+    dx = 0.01
+    cut = 2.0 * 1.3 * dx
+    pts = []
+    for c in rng.uniform(0.1, 0.4, (12, 3)):
+        d = rng.normal(size=(60, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        r = cut * (1.0 + rng.uniform(-1e-6, 1e-6, 60))
+        pts.append(c)
+        pts.append(c + d * r[:, None])
+    pos = np.concatenate([np.atleast_2d(p) for p in pts])
+    spec = scenes.SPHSceneSpec(fluid_pos=pos, dummy_pos=np.zeros((0, 3)), dx=dx, rho0=1000.0, c0=10.0, nu=1e-6,
+                               min_bound=np.zeros(3), max_bound=np.full(3, 0.5), gravity=np.array([0.0, 0.0, -9.81]),
+                               dt=1e-5)
+    o = oracle.OracleSPH(spec)
+    o.neighbor_search()
+    want = o.get("pointing_ff")
+    for pre in (1, 0):
+        for L in (2, 8):
+            s = make_gpu(spec)
+            s.tune("listPrefilter", pre)
+            s.tune("listLanes", L)
+            s.neighbor_search()
+            assert np.array_equal(s.get("neighborCount"), o.get("count_ff")), (pre, L)
+            assert np.array_equal(s.get("objectPointing"), want), (pre, L)
+
+
 def test_tait_eos_matches_oracle():
     spec = scenes.small_dam_break(eos=1)
     o = oracle.OracleSPH(spec)

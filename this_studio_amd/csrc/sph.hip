@@ -262,6 +262,16 @@ __global__ __launch_bounds__(kThreads) void k_hash(int n, GridGeom g, long long 
     key[orig[s]] = (uint32_t)cell_hash(g, p.x, p.y, p.z);
 }
 
+// FP32 copy of a position relative to the grid origin for the list
+// prefilter; w = 1 marks a point farther than R from the origin on some
+// axis, where the FP32 rounding could reach the prefilter's band.
+__device__ __forceinline__ float4 pos_f32(const GridGeom& g, double R, double x, double y, double z)
+{
+    const double rx = x - g.minx, ry = y - g.miny, rz = z - g.minz;
+    const bool far = fabs(rx) > R || fabs(ry) > R || fabs(rz) > R;
+    return make_float4((float)rx, (float)ry, (float)rz, far ? 1.f : 0.f);
+}
+
 struct ReorderArgs {
     int n;
     long long numGrids;
@@ -278,6 +288,9 @@ struct ReorderArgs {
     double *ovx, *ovy, *ovz;
     float* odrho;
     int2* cells;  // (start, end) per cell
+    float4* oPf;  // FP32 positions for the list prefilter (null: none)
+    GridGeom grid;
+    double R;
 };
 
 __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
@@ -288,7 +301,9 @@ __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
     const int s = a.rank[o];
     a.rank[o] = k;
     a.orig_out[k] = o;
-    a.oP[k] = a.P[s];
+    const FluidPos p = a.P[s];
+    a.oP[k] = p;
+    if (a.oPf) a.oPf[k] = pos_f32(a.grid, a.R, p.x, p.y, p.z);
     a.oV[k] = a.V[s];
     a.ovx[k] = a.vx[s]; a.ovy[k] = a.vy[s]; a.ovz[k] = a.vz[s];
     a.odrho[k] = a.drho[s];
@@ -321,6 +336,11 @@ struct ListArgs {
     const SolidPos* SP;
     int *nbr_fs, *cnt_fs;
     int K_fs;
+    // FP32 prefilter of the lane walks: positions relative to the grid
+    // origin (w = 1: far from it), and the relative half-width of the band
+    // around cut^2 in which the exact FP64 test decides
+    const float4 *Pf, *DPf;
+    float band;
 };
 
 enum WalkKind { kFluid = 0, kDummy = 1, kSolid = 2 };
@@ -424,10 +444,16 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
 // consecutive candidates (kWalkAhead per lane in flight), and a ballot over
 // the lane group ranks the hits, so the list holds the same entries in the
 // same order as walk_list's; entries are stored one int at a time.
-template <bool UNIF, int KIND, class PosT, int L>
+//
+// PRE: candidates are first tested in FP32 on the prefilter copy Bf; only
+// those within the band around cut^2 (or marked far from the origin) take
+// the exact FP64 test, so the decisions are those of within_cut (the host
+// sizes the band above the FP32 rounding of points within R of the origin).
+template <bool UNIF, int KIND, class PosT, int L, bool PRE>
 __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __restrict__ B,
-                                               const int2* __restrict__ cells, double x, double y, double z,
-                                               double hA, int self, int* __restrict__ out, int K, int q)
+                                               const float4* __restrict__ Bf, const int2* __restrict__ cells,
+                                               double x, double y, double z, float4 xf, double hA, int self,
+                                               int* __restrict__ out, int K, int q)
 {
     const GridGeom& g = a.grid;
     const double cut_u = KIND == kSolid ? 2.0 * hA : 2.0 * fmax(hA, KIND == kDummy ? a.hd : a.hf);
@@ -461,10 +487,16 @@ __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __
         const int e = re[k];
         for (int j0 = rs[k]; j0 < e; j0 += L * kWalkAhead) {
             double qx[kWalkAhead], qy[kWalkAhead], qz[kWalkAhead];
+            float4 qf[kWalkAhead];
 #pragma unroll
             for (int u = 0; u < kWalkAhead; u++) {
-                const PosT* b = B + min(j0 + u * L + q, e - 1);
-                qx[u] = b->x; qy[u] = b->y; qz[u] = b->z;
+                const int jj = min(j0 + u * L + q, e - 1);
+                if (PRE) {
+                    qf[u] = Bf[jj];
+                } else {
+                    const PosT* b = B + jj;
+                    qx[u] = b->x; qy[u] = b->y; qz[u] = b->z;
+                }
             }
 #pragma unroll
             for (int u = 0; u < kWalkAhead; u++) {
@@ -477,7 +509,20 @@ __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __
                             KIND == kDummy ? a.phases[a.dphase[j]].h : a.phases[phase_bits(a.V[j].w)].h;
                         cut = 2.0 * fmax(hA, hB);
                     }
-                    hit = within_cut(cut, x - qx[u], y - qy[u], z - qz[u]);
+                    if (PRE) {
+                        const float fx = xf.x - qf[u].x, fy = xf.y - qf[u].y, fz = xf.z - qf[u].z;
+                        const float d2 = fx * fx + fy * fy + fz * fz;
+                        const float c2 = (float)(cut * cut);
+                        const bool exact = xf.w != 0.f || qf[u].w != 0.f || fabsf(d2 - c2) <= a.band * c2;
+                        if (exact) {
+                            const PosT* b = B + j;
+                            hit = within_cut(cut, x - b->x, y - b->y, z - b->z);
+                        } else {
+                            hit = d2 < c2;
+                        }
+                    } else {
+                        hit = within_cut(cut, x - qx[u], y - qy[u], z - qz[u]);
+                    }
                 }
                 const unsigned gm = (unsigned)(__ballot(hit) >> gshift) & ((1u << L) - 1u);
                 if (hit) {
@@ -491,24 +536,25 @@ __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __
     return count;
 }
 
-template <bool UNIF, int LANES>
+template <bool UNIF, int LANES, bool PRE>
 __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
 {
     const int gt = xcd_block() * kThreads + threadIdx.x;
     const int i = gt / LANES, q = gt % LANES;
     if (i >= a.nf) return;  // uniform over a lane group
     const FluidPos p = a.P[i];
+    const float4 pf = PRE ? a.Pf[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
     int* of = a.nbr_ff + (size_t)i * kChunk;
     const int c1 = LANES == 1 ? walk_list<UNIF, kFluid>(a, a.P, a.cells, p.x, p.y, p.z, h, i, of, a.K_ff)
-                              : walk_list_lanes<UNIF, kFluid, FluidPos, LANES>(a, a.P, a.cells, p.x, p.y, p.z, h, i,
-                                                                               of, a.K_ff, q);
+                              : walk_list_lanes<UNIF, kFluid, FluidPos, LANES, PRE>(
+                                    a, a.P, a.Pf, a.cells, p.x, p.y, p.z, pf, h, i, of, a.K_ff, q);
     int c2 = 0;
     if (a.nd > 0 && a.dummy_near[cell_hash(a.grid, p.x, p.y, p.z)]) {
         int* od = a.nbr_fd + (size_t)i * kChunk;
         c2 = LANES == 1 ? walk_list<UNIF, kDummy>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, od, a.K_fd)
-                        : walk_list_lanes<UNIF, kDummy, DummyPos, LANES>(a, a.DP, a.dcells, p.x, p.y, p.z, h, -1, od,
-                                                                         a.K_fd, q);
+                        : walk_list_lanes<UNIF, kDummy, DummyPos, LANES, PRE>(
+                              a, a.DP, a.DPf, a.dcells, p.x, p.y, p.z, pf, h, -1, od, a.K_fd, q);
     }
     if (q != 0) return;
     a.cnt_ff[i] = c1;
@@ -1050,6 +1096,9 @@ struct DummyReorderArgs {
     uint8_t* ophase;
     int* orig;
     int2* cells;
+    float4* DPf;  // FP32 positions for the list prefilter (null: none)
+    GridGeom grid;
+    double R;
 };
 
 __global__ __launch_bounds__(kThreads) void k_dummy_reorder(DummyReorderArgs a)
@@ -1062,6 +1111,7 @@ __global__ __launch_bounds__(kThreads) void k_dummy_reorder(DummyReorderArgs a)
     d.x = a.px[o]; d.y = a.py[o]; d.z = a.pz[o];
     d.nx = 0.f; d.ny = 0.f;
     a.DP[k] = d;
+    if (a.DPf) a.DPf[k] = pos_f32(a.grid, a.R, d.x, d.y, d.z);
     a.DV[k] = make_float4((float)a.vx[o], (float)a.vy[o], (float)a.vz[o], 0.f);
     a.ophase[k] = a.phase[o];
     const uint32_t h = a.skey[k];
@@ -1250,6 +1300,11 @@ struct demsph_sph {
     StepStamps stamps;  // step-end timestamps (batched serving)
     int pair_lanes_set = 0;  // 0: chosen from the particle count
     int list_lanes_set = 0;
+    // FP32 prefilter of the list walks (exact decisions; see pos_f32)
+    bool prefilter = true;
+    static constexpr float kPrefilterBand = 1.0e-3f;
+    double pref_R = 0.0;
+    float4 *Pf = nullptr, *DPf = nullptr;
 
     ~demsph_sph() { release(); }
 
@@ -1270,7 +1325,7 @@ struct demsph_sph {
         dfree(rank); dfree(key); dfree(skey); dfree(ktmp); dfree(perm); dfree(vtmp);
         if (sort_temp) (void)hipFree(sort_temp);
         sort_temp = nullptr;
-        dfree(cells); dfree(nbr_ff); dfree(cnt_ff); dfree(nbr_fd); dfree(cnt_fd);
+        dfree(cells); dfree(nbr_ff); dfree(cnt_ff); dfree(nbr_fd); dfree(cnt_fd); dfree(Pf); dfree(DPf);
         dfree(maxcount);
         if (maxcount_h) (void)hipHostFree(maxcount_h);
         maxcount_h = nullptr;
@@ -1359,10 +1414,11 @@ struct demsph_sph {
         const size_t cells = grid_set(gg, minb, maxb, staged_cell_size()) ? (size_t)gg.numGrids() : 0;
         const size_t rec = sizeof(FluidPos) + sizeof(float4) + 3 * sizeof(double) + sizeof(float) + sizeof(int);
         const size_t per_fluid = 2 * rec + rec + sizeof(int)             // two buffers, checkpoint, rank
+                                 + sizeof(float4)                         // FP32 list-prefilter copy
                                  + 2 * sizeof(int)                        // list counts
                                  + sizeof(int) * (size_t)(round_chunk(K_ff) + round_chunk(K_fd));
         const size_t per_sort = 3 * sizeof(uint32_t) + 2 * sizeof(int);
-        const size_t per_dummy = sizeof(DummyPos) + sizeof(float4) + 1 + sizeof(int);
+        const size_t per_dummy = sizeof(DummyPos) + 2 * sizeof(float4) + 1 + sizeof(int);
         // initialisation uploads positions, velocities and phases through
         // temporary buffers, which the runtime's allocator keeps cached
         const size_t upload = 6 * sizeof(double) + 1;
@@ -1418,6 +1474,17 @@ struct demsph_sph {
         perm = dmalloc<int>(nsort); vtmp = dmalloc<int>(nsort);
         DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(nsort)));
         cells = dmalloc<int2>(numGrids);
+        Pf = dmalloc<float4>(nf);
+        DPf = dmalloc<float4>(nd > 0 ? nd : 1);
+        // Points within pref_R of the grid origin on every axis carry FP32
+        // copies whose rounding stays inside the prefilter band for pairs at
+        // the smallest cut: |d2_f32 - d2| <= 2 sqrt(3) cut 2^-22 R + O(2^-24)
+        // cut^2 <= band cut^2 / 2 for R = 500 cut_min.
+        {
+            double hmin = 1e300;
+            for (const Phase& q : phases) hmin = std::min(hmin, q.h);
+            pref_R = 500.0 * 2.0 * hmin;
+        }
         cnt_ff = dmalloc<int>(n); cnt_fd = dmalloc<int>(n);
         maxcount = dmalloc<int>(3);
         DEMSPH_HIP(hipHostMalloc((void**)&maxcount_h, 3 * sizeof(int), hipHostMallocDefault));
@@ -1476,6 +1543,7 @@ struct demsph_sph {
             ra.px = ux; ra.py = uy; ra.pz = uz; ra.vx = wx; ra.vy = wy; ra.vz = wz; ra.phase = uph;
             ra.DP = DP; ra.DV = DV; ra.ophase = dphase;
             ra.orig = dorig; ra.cells = dcells;
+            ra.DPf = DPf; ra.grid = grid; ra.R = pref_R;
             hipLaunchKernelGGL(k_dummy_reorder, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, stream, ra);
             NormalArgs na{};
             na.nd = (int)m; na.grid = grid; na.phases = d_phases; na.DP = DP; na.DV = DV;
@@ -1536,6 +1604,7 @@ struct demsph_sph {
         ra.oP = P[1 - cP]; ra.oV = V[1 - cV]; ra.ovx = vx[1 - cV]; ra.ovy = vy[1 - cV]; ra.ovz = vz[1 - cV];
         ra.odrho = drho[1 - cD];
         ra.cells = cells;
+        ra.oPf = prefilter ? Pf : nullptr; ra.grid = grid; ra.R = pref_R;
         DEMSPH_LAUNCH(timer, "sph_reorder", k_reorder, gb, tb, 0, stream, ra);
         cP ^= 1; cV ^= 1; cD ^= 1; cO ^= 1;
         cells_track_skey = true;
@@ -1545,6 +1614,7 @@ struct demsph_sph {
         la.DP = DP; la.dphase = dphase; la.dcells = dcells; la.dummy_near = dummy_near;
         la.nbr_ff = nbr_ff; la.cnt_ff = cnt_ff; la.nbr_fd = nbr_fd; la.cnt_fd = cnt_fd;
         la.K_ff = K_ff; la.K_fd = K_fd; la.maxcount = maxcount;
+        la.Pf = Pf; la.DPf = DPf; la.band = kPrefilterBand;
         if (uniform) lists_lanes<true>(la);
         else lists_lanes<false>(la);
         lists_valid = true;
@@ -1568,7 +1638,8 @@ struct demsph_sph {
     void lists_launch(const ListArgs& la)
     {
         const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
-        DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L>), gb, tb, 0, stream, la);
+        if (L > 1 && prefilter) DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, true>), gb, tb, 0, stream, la);
+        else DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, false>), gb, tb, 0, stream, la);
     }
     template <bool U>
     void lists_lanes(const ListArgs& la)
@@ -2129,6 +2200,12 @@ extern "C" int demsph_sph_tune(demsph_sph* s, const char* knob, int value)
     return guarded([&] {
         if (!s || !knob) throw InvalidArg("null argument");
         const std::string k = knob;
+        if (k == "listPrefilter") {
+            if (value != 0 && value != 1) throw InvalidArg("listPrefilter must be 0 or 1");
+            s->drop_graphs();
+            s->prefilter = value != 0;
+            return;
+        }
         if (k == "pairLanes" || k == "listLanes") {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
                 throw InvalidArg(k + " must be 0 (automatic), 1, 2, 4 or 8");
@@ -2147,6 +2224,10 @@ extern "C" int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value)
         const std::string k = knob;
         if (k == "pairLanes" || k == "listLanes") {
             *value = k == "pairLanes" ? s->pair_lanes() : s->list_lanes();
+            return;
+        }
+        if (k == "listPrefilter") {
+            *value = s->prefilter ? 1 : 0;
             return;
         }
         throw InvalidArg("unknown tuning knob '" + k + "'");
