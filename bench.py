@@ -50,9 +50,9 @@ def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> flo
     if kernel == "sph_continuity":
         return nf * (60 + 36) + lists      # read P, V, drho, counts; write P', drho'
     if kernel == "sph_lists":
-        return nf * 24 + lists             # read own x; write the lists and counts
+        return nf * 40 + lists             # read own x and its FP32 copy; write the lists and counts
     if kernel == "sph_reorder":
-        return nf * 168                    # gather P, V, v64, drho, ids; write them
+        return nf * 184                    # gather P, V, v64, drho, ids; write them and the FP32 copy
     return 0.0
 
 
@@ -60,11 +60,21 @@ def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> flo
 # this workload (HBM bytes = 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE, from a
 # plain 256 MiB copy), as recorded in profiles/wcsph_dam_break_256k_kernels.md.
 MEASURED_HBM_BYTES = {
-    "sph_lists": (24.3 + 102.1) * 1e6,
-    "sph_momentum_first": (120.7 + 18.4) * 1e6,
-    "sph_momentum_second": (120.4 + 10.2) * 1e6,
-    "sph_continuity": (115.4 + 9.2) * 1e6,
-    "sph_reorder": (25.3 + 22.9) * 1e6,
+    "sph_lists": (23.0 + 83.8) * 1e6,
+    "sph_momentum_first": (124.2 + 18.4) * 1e6,
+    "sph_momentum_second": (123.3 + 10.2) * 1e6,
+    "sph_continuity": (118.1 + 9.2) * 1e6,
+    "sph_reorder": (24.4 + 26.0) * 1e6,
+}
+
+
+# What bounds each step kernel on this workload (hardware counters, kernel trace).
+LIMITER = {
+    "sph_lists": "instruction issue: ~420 candidate cut tests per particle (SQ counters), not HBM",
+    "sph_momentum_first": "latency of dependent neighbour gathers",
+    "sph_momentum_second": "latency of dependent neighbour gathers",
+    "sph_continuity": "latency of dependent neighbour gathers",
+    "sph_reorder": "HBM",
 }
 
 
@@ -278,7 +288,8 @@ def main() -> int:
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": MEASURED_HBM_BYTES.get(dom),
-                "avg_launch_us": round(d_avg_s * 1e6, 2), "algorithmic_bytes": int(d_bytes)}
+                "avg_launch_us": round(d_avg_s * 1e6, 2), "algorithmic_bytes": int(d_bytes),
+                "limiter": LIMITER.get(dom)}
     fl = algorithmic_flops(dom, pairs_ff, pairs_fd)
     if fl:
         roofline["fp32_valu_tflops"] = round(fl / d_avg_s / 1e12, 2)
