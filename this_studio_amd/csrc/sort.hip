@@ -23,6 +23,45 @@ constexpr int kItems = 8;                     // keys per lane
 constexpr int kTile = kSortThreads * kItems;  // keys per workgroup
 
 int sort_blocks(int n) { return (n + kTile - 1) / kTile; }
+// Up to this many tiles (64k keys) a scatter workgroup reads the whole
+// histogram table itself instead of waiting for a device-wide scan.
+constexpr int kFusedScatterTiles = 32;
+
+// ----------------------------------------------------------------- scan
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int wave_inclusive_sum(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ int block_exclusive(int v, int& total)
+{
+    __shared__ int wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inc = wave_inclusive_sum(v);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int wprefix = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; w++) {
+        if (w < wave) wprefix += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    total = tot;
+    return wprefix + inc - v;
+}
+
 
 // Histogram of one digit per tile, written digit-major so that one
 // exclusive scan over [digit][block] yields every scatter base.
@@ -44,18 +83,39 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
 
 // Stable scatter.  Tile element order is wave-major, then item, then lane,
 // which is the global index order inside the tile.
+//
+// FUSED (few tiles): `table` is the digit-major histogram itself and each
+// workgroup derives its scatter bases from it (thread d sums digit d over
+// all tiles and over the tiles before its own, then a workgroup scan over
+// the digits), which replaces the device-wide scan's three launches.
+// Otherwise `table` holds the scanned offsets.
+static_assert(kBins == kSortThreads, "one thread per digit in the fused scatter");
+template <bool FUSED>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
                                                                 const int* __restrict__ vals_in,
                                                                 uint32_t* __restrict__ keys_out,
                                                                 int* __restrict__ vals_out, int n, int shift,
-                                                                const uint32_t* __restrict__ offsets, int nblocks)
+                                                                const uint32_t* __restrict__ table, int nblocks)
 {
     __shared__ uint32_t wave_count[kWaves][kBins];
     __shared__ uint32_t block_base[kBins];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * kBins; i += kSortThreads) (&wave_count[0][0])[i] = 0;
-    for (int d = threadIdx.x; d < kBins; d += kSortThreads) block_base[d] = offsets[(size_t)d * nblocks + blockIdx.x];
+    if (FUSED) {
+        const int d = threadIdx.x;
+        uint32_t tot = 0, before = 0;
+        for (int b = 0; b < nblocks; b++) {
+            const uint32_t c = table[(size_t)d * nblocks + b];
+            tot += c;
+            before += b < (int)blockIdx.x ? c : 0u;
+        }
+        int total;
+        const int ex = block_exclusive((int)tot, total);
+        block_base[d] = (uint32_t)ex + before;
+    } else {
+        for (int d = threadIdx.x; d < kBins; d += kSortThreads) block_base[d] = table[(size_t)d * nblocks + blockIdx.x];
+    }
     __syncthreads();
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -106,41 +166,6 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
         keys_out[dst] = key[it];
         vals_out[dst] = val[it];
     }
-}
-
-// ----------------------------------------------------------------- scan
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-__device__ __forceinline__ int wave_inclusive_sum(int v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// Block-wide exclusive scan of one value per thread; returns the block total.
-__device__ __forceinline__ int block_exclusive(int v, int& total)
-{
-    __shared__ int wsum[kScanThreads / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int inc = wave_inclusive_sum(v);
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    int wprefix = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kScanThreads / 64; w++) {
-        if (w < wave) wprefix += wsum[w];
-        tot += wsum[w];
-    }
-    __syncthreads();
-    total = tot;
-    return wprefix + inc - v;
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ partial)
@@ -278,8 +303,15 @@ void radix_sort_pairs(const uint32_t* keys_in, const int* vals_in, uint32_t* key
         (void)last;
         const int shift = p * kRadixBits;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
-        exclusive_scan(reinterpret_cast<const int*>(hist), reinterpret_cast<int*>(offs), kBins * nb, scan_tmp, nullptr, s);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst, vdst, n, shift, offs, nb);
+        if (nb <= kFusedScatterTiles) {
+            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst, vdst, n,
+                               shift, hist, nb);
+        } else {
+            exclusive_scan(reinterpret_cast<const int*>(hist), reinterpret_cast<int*>(offs), kBins * nb, scan_tmp,
+                           nullptr, s);
+            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst, vdst, n,
+                               shift, offs, nb);
+        }
         DEMSPH_HIP(hipGetLastError());
         ksrc = kdst;
         vsrc = vdst;
