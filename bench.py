@@ -147,6 +147,69 @@ def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     return out
 
 
+def cantilever_clumps(steps: int) -> dict:
+    """The bonded cantilever with falling clumps: a 128 x 16 x 16 lattice of
+    32,768 bonded 2 mm spheres, clamped at x = 0, with a 10 N tip load ramped by
+    the hook, and 20,000 tetrahedral 4-sphere clumps above it (112,768 balls in
+    all), dt = 5e-7 s.  Ball-steps per second over `steps` steps after solve()
+    setup (bonds from the initial pairs) and 20 warm-up steps, state
+    resident."""
+    import this_studio_amd as ts
+    from this_studio_amd import scenes
+
+    s = scenes.cantilever_with_clumps()
+    g = ts.DEMSceneGPU(s)
+    g.step(20)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    n, nc = g.sizes()[0], g.sizes()[1]
+    nbond = int(g.get("bondPointed").size)
+    tip = float(g.get("position")[:, 2].min())
+    g.close()
+    return {"workload": "dem_cantilever_clumps", "balls": n, "clumps": nc, "bonds": nbond, "steps": steps,
+            "value": round(n * steps / el, 1), "unit": "ball-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
+            "lowest_ball_z_m": tip,
+            "precision": "fp32 contact and bond forces, fp64 positions/velocities/springs/bond state"}
+
+
+def ice_cylinder_coupled(steps: int) -> dict:
+    """The coupled ice sheet on a current: ~980k WCSPH fluid particles at dx =
+    4 mm moving at 0.5 m/s in a 2.0 x 0.8 m tank, ~50k bonded ice balls (r =
+    4 mm, rho = 917) in two layers on the surface, a 2,048-facet cylinder of R
+    = 0.1 m (DEM wall, and dummy particles for the fluid), two-way coupling,
+    dt = 5e-6 s.  Particle-steps (fluid + balls) per second over `steps`
+    coupled steps after setup and 5 warm-up steps, state resident."""
+    import this_studio_amd as ts
+    from this_studio_amd import scenes, serving
+
+    spec, d = scenes.ice_cylinder_coupled()
+    f = serving.staged_sph(spec)
+    f.initialize(spec.min_bound, spec.max_bound, spec.gravity)
+    g = ts.DEMSceneGPU(d)
+    c = ts.CoupledScene(f, g)
+    c.step(spec.dt, 5, spec.gravity)
+    f.synchronize()
+    t0 = time.perf_counter()
+    c.step(spec.dt, steps, spec.gravity)
+    f.synchronize()
+    el = time.perf_counter() - t0
+    nf, nd = f.sizes()[0], f.sizes()[1]
+    nb, ntri = g.sizes()[0], g.sizes()[3]
+    bonds = g.get("bondIsBonded")
+    out = {"workload": "ice_cylinder_coupled", "fluid_particles": nf, "dummy_particles": nd, "ice_balls": nb,
+           "bonds": int(bonds.size), "bonds_intact": int(bonds.sum()), "triangles": ntri, "steps": steps,
+           "dt": spec.dt, "value": round((nf + nb) * steps / el, 1), "unit": "particle-steps/s (fluid + balls)",
+           "ms_per_step": round(el / steps * 1e3, 4),
+           "precision": "fp32 pair and contact forces, fp64 positions/velocities/springs"}
+    c.close()
+    g.close()
+    f.close()
+    return out
+
+
 def serving_mix(count: int, scale: float, slots: int) -> dict:
     """The batched-serving job stream (256 scenes of 2k-200k particles,
     log-uniform; 50/30/20 DEM/WCSPH/coupled; 1k-20k steps; Poisson arrivals
@@ -195,6 +258,8 @@ def main() -> int:
     ap.add_argument("--dem-steps", type=int, default=2000, help="steps of the DEM collapse leg (0: skip it)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--mix-scenes", type=int, default=48, help="scenes of the serving-mix leg (0: skip it)")
+    ap.add_argument("--beam-steps", type=int, default=400, help="steps of the cantilever-with-clumps leg (0: skip it)")
+    ap.add_argument("--ice-steps", type=int, default=40, help="steps of the coupled ice-cylinder leg (0: skip it)")
     ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
     ap.add_argument("--mix-slots", type=int, default=16, help="concurrent jobs of the batched server")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -306,6 +371,18 @@ def main() -> int:
         print("[bench] dem granular collapse ...", file=sys.stderr, flush=True)
         dem = dem_collapse(args.dem_steps, min(args.cpu_budget, 10.0), not args.no_cpu_baseline)
         print(f"[bench] dem collapse: {dem['ms_per_step'] * 1e3:.1f} us/step", file=sys.stderr, flush=True)
+    beam = None
+    if args.beam_steps > 0:
+        print("[bench] bonded cantilever + clumps ...", file=sys.stderr, flush=True)
+        beam = cantilever_clumps(args.beam_steps)
+        print(f"[bench] cantilever + clumps: {beam['balls']} balls, {beam['bonds']} bonds, "
+              f"{beam['ms_per_step'] * 1e3:.1f} us/step", file=sys.stderr, flush=True)
+    ice = None
+    if args.ice_steps > 0:
+        print("[bench] coupled ice cylinder ...", file=sys.stderr, flush=True)
+        ice = ice_cylinder_coupled(args.ice_steps)
+        print(f"[bench] ice cylinder: {ice['fluid_particles']} fluid + {ice['ice_balls']} balls, "
+              f"{ice['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     mix = None
     if args.mix_scenes > 0:
         mix = serving_mix(args.mix_scenes, args.mix_scale, args.mix_slots)
@@ -337,6 +414,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "dem_collapse": dem,
         "serving_mix": mix,
+        "cantilever_clumps": beam,
+        "ice_cylinder_coupled": ice,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }

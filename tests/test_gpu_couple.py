@@ -123,3 +123,32 @@ def test_coupled_steps_match_oracle():
     assert np.abs(gd.get("velocity") - od.get("vel")).max() < 1e-5 * 0.1
     assert np.abs(gf.get("position") - of.get("pos")).max() < 1e-9
     assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5
+
+
+def test_ice_sheet_on_current_matches_oracle():
+    """A reduced coupled ice scene of the bench's kind (40 x 20 x 6 fluid on
+    a current, bonded two-layer ice sheet, a 64-facet cylinder as DEM wall and
+    fluid dummies): bonds from the initial pairs, 10 coupled steps against
+    the FP64 oracle of the same model."""
+    import oracle
+
+    spec, d = scenes.ice_cylinder_coupled(nx=40, ny=20, nz=6, R=0.012, n_around=8, n_height=4)
+    of = oracle.OracleSPH(spec)
+    od = oracle.OracleDEM(d)
+    oc = oracle.OracleCoupled(of, od)
+    gf, gd = fluid_scene(spec), ts.DEMSceneGPU(d)
+    assert np.array_equal(gd.get("bondPointed"), od.get("bond_pointed"))
+    gc = ts.CoupledScene(gf, gd)
+    oc.step(spec.dt, 10, spec.gravity)
+    gc.step(spec.dt, 10, spec.gravity)
+    Fo, Fg = oc.get("couplingForce"), gc.get("couplingForce")
+    # The sheet rests on the surface: most balls meet fluid only near the
+    # edge of the support, where FP32 evaluation of (1 - q/2) -> 0 has a large
+    # relative but tiny absolute error (forces ~1e-7 N).  Compare on the scale
+    # of the largest of the coupling force and a ball's weight.
+    r = 0.004
+    weight = 917.0 * 4.0 / 3.0 * np.pi * r ** 3 * 9.81
+    assert np.abs(Fg - Fo).max() < 1e-4 * max(np.abs(Fo).max(), weight)
+    assert np.abs(gd.get("position") - od.get("pos")).max() < 1e-9
+    assert np.abs(gf.get("position") - of.get("pos")).max() < 1e-9
+    assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5

@@ -180,6 +180,28 @@ def test_clumps_on_plane_match_oracle():
     assert np.abs(g.get("position") - o.get("pos")).max() < 1e-7
 
 
+def test_cantilever_with_clumps_matches_oracle():
+    """A reduced bonded cantilever with falling tetrahedral clumps (16 x 4 x 4
+    beam, 40 clumps; the bench's scene at full size): bonds from the initial
+    pairs, ramped tip load through the hook, clumps under gravity; 3,000 steps
+    against the FP64 oracle (FP32 contact and bond forces: 1e-5 of the
+    diameter)."""
+    s = scenes.cantilever_with_clumps(nx=16, ny=4, nz=4, n_clumps=40, ramp_steps=1000)
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s)
+    assert np.array_equal(g.get("bondPointed"), o.get("bond_pointed"))
+    assert np.array_equal(g.get("bondPointing"), o.get("bond_pointing"))
+    o.step(3000)
+    g.step(3000)
+    d = 0.004
+    assert np.abs(g.get("position") - o.get("pos")).max() / d < 1e-5
+    assert np.abs(g.get("clumpPosition") - o.get("clump_pos")).max() / d < 1e-5
+    # the tip layer sags under the ramped load
+    x0 = np.asarray(s.initial_positions())
+    tip = np.isclose(x0[:, 0], 15 * d) & (x0[:, 2] < 4 * d)
+    assert g.get("position")[tip, 2].mean() < x0[tip, 2].mean() - 1e-7
+
+
 def test_user_hook_through_views_equals_builtin_hook():
     """addExternalForceTorque through the C-ABI callback, calling the HIP
     launcher on the insertion-ordered FP64 views, against the built-in

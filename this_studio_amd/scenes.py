@@ -585,3 +585,157 @@ def serving_mix(count: int = 256, seed: int = 20261202, n_range=(2.0e3, 2.0e5), 
             t += -mean_interarrival * math.log(max(1.0 - u[k, 3], 1e-300))
         jobs.append(MixJob(kind, n, steps, t, seed + 7919 * (k + 1)))
     return jobs
+
+
+# ---------------------------------------------------------------------------
+# Bonded cantilever with falling tetrahedral clumps
+# ---------------------------------------------------------------------------
+def quat_uniform(u1: float, u2: float, u3: float) -> np.ndarray:
+    """Uniformly distributed unit quaternion (w, x, y, z) from three uniforms
+    (Shoemake, Graphics Gems III, "Uniform random rotations")."""
+    a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+    t1, t2 = 2.0 * math.pi * u2, 2.0 * math.pi * u3
+    x, y, z, w = a * math.sin(t1), a * math.cos(t1), b * math.sin(t2), b * math.cos(t2)
+    return np.array([w, x, y, z])
+
+
+def quat_rotate(q: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Rotates the rows of v by the unit quaternion q = (w, x, y, z)."""
+    w, u = q[0], q[1:]
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def cantilever_with_clumps(nx: int = 128, ny: int = 16, nz: int = 16, r: float = 0.002, n_clumps: int = 20000,
+                           E: float = 1e9, tip_force: float = 10.0, ramp_steps: int = 10000,
+                           seed: int = 20261202) -> DEMScene:
+    """Bonded DEM cantilever with falling clumps.
+
+    * The beam is an nx x ny x nz simple-cubic lattice of touching spheres (spacing 2r, rho 2500). Its x = 0
+      layer is fixed.
+    * Parallel bonds come from the initial pair list: E = 1e9, gamma = 1, kn/ks = 2.6, sigma_t = C = 0 (never
+      breaking).
+    * The tip load is split evenly over the x = max layer and ramped linearly over `ramp_steps` steps by the hook.
+    * Above the beam are n_clumps rigid tetrahedral 4-sphere clumps (pebble radius r, pebble centres r from the
+      clump centre), with seeded Shoemake-uniform orientations on a 3r-clearance lattice over the beam's
+      footprint.
+    * Hertz-Mindlin contacts everywhere (E* and G* of E, nu = 0.3), e = 0.5, mu = 0.3.
+    * Gravity -z.  dt = 5e-7 s, 1/65 of the bond oscillation period 2 pi sqrt(m / (E pi r / 2)).
+    """
+    s = DEMScene("cantilever_clumps")
+    rho, nu = 2500.0, 0.3
+    Es, Gs = E / (2 * (1 - nu * nu)), E / (4 * (2 - nu) * (1 + nu))
+    s.set_bonded_pair(0, 0, 1.0, E, 2.6, 0.0, 0.0, 0.0)
+    for a, b in ((0, 0), (0, 1), (1, 1)):
+        s.set_hertzian_pair(a, b, Es, Gs, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    beam = lattice(nx, ny, nz, 2.0 * r)
+    fixed = beam[:, 0] < 0.5 * r
+    s.add_fixed_balls(beam[fixed], r, 0)
+    s.add_balls(beam[~fixed], [0, 0, 0], [0, 0, 0], r, rho, 0)
+    n_beam = beam.shape[0]
+    tip = beam[~fixed][:, 0] > (nx - 1.5) * 2.0 * r
+    # clumps
+    d = r
+    tet = np.array([[1.0, 1.0, 1.0], [1.0, -1.0, -1.0], [-1.0, 1.0, -1.0], [-1.0, -1.0, 1.0]]) / math.sqrt(3.0) * d
+    mp = rho * 4.0 / 3.0 * math.pi * r ** 3
+    I = 4.0 * (0.4 * mp * r * r + mp * (2.0 / 3.0) * d * d)
+    pitch = 2.0 * (d + r) + 3.0 * r
+    cx = max(1, int((nx * 2.0 * r) // pitch))
+    cy = max(1, int((ny * 2.0 * r) // pitch))
+    u = uniform01(seed, 3 * n_clumps).reshape(n_clumps, 3)
+    z0 = nz * 2.0 * r + pitch
+    for k in range(n_clumps):
+        i, j, l = k % cx, (k // cx) % cy, k // (cx * cy)
+        c = np.array([(i + 0.5) * pitch, (j + 0.5) * pitch, z0 + l * pitch])
+        q = quat_uniform(*u[k])
+        s.add_clump(quat_rotate(q, tet) + c, np.full(4, r), c, [0, 0, 0], [0, 0, 0], 4 * mp, [I, I, I, 0, 0, 0], 1)
+    n_total = n_beam + 4 * n_clumps
+    f = np.zeros((n_total, 3))
+    # insertion order: fixed layer, free beam balls, then the clumps' pebbles
+    nfix = int(fixed.sum())
+    f[nfix + np.flatnonzero(tip), 2] = -tip_force / int(tip.sum())
+    dt = 5e-7
+    s.hooks = [("const", f, ramp_steps * dt)]
+    s.bonds_from_initial_pairs = True
+    top = z0 + (n_clumps // (cx * cy) + 1) * pitch
+    s.min_bound = np.array([-4.0 * r, -4.0 * r, -0.5])
+    s.max_bound = np.array([nx * 2.0 * r + 4.0 * r, ny * 2.0 * r + 4.0 * r, top])
+    s.gravity = np.array([0.0, 0.0, -9.81])
+    s.dt = dt
+    return s
+
+
+# ---------------------------------------------------------------------------
+# Coupled ice sheet, current and cylinder
+# ---------------------------------------------------------------------------
+def ice_cylinder_coupled(nx: int = 500, ny: int = 200, nz: int = 10, dx: float = 0.004, R: float = 0.1,
+                         n_around: int = 32, n_height: int = 32, current: float = 0.5, ice_cover: float = 0.93,
+                         eos: int = 1):
+    """Coupled DEM-SPH ice sheet pushed by a current onto a rigid cylinder.
+
+    * Water: an nx x ny x nz lattice at spacing dx filling an open tank (3-layer dummy shell), moving at
+      `current` m/s along +x, Tait EOS, c0 = 10 sqrt(2 g H).
+    * Cylinder: vertical, radius R, at the tank centre; a DEM mesh wall of 2 n_around n_height triangles, and,
+      for the fluid, dummy particles in the annulus R - 2.5 dx <= r <= R - 0.5 dx.  Fluid within R + 0.5 dx of
+      its axis is removed.
+    * Ice: two stacked hex layers of bonded balls (r = dx, rho = 917) on the water surface over the first
+      `ice_cover` of the tank's length, drifting with the current.  Bonds come from the initial pairs
+      (sigma_t = C = 0.5 MPa, mu = 0.1, kn/ks = 2.6, as iceCylinderInteraction.cpp:85).  Linear contacts use
+      kn from a 50 dt contact time.
+    * The tank box is also a DEM wall.
+
+    dt = 5e-6 s.  Returns (fluid spec, DEM scene), sharing the fluid's bounds.
+    """
+    g, rho0, dt = 9.81, 1000.0, 5e-6
+    Lx, Ly = nx * dx, ny * dx
+    H = nz * dx
+    tank = (Lx, Ly + dx, 2.5 * H + 4 * dx)
+    fluid = lattice(nx, ny, nz, dx)
+    cxy = np.array([0.5 * Lx, 0.5 * Ly])
+    rf = np.linalg.norm(fluid[:, :2] - cxy, axis=1)
+    fluid = fluid[rf >= R + 0.5 * dx]
+    shell, mn, mx = tank_dummies(tank, dx)
+    # cylinder annulus on the same lattice, from the floor shell to the tank top
+    ca = int(math.ceil(R / dx)) + 1
+    ix = np.arange(int(cxy[0] / dx) - ca, int(cxy[0] / dx) + ca + 1)
+    iy = np.arange(int(cxy[1] / dx) - ca, int(cxy[1] / dx) + ca + 1)
+    iz = np.arange(-3, int(tank[2] / dx) + 1)
+    zz, yy, xx = np.meshgrid(iz * dx, iy * dx, ix * dx, indexing="ij")
+    cand = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1)
+    rc = np.linalg.norm(cand[:, :2] - cxy, axis=1)
+    ring = cand[(rc >= R - 2.5 * dx) & (rc <= R - 0.5 * dx)]
+    dummies = np.concatenate([shell, ring])
+    c0 = 10.0 * math.sqrt(2.0 * g * H)
+    vel = np.tile([current, 0.0, 0.0], (fluid.shape[0], 1))
+    spec = SPHSceneSpec(fluid_pos=fluid, dummy_pos=dummies, dx=dx, rho0=rho0, c0=c0, nu=1.0e-6, min_bound=mn,
+                        max_bound=mx, gravity=np.array([0.0, 0.0, -g]), dt=dt, eos=eos, fluid_vel=vel,
+                        name="ice_cylinder_coupled")
+    # ice
+    r, den = dx, 917.0
+    tc = 50.0 * dt
+    m = den * 4.0 / 3.0 * math.pi * r ** 3
+    kn = 0.5 * m * math.pi ** 2 / tc / tc
+    E = kn * 2.0 * r / (math.pi * r * r)
+    d = DEMScene("ice_cylinder_coupled")
+    d.set_bonded_pair(0, 0, 1.0, E, 2.6, 0.5e6, 0.5e6, 0.1)
+    d.set_linear_pair(0, 0, kn, kn / 2.6, 0.0, 0.0, 0.1, 0.1, 0.0, 0.0, 0.1, 0.0, 0.0)
+    d.set_linear_pair(0, 1, kn, kn / 2.6, 0.0, 0.0, 0.1, 0.1, 0.0, 0.0, 0.1, 0.0, 0.0)
+    ztop = (nz - 1) * dx
+    lo = [2.0 * r, 2.0 * r, ztop + r]
+    hi = [ice_cover * Lx, Ly - 2.0 * r, ztop + r]
+    l1 = hex_sheet(lo, hi, 2.0 * r)
+    l2 = hex_sheet([lo[0] + r, lo[1] + r / math.sqrt(3.0), ztop + r + 2.0 * r * math.sqrt(2.0 / 3.0)],
+                   [hi[0], hi[1], 0.0], 2.0 * r)
+    l2[:, 2] = ztop + r + 2.0 * r * math.sqrt(2.0 / 3.0)
+    ice = np.concatenate([l1, l2])
+    ice = ice[np.linalg.norm(ice[:, :2] - cxy, axis=1) >= R + 2.0 * r]
+    d.add_balls(ice, [current, 0.0, 0.0], [0.0, 0.0, 0.0], r, den, 0)
+    v, t0, t1, t2 = cylinder_mesh([cxy[0], cxy[1], 0.5 * tank[2]], [0.0, 0.0, 1.0], R, tank[2], n_around, n_height,
+                                  caps=False)
+    d.add_mesh_wall(v, t0, t1, t2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 1)
+    bv, b0, b1, b2 = box_mesh([0.0, 0.0, 0.0], [Lx, Ly + dx, tank[2]])
+    d.add_mesh_wall(bv, b0, b1, b2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 1)
+    d.bonds_from_initial_pairs = True
+    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
+    d.dt = dt
+    return spec, d
