@@ -877,6 +877,7 @@ struct RevArgs {
     const int* rstart;
     int* rfluid;
     int rcap;  // entries rfluid holds; a list overflow (reported by the fluid side) may exceed it
+    double hf; // the fluid's smoothing length when it has one phase
 };
 
 // Reverse index of the fluid -> ball lists: for ball j, the fluid slots i
@@ -886,7 +887,7 @@ struct RevArgs {
 // finds every such i; rows are walked in cell order, so the slots come out
 // ascending.  COUNT: rcnt[j] = their number; else they fill
 // rfluid[rstart[j] ...] through a wave ballot (lane order = slot order).
-template <bool COUNT>
+template <bool COUNT, bool UNIF>
 __global__ __launch_bounds__(256) void k_fs_rev_walk(RevArgs a)
 {
     const int lane = wave_lane();
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(256) void k_fs_rev_walk(RevArgs a)
             bool hit = false;
             if (i < e) {
                 const FluidPos p = a.P[i];
-                const double h = a.phases[phase_bits(a.V[i].w)].h;
+                const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
                 hit = within_cut(2.0 * h, p.x - bx, p.y - by, p.z - bz);
             }
             const unsigned long long m = __ballot(hit);
@@ -1238,6 +1239,8 @@ struct demsph_sph {
     std::vector<uint8_t> fphase_h, dphase_h;
     Phase* d_phases = nullptr;
     bool uniform = false;  // one fluid phase and at most one dummy phase
+    bool fluid_one_phase = false;
+    bool fluid_uniform() const { return fluid_one_phase; }
     SolidView solids;      // coupled balls (ns == 0 outside coupled stepping)
     Phase phf{}, phd{};
 
@@ -1457,6 +1460,7 @@ struct demsph_sph {
         for (int i = 1; i < nf; i++) fu = fu && fphase_h[i] == fphase_h[0];
         for (int i = 1; i < nd; i++) du = du && dphase_h[i] == dphase_h[0];
         uniform = fu && du;
+        fluid_one_phase = fu;
         phf = phases[fphase_h[0]];
         phd = nd > 0 ? phases[dphase_h[0]] : phf;
         d_phases = dmalloc<Phase>(phases.size());
@@ -2406,11 +2410,16 @@ struct demsph_couple {
         RevArgs ra{};
         ra.ns = ns; ra.grid = f->grid; ra.cells = f->cells; ra.P = f->P[f->cP]; ra.V = f->V[f->cV];
         ra.phases = f->d_phases; ra.SP = SP; ra.rcnt = rcnt; ra.rstart = rstart; ra.rfluid = rfluid; ra.rcap = rcap();
+        ra.hf = f->phf.h;
         const dim3 wb(blocks_for(ns, kBallsPerBlock)), wt(64 * kBallsPerBlock);
-        hipLaunchKernelGGL(k_fs_rev_walk<true>, wb, wt, 0, st, ra);
+        // one fluid phase: every fluid particle has the same support
+        const bool unif = f->fluid_uniform();
+        if (unif) hipLaunchKernelGGL((k_fs_rev_walk<true, true>), wb, wt, 0, st, ra);
+        else hipLaunchKernelGGL((k_fs_rev_walk<true, false>), wb, wt, 0, st, ra);
         DEMSPH_HIP(hipGetLastError());
         exclusive_scan(rcnt, rstart, ns, scan_temp, rstart + ns, st);
-        hipLaunchKernelGGL(k_fs_rev_walk<false>, wb, wt, 0, st, ra);
+        if (unif) hipLaunchKernelGGL((k_fs_rev_walk<false, true>), wb, wt, 0, st, ra);
+        else hipLaunchKernelGGL((k_fs_rev_walk<false, false>), wb, wt, 0, st, ra);
         DEMSPH_HIP(hipGetLastError());
         f->timer.end(st);
         pressure(st);
