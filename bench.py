@@ -239,11 +239,14 @@ def serving_mix(count: int, scale: float, slots: int) -> dict:
                      "p50_step_ms": round(st["p50"] * 1e3, 4), "p99_step_ms": round(st["p99"] * 1e3, 4),
                      "jobs_done": st["jobsDone"], "jobs_failed": st["jobsFailed"], "peak_active": st["peakActive"],
                      "peak_admitted_GB": round(st["peakBytes"] / 1e9, 3)}
+        errs = [f"{j.kind} {j.n}: {srv.info(i)['error']}" for j, i in zip(jobs, ids) if srv.info(i)["status"]]
+        if errs:
+            out[name]["errors"] = errs[:5]
         results[name] = [(srv.result(i, "position") if j.kind != "dem" else srv.result(i, "ballPosition"))
-                         for j, i in zip(jobs, ids)]
+                         if not srv.info(i)["status"] else None for j, i in zip(jobs, ids)]
         srv.close()
     out["speedup"] = round(out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"], 3)
-    out["results_equal_alone_and_mixed"] = all(np.array_equal(a, b)
+    out["results_equal_alone_and_mixed"] = all(a is not None and b is not None and np.array_equal(a, b)
                                                for a, b in zip(results["plain"], results["batched"]))
     return out
 
