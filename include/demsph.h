@@ -288,6 +288,14 @@ int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity,
  * reference's getMeshWallBody() write path). */
 int demsph_dem_set(demsph_dem* d, const char* field, const void* in, size_t count);
 
+/* Step graphs (no counterpart in the reference): demsph_dem_step replays the
+ * four steps between two pair rebuilds as one captured hipGraph when the
+ * scene runs on a stream of its own (not the null stream), has no hooks, its
+ * walls are static and timing is off; rebuild steps stay eager.  On by
+ * default; bit-identical to eager steps.  demsph_dem_graph_captures counts
+ * the captures. */
+int demsph_dem_set_graphs(demsph_dem* d, int enable);
+int demsph_dem_graph_captures(demsph_dem* d, long* count);
 int demsph_dem_enable_timing(demsph_dem* d, int enable);
 int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long* launches, double* totalMs);
 

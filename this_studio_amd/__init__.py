@@ -287,6 +287,8 @@ def _declare_dem(L):
         "demsph_dem_sizes": [_vp] + [C.POINTER(_sz)] * 6,
         "demsph_dem_get": [_vp, C.c_char_p, _vp, _sz, C.POINTER(_sz)],
         "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
+        "demsph_dem_set_graphs": [_vp, C.c_int],
+        "demsph_dem_graph_captures": [_vp, C.POINTER(C.c_long)],
         "demsph_dem_enable_timing": [_vp, C.c_int],
         "demsph_dem_timing": [_vp, C.c_int, C.c_char_p, _sz, C.POINTER(C.c_long), C.POINTER(C.c_double)],
     }
@@ -467,6 +469,16 @@ class DEMSceneGPU:
     def set(self, field: str, values):
         v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))
         check(self.L.demsph_dem_set(self._h, field.encode(), v.ctypes.data_as(_vp), v.size))
+
+    def set_graphs(self, on: bool = True):
+        """Replay captured four-step graphs between pair rebuilds (on by
+        default; needs a scene stream other than the null stream)."""
+        check(self.L.demsph_dem_set_graphs(self._h, 1 if on else 0))
+
+    def graph_captures(self) -> int:
+        n = C.c_long()
+        check(self.L.demsph_dem_graph_captures(self._h, C.byref(n)))
+        return n.value
 
     def enable_timing(self, on: bool = True):
         check(self.L.demsph_dem_enable_timing(self._h, 1 if on else 0))

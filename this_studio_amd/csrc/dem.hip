@@ -1017,10 +1017,39 @@ struct demsph_dem {
     // clock
     size_t iStep = 0;
     double time = 0.0;
+    // Captured graphs of the four steps between two pair rebuilds.  The
+    // rebuild flips the ball and pair buffers (cb, cp) and may reallocate the
+    // pair arrays, so a graph is keyed by what its launches bake in.
+    struct GraphKey {
+        double dt, gx, gy, gz;
+        int cb, cp, nbond, fp64;
+        const void *pairs, *rev, *cand, *stamp;
+        bool operator==(const GraphKey& o) const
+        {
+            return dt == o.dt && gx == o.gx && gy == o.gy && gz == o.gz && cb == o.cb && cp == o.cp &&
+                   nbond == o.nbond && fp64 == o.fp64 && pairs == o.pairs && rev == o.rev && cand == o.cand &&
+                   stamp == o.stamp;
+        }
+    };
+    struct GraphEntry {
+        GraphKey key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    bool use_graphs = true;
+    long graph_captures = 0;
+
+    void drop_graphs()
+    {
+        if (!graphs.empty()) (void)hipStreamSynchronize(stream);
+        for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
+        graphs.clear();
+    }
 
     ~demsph_dem()
     {
         (void)hipStreamSynchronize(stream);
+        drop_graphs();
         if (sort_temp) (void)hipFree(sort_temp);
         if (tsort_temp) (void)hipFree(tsort_temp);
         if (tscan_temp) (void)hipFree(tscan_temp);
@@ -1615,6 +1644,61 @@ struct demsph_dem {
         check_overflow();
     }
 
+    // Steps between rebuilds can replay a captured graph when nothing in
+    // them involves the host: no hooks (they take host times or call back),
+    // static or no walls (moving walls re-sort their triangles with a host
+    // read), a stream of the scene's own (the null stream cannot be
+    // captured) and no kernel timing.
+    bool graphs_ok() const
+    {
+        return use_graphs && !timer.enabled && stream != nullptr && hooks.empty() && !hook_fn &&
+               (nt == 0 || (walls_static && tri_cells_valid && walls_valid));
+    }
+
+    void plain_steps(const double* g, double dt, int count)
+    {
+        const double h = 0.5 * dt;
+        for (int k = 0; k < count; k++) {
+            evaluate(h, dt);
+            half(true, g, h);
+            evaluate(h, dt);
+            half(false, g, h);
+            step_stamp(stamps, stream);
+        }
+    }
+
+    hipGraphExec_t quad_graph(const double* g, double dt)
+    {
+        GraphKey key{};
+        key.dt = dt; key.gx = g[0]; key.gy = g[1]; key.gz = g[2];
+        key.cb = cb; key.cp = cp; key.nbond = nbond; key.fp64 = fp64 ? 1 : 0;
+        key.pairs = ppartner[cp].p; key.rev = rpair.p; key.cand = tcand_tri.p; key.stamp = stamps.buf;
+        for (auto& e : graphs)
+            if (e.key == key) return e.exec;
+        hipGraph_t gr = nullptr;
+        DEMSPH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            plain_steps(g, dt, 4);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        DEMSPH_HIP(hipStreamEndCapture(stream, &gr));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        DEMSPH_HIP(e);
+        if (graphs.size() >= 4) {
+            DEMSPH_HIP(hipStreamSynchronize(stream));
+            (void)hipGraphExecDestroy(graphs.front().exec);
+            graphs.erase(graphs.begin());
+        }
+        graphs.push_back({key, ex});
+        graph_captures++;
+        return ex;
+    }
+
     // loop_ball / loop_ball_wall (DEMSolver.h:1376-1405, 1452-1485)
     void step(const double* g, double dt, int nsteps)
     {
@@ -1622,7 +1706,17 @@ struct demsph_dem {
         if (!has_list) throw InvalidArg("solve_begin() (or neighbor_search) must precede step()");
         if (!(dt > 0)) throw InvalidArg("time step must be positive");
         const double h = 0.5 * dt;
-        for (int k = 0; k < nsteps; k++) {
+        for (int k = 0; k < nsteps;) {
+            // right after a rebuild (or at the start) the next four steps rebuild nothing
+            if (iStep % 5 == 0 && nsteps - k >= 4 && graphs_ok()) {
+                DEMSPH_HIP(hipGraphLaunch(quad_graph(g, dt), stream));
+                for (int q = 0; q < 4; q++) {
+                    iStep++;
+                    time += dt;
+                }
+                k += 4;
+                continue;
+            }
             iStep++;
             time += dt;
             if (iStep % 5 == 0) neighbor_search();
@@ -1633,6 +1727,7 @@ struct demsph_dem {
             run_hooks(time);
             half(false, g, h);
             step_stamp(stamps, stream);
+            k++;
         }
         check_overflow();
     }
@@ -1674,7 +1769,11 @@ size_t dem_bytes_estimate(demsph_dem* d, const double* minb, const double* maxb)
     return d->bytes_estimate(minb, maxb);
 }
 int dem_num_balls(demsph_dem* d) { return d->initialized ? d->n : (int)d->hb.size(); }
-void dem_set_stamps(demsph_dem* d, const StepStamps& st) { d->stamps = st; }
+void dem_set_stamps(demsph_dem* d, const StepStamps& st)
+{
+    d->drop_graphs();  // captured graphs hold the previous stamp buffer
+    d->stamps = st;
+}
 
 }  // namespace demsph
 
@@ -2224,6 +2323,22 @@ extern "C" int demsph_dem_set(demsph_dem* d, const char* field, const void* in, 
             return;
         }
         throw InvalidArg("unknown field '" + f + "'");
+    });
+}
+
+extern "C" int demsph_dem_set_graphs(demsph_dem* d, int enable)
+{
+    DEM_GUARD({
+        d->use_graphs = enable != 0;
+        if (!d->use_graphs) d->drop_graphs();
+    });
+}
+
+extern "C" int demsph_dem_graph_captures(demsph_dem* d, long* count)
+{
+    DEM_GUARD({
+        if (!count) throw InvalidArg("null argument");
+        *count = d->graph_captures;
     });
 }
 

@@ -10,8 +10,8 @@
 // destroys it, which returns its memory and slot.  Each slot is a host
 // thread, so a host synchronisation inside one job's step block (the DEM
 // pair-list resize) stalls only that slot while the other slots' jobs keep
-// the device busy.  WCSPH jobs replay captured two-step graphs when graphs
-// are on.  Every step ends with a device timestamp, from which the per-job
+// the device busy.  With graphs on, WCSPH jobs replay captured two-step
+// graphs and DEM jobs the four steps between pair rebuilds.  Every step ends with a device timestamp, from which the per-job
 // step latencies (intervals between consecutive step completions) come.
 //
 // One slot with graphs off is the plain path: the same jobs served one at a
@@ -160,6 +160,7 @@ struct demsph_server {
                 st = own;
             }
             if (j.dem) {
+                ok(demsph_dem_set_graphs(j.dem, graphs ? 1 : 0));
                 dem_set_stream(j.dem, st);
                 ok(demsph_dem_initialize(j.dem, j.dmin, j.dmax));
                 ok(demsph_dem_solve_begin(j.dem));
