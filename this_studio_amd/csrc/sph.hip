@@ -123,13 +123,15 @@ __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x);
 // sigma/h * [-2 t^3 (1+2q) + 2 t^4] / r equals -5 sigma t^3 / h^2 with
 // t = 1 - q/2, q = r/h, sigma = 21 / (16 pi h^3).  With a uniform h every
 // factor but t is loop-invariant and hoisted by the compiler.
+// Evaluated unconditionally and selected, so that a pair loop has no
+// divergent branch between its gathers and their use.
 __device__ __forceinline__ float grad_factor(float r, float h)
 {
-    if (r < 1.e-20f || r >= 2.0f * h) return 0.0f;
     const float invh = 1.0f / h;
     const float t = 1.0f - 0.5f * r * invh;
     const float sigma = (float)(21.0 / (16.0 * M_PI)) * invh * invh * invh;
-    return -5.0f * sigma * invh * invh * (t * t * t);
+    const float f = -5.0f * sigma * invh * invh * (t * t * t);
+    return (r < 1.e-20f || r >= 2.0f * h) ? 0.0f : f;
 }
 
 // Inverse EOS for the dummy density (WCSPHIntegrationKernel.cu:132,249):
@@ -622,6 +624,50 @@ __device__ __forceinline__ void for_chunks(const int* __restrict__ list, int cnt
     }
 }
 
+// for_chunks in two phases per batch: every entry's records are gathered
+// (load) before any is used (use), so kBatch independent gathers are in
+// flight per lane.  Same entries, same order as for_chunks.
+template <int LANES, class LoadF, class UseF>
+__device__ __forceinline__ void for_chunks_gathered(const int* __restrict__ list, int cnt, int K, int nf, int i, int q,
+                                                    int safe, LoadF&& load, UseF&& use)
+{
+    const int n = min(cnt, K);
+    const int* L = list + (size_t)i * kChunk;
+    const size_t cs = (size_t)nf * kChunk;
+    for (int c = q; c * kChunk < n; c += LANES) {
+#pragma unroll
+        for (int half = 0; half < kChunk; half += kBatch) {
+            const int base = c * kChunk + half;
+            if (base >= n) break;
+            const int* src = L + (size_t)c * cs + half;
+            int js[kBatch];
+            const int4 lo = reinterpret_cast<const int4*>(src)[0];
+            js[0] = lo.x; js[1] = lo.y; js[2] = lo.z; js[3] = lo.w;
+            if (kBatch == 8) {
+                const int4 hi = reinterpret_cast<const int4*>(src)[1];
+                js[4 % kBatch] = hi.x; js[5 % kBatch] = hi.y; js[6 % kBatch] = hi.z; js[7 % kBatch] = hi.w;
+            }
+            decltype(load(0)) rec[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                js[u] = base + u < n ? js[u] : safe;
+                rec[u] = load(js[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) use(rec[u], js[u], base + u < n);
+        }
+    }
+}
+
+struct FluidRec {
+    FluidPos P;
+    float4 V;
+};
+struct DummyRec {
+    DummyPos D;
+    float4 V;
+};
+
 // Sum over the LANES lanes of a particle (a fixed butterfly order).
 template <int LANES, class T>
 __device__ __forceinline__ T lane_group_sum(T v)
@@ -648,9 +694,11 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
     const float inv_rhoi = 1.0f / rhoi;
     float ax = 0.f, ay = 0.f, az = 0.f;
 
-    for_chunks<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, [&](int j, bool valid) {
-        const FluidPos Pj = a.P[j];
-        const float4 Vj = a.V[j];
+    const auto load_f = [&](int j) { return FluidRec{a.P[j], a.V[j]}; };
+    const auto load_d = [&](int j) { return DummyRec{a.DP[j], a.DV[j]}; };
+    for_chunks_gathered<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, load_f, [&](const FluidRec& R, int, bool valid) {
+        const FluidPos& Pj = R.P;
+        const float4& Vj = R.V;
         const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
         const float rx = (float)(xi - Pj.x), ry = (float)(yi - Pj.y), rz = (float)(zi - Pj.z);
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -672,9 +720,9 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
         az += kp * rz + kv * (viz - Vj.z);
     });
     if (a.nd > 0) {
-        for_chunks<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, [&](int j, bool valid) {
-            const DummyPos Dj = a.DP[j];
-            const float4 DVj = a.DV[j];
+        for_chunks_gathered<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, load_d, [&](const DummyRec& R, int j, bool valid) {
+            const DummyPos& Dj = R.D;
+            const float4& DVj = R.V;
             const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
             const float rx = (float)(xi - Dj.x), ry = (float)(yi - Dj.y), rz = (float)(zi - Dj.z);
             const float r2 = rx * rx + ry * ry + rz * rz;
@@ -759,9 +807,11 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
     const Phase pi = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
     const float rhoi = Pi.rho, PL = Pi.p;
     float d = 0.f;
-    for_chunks<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, [&](int j, bool valid) {
-        const FluidPos Pj = a.P[j];
-        const float4 Vj = a.V[j];
+    const auto load_f = [&](int j) { return FluidRec{a.P[j], a.V[j]}; };
+    const auto load_d = [&](int j) { return DummyRec{a.DP[j], a.DV[j]}; };
+    for_chunks_gathered<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, load_f, [&](const FluidRec& R, int, bool valid) {
+        const FluidPos& Pj = R.P;
+        const float4& Vj = R.V;
         const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
         const float rx = (float)(xi - Pj.x), ry = (float)(yi - Pj.y), rz = (float)(zi - Pj.z);
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -774,9 +824,9 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
         d += pj.m * rcp(Pj.rho) * f * t;
     });
     if (a.nd > 0) {
-        for_chunks<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, [&](int j, bool valid) {
-            const DummyPos Dj = a.DP[j];
-            const float4 DVj = a.DV[j];
+        for_chunks_gathered<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, load_d, [&](const DummyRec& R, int j, bool valid) {
+            const DummyPos& Dj = R.D;
+            const float4& DVj = R.V;
             const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
             const float rx = (float)(xi - Dj.x), ry = (float)(yi - Dj.y), rz = (float)(zi - Dj.z);
             const float r2 = rx * rx + ry * ry + rz * rz;
