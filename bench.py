@@ -60,10 +60,10 @@ def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> flo
 # this workload (HBM bytes = 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE, from a
 # plain 256 MiB copy), as recorded in profiles/wcsph_dam_break_256k_kernels.md.
 MEASURED_HBM_BYTES = {
-    "sph_lists": (23.0 + 83.8) * 1e6,
-    "sph_momentum_first": (124.2 + 18.4) * 1e6,
-    "sph_momentum_second": (123.3 + 10.2) * 1e6,
-    "sph_continuity": (118.1 + 9.2) * 1e6,
+    "sph_lists": (23.0 + 83.7) * 1e6,
+    "sph_momentum_first": (125.7 + 18.4) * 1e6,
+    "sph_momentum_second": (124.9 + 10.2) * 1e6,
+    "sph_continuity": (120.1 + 9.2) * 1e6,
     "sph_reorder": (24.4 + 26.0) * 1e6,
 }
 
@@ -103,10 +103,26 @@ def cpu_baseline(spec, budget_s: float = 15.0) -> dict:
         if el >= budget_s or steps >= 50:
             break
     threads = int(oracle.lib().o_num_threads())
+    # The GPU path over the same 1 + steps steps from the same state: error
+    # against the FP64 oracle at full size.
+    import this_studio_amd as ts
+
+    g = ts.SPHScene()
+    g.set_eos(spec.eos, spec.gamma)
+    g.add_fluid(spec.fluid_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
+    g.add_dummy(spec.dummy_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
+    g.initialize(spec.min_bound, spec.max_bound, spec.gravity)
+    g.step(spec.dt, 1 + steps)
+    parity = {"steps": 1 + steps,
+              "max_dx_over_spacing": float(abs(g.get("position") - o.get("pos")).max() / spec.dx),
+              "max_dv_m_per_s": float(abs(g.get("velocity") - o.get("vel")).max()),
+              "max_drho_kg_per_m3": float(abs(g.get("density") - o.get("rho")).max())}
+    g.close()
     o.close()
     return {"value": spec.nf * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle (C, OpenMP) on the same {spec.nf}-fluid dam break, {steps} steps after "
-                      f"1 warm-up step ({el:.1f} s; setup {t1 - t0:.1f} s excluded)"}
+                      f"1 warm-up step ({el:.1f} s; setup {t1 - t0:.1f} s excluded)",
+            "gpu_vs_oracle": parity}
 
 
 def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
