@@ -146,9 +146,15 @@ class OracleSPH:
 
         fv = spec.fluid_vel if spec.fluid_vel is not None else np.zeros((nf, 3))
         dv = spec.dummy_vel if spec.dummy_vel is not None else np.zeros((nd, 3))
+        # per-particle fluid properties when the spec carries them (several phases)
+        fc = getattr(spec, "fluid_c0", None)
+        fdx = getattr(spec, "fluid_dx", None)
+        fr = getattr(spec, "fluid_rho0", None)
+        fnu = getattr(spec, "fluid_nu", None)
         self.h = lib().o_sph_create(
-            nf, arr(spec.fluid_pos, nf, 3), arr(fv, nf, 3), arr(spec.c0, nf), arr(spec.dx, nf), arr(spec.rho0, nf),
-            arr(spec.nu, nf), nd, arr(spec.dummy_pos, max(nd, 1), 3) if nd else None, arr(dv, max(nd, 1), 3) if nd else None,
+            nf, arr(spec.fluid_pos, nf, 3), arr(fv, nf, 3), arr(spec.c0 if fc is None else fc, nf),
+            arr(spec.dx if fdx is None else fdx, nf), arr(spec.rho0 if fr is None else fr, nf),
+            arr(spec.nu if fnu is None else fnu, nf), nd, arr(spec.dummy_pos, max(nd, 1), 3) if nd else None, arr(dv, max(nd, 1), 3) if nd else None,
             arr(spec.c0, max(nd, 1)), arr(spec.dx, max(nd, 1)), arr(spec.rho0, max(nd, 1)), arr(spec.nu, max(nd, 1)),
             arr(spec.min_bound, 3), arr(spec.max_bound, 3), arr(spec.gravity, 3), int(spec.eos), float(spec.gamma),
             int(nthreads))

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 def fluid_scene(spec, drho=None):
     s = ts.SPHScene()
     s.set_eos(spec.eos, spec.gamma)
-    s.add_fluid(spec.fluid_pos, spec.fluid_vel, spec.c0, spec.dx, spec.rho0, spec.nu)
+    s.add_fluid(spec.fluid_pos, spec.fluid_vel, *spec.fluid_props())
     if spec.nd:
         s.add_dummy(spec.dummy_pos, spec.dummy_vel, spec.c0, spec.dx, spec.rho0, spec.nu)
     s.initialize(spec.min_bound, spec.max_bound, spec.gravity)
@@ -151,4 +151,28 @@ def test_ice_sheet_on_current_matches_oracle():
     assert np.abs(Fg - Fo).max() < 1e-4 * max(np.abs(Fo).max(), weight)
     assert np.abs(gd.get("position") - od.get("pos")).max() < 1e-9
     assert np.abs(gf.get("position") - of.get("pos")).max() < 1e-9
+    assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5
+
+
+def test_balls_in_two_fluid_phases_match_oracle():
+    """Balls in each block of the two-phase tank (the coupling's per-phase
+    supports and the reverse walk's non-uniform path), 10 coupled steps
+    against the FP64 oracle."""
+    import oracle
+
+    spec = scenes.two_phase_tank()
+    d = scenes.DEMScene("balls in two phases")
+    d.add_balls([[0.035, 0.025, 0.04], [0.13, 0.03, 0.04]], [0.0, 0.0, -0.05], [0.0, 0.0, 0.0], 0.006, 2500.0, 0)
+    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
+    of, od = oracle.OracleSPH(spec), oracle.OracleDEM(d)
+    oc = oracle.OracleCoupled(of, od)
+    gf, gd = fluid_scene(spec), ts.DEMSceneGPU(d)
+    gc = ts.CoupledScene(gf, gd)
+    oc.step(spec.dt, 10, spec.gravity)
+    gc.step(spec.dt, 10, spec.gravity)
+    Fo, Fg = oc.get("couplingForce"), gc.get("couplingForce")
+    assert np.abs(Fo).min() > 0.0  # both balls couple to their phase
+    assert np.abs(Fg - Fo).max() < 1e-4 * np.abs(Fo).max()
+    assert np.abs(gd.get("position") - od.get("pos")).max() < 1e-9
     assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5

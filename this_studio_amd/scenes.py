@@ -71,6 +71,17 @@ class SPHSceneSpec:
     fluid_vel: np.ndarray | None = None
     dummy_vel: np.ndarray | None = None
     name: str = ""
+    # per-fluid-particle properties (several phases); None: the scalars above
+    fluid_c0: np.ndarray | None = None
+    fluid_dx: np.ndarray | None = None
+    fluid_rho0: np.ndarray | None = None
+    fluid_nu: np.ndarray | None = None
+
+    def fluid_props(self):
+        """(sound speed, spacing, density, viscosity) of the fluid particles."""
+        pick = lambda a, d: d if a is None else a
+        return (pick(self.fluid_c0, self.c0), pick(self.fluid_dx, self.dx), pick(self.fluid_rho0, self.rho0),
+                pick(self.fluid_nu, self.nu))
 
     @property
     def nf(self) -> int:
@@ -738,3 +749,28 @@ def ice_cylinder_coupled(nx: int = 500, ny: int = 200, nz: int = 10, dx: float =
     d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
     d.dt = dt
     return spec, d
+
+
+
+def two_phase_tank(eos: int = 0) -> SPHSceneSpec:
+    """Two fluid phases side by side in a small tank: 8 x 6 x 8 particles at
+    dx = 1 cm (1000 kg/m^3, nu = 1e-6) and, 1 cm to their right, 6 x 5 x 6 at
+    1.25 cm (800 kg/m^3, nu = 1e-3), with a 3-layer dummy shell at 1 cm.
+    Per-pair cuts 2 max(h_A, h_B) and averaged smoothing lengths come into
+    play across the interface.  dt = 0.2 h_min / c0."""
+    dx1, dx2 = 0.01, 0.0125
+    a = lattice(8, 6, 8, dx1)
+    b = lattice(6, 5, 6, dx2, origin=(8 * dx1 + dx1, 0.0, 0.0))
+    pos = np.concatenate([a, b])
+    na, nb = a.shape[0], b.shape[0]
+    tank = (0.3, 6 * dx1 + dx1, 0.15)
+    dummies, mn, mx = tank_dummies(tank, dx1)
+    c0 = 10.0 * math.sqrt(2.0 * 9.81 * 8 * dx1)
+    spec = SPHSceneSpec(fluid_pos=pos, dummy_pos=dummies, dx=dx1, rho0=1000.0, c0=c0, nu=1e-6, min_bound=mn,
+                        max_bound=mx, gravity=np.array([0.0, 0.0, -9.81]), dt=0.2 * 1.3 * dx1 / c0, eos=eos,
+                        name="two_phase_tank")
+    spec.fluid_dx = np.concatenate([np.full(na, dx1), np.full(nb, dx2)])
+    spec.fluid_rho0 = np.concatenate([np.full(na, 1000.0), np.full(nb, 800.0)])
+    spec.fluid_nu = np.concatenate([np.full(na, 1e-6), np.full(nb, 1e-3)])
+    spec.fluid_c0 = np.full(na + nb, c0)
+    return spec

@@ -12,7 +12,7 @@ from . import scenes
 def staged_sph(spec: scenes.SPHSceneSpec) -> SPHScene:
     s = SPHScene()
     s.set_eos(spec.eos, spec.gamma)
-    s.add_fluid(spec.fluid_pos, spec.fluid_vel, spec.c0, spec.dx, spec.rho0, spec.nu)
+    s.add_fluid(spec.fluid_pos, spec.fluid_vel, *spec.fluid_props())
     if spec.nd:
         s.add_dummy(spec.dummy_pos, spec.dummy_vel, spec.c0, spec.dx, spec.rho0, spec.nu)
     return s
