@@ -3,8 +3,8 @@
 // The reference sorts (cell hash, particle index) pairs with Thrust's
 // radix sort_by_key over an iota value array (buildHashStartEnd.cu:50-76),
 // so its permutation is the lexicographic (hash, index) order.  This file
-// implements that order natively: 8-bit digits, one histogram kernel and one
-// scatter kernel per digit, a wave64 ballot "match-any" to rank equal digits
+// implements that order natively: 8- or 9-bit digits, one histogram kernel
+// and one scatter kernel per digit, a wave64 ballot "match-any" to rank equal digits
 // stably inside a wavefront, and LDS prefix sums across the wavefronts of a
 // workgroup.  Only the bits that hold hashes in [0, numGrids] are sorted.
 #include "common.hpp"
@@ -15,8 +15,9 @@
 namespace demsph {
 
 namespace {
-constexpr int kRadixBits = 8;
-constexpr int kBins = 1 << kRadixBits;
+// Digits of 8 bits, or of 9 when that saves a pass (17- and 18-bit keys,
+// grids of up to 262,144 cells, sort in two passes instead of three).
+constexpr int kMaxBins = 512;
 constexpr int kSortThreads = 256;             // 4 wavefronts
 constexpr int kWaves = kSortThreads / 64;
 constexpr int kItems = 8;                     // keys per lane
@@ -65,20 +66,22 @@ __device__ __forceinline__ int block_exclusive(int v, int& total)
 
 // Histogram of one digit per tile, written digit-major so that one
 // exclusive scan over [digit][block] yields every scatter base.
+template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
                                                              uint32_t* __restrict__ hist, int nblocks)
 {
-    __shared__ uint32_t h[kBins];
-    for (int i = threadIdx.x; i < kBins; i += kSortThreads) h[i] = 0;
+    constexpr int BINS = 1 << BITS;
+    __shared__ uint32_t h[BINS];
+    for (int i = threadIdx.x; i < BINS; i += kSortThreads) h[i] = 0;
     __syncthreads();
     const int base = blockIdx.x * kTile;
 #pragma unroll
     for (int it = 0; it < kItems; it++) {
         const int e = base + it * kSortThreads + threadIdx.x;
-        if (e < n) atomicAdd(&h[(keys[e] >> shift) & (kBins - 1)], 1u);
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & (BINS - 1)], 1u);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kBins; d += kSortThreads) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
+    for (int d = threadIdx.x; d < BINS; d += kSortThreads) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
 // Stable scatter.  Tile element order is wave-major, then item, then lane,
@@ -89,32 +92,46 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
 // all tiles and over the tiles before its own, then a workgroup scan over
 // the digits), which replaces the device-wide scan's three launches.
 // Otherwise `table` holds the scanned offsets.
-static_assert(kBins == kSortThreads, "one thread per digit in the fused scatter");
-template <bool FUSED>
+template <bool FUSED, int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
                                                                 const int* __restrict__ vals_in,
                                                                 uint32_t* __restrict__ keys_out,
                                                                 int* __restrict__ vals_out, int n, int shift,
                                                                 const uint32_t* __restrict__ table, int nblocks)
 {
-    __shared__ uint32_t wave_count[kWaves][kBins];
-    __shared__ uint32_t block_base[kBins];
+    constexpr int BINS = 1 << BITS;
+    constexpr int DPT = BINS / kSortThreads;  // digits per thread in the fused bases: 1 or 2
+    static_assert(DPT == 1 || DPT == 2, "8- or 9-bit digits");
+    __shared__ uint32_t wave_count[kWaves][BINS];
+    __shared__ uint32_t block_base[BINS];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < kWaves * kBins; i += kSortThreads) (&wave_count[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < kWaves * BINS; i += kSortThreads) (&wave_count[0][0])[i] = 0;
     if (FUSED) {
-        const int d = threadIdx.x;
-        uint32_t tot = 0, before = 0;
-        for (int b = 0; b < nblocks; b++) {
-            const uint32_t c = table[(size_t)d * nblocks + b];
-            tot += c;
-            before += b < (int)blockIdx.x ? c : 0u;
+        // thread t owns digits DPT t .. DPT t + DPT - 1
+        uint32_t tot[DPT], before[DPT];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < DPT; k++) {
+            const int d = DPT * threadIdx.x + k;
+            tot[k] = 0;
+            before[k] = 0;
+            for (int b = 0; b < nblocks; b++) {
+                const uint32_t c = table[(size_t)d * nblocks + b];
+                tot[k] += c;
+                before[k] += b < (int)blockIdx.x ? c : 0u;
+            }
+            sum += tot[k];
         }
         int total;
-        const int ex = block_exclusive((int)tot, total);
-        block_base[d] = (uint32_t)ex + before;
+        uint32_t run = (uint32_t)block_exclusive((int)sum, total);
+#pragma unroll
+        for (int k = 0; k < DPT; k++) {
+            block_base[DPT * threadIdx.x + k] = run + before[k];
+            run += tot[k];
+        }
     } else {
-        for (int d = threadIdx.x; d < kBins; d += kSortThreads) block_base[d] = table[(size_t)d * nblocks + blockIdx.x];
+        for (int d = threadIdx.x; d < BINS; d += kSortThreads) block_base[d] = table[(size_t)d * nblocks + blockIdx.x];
     }
     __syncthreads();
 
@@ -129,10 +146,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
         const bool valid = e < n;
         key[it] = valid ? keys_in[e] : 0u;
         val[it] = valid ? (vals_in ? vals_in[e] : e) : 0;
-        const uint32_t digit = (key[it] >> shift) & (kBins - 1);
+        const uint32_t digit = (key[it] >> shift) & (BINS - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < kRadixBits; b++) {
+        for (int b = 0; b < BITS; b++) {
             const bool bit = (digit >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -148,7 +165,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     }
     __syncthreads();
     // Exclusive prefix of the per-wave counts, per digit.
-    for (int d = threadIdx.x; d < kBins; d += kSortThreads) {
+    for (int d = threadIdx.x; d < BINS; d += kSortThreads) {
         uint32_t run = block_base[d];
 #pragma unroll
         for (int w = 0; w < kWaves; w++) {
@@ -161,7 +178,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
 #pragma unroll
     for (int it = 0; it < kItems; it++) {
         if (pos[it] == 0xffffffffu) continue;
-        const uint32_t digit = (key[it] >> shift) & (kBins - 1);
+        const uint32_t digit = (key[it] >> shift) & (BINS - 1);
         const uint32_t dst = wave_count[wave][digit] + pos[it];
         keys_out[dst] = key[it];
         vals_out[dst] = val[it];
@@ -278,44 +295,53 @@ void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, 
 size_t radix_sort_temp_bytes(int n)
 {
     const int nb = sort_blocks(n > 0 ? n : 1);
-    const size_t hist = sizeof(uint32_t) * (size_t)kBins * nb;
-    return hist * 2 + scan_temp_bytes(kBins * nb) + 256;
+    const size_t hist = sizeof(uint32_t) * (size_t)kMaxBins * nb;
+    return hist * 2 + scan_temp_bytes(kMaxBins * nb) + 256;
 }
+
+namespace {
+template <int BITS>
+void radix_passes(const uint32_t* keys_in, const int* vals_in, uint32_t* keys_out, int* vals_out, uint32_t* keys_tmp,
+                  int* vals_tmp, void* temp, int n, int passes, hipStream_t s)
+{
+    constexpr int BINS = 1 << BITS;
+    const int nb = sort_blocks(n);
+    uint32_t* hist = static_cast<uint32_t*>(temp);
+    uint32_t* offs = hist + (size_t)kMaxBins * nb;
+    void* scan_tmp = offs + (size_t)kMaxBins * nb;
+    // Ping-pong so that the last pass writes keys_out / vals_out.
+    const uint32_t* ksrc = keys_in;
+    const int* vsrc = vals_in;
+    for (int p = 0; p < passes; p++) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t* kdst = to_out ? keys_out : keys_tmp;
+        int* vdst = to_out ? vals_out : vals_tmp;
+        const int shift = p * BITS;
+        hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
+        if (nb <= kFusedScatterTiles) {
+            hipLaunchKernelGGL((k_radix_scatter<true, BITS>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst,
+                               vdst, n, shift, hist, nb);
+        } else {
+            exclusive_scan(reinterpret_cast<const int*>(hist), reinterpret_cast<int*>(offs), BINS * nb, scan_tmp,
+                           nullptr, s);
+            hipLaunchKernelGGL((k_radix_scatter<false, BITS>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst,
+                               vdst, n, shift, offs, nb);
+        }
+        DEMSPH_HIP(hipGetLastError());
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+}
+}  // namespace
 
 void radix_sort_pairs(const uint32_t* keys_in, const int* vals_in, uint32_t* keys_out, int* vals_out,
                       uint32_t* keys_tmp, int* vals_tmp, void* temp, int n, int nbits, hipStream_t s)
 {
     if (n <= 0) return;
     if (nbits < 1) nbits = 1;
-    const int nb = sort_blocks(n);
-    uint32_t* hist = static_cast<uint32_t*>(temp);
-    uint32_t* offs = hist + (size_t)kBins * nb;
-    void* scan_tmp = offs + (size_t)kBins * nb;
-    const int passes = (nbits + kRadixBits - 1) / kRadixBits;
-    // Ping-pong so that the last pass writes keys_out / vals_out.
-    const uint32_t* ksrc = keys_in;
-    const int* vsrc = vals_in;
-    for (int p = 0; p < passes; p++) {
-        const bool last = (p == passes - 1);
-        const bool to_out = ((passes - 1 - p) % 2) == 0;
-        uint32_t* kdst = to_out ? keys_out : keys_tmp;
-        int* vdst = to_out ? vals_out : vals_tmp;
-        (void)last;
-        const int shift = p * kRadixBits;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
-        if (nb <= kFusedScatterTiles) {
-            hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst, vdst, n,
-                               shift, hist, nb);
-        } else {
-            exclusive_scan(reinterpret_cast<const int*>(hist), reinterpret_cast<int*>(offs), kBins * nb, scan_tmp,
-                           nullptr, s);
-            hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst, vdst, n,
-                               shift, offs, nb);
-        }
-        DEMSPH_HIP(hipGetLastError());
-        ksrc = kdst;
-        vsrc = vdst;
-    }
+    const int p8 = (nbits + 7) / 8, p9 = (nbits + 8) / 9;
+    if (p9 < p8) radix_passes<9>(keys_in, vals_in, keys_out, vals_out, keys_tmp, vals_tmp, temp, n, p9, s);
+    else radix_passes<8>(keys_in, vals_in, keys_out, vals_out, keys_tmp, vals_tmp, temp, n, p8, s);
 }
 
 void iota_i32(int* p, int n, hipStream_t s)
