@@ -226,7 +226,7 @@ def ice_cylinder_coupled(steps: int) -> dict:
     return out
 
 
-def serving_mix(count: int, scale: float, slots: int) -> dict:
+def serving_mix(count: int, scale: float, slots: int, autotune: str | None = None) -> dict:
     """The batched-serving job stream (256 scenes of 2k-200k particles,
     log-uniform; 50/30/20 DEM/WCSPH/coupled; 1k-20k steps; Poisson arrivals
     of mean 50 ms) scaled to finish in seconds: `count` scenes, step counts
@@ -235,9 +235,12 @@ def serving_mix(count: int, scale: float, slots: int) -> dict:
     launch per stage from the host, every job alone) and batched (`slots`
     concurrent jobs, WCSPH steps as graph replays).  Plain runs every job
     alone, so equal results in the batched run show that the mix changes
-    nothing."""
+    nothing.  With `autotune`, a third run serves the batched path with
+    every fluid scene autotuned on admission (the tuning inside the wall
+    time; "exact" keeps results bit-identical)."""
     import numpy as np
 
+    import this_studio_amd as ts
     from this_studio_amd import scenes, serving
 
     lo, hi = max(1, int(round(1000 * scale))), max(2, int(round(20000 * scale)))
@@ -246,11 +249,16 @@ def serving_mix(count: int, scale: float, slots: int) -> dict:
            "kinds": {k: sum(j.kind == k for j in jobs) for k in ("dem", "wcsph", "coupled")},
            "memory_budget_GB": 288}
     results = {}
-    for name, sl, graphs in (("plain", 1, False), ("batched", slots, True)):
-        print(f"[bench] serving mix: {name} ({sl} slot(s), graphs={graphs}) ...", file=sys.stderr, flush=True)
-        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs)
+    runs = [("plain", 1, False, None), ("batched", slots, True, None)]
+    if autotune:
+        runs.append(("batched_autotuned", slots, True, autotune))
+        ts.SPHScene.clear_autotune_cache()
+    for name, sl, graphs, tune in runs:
+        print(f"[bench] serving mix: {name} ({sl} slot(s), graphs={graphs}, autotune={tune}) ...", file=sys.stderr,
+              flush=True)
+        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs, autotune=tune)
         st = srv.stats()
-        out[name] = {"slots": sl, "graphs": graphs, "wall_s": round(st["wall"], 4),
+        out[name] = {"slots": sl, "graphs": graphs, "autotune": tune, "wall_s": round(st["wall"], 4),
                      "particle_steps_per_s": round(st["particleStepsPerSecond"], 1),
                      "p50_step_ms": round(st["p50"] * 1e3, 4), "p99_step_ms": round(st["p99"] * 1e3, 4),
                      "jobs_done": st["jobsDone"], "jobs_failed": st["jobsFailed"], "peak_active": st["peakActive"],
@@ -262,8 +270,12 @@ def serving_mix(count: int, scale: float, slots: int) -> dict:
                          if not srv.info(i)["status"] else None for j, i in zip(jobs, ids)]
         srv.close()
     out["speedup"] = round(out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"], 3)
-    out["results_equal_alone_and_mixed"] = all(a is not None and b is not None and np.array_equal(a, b)
-                                               for a, b in zip(results["plain"], results["batched"]))
+    if autotune:
+        out["speedup_autotuned"] = round(out["batched_autotuned"]["particle_steps_per_s"]
+                                         / out["plain"]["particle_steps_per_s"], 3)
+    out["results_equal_alone_and_mixed"] = all(
+        all(r[k] is not None and np.array_equal(r[k], results["plain"][k]) for r in results.values())
+        for k in range(len(jobs)))
     return out
 
 
@@ -281,6 +293,10 @@ def main() -> int:
     ap.add_argument("--ice-steps", type=int, default=40, help="steps of the coupled ice-cylinder leg (0: skip it)")
     ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
     ap.add_argument("--mix-slots", type=int, default=16, help="concurrent jobs of the batched server")
+    ap.add_argument("--mix-autotune", choices=("off", "exact", "pairs"), default="exact",
+                    help="also serve the batched path with fluid scenes autotuned on admission")
+    ap.add_argument("--no-autotune-report", action="store_true",
+                    help="skip the autotuner trials on the dam break after the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate HBM counters")
     args = ap.parse_args()
@@ -379,6 +395,15 @@ def main() -> int:
         roofline["fp32_valu_tflops"] = round(fl / d_avg_s / 1e12, 2)
         roofline["fp32_valu_frac"] = round(fl / d_avg_s / 1e12 / FP32_PEAK_TFLOPS, 4)
 
+    # The autotuner on the same scene, after the timed steps (its trials
+    # restore the state): what it picks against the built-in lane rules.
+    tune = None
+    if not args.no_autotune_report:
+        ts.SPHScene.clear_autotune_cache()
+        tune = s.autotune(spec.dt, trial_steps=5, mode="pairs", use_cache=False)
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
+        ts.SPHScene.clear_autotune_cache()
+
     # whole-step algorithmic HBM rate (sum over the step's kernels)
     step_bytes = sum(algorithmic_bytes(k, nf, pairs_ff, pairs_fd) * timing[k][0] for k in timing) / args.steps
     cpu = None
@@ -404,12 +429,18 @@ def main() -> int:
               f"{ice['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     mix = None
     if args.mix_scenes > 0:
-        mix = serving_mix(args.mix_scenes, args.mix_scale, args.mix_slots)
+        mix = serving_mix(args.mix_scenes, args.mix_scale, args.mix_slots,
+                          None if args.mix_autotune == "off" else args.mix_autotune)
         print(f"[bench] serving mix: plain {mix['plain']['particle_steps_per_s']:.3e} particle-steps/s "
               f"p50 {mix['plain']['p50_step_ms']:.3f} ms p99 {mix['plain']['p99_step_ms']:.3f} ms | batched "
               f"{mix['batched']['particle_steps_per_s']:.3e} p50 {mix['batched']['p50_step_ms']:.3f} ms p99 "
               f"{mix['batched']['p99_step_ms']:.3f} ms | x{mix['speedup']:.2f}, equal={mix['results_equal_alone_and_mixed']}",
               file=sys.stderr, flush=True)
+        if "batched_autotuned" in mix:
+            bt = mix["batched_autotuned"]
+            print(f"[bench] serving mix, autotuned: {bt['particle_steps_per_s']:.3e} particle-steps/s p50 "
+                  f"{bt['p50_step_ms']:.3f} ms p99 {bt['p99_step_ms']:.3f} ms | x{mix['speedup_autotuned']:.2f}",
+                  file=sys.stderr, flush=True)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -433,6 +464,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "dem_collapse": dem,
         "serving_mix": mix,
+        "autotune": tune,
         "cantilever_clumps": beam,
         "ice_cylinder_coupled": ice,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}

@@ -157,6 +157,32 @@ int demsph_sph_graph_captures(demsph_sph* s, long* count);
 int demsph_sph_tune(demsph_sph* s, const char* knob, int value);
 int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value);
 
+/* Autotuner (no counterpart in the reference: its block shapes are fixed).
+ * Times the variants on the initialised scene's own steps, each trial one
+ * untimed step and then trialSteps timed steps (HIP events) from a
+ * checkpoint that is restored afterwards, so the scene's state and its
+ * later results are those of a scene never tuned with the chosen knobs.
+ * mode 0 tries only result-invariant knobs (listLanes 1, 2, 4, 8 with and
+ * without listPrefilter: every variant builds the same lists); mode 1 also
+ * tries pairLanes (FP32 summation order changes).  A variant within 2 % of
+ * the built-in rule keeps the rule.  The cell size is not tuned: it fixes
+ * the list order and the sort (2 max h, WCSPHSolver.h:127-135).  The choice
+ * is set on the scene and cached per size class (uniform phase, fluid count
+ * to a quarter octave, dummy count to a half octave, mode) for the process;
+ * with useCache a cached class is applied without timing.  Not while
+ * coupled stepping is in flight on the scene's stream. */
+typedef struct demsph_sph_tune_report {
+    int listLanes, listPrefilter, pairLanes; /* now in effect */
+    int rulesListLanes, rulesPairLanes;      /* the built-in lane rules */
+    int configs;                             /* trials timed (0 when cached) */
+    int cached;                              /* 1: taken from the cache */
+    double listUs, pairUs;                   /* per step: list build; the three pair passes */
+    double rulesListUs, rulesPairUs;         /* the same under the rules (0 if not timed) */
+} demsph_sph_tune_report;
+int demsph_sph_autotune(demsph_sph* s, double dt, int trialSteps, int mode, int useCache,
+                        demsph_sph_tune_report* report);
+int demsph_sph_autotune_clear_cache(void);
+
 /* ------------------------------------------------------------------ DEM
  * Opaque DEMSolver scene (kernel/DEMSolver.h:144-2079). */
 typedef struct demsph_dem demsph_dem;
@@ -370,6 +396,10 @@ int demsph_server_create(demsph_server** out, int slots, double memoryBudgetByte
 int demsph_server_destroy(demsph_server* v);
 /* Steps per step() call inside a job (default 100). */
 int demsph_server_set_block(demsph_server* v, long steps);
+/* Autotune every job's fluid scene on admission (demsph_sph_autotune with
+ * the cache on; mode -1 off (default), 0 result-invariant knobs, 1 also
+ * pair lanes).  Set before demsph_server_run. */
+int demsph_server_set_autotune(demsph_server* v, int mode, int trialSteps);
 /* fluid and/or dem non-NULL; fluidMin/Max with a fluid, demMin/Max with a
  * DEM scene (a coupled DEM scene usually shares the fluid bounds). */
 int demsph_server_submit(demsph_server* v, demsph_sph* fluid, demsph_dem* dem, const double* fluidMin,

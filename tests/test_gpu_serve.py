@@ -51,6 +51,40 @@ def test_mix_results_equal_alone():
         alone.close()
 
 
+def test_many_short_jobs_capture_while_others_allocate():
+    """Many short jobs on 8 slots: graph captures in some slots overlap the
+    set-up, read-back and teardown (allocation, release, copies) of others.
+    Every job must complete and equal its plain run."""
+    kinds = ("dem", "wcsph", "dem", "coupled")
+    jobs = [MixJob(kinds[k % 4], 2000 + 300 * k, 24 + (k % 5) * 4, 0.0, 100 + k) for k in range(24)]
+    srv, ids = serving.serve(jobs, slots=8, graphs=True, block=8)
+    for job, i in zip(jobs, ids):
+        assert srv.info(i)["status"] == 0, (job, srv.info(i)["error"])
+    plain, pids = serving.serve(jobs, slots=1, graphs=False)
+    for job, i, k in zip(jobs, ids, pids):
+        for f in _fields(job.kind):
+            assert np.array_equal(srv.result(i, f), plain.result(k, f)), (job, f)
+    srv.close()
+    plain.close()
+
+
+def test_autotuned_serving_keeps_results():
+    """Jobs autotuned on admission ("exact": list variants only) give the
+    same results, bit for bit, as the plain untuned path."""
+    ts.SPHScene.clear_autotune_cache()
+    jobs = [SMALL[0], SMALL[2], SMALL[3]]
+    srv, ids = serving.serve(jobs, slots=3, graphs=True, block=32, autotune="exact")
+    for job, i in zip(jobs, ids):
+        assert srv.info(i)["status"] == 0, srv.info(i)["error"]
+    plain, pids = serving.serve(jobs, slots=1, graphs=False)
+    for job, i, k in zip(jobs, ids, pids):
+        for f in _fields(job.kind):
+            assert np.array_equal(srv.result(i, f), plain.result(k, f)), (job, f)
+    srv.close()
+    plain.close()
+    ts.SPHScene.clear_autotune_cache()
+
+
 def test_served_jobs_match_oracle():
     """A WCSPH and a DEM job served together against the FP64 oracle."""
     jobs = [MixJob("wcsph", 2000, 40, 0.0, 21), MixJob("dem", 2000, 200, 0.0, 22)]

@@ -180,6 +180,52 @@ def test_lane_settings_build_identical_lists_and_agree(small):
         assert np.abs(runs[0] - base).max() < 1e-6 * small.dx, L
 
 
+def test_autotune_restores_state_and_keeps_results(small):
+    """Autotuning times its trials from a checkpoint and restores it: a tuned
+    scene steps exactly like an untuned one.  In "exact" mode the result
+    is bit-identical to the default knobs; in "pairs" mode it is
+    bit-identical to a scene given the chosen knobs by hand."""
+    ts.SPHScene.clear_autotune_cache()
+    ref = make_gpu(small)
+    ref.step(small.dt, 7)
+    ref.step(small.dt, 20)
+    a = make_gpu(small)
+    a.step(small.dt, 7)  # tune mid-run, not only right after initialisation
+    rep = a.autotune(small.dt, trial_steps=2, mode="exact", use_cache=False)
+    print("[autotune] exact:", rep, flush=True)
+    assert rep["configs"] == 7 and rep["cached"] == 0
+    assert rep["listLanes"] in (1, 2, 4, 8) and rep["listPrefilter"] in (0, 1)
+    assert a.tuned("listLanes") == rep["listLanes"]
+    assert a.tuned("listPrefilter") == rep["listPrefilter"]
+    assert a.tuned("pairLanes") == rep["pairLanes"] == rep["rulesPairLanes"]
+    assert rep["listUs"] > 0 and rep["pairUs"] > 0 and rep["rulesListUs"] > 0
+    a.step(small.dt, 20)
+    for f in ("position", "velocity", "densityChange"):
+        assert np.array_equal(a.get(f), ref.get(f)), f
+
+    b = make_gpu(small)
+    rep = b.autotune(small.dt, trial_steps=2, mode="pairs", use_cache=False)
+    print("[autotune] pairs:", rep, flush=True)
+    assert rep["configs"] == 11
+    b.step(small.dt, 25)
+    c = make_gpu(small)
+    c.tune("listLanes", rep["listLanes"])
+    c.tune("listPrefilter", rep["listPrefilter"])
+    c.tune("pairLanes", rep["pairLanes"])
+    c.step(small.dt, 25)
+    for f in ("position", "velocity", "densityChange"):
+        assert np.array_equal(b.get(f), c.get(f)), f
+
+    # the size class is cached: a second scene takes the choice untimed
+    d = make_gpu(small)
+    rep2 = d.autotune(small.dt, mode="pairs")
+    assert rep2["cached"] == 1 and rep2["configs"] == 0
+    assert (rep2["listLanes"], rep2["listPrefilter"], rep2["pairLanes"]) == \
+        (rep["listLanes"], rep["listPrefilter"], rep["pairLanes"])
+    assert d.tuned("pairLanes") == rep["pairLanes"]
+    ts.SPHScene.clear_autotune_cache()
+
+
 def test_prefilter_keeps_pairs_at_the_cut_exact():
     """Shells of particles at |r| = cut (1 + u 1e-6), u in [-1, 1], around
     seeded centres: every decision falls inside the FP32 prefilter's band and

@@ -44,6 +44,17 @@ _vp = C.c_void_p
 _sz = C.c_size_t
 
 
+class TuneReport(C.Structure):
+    """demsph_sph_tune_report: what the autotuner chose and measured."""
+    _fields_ = [("listLanes", C.c_int), ("listPrefilter", C.c_int), ("pairLanes", C.c_int),
+                ("rulesListLanes", C.c_int), ("rulesPairLanes", C.c_int), ("configs", C.c_int),
+                ("cached", C.c_int), ("listUs", C.c_double), ("pairUs", C.c_double),
+                ("rulesListUs", C.c_double), ("rulesPairUs", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def _declare(L: C.CDLL) -> None:
     L.demsph_last_error.restype = C.c_char_p
     L.demsph_version.restype = C.c_char_p
@@ -71,6 +82,8 @@ def _declare(L: C.CDLL) -> None:
         "demsph_sph_graph_captures": [_vp, C.POINTER(C.c_long)],
         "demsph_sph_tune": [_vp, C.c_char_p, C.c_int],
         "demsph_sph_tune_get": [_vp, C.c_char_p, C.POINTER(C.c_int)],
+        "demsph_sph_autotune": [_vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(TuneReport)],
+        "demsph_sph_autotune_clear_cache": [],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -198,6 +211,23 @@ class SPHScene:
         v = C.c_int()
         check(lib().demsph_sph_tune_get(self._h, knob.encode(), C.byref(v)))
         return v.value
+
+    def autotune(self, dt: float, trial_steps: int = 4, mode: str = "exact", use_cache: bool = True) -> dict:
+        """Time the list-build and pair-pass variants on this scene's own steps
+        (state restored afterwards) and keep the fastest.  mode "exact" tries
+        only knobs that leave every result bit-identical; "pairs" also tries
+        pair lanes (FP32 summation order).  Returns the report as a dict."""
+        modes = {"exact": 0, "pairs": 1}
+        if mode not in modes:
+            raise ValueError(f"mode must be one of {sorted(modes)}")
+        r = TuneReport()
+        check(lib().demsph_sph_autotune(self._h, float(dt), int(trial_steps), modes[mode],
+                                        1 if use_cache else 0, C.byref(r)))
+        return r.as_dict()
+
+    @staticmethod
+    def clear_autotune_cache() -> None:
+        check(lib().demsph_sph_autotune_clear_cache())
 
     def graph_captures(self) -> int:
         n = C.c_long()
@@ -563,6 +593,7 @@ def _declare_server(L):
         "demsph_server_create": [C.POINTER(_vp), C.c_int, C.c_double, C.c_int],
         "demsph_server_destroy": [_vp],
         "demsph_server_set_block": [_vp, C.c_long],
+        "demsph_server_set_autotune": [_vp, C.c_int, C.c_int],
         "demsph_server_submit": [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_long, C.c_double,
                                  C.POINTER(C.c_int)],
         "demsph_server_run": [_vp],
@@ -586,13 +617,20 @@ class Server:
 
     KINDS = ("wcsph", "dem", "coupled")
 
-    def __init__(self, slots: int = 4, memory_budget: float = 288e9, graphs: bool = True, block: int = 100):
+    def __init__(self, slots: int = 4, memory_budget: float = 288e9, graphs: bool = True, block: int = 100,
+                 autotune: str | None = None, tune_steps: int = 3):
+        """autotune: None (off), "exact" (result-invariant knobs) or "pairs"
+        (also pair lanes): every job's fluid scene is tuned on admission."""
+        modes = {None: -1, "exact": 0, "pairs": 1}
+        if autotune not in modes:
+            raise ValueError("autotune must be None, 'exact' or 'pairs'")
         L = lib()
         _declare_server(L)
         self.L = L
         self._h = _vp()
         check(L.demsph_server_create(C.byref(self._h), int(slots), float(memory_budget), 1 if graphs else 0))
         check(L.demsph_server_set_block(self._h, int(block)))
+        check(L.demsph_server_set_autotune(self._h, modes[autotune], int(tune_steps)))
 
     def close(self) -> None:
         if self._h:

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <stdexcept>
 
@@ -59,19 +60,69 @@ int guarded(F&& f)
     }
 }
 
+// ------------------------------------------------- capture-safe runtime calls
+// Scenes served together capture step graphs on their own streams from
+// several host threads.  HIP rejects some runtime calls made by one thread
+// while another captures (allocation, release, device-wide and
+// legacy-stream synchronisation), and the rejection can also invalidate the
+// other thread's capture.  Every such call in this library and every
+// capture hold this one process-wide lock; captures only enqueue, so they
+// hold it briefly.  Recursive: a capture that allocates does not deadlock.
+std::recursive_mutex& capture_lock();
+using CaptureGuard = std::lock_guard<std::recursive_mutex>;
+
+// Device and pinned host memory.  While a scene server runs (block_cache_
+// begin .. block_cache_end), released blocks are cached by size class and
+// reused instead of being returned with hipFree / hipHostFree, which wait
+// for the work of every stream on the device and so would stall all the
+// other jobs' streams at each job's set-up and teardown.  Outside serving
+// (or for blocks not from the cache) the calls go straight to HIP.
+hipError_t dev_malloc(void** p, size_t bytes);
+void dev_free(void* p);
+hipError_t host_malloc(void** p, size_t bytes);
+void host_free(void* p);
+// A server starts / ends using the cache; capBytes bounds what it holds
+// (live and cached); the last server to end releases the cached blocks.
+void block_cache_begin(size_t capBytes);
+void block_cache_end();
+inline hipError_t stream_create(hipStream_t* s)
+{
+    CaptureGuard g(capture_lock());
+    return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+inline void stream_destroy(hipStream_t s)
+{
+    if (!s) return;
+    CaptureGuard g(capture_lock());
+    (void)hipStreamDestroy(s);
+}
+inline void graph_exec_destroy(hipGraphExec_t e)
+{
+    if (!e) return;
+    CaptureGuard g(capture_lock());
+    (void)hipGraphExecDestroy(e);
+}
+// Host <-> device copy ordered on stream s, complete on return.
+inline void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    if (!bytes) return;
+    DEMSPH_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+    DEMSPH_HIP(hipStreamSynchronize(s));
+}
+
 // ---------------------------------------------------------------- memory
 template <class T>
 T* dmalloc(size_t n)
 {
     void* p = nullptr;
     if (n == 0) n = 1;
-    DEMSPH_HIP(hipMalloc(&p, n * sizeof(T)));
+    DEMSPH_HIP(dev_malloc(&p, n * sizeof(T)));
     return static_cast<T*>(p);
 }
 template <class T>
 void dfree(T*& p)
 {
-    if (p) (void)hipFree(p);
+    dev_free(p);
     p = nullptr;
 }
 

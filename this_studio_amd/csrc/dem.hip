@@ -1042,7 +1042,7 @@ struct demsph_dem {
     void drop_graphs()
     {
         if (!graphs.empty()) (void)hipStreamSynchronize(stream);
-        for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
+        for (auto& e : graphs) graph_exec_destroy(e.exec);
         graphs.clear();
     }
 
@@ -1050,13 +1050,13 @@ struct demsph_dem {
     {
         (void)hipStreamSynchronize(stream);
         drop_graphs();
-        if (sort_temp) (void)hipFree(sort_temp);
-        if (tsort_temp) (void)hipFree(tsort_temp);
-        if (tscan_temp) (void)hipFree(tscan_temp);
+        dev_free(sort_temp);
+        dev_free(tsort_temp);
+        dev_free(tscan_temp);
         tscan_temp = nullptr;
-        if (scan_temp) (void)hipFree(scan_temp);
-        if (total_h) (void)hipHostFree(total_h);
-        if (overflow_h) (void)hipHostFree(overflow_h);
+        dev_free(scan_temp);
+        host_free(total_h);
+        host_free(overflow_h);
         for (auto* b : hook_bufs) delete b;
     }
 
@@ -1163,10 +1163,10 @@ struct demsph_dem {
                                 &dbFn, &dbTt, &dbsig, &dbtau};
         for (int k = 0; k < 16; k++) {
             keep[k].assign(nb_new, 0.0);
-            if (nbond_dev) DEMSPH_HIP(hipMemcpy(keep[k].data(), st[k]->p, nbond_dev * 8, hipMemcpyDeviceToHost));
+            if (nbond_dev) copy_sync(keep[k].data(), st[k]->p, nbond_dev * 8, hipMemcpyDeviceToHost, stream);
         }
         keep_bonded.assign(nb_new, 1);
-        if (nbond_dev) DEMSPH_HIP(hipMemcpy(keep_bonded.data(), dbonded.p, nbond_dev * 4, hipMemcpyDeviceToHost));
+        if (nbond_dev) copy_sync(keep_bonded.data(), dbonded.p, nbond_dev * 4, hipMemcpyDeviceToHost, stream);
         for (int k = nbond_dev; k < nb_new; k++) {
             keep[0][k] = bond_point[3 * k]; keep[1][k] = bond_point[3 * k + 1]; keep[2][k] = bond_point[3 * k + 2];
             keep[3][k] = bond_normal[3 * k]; keep[4][k] = bond_normal[3 * k + 1]; keep[5][k] = bond_normal[3 * k + 2];
@@ -1270,12 +1270,12 @@ struct demsph_dem {
         for (auto* d : {&fx, &fy, &fz, &tx, &ty, &tz}) d->alloc(n);
         for (auto* d : {&ofx, &ofy, &ofz, &otx, &oty, &otz}) d->alloc(n);
         key.alloc(n); skey.alloc(n); ktmp.alloc(n); perm.alloc(n); vtmp.alloc(n);
-        DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(n)));
+        DEMSPH_HIP(dev_malloc(&sort_temp, radix_sort_temp_bytes(n)));
         cellStart.alloc(bng); cellEnd.alloc(bng);
         cnt.alloc(n + 1); rcnt.alloc(n + 1); rstart.alloc(n + 1); rcursor.alloc(n);
-        DEMSPH_HIP(hipMalloc(&scan_temp, scan_temp_bytes(n + 1)));
+        DEMSPH_HIP(dev_malloc(&scan_temp, scan_temp_bytes(n + 1)));
         total.alloc(2);
-        DEMSPH_HIP(hipHostMalloc((void**)&total_h, 2 * sizeof(int), hipHostMallocDefault));
+        DEMSPH_HIP(host_malloc((void**)&total_h, 2 * sizeof(int)));
         for (int w = 0; w < 2; w++) {
             pstart[w].alloc(n + 1);
             alloc_pairs(w, 6 * n);  // initial capacity as DEMSolver.h:174
@@ -1320,10 +1320,10 @@ struct demsph_dem {
             tbox.alloc(6 * (size_t)nt);
             tri_cells_valid = false;
             walls_static = true;
-            DEMSPH_HIP(hipMalloc(&tsort_temp, radix_sort_temp_bytes(nt)));
+            DEMSPH_HIP(dev_malloc(&tsort_temp, radix_sort_temp_bytes(nt)));
             tcs.alloc(tng); tce.alloc(tng);
             tcand_cnt.alloc(tng + 1); tcand_start.alloc(tng + 1);
-            DEMSPH_HIP(hipMalloc(&tscan_temp, scan_temp_bytes((int)tng + 1)));
+            DEMSPH_HIP(dev_malloc(&tscan_temp, scan_temp_bytes((int)tng + 1)));
             rmax = 0.0;
             for (double rr : hb.r) rmax = std::max(rmax, rr);
             const size_t hn = (size_t)KW * n;
@@ -1335,11 +1335,10 @@ struct demsph_dem {
         }
         overflow.alloc(1);
         DEMSPH_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), stream));
-        DEMSPH_HIP(hipHostMalloc((void**)&overflow_h, sizeof(int), hipHostMallocDefault));
+        DEMSPH_HIP(host_malloc((void**)&overflow_h, sizeof(int)));
         nbond_dev = 0;
         upload_bonds();
         DEMSPH_HIP(hipStreamSynchronize(stream));
-        DEMSPH_HIP(hipDeviceSynchronize());
         initialized = true;
     }
 
@@ -1676,6 +1675,9 @@ struct demsph_dem {
         for (auto& e : graphs)
             if (e.key == key) return e.exec;
         hipGraph_t gr = nullptr;
+        // No other thread may allocate, free or synchronise the device
+        // until the graph is instantiated (see capture_lock).
+        CaptureGuard capture_guard(capture_lock());
         DEMSPH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
             plain_steps(g, dt, 4);
@@ -1691,7 +1693,7 @@ struct demsph_dem {
         DEMSPH_HIP(e);
         if (graphs.size() >= 4) {
             DEMSPH_HIP(hipStreamSynchronize(stream));
-            (void)hipGraphExecDestroy(graphs.front().exec);
+            graph_exec_destroy(graphs.front().exec);
             graphs.erase(graphs.begin());
         }
         graphs.push_back({key, ex});
@@ -2103,7 +2105,7 @@ extern "C" int demsph_dem_hook_constant_force(demsph_dem* d, const double* force
         auto* b = new DBuf<double>();
         d->hook_bufs.push_back(b);
         b->alloc(3 * (size_t)d->n);
-        DEMSPH_HIP(hipMemcpy(b->p, force, 3 * (size_t)d->n * 8, hipMemcpyHostToDevice));
+        copy_sync(b->p, force, 3 * (size_t)d->n * 8, hipMemcpyHostToDevice, d->stream);
         demsph_dem::HookOp h{};
         h.kind = 0; h.ramp = ramp; h.f = b;
         d->hooks.push_back(h);
@@ -2294,9 +2296,9 @@ extern "C" int demsph_dem_set(demsph_dem* d, const char* field, const void* in, 
             if (f == "position") { t[0] = &d->bpx[cb]; t[1] = &d->bpy[cb]; t[2] = &d->bpz[cb]; }
             else if (f == "velocity") { t[0] = &d->bvx[cb]; t[1] = &d->bvy[cb]; t[2] = &d->bvz[cb]; }
             else { t[0] = &d->bwx[cb]; t[1] = &d->bwy[cb]; t[2] = &d->bwz[cb]; }
-            DEMSPH_HIP(hipMemcpy(t[0]->p, hx.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(t[1]->p, hy.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(t[2]->p, hz.data(), n * 8, hipMemcpyHostToDevice));
+            copy_sync(t[0]->p, hx.data(), n * 8, hipMemcpyHostToDevice, d->stream);
+            copy_sync(t[1]->p, hy.data(), n * 8, hipMemcpyHostToDevice, d->stream);
+            copy_sync(t[2]->p, hz.data(), n * 8, hipMemcpyHostToDevice, d->stream);
             return;
         }
         if (f == "wallVelocity" || f == "wallAngularVelocity") {
@@ -2312,13 +2314,13 @@ extern "C" int demsph_dem_set(demsph_dem* d, const char* field, const void* in, 
                 }
             DEMSPH_HIP(hipStreamSynchronize(st));
             if (f == "wallVelocity") {
-                DEMSPH_HIP(hipMemcpy(d->wvx.p, hx.data(), m * 8, hipMemcpyHostToDevice));
-                DEMSPH_HIP(hipMemcpy(d->wvy.p, hy.data(), m * 8, hipMemcpyHostToDevice));
-                DEMSPH_HIP(hipMemcpy(d->wvz.p, hz.data(), m * 8, hipMemcpyHostToDevice));
+                copy_sync(d->wvx.p, hx.data(), m * 8, hipMemcpyHostToDevice, d->stream);
+                copy_sync(d->wvy.p, hy.data(), m * 8, hipMemcpyHostToDevice, d->stream);
+                copy_sync(d->wvz.p, hz.data(), m * 8, hipMemcpyHostToDevice, d->stream);
             } else {
-                DEMSPH_HIP(hipMemcpy(d->wwx.p, hx.data(), m * 8, hipMemcpyHostToDevice));
-                DEMSPH_HIP(hipMemcpy(d->wwy.p, hy.data(), m * 8, hipMemcpyHostToDevice));
-                DEMSPH_HIP(hipMemcpy(d->wwz.p, hz.data(), m * 8, hipMemcpyHostToDevice));
+                copy_sync(d->wwx.p, hx.data(), m * 8, hipMemcpyHostToDevice, d->stream);
+                copy_sync(d->wwy.p, hy.data(), m * 8, hipMemcpyHostToDevice, d->stream);
+                copy_sync(d->wwz.p, hz.data(), m * 8, hipMemcpyHostToDevice, d->stream);
             }
             return;
         }

@@ -80,6 +80,8 @@ struct demsph_server {
     size_t budget = 0;
     bool graphs = true;
     long block = 100;  // steps per step() call
+    int autotune = -1;   // WCSPH autotuner mode on admission (-1: off)
+    int tune_steps = 3;  // timed steps per autotuner trial
     int device = 0;
     double clock_hz = 1.0e8;
     std::vector<std::unique_ptr<Job>> jobs;
@@ -154,9 +156,11 @@ struct demsph_server {
             if (j.fluid) {
                 ok(demsph_sph_set_graphs(j.fluid, graphs ? 1 : 0));
                 ok(demsph_sph_initialize(j.fluid, j.fmin, j.fmax, j.g));
+                // trials restore the state; a cached size class costs nothing
+                if (autotune >= 0) ok(demsph_sph_autotune(j.fluid, j.dt, tune_steps, autotune, 1, nullptr));
                 st = sph_stream(j.fluid);
             } else {
-                DEMSPH_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+                DEMSPH_HIP(stream_create(&own));
                 st = own;
             }
             if (j.dem) {
@@ -166,8 +170,8 @@ struct demsph_server {
                 ok(demsph_dem_solve_begin(j.dem));
             }
             if (j.kind == 2) ok(demsph_couple_create(&j.couple, j.fluid, j.dem));
-            DEMSPH_HIP(hipMalloc(&sbuf, sizeof(unsigned long long) * (size_t)(j.steps + 1)));
-            DEMSPH_HIP(hipMalloc(&scount, sizeof(int)));
+            DEMSPH_HIP(dev_malloc((void**)&sbuf, sizeof(unsigned long long) * (size_t)(j.steps + 1)));
+            DEMSPH_HIP(dev_malloc((void**)&scount, sizeof(int)));
             fill_i32(scount, 1, 0, st);
             StepStamps ss;
             ss.buf = sbuf;
@@ -187,12 +191,10 @@ struct demsph_server {
             DEMSPH_HIP(hipStreamSynchronize(st));
             j.stepped = now();
             int cnt = 0;
-            DEMSPH_HIP(hipMemcpy(&cnt, scount, sizeof(int), hipMemcpyDeviceToHost));
+            copy_sync(&cnt, scount, sizeof(int), hipMemcpyDeviceToHost, st);
             cnt = std::min<int>(cnt, (int)(j.steps + 1));
             j.stamps.resize((size_t)cnt);
-            if (cnt)
-                DEMSPH_HIP(hipMemcpy(j.stamps.data(), sbuf, sizeof(unsigned long long) * (size_t)cnt,
-                                     hipMemcpyDeviceToHost));
+            copy_sync(j.stamps.data(), sbuf, sizeof(unsigned long long) * (size_t)cnt, hipMemcpyDeviceToHost, st);
             if (j.fluid) sph_set_stamps(j.fluid, StepStamps());
             if (j.dem) dem_set_stamps(j.dem, StepStamps());
             auto field = [](auto get, auto* h, const char* f, std::vector<double>& out) {
@@ -220,9 +222,9 @@ struct demsph_server {
             j.error = e.what();
         }
         j.destroy_scenes();
-        if (sbuf) (void)hipFree(sbuf);
-        if (scount) (void)hipFree(scount);
-        if (own) (void)hipStreamDestroy(own);
+        dev_free(sbuf);
+        dev_free(scount);
+        stream_destroy(own);
         j.finished = now();
     }
 
@@ -281,6 +283,12 @@ struct demsph_server {
         for (auto& j : jobs) order.push_back(j.get());
         std::stable_sort(order.begin(), order.end(), [](const Job* a, const Job* b) { return a->arrival < b->arrival; });
         next = 0;
+        // Jobs' device blocks are recycled through the block cache while the
+        // server runs (no device-wide waits in hipFree between jobs).
+        struct CacheUse {
+            explicit CacheUse(size_t cap) { block_cache_begin(cap); }
+            ~CacheUse() { block_cache_end(); }
+        } cache_use(budget);
         t0 = std::chrono::steady_clock::now();
         std::vector<std::thread> pool;
         for (int s = 0; s < slots; s++) pool.emplace_back([this] { worker(); });
@@ -363,6 +371,18 @@ extern "C" int demsph_server_set_block(demsph_server* v, long steps)
         if (!v) throw InvalidArg("null server");
         if (steps < 1) throw InvalidArg("block must be positive");
         v->block = steps;
+    });
+}
+
+extern "C" int demsph_server_set_autotune(demsph_server* v, int mode, int trialSteps)
+{
+    return guarded([&] {
+        if (!v) throw InvalidArg("null server");
+        if (mode < -1 || mode > 1) throw InvalidArg("autotune mode must be -1 (off), 0 or 1");
+        if (trialSteps < 1 || trialSteps > 1000) throw InvalidArg("trial steps must be in 1..1000");
+        if (v->ran) throw InvalidArg("the server has already run");
+        v->autotune = mode;
+        v->tune_steps = trialSteps;
     });
 }
 

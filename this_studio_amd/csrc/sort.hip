@@ -9,7 +9,11 @@
 // workgroup.  Only the bits that hold hashes in [0, numGrids] are sorted.
 #include "common.hpp"
 
+#include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
+#include <unordered_map>
 #include <vector>
 
 namespace demsph {
@@ -373,6 +377,163 @@ void step_stamp(const StepStamps& st, hipStream_t s)
     DEMSPH_HIP(hipGetLastError());
 }
 
+std::recursive_mutex& capture_lock()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
+// ------------------------------------------------------------ block cache
+namespace {
+
+// Size classes: 256 B, then 2^k x {1, 1.25, 1.5, 1.75}, so a cached block
+// is at most 25 % larger than the request it serves.
+size_t size_class(size_t b)
+{
+    if (b <= 256) return 256;
+    const int k = 63 - __builtin_clzll((unsigned long long)b);
+    const size_t base = (size_t)1 << k, step = base >> 2;
+    return base + (b - base + step - 1) / step * step;
+}
+
+struct BlockCache {
+    struct Live {
+        size_t bytes;
+        int device;
+        bool host;
+    };
+    using Key = std::tuple<bool, int, size_t>;  // host, device, size class
+    std::mutex mu;
+    int users = 0;
+    size_t cap = 0;
+    size_t held = 0;  // device bytes owned by the cache, live and free
+    size_t free_bytes = 0;
+    std::unordered_map<void*, Live> live;
+    std::multimap<Key, void*> free_blocks;
+};
+BlockCache g_cache;
+
+// Takes every free block out of the cache (caller holds g_cache.mu).
+std::vector<std::pair<void*, bool>> take_free_blocks()
+{
+    std::vector<std::pair<void*, bool>> out;
+    for (const auto& kv : g_cache.free_blocks) out.emplace_back(kv.second, std::get<0>(kv.first));
+    g_cache.free_blocks.clear();
+    g_cache.held -= g_cache.free_bytes;
+    g_cache.free_bytes = 0;
+    return out;
+}
+
+// Returns blocks to HIP (without g_cache.mu: the capture lock comes second).
+void release_blocks(const std::vector<std::pair<void*, bool>>& blocks)
+{
+    if (blocks.empty()) return;
+    CaptureGuard cg(capture_lock());
+    for (const auto& b : blocks) {
+        if (b.second) (void)hipHostFree(b.first);
+        else (void)hipFree(b.first);
+    }
+}
+
+hipError_t cached_alloc(void** p, size_t bytes, bool host)
+{
+    *p = nullptr;
+    size_t cls = bytes;
+    int dev = 0;
+    bool pooled = false;
+    std::vector<std::pair<void*, bool>> evict;
+    {
+        std::lock_guard<std::mutex> lk(g_cache.mu);
+        if (g_cache.users > 0) {
+            pooled = true;
+            if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+            cls = size_class(bytes);
+            const auto it = g_cache.free_blocks.find(BlockCache::Key{host, dev, cls});
+            if (it != g_cache.free_blocks.end()) {
+                *p = it->second;
+                g_cache.free_blocks.erase(it);
+                if (!host) g_cache.free_bytes -= cls;
+                g_cache.live[*p] = BlockCache::Live{cls, dev, host};
+                return hipSuccess;
+            }
+            if (!host && g_cache.held + cls > g_cache.cap) evict = take_free_blocks();
+        }
+    }
+    release_blocks(evict);
+    hipError_t e;
+    {
+        CaptureGuard cg(capture_lock());
+        e = host ? hipHostMalloc(p, cls, hipHostMallocDefault) : hipMalloc(p, cls);
+    }
+    if (e == hipErrorOutOfMemory && pooled) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(g_cache.mu);
+            evict = take_free_blocks();
+        }
+        release_blocks(evict);
+        CaptureGuard cg(capture_lock());
+        e = host ? hipHostMalloc(p, cls, hipHostMallocDefault) : hipMalloc(p, cls);
+    }
+    if (e == hipSuccess && pooled) {
+        std::lock_guard<std::mutex> lk(g_cache.mu);
+        g_cache.live[*p] = BlockCache::Live{cls, dev, host};
+        if (!host) g_cache.held += cls;
+    }
+    return e;
+}
+
+// Returns a block: to the cache while a server runs (if the cache made it),
+// else to HIP.  A block the cache never saw is of the caller's kind.
+void cached_free(void* p, bool host_if_unknown)
+{
+    if (!p) return;
+    bool host = host_if_unknown;
+    {
+        std::lock_guard<std::mutex> lk(g_cache.mu);
+        const auto it = g_cache.live.find(p);
+        if (it != g_cache.live.end()) {
+            const BlockCache::Live L = it->second;
+            g_cache.live.erase(it);
+            host = L.host;
+            if (g_cache.users > 0) {
+                g_cache.free_blocks.emplace(BlockCache::Key{L.host, L.device, L.bytes}, p);
+                if (!L.host) g_cache.free_bytes += L.bytes;
+                return;
+            }
+            if (!L.host) g_cache.held -= L.bytes;
+        }
+    }
+    release_blocks({{p, host}});
+}
+
+}  // namespace
+
+hipError_t dev_malloc(void** p, size_t bytes) { return cached_alloc(p, bytes, false); }
+void dev_free(void* p) { cached_free(p, false); }
+hipError_t host_malloc(void** p, size_t bytes) { return cached_alloc(p, bytes, true); }
+void host_free(void* p) { cached_free(p, true); }
+
+void block_cache_begin(size_t capBytes)
+{
+    std::lock_guard<std::mutex> lk(g_cache.mu);
+    g_cache.users++;
+    g_cache.cap = std::max(g_cache.cap, capBytes);
+}
+
+void block_cache_end()
+{
+    std::vector<std::pair<void*, bool>> evict;
+    {
+        std::lock_guard<std::mutex> lk(g_cache.mu);
+        if (g_cache.users > 0 && --g_cache.users == 0) {
+            evict = take_free_blocks();
+            g_cache.cap = 0;
+        }
+    }
+    release_blocks(evict);
+}
+
 void fill_i32(int* p, long long n, int v, hipStream_t s)
 {
     if (n <= 0) return;
@@ -490,15 +651,18 @@ extern "C" int demsph_build_cells(const double* position, int* hashIndex, int* h
         DEMSPH_HIP(hipGetDevice(&dev));
         if (g_raw.device != dev || g_raw.cap < numObject) {
             // Older buffers may still be read by queued work on another stream.
-            DEMSPH_HIP(hipDeviceSynchronize());
+            {
+                CaptureGuard cg(capture_lock());
+                DEMSPH_HIP(hipDeviceSynchronize());
+            }
             dfree(g_raw.keys); dfree(g_raw.keys_tmp); dfree(g_raw.vals_tmp);
-            if (g_raw.temp) (void)hipFree(g_raw.temp);
+            dev_free(g_raw.temp);
             g_raw.cap = numObject + numObject / 4 + 1024;
             g_raw.keys = dmalloc<uint32_t>(g_raw.cap);
             g_raw.keys_tmp = dmalloc<uint32_t>(g_raw.cap);
             g_raw.vals_tmp = dmalloc<int>(g_raw.cap);
             g_raw.temp_bytes = radix_sort_temp_bytes((int)g_raw.cap);
-            DEMSPH_HIP(hipMalloc(&g_raw.temp, g_raw.temp_bytes));
+            DEMSPH_HIP(dev_malloc(&g_raw.temp, g_raw.temp_bytes));
             g_raw.device = dev;
         }
         hash_aos(position, n, g, g_raw.keys, s);

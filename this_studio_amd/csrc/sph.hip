@@ -31,9 +31,11 @@
 #include "timing.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -1364,7 +1366,7 @@ struct demsph_sph {
     void drop_graphs()
     {
         if (!graphs.empty() && stream) (void)hipStreamSynchronize(stream);
-        for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
+        for (auto& e : graphs) graph_exec_destroy(e.exec);
         graphs.clear();
     }
 
@@ -1376,16 +1378,16 @@ struct demsph_sph {
             dfree(P[b]); dfree(V[b]); dfree(vx[b]); dfree(vy[b]); dfree(vz[b]); dfree(drho[b]); dfree(orig[b]);
         }
         dfree(rank); dfree(key); dfree(skey); dfree(ktmp); dfree(perm); dfree(vtmp);
-        if (sort_temp) (void)hipFree(sort_temp);
+        dev_free(sort_temp);
         sort_temp = nullptr;
         dfree(cells); dfree(nbr_ff); dfree(cnt_ff); dfree(nbr_fd); dfree(cnt_fd); dfree(Pf); dfree(DPf);
         dfree(maxcount);
-        if (maxcount_h) (void)hipHostFree(maxcount_h);
+        host_free(maxcount_h);
         maxcount_h = nullptr;
         dfree(DP); dfree(DV); dfree(dphase); dfree(dorig); dfree(dcells); dfree(dummy_near); dfree(d_phases);
         dfree(ck_P); dfree(ck_V); dfree(ck_vx); dfree(ck_vy); dfree(ck_vz); dfree(ck_drho); dfree(ck_orig);
         dfree(ck_rank);
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        if (own_stream) stream_destroy(stream);
         stream = nullptr;
     }
 
@@ -1484,7 +1486,7 @@ struct demsph_sph {
     {
         if (initialized) release_state_keep_host();
         if (!stream) {
-            DEMSPH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            DEMSPH_HIP(stream_create(&stream));
             own_stream = true;
         }
         g[0] = gravity[0]; g[1] = gravity[1]; g[2] = gravity[2];
@@ -1526,7 +1528,7 @@ struct demsph_sph {
         const int nsort = std::max(nf, nd);
         key = dmalloc<uint32_t>(nsort); skey = dmalloc<uint32_t>(nsort); ktmp = dmalloc<uint32_t>(nsort);
         perm = dmalloc<int>(nsort); vtmp = dmalloc<int>(nsort);
-        DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(nsort)));
+        DEMSPH_HIP(dev_malloc(&sort_temp, radix_sort_temp_bytes(nsort)));
         cells = dmalloc<int2>(numGrids);
         Pf = dmalloc<float4>(nf);
         DPf = dmalloc<float4>(nd > 0 ? nd : 1);
@@ -1541,7 +1543,7 @@ struct demsph_sph {
         }
         cnt_ff = dmalloc<int>(n); cnt_fd = dmalloc<int>(n);
         maxcount = dmalloc<int>(3);
-        DEMSPH_HIP(hipHostMalloc((void**)&maxcount_h, 3 * sizeof(int), hipHostMallocDefault));
+        DEMSPH_HIP(host_malloc((void**)&maxcount_h, 3 * sizeof(int)));
         alloc_lists();
         cP = cV = cD = cO = 0;
 
@@ -1873,6 +1875,9 @@ struct demsph_sph {
             if (e.key == key) return e.exec;
         const int sP = cP, sV = cV, sD = cD, sO = cO;
         hipGraph_t g = nullptr;
+        // No other thread may allocate, free or synchronise the device
+        // until the graph is instantiated (see capture_lock).
+        CaptureGuard capture_guard(capture_lock());
         DEMSPH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
             one_step(dt);
@@ -1894,7 +1899,7 @@ struct demsph_sph {
         DEMSPH_HIP(e);
         if (graphs.size() >= 4) {
             DEMSPH_HIP(hipStreamSynchronize(stream));
-            (void)hipGraphExecDestroy(graphs.front().exec);
+            graph_exec_destroy(graphs.front().exec);
             graphs.erase(graphs.begin());
         }
         graphs.push_back({key, ex});
@@ -1932,6 +1937,194 @@ struct demsph_sph {
             checkpoint(false);
         }
         throw std::runtime_error("neighbour list capacity did not converge");
+    }
+
+    // ------------------------------------------------------------ autotuner
+    // Times the list-build and pair-pass variants on this scene's own steps,
+    // from a checkpoint that is restored after every trial, and keeps the
+    // fastest.  Mode 0 tries only knobs that leave every result bit-identical
+    // (list lanes, the FP32 prefilter: the lists are the same entries in the
+    // same order); mode 1 also tries pair lanes, which change the FP32
+    // summation order of the pair passes.  The cell size is not a knob: it
+    // fixes the pair sets' order and the sort (2 max h, WCSPHSolver.h:127-135).
+    // Choices are kept per size class in a process-wide cache, so that scenes
+    // of one class get the same variants whichever is tuned first.
+    struct TuneChoice {
+        int list_lanes = 0, prefilter = 1, pair_lanes = 0;
+    };
+    struct TuneOutcome {
+        TuneChoice chosen, rules;
+        int configs = 0;
+        bool cached = false;
+        double list_us = 0.0, pair_us = 0.0;              // chosen variants, per step
+        double rules_list_us = 0.0, rules_pair_us = 0.0;  // built-in lane rules, per step
+    };
+    using TuneKey = std::array<int, 4>;
+    static std::mutex& tune_mutex()
+    {
+        static std::mutex m;
+        return m;
+    }
+    static std::map<TuneKey, TuneChoice>& tune_cache()
+    {
+        static std::map<TuneKey, TuneChoice> c;
+        return c;
+    }
+    // Size class: uniform phase, fluid count to a quarter octave, dummy
+    // count to a half octave, mode.
+    TuneKey tune_key(int mode) const
+    {
+        const auto oct = [](double v, double per) { return (int)std::lround(per * std::log2(v + 1.0)); };
+        return {uniform ? 1 : 0, oct(nf, 4.0), oct(nd, 2.0), mode};
+    }
+
+    void apply_choice(const TuneChoice& c, int mode)
+    {
+        drop_graphs();
+        list_lanes_set = c.list_lanes;
+        prefilter = c.prefilter != 0;
+        if (mode == 1) pair_lanes_set = c.pair_lanes;
+    }
+
+    // Mean ms per step of the list build and of the three pair passes over
+    // `steps` eager steps from the checkpoint (one untimed step first).
+    std::pair<double, double> time_trial(double dt, int steps, int sP, int sV, int sD, int sO)
+    {
+        for (int attempt = 0; attempt < 8; attempt++) {
+            cP = sP; cV = sV; cD = sD; cO = sO;
+            checkpoint(false);
+            reset_overflow();
+            timer.enabled = false;
+            one_step(dt);
+            timer.enabled = true;
+            timer.reset();
+            for (int k = 0; k < steps; k++) one_step(dt);
+            if (check_overflow()) continue;  // grown lists: time the trial again
+            double lists = 0.0, pairs = 0.0;
+            for (const auto& kv : timer.totals()) {
+                if (kv.first == "sph_lists") lists += kv.second.second;
+                else if (kv.first == "sph_momentum_first" || kv.first == "sph_momentum_second" ||
+                         kv.first == "sph_continuity")
+                    pairs += kv.second.second;
+            }
+            timer.reset();
+            return {lists / steps, pairs / steps};
+        }
+        throw std::runtime_error("neighbour list capacity did not converge");
+    }
+
+    TuneOutcome autotune(double dt, int steps, int mode, bool use_cache)
+    {
+        require_init();
+        if (dt <= 0.0) throw InvalidArg("time step must be positive");
+        if (steps < 1 || steps > 1000) throw InvalidArg("trial steps must be in 1..1000");
+        if (mode != 0 && mode != 1) throw InvalidArg("mode must be 0 (result-invariant knobs) or 1 (also pair lanes)");
+        TuneOutcome out;
+        out.rules = TuneChoice{lanes_for(600000), 1, lanes_for(400000)};
+        const TuneKey key = tune_key(mode);
+        if (use_cache) {
+            std::lock_guard<std::mutex> lk(tune_mutex());
+            const auto it = tune_cache().find(key);
+            if (it != tune_cache().end()) {
+                out.chosen = it->second;
+                out.cached = true;
+                apply_choice(out.chosen, mode);
+                if (mode == 0) out.chosen.pair_lanes = pair_lanes();
+                return out;
+            }
+        }
+        DEMSPH_HIP(hipStreamSynchronize(stream));
+        timer.harvest();
+        drop_graphs();
+        std::map<std::string, std::pair<long, double>> saved_totals;
+        timer.swap_totals(saved_totals);
+        const bool saved_enabled = timer.enabled;
+        const StepStamps saved_stamps = stamps;
+        const int saved_list = list_lanes_set, saved_pair = pair_lanes_set;
+        const bool saved_pre = prefilter;
+        stamps = StepStamps{};  // trials leave no step timestamps
+        ensure_checkpoint();
+        checkpoint(true);
+        const int sP = cP, sV = cV, sD = cD, sO = cO;
+        const auto restore = [&] {
+            cP = sP; cV = sV; cD = sD; cO = sO;
+            checkpoint(false);
+            DEMSPH_HIP(hipStreamSynchronize(stream));
+            timer.harvest();
+            timer.swap_totals(saved_totals);
+            timer.enabled = saved_enabled;
+            stamps = saved_stamps;
+            lists_valid = false;  // the lists hold the last trial's step
+        };
+        TuneChoice best;
+        try {
+            // List build: every lane count, with and without the prefilter
+            // (one lane has no prefilter variant).  Near-ties keep the rule.
+            double best_l = 0.0, pair_sum = 0.0;
+            for (int L : {1, 2, 4, 8}) {
+                for (int pre = 0; pre < 2; pre++) {
+                    if (L == 1 && pre) continue;
+                    list_lanes_set = L;
+                    prefilter = pre != 0;
+                    const auto t = time_trial(dt, steps, sP, sV, sD, sO);
+                    out.configs++;
+                    const bool rule = L == out.rules.list_lanes && pre == 1;
+                    if (rule) out.rules_list_us = t.first * 1e3;
+                    if (out.configs == 1 || t.first < best_l) {
+                        best_l = t.first;
+                        best.list_lanes = L;
+                        best.prefilter = pre;
+                    }
+                    pair_sum += t.second;
+                }
+            }
+            // the pair passes ran with the scene's pair lanes in every trial
+            out.pair_us = pair_sum / out.configs * 1e3;
+            if (out.rules_list_us <= best_l * 1e3 * 1.02) {
+                best.list_lanes = out.rules.list_lanes;
+                best.prefilter = 1;
+                best_l = out.rules_list_us * 1e-3;
+            }
+            out.list_us = best_l * 1e3;
+            list_lanes_set = best.list_lanes;
+            prefilter = best.prefilter != 0;
+            if (mode == 1) {
+                double best_p = 0.0;
+                int k = 0;
+                for (int L : {1, 2, 4, 8}) {
+                    pair_lanes_set = L;
+                    const auto t = time_trial(dt, steps, sP, sV, sD, sO);
+                    out.configs++;
+                    if (L == out.rules.pair_lanes) out.rules_pair_us = t.second * 1e3;
+                    if (k++ == 0 || t.second < best_p) {
+                        best_p = t.second;
+                        best.pair_lanes = L;
+                    }
+                }
+                if (out.rules_pair_us <= best_p * 1e3 * 1.02) {
+                    best.pair_lanes = out.rules.pair_lanes;
+                    best_p = out.rules_pair_us * 1e-3;
+                }
+                out.pair_us = best_p * 1e3;
+            } else {
+                best.pair_lanes = pair_lanes();
+                if (best.pair_lanes == out.rules.pair_lanes) out.rules_pair_us = out.pair_us;
+            }
+        } catch (...) {
+            list_lanes_set = saved_list;
+            pair_lanes_set = saved_pair;
+            prefilter = saved_pre;
+            restore();
+            throw;
+        }
+        restore();
+        apply_choice(best, mode);
+        out.chosen = best;
+        {
+            std::lock_guard<std::mutex> lk(tune_mutex());
+            tune_cache()[key] = best;
+        }
+        return out;
     }
 
     void refresh_eos()
@@ -2196,7 +2389,7 @@ extern "C" int demsph_sph_set(demsph_sph* s, const char* field, const void* in, 
                 const size_t o = orig[i];
                 h[i].x = src[3 * o]; h[i].y = src[3 * o + 1]; h[i].z = src[3 * o + 2];
             }
-            DEMSPH_HIP(hipMemcpy(s->P[s->cP], h.data(), n * sizeof(FluidPos), hipMemcpyHostToDevice));
+            copy_sync(s->P[s->cP], h.data(), n * sizeof(FluidPos), hipMemcpyHostToDevice, s->stream);
             s->lists_valid = false;
             return;
         }
@@ -2210,17 +2403,17 @@ extern "C" int demsph_sph_set(demsph_sph* s, const char* field, const void* in, 
                 hv[i].x = (float)hx[i]; hv[i].y = (float)hy[i]; hv[i].z = (float)hz[i];
             }
             const int b = s->cV;
-            DEMSPH_HIP(hipMemcpy(s->vx[b], hx.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(s->vy[b], hy.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(s->vz[b], hz.data(), n * 8, hipMemcpyHostToDevice));
-            DEMSPH_HIP(hipMemcpy(s->V[b], hv.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+            copy_sync(s->vx[b], hx.data(), n * 8, hipMemcpyHostToDevice, s->stream);
+            copy_sync(s->vy[b], hy.data(), n * 8, hipMemcpyHostToDevice, s->stream);
+            copy_sync(s->vz[b], hz.data(), n * 8, hipMemcpyHostToDevice, s->stream);
+            copy_sync(s->V[b], hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, s->stream);
             return;
         }
         if (f == "densityChange") {
             if (count != n) throw InvalidArg("expected numFluid values");
             std::vector<float> h(n);
             for (size_t i = 0; i < n; i++) h[i] = (float)src[orig[i]];
-            DEMSPH_HIP(hipMemcpy(s->drho[s->cD], h.data(), n * 4, hipMemcpyHostToDevice));
+            copy_sync(s->drho[s->cD], h.data(), n * 4, hipMemcpyHostToDevice, s->stream);
             s->refresh_eos();
             DEMSPH_HIP(hipStreamSynchronize(st));
             return;
@@ -2285,6 +2478,35 @@ extern "C" int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value)
             return;
         }
         throw InvalidArg("unknown tuning knob '" + k + "'");
+    });
+}
+
+extern "C" int demsph_sph_autotune(demsph_sph* s, double dt, int trialSteps, int mode, int useCache,
+                                   demsph_sph_tune_report* report)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArg("null scene");
+        const auto o = s->autotune(dt, trialSteps, mode, useCache != 0);
+        if (!report) return;
+        report->listLanes = o.chosen.list_lanes;
+        report->listPrefilter = o.chosen.prefilter;
+        report->pairLanes = o.chosen.pair_lanes;
+        report->rulesListLanes = o.rules.list_lanes;
+        report->rulesPairLanes = o.rules.pair_lanes;
+        report->configs = o.configs;
+        report->cached = o.cached ? 1 : 0;
+        report->listUs = o.list_us;
+        report->pairUs = o.pair_us;
+        report->rulesListUs = o.rules_list_us;
+        report->rulesPairUs = o.rules_pair_us;
+    });
+}
+
+extern "C" int demsph_sph_autotune_clear_cache(void)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(demsph_sph::tune_mutex());
+        demsph_sph::tune_cache().clear();
     });
 }
 
@@ -2361,11 +2583,11 @@ struct demsph_couple {
     {
         if (f && f->stream) (void)hipStreamSynchronize(f->stream);
         dfree(key); dfree(skey); dfree(ktmp); dfree(perm); dfree(vtmp);
-        if (sort_temp) (void)hipFree(sort_temp);
+        dev_free(sort_temp);
         sort_temp = nullptr;
         dfree(cells); dfree(SP); dfree(SV); dfree(SW); dfree(nbr_fs); dfree(cnt_fs);
         dfree(rcnt); dfree(rstart); dfree(rfluid);
-        if (scan_temp) (void)hipFree(scan_temp);
+        dev_free(scan_temp);
         scan_temp = nullptr;
         dfree(Fc); dfree(acs);
         ready = false;
@@ -2390,13 +2612,13 @@ struct demsph_couple {
         const size_t nf = f->nf;
         key = dmalloc<uint32_t>(ns); skey = dmalloc<uint32_t>(ns); ktmp = dmalloc<uint32_t>(ns);
         perm = dmalloc<int>(ns); vtmp = dmalloc<int>(ns);
-        DEMSPH_HIP(hipMalloc(&sort_temp, radix_sort_temp_bytes(ns)));
+        DEMSPH_HIP(dev_malloc(&sort_temp, radix_sort_temp_bytes(ns)));
         cells = dmalloc<int2>(f->numGrids);
         SP = dmalloc<SolidPos>(ns); SV = dmalloc<float4>(ns); SW = dmalloc<float4>(ns);
         nbr_fs = dmalloc<int>((size_t)K_fs * nf); cnt_fs = dmalloc<int>(nf);
         rcnt = dmalloc<int>(ns + 1); rstart = dmalloc<int>(ns + 1);
         rfluid = dmalloc<int>(rcap());
-        DEMSPH_HIP(hipMalloc(&scan_temp, scan_temp_bytes(ns + 1)));
+        DEMSPH_HIP(dev_malloc(&scan_temp, scan_temp_bytes(ns + 1)));
         Fc = dmalloc<double>(6 * (size_t)ns);
         acs = dmalloc<float>(3 * nf);
         DEMSPH_HIP(hipMemsetAsync(Fc, 0, 6 * (size_t)ns * sizeof(double), f->stream));

@@ -48,6 +48,9 @@ public:
         totals_.clear();
     }
     const std::map<std::string, std::pair<long, double>>& totals() const { return totals_; }
+    // Exchanges the accumulated totals (lets a caller time its own launches
+    // and then put back what was recorded before).
+    void swap_totals(std::map<std::string, std::pair<long, double>>& other) { totals_.swap(other); }
     ~KernelTimer()
     {
         for (auto& r : pending_) {
