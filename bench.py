@@ -292,7 +292,7 @@ def main() -> int:
     ap.add_argument("--beam-steps", type=int, default=400, help="steps of the cantilever-with-clumps leg (0: skip it)")
     ap.add_argument("--ice-steps", type=int, default=40, help="steps of the coupled ice-cylinder leg (0: skip it)")
     ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
-    ap.add_argument("--mix-slots", type=int, default=16, help="concurrent jobs of the batched server")
+    ap.add_argument("--mix-slots", type=int, default=8, help="concurrent jobs of the batched server")
     ap.add_argument("--mix-autotune", choices=("off", "exact", "pairs"), default="exact",
                     help="also serve the batched path with fluid scenes autotuned on admission")
     ap.add_argument("--no-autotune-report", action="store_true",
