@@ -60,10 +60,10 @@ def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> flo
 # this workload (HBM bytes = 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE, from a
 # plain 256 MiB copy), as recorded in profiles/wcsph_dam_break_256k_kernels.md.
 MEASURED_HBM_BYTES = {
-    "sph_lists": (23.0 + 83.7) * 1e6,
-    "sph_momentum_first": (125.7 + 18.4) * 1e6,
-    "sph_momentum_second": (124.9 + 10.2) * 1e6,
-    "sph_continuity": (120.1 + 9.2) * 1e6,
+    "sph_lists": (20.8 + 88.3) * 1e6,
+    "sph_momentum_first": (124.1 + 18.4) * 1e6,
+    "sph_momentum_second": (123.5 + 10.2) * 1e6,
+    "sph_continuity": (118.1 + 9.2) * 1e6,
     "sph_reorder": (24.4 + 26.0) * 1e6,
 }
 

@@ -629,7 +629,7 @@ __global__ __launch_bounds__(kThreads) void k_build_fs_lists(ListArgs a)
 // under 128 VGPRs; 8 doubles the gathers in flight per wavefront at lower
 // occupancy.
 constexpr int kBatch = DEMSPH_PAIR_BATCH;
-static_assert(kBatch == 4 || kBatch == 8, "pair batch is half or all of a chunk");
+static_assert(kBatch >= 1 && kBatch <= 8, "pair batch within a chunk");
 
 // LANES consecutive lanes share one particle: lane q takes the list chunks
 // q, q + LANES, ... (LANES = 1: every chunk in order), so a small scene
@@ -642,22 +642,19 @@ __device__ __forceinline__ void for_chunks(const int* __restrict__ list, int cnt
     const int* L = list + (size_t)i * kChunk;
     const size_t cs = (size_t)nf * kChunk;
     for (int c = q; c * kChunk < n; c += LANES) {
+        const int base = c * kChunk;
+        const int4* src = reinterpret_cast<const int4*>(L + (size_t)c * cs);
+        const int4 lo = src[0];
+        const int4 hi = base + 4 < n ? src[1] : make_int4(safe, safe, safe, safe);
+        const int jall[kChunk] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
-        for (int half = 0; half < kChunk; half += kBatch) {
-            const int base = c * kChunk + half;
-            if (base >= n) break;
-            const int* src = L + (size_t)c * cs + half;
-            int js[kBatch];
-            const int4 lo = reinterpret_cast<const int4*>(src)[0];
-            js[0] = lo.x; js[1] = lo.y; js[2] = lo.z; js[3] = lo.w;
-            if (kBatch == 8) {
-                const int4 hi = reinterpret_cast<const int4*>(src)[1];
-                js[4 % kBatch] = hi.x; js[5 % kBatch] = hi.y; js[6 % kBatch] = hi.z; js[7 % kBatch] = hi.w;
-            }
+        for (int b0 = 0; b0 < kChunk; b0 += kBatch) {
+            if (base + b0 >= n) break;
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-                const bool valid = base + u < n;
-                body(valid ? js[u] : safe, valid);
+                if (b0 + u >= kChunk) break;
+                const bool valid = base + b0 + u < n;
+                body(valid ? jall[b0 + u] : safe, valid);
             }
         }
     }
@@ -665,7 +662,8 @@ __device__ __forceinline__ void for_chunks(const int* __restrict__ list, int cnt
 
 // for_chunks in two phases per batch: every entry's records are gathered
 // (load) before any is used (use), so kBatch independent gathers are in
-// flight per lane.  Same entries, same order as for_chunks.
+// flight per lane.  A chunk's eight indices are read at once; batches of
+// kBatch entries follow in order.  Same entries, same order as for_chunks.
 template <int LANES, class LoadF, class UseF>
 __device__ __forceinline__ void for_chunks_gathered(const int* __restrict__ list, int cnt, int K, int nf, int i, int q,
                                                     int safe, LoadF&& load, UseF&& use)
@@ -674,26 +672,27 @@ __device__ __forceinline__ void for_chunks_gathered(const int* __restrict__ list
     const int* L = list + (size_t)i * kChunk;
     const size_t cs = (size_t)nf * kChunk;
     for (int c = q; c * kChunk < n; c += LANES) {
+        const int base = c * kChunk;
+        const int4* src = reinterpret_cast<const int4*>(L + (size_t)c * cs);
+        const int4 lo = src[0];
+        const int4 hi = base + 4 < n ? src[1] : make_int4(safe, safe, safe, safe);
+        const int jall[kChunk] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
-        for (int half = 0; half < kChunk; half += kBatch) {
-            const int base = c * kChunk + half;
-            if (base >= n) break;
-            const int* src = L + (size_t)c * cs + half;
+        for (int b0 = 0; b0 < kChunk; b0 += kBatch) {
+            if (base + b0 >= n) break;
             int js[kBatch];
-            const int4 lo = reinterpret_cast<const int4*>(src)[0];
-            js[0] = lo.x; js[1] = lo.y; js[2] = lo.z; js[3] = lo.w;
-            if (kBatch == 8) {
-                const int4 hi = reinterpret_cast<const int4*>(src)[1];
-                js[4 % kBatch] = hi.x; js[5 % kBatch] = hi.y; js[6 % kBatch] = hi.z; js[7 % kBatch] = hi.w;
-            }
             decltype(load(0)) rec[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; u++) {
-                js[u] = base + u < n ? js[u] : safe;
+                if (b0 + u >= kChunk) break;
+                js[u] = base + b0 + u < n ? jall[b0 + u] : safe;
                 rec[u] = load(js[u]);
             }
 #pragma unroll
-            for (int u = 0; u < kBatch; u++) use(rec[u], js[u], base + u < n);
+            for (int u = 0; u < kBatch; u++) {
+                if (b0 + u >= kChunk) break;
+                use(rec[u], js[u], base + b0 + u < n);
+            }
         }
     }
 }
