@@ -215,10 +215,8 @@ struct PairForceArgs {
 // (contactKernel.cu:135-259), one thread per owner ball over its pairs, and
 // the owner half of sumObjectPointedForceTorque (:657-681).
 template <class T>
-__global__ __launch_bounds__(kT) void k_pair_forces(PairForceArgs a)
+__device__ __forceinline__ void pair_forces_ball(const PairForceArgs& a, int i)
 {
-    const int i = xcd_block() * kT + threadIdx.x;
-    if (i >= a.b.n) return;
     const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i);
     const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i);
     const D3 wi = ld3(a.b.wx, a.b.wy, a.b.wz, i);
@@ -263,6 +261,13 @@ __global__ __launch_bounds__(kT) void k_pair_forces(PairForceArgs a)
     }
     ((T*)a.ofx)[i] = Fo.x; ((T*)a.ofy)[i] = Fo.y; ((T*)a.ofz)[i] = Fo.z;
     ((T*)a.otx)[i] = To.x; ((T*)a.oty)[i] = To.y; ((T*)a.otz)[i] = To.z;
+}
+template <class T>
+__global__ __launch_bounds__(kT) void k_pair_forces(PairForceArgs a)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    pair_forces_ball<T>(a, i);
 }
 
 struct BondArgs {
@@ -448,11 +453,9 @@ struct WallArgs {
 // candidate list, its spring history and cancel flags live in registers
 // and are written back as the new history.
 template <class T>
-__global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
+__device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
 {
     constexpr int KMAX = 16;
-    const int i = xcd_block() * kT + threadIdx.x;
-    if (i >= a.b.n) return;
     const int o = a.b.orig[i];
     const int N = a.nballs;
     const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i);
@@ -579,6 +582,13 @@ __global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
     ((T*)a.ofx)[i] = Fw.x; ((T*)a.ofy)[i] = Fw.y; ((T*)a.ofz)[i] = Fw.z;
     ((T*)a.otx)[i] = Tw.x; ((T*)a.oty)[i] = Tw.y; ((T*)a.otz)[i] = Tw.z;
 }
+template <class T>
+__global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    wall_contacts_ball<T>(a, i);
+}
 
 // ----------------------------------------------------------------- gather
 struct GatherArgs {
@@ -598,10 +608,8 @@ __device__ __forceinline__ T at(const void* p, int i) { return ((const T*)p)[i];
 // sumObjectPointingForceTorqueFromInteraction (contactKernel.cu:683-714) and
 // the per-ball totals: owner sums - partner-side pair terms + bonds + walls.
 template <class T>
-__global__ __launch_bounds__(kT) void k_gather(GatherArgs a)
+__device__ __forceinline__ void gather_ball(const GatherArgs& a, int j)
 {
-    const int j = xcd_block() * kT + threadIdx.x;
-    if (j >= a.b.n) return;
     T fx = at<T>(a.ofx, j), fy = at<T>(a.ofy, j), fz = at<T>(a.ofz, j);
     T tx = at<T>(a.otx, j), ty = at<T>(a.oty, j), tz = at<T>(a.otz, j);
     for (int q = a.rstart[j]; q < a.rstart[j + 1]; q++) {
@@ -629,6 +637,13 @@ __global__ __launch_bounds__(kT) void k_gather(GatherArgs a)
     }
     a.out.fx[j] = fx; a.out.fy[j] = fy; a.out.fz[j] = fz;
     a.out.tx[j] = tx; a.out.ty[j] = ty; a.out.tz[j] = tz;
+}
+template <class T>
+__global__ __launch_bounds__(kT) void k_gather(GatherArgs a)
+{
+    const int j = xcd_block() * kT + threadIdx.x;
+    if (j >= a.b.n) return;
+    gather_ball<T>(a, j);
 }
 
 // ----------------------------------------------------------------- integration
@@ -692,10 +707,8 @@ struct BallHalfArgs {
 // ballVelocityAngularVelocityIntegrationKernel (:30-56) and, in the first
 // half, positionIntegrationKernel (:112-121) for every ball.
 template <bool FIRST>
-__global__ __launch_bounds__(kT) void k_ball_half(BallHalfArgs a)
+__device__ __forceinline__ void ball_half_ball(const BallHalfArgs& a, int i)
 {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= a.b.n) return;
     D3 v = ld3(a.b.vx, a.b.vy, a.b.vz, i);
     const int c = a.b.clump[i];
     // A clump id without a clump object only excludes contacts; such a ball
@@ -725,6 +738,39 @@ __global__ __launch_bounds__(kT) void k_ball_half(BallHalfArgs a)
         a.b.py[i] += dt * v.y;
         a.b.pz[i] += dt * v.z;
     }
+}
+template <bool FIRST>
+__global__ __launch_bounds__(kT) void k_ball_half(BallHalfArgs a)
+{
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= a.b.n) return;
+    ball_half_ball<FIRST>(a, i);
+}
+
+// The two per-ball loops of a force evaluation as one launch: contacts over
+// the ball's owned pairs, then its wall contacts.  Neither reads what the
+// other writes.
+template <class T>
+__global__ __launch_bounds__(kT) void k_pair_wall_forces(PairForceArgs pa, WallArgs wa)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= pa.b.n) return;
+    pair_forces_ball<T>(pa, i);
+    wall_contacts_ball<T>(wa, i);
+}
+
+// A ball's force and torque gather and its half kick (with the drift in the
+// first half) as one launch: the gather reads only pair, bond and wall
+// terms, and the kick only the ball's own state.  Used without clumps (a
+// clump's kick sums its pebbles' forces first) and without hooks (they act
+// between the two).
+template <class T, bool FIRST>
+__global__ __launch_bounds__(kT) void k_gather_half(GatherArgs ga, BallHalfArgs ha)
+{
+    const int i = xcd_block() * kT + threadIdx.x;
+    if (i >= ga.b.n) return;
+    gather_ball<T>(ga, i);
+    ball_half_ball<FIRST>(ha, i);
 }
 
 // launchWallIntegration / vertex update / circumcentres
@@ -1423,15 +1469,29 @@ struct demsph_dem {
         }
     }
 
-    void evaluate_pairs(double springDt)
+    PairForceArgs pair_args(double springDt) const
+    {
+        PairForceArgs pa{};
+        pa.b = cur(); pa.p = pairs(cp); pa.tb = tables(); pa.dt = springDt;
+        pa.ofx = ofx.p; pa.ofy = ofy.p; pa.ofz = ofz.p; pa.otx = otx.p; pa.oty = oty.p; pa.otz = otz.p;
+        return pa;
+    }
+
+    void evaluate_pairs(double springDt, bool with_walls = false, double wallDt = 0.0)
     {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         Balls b = cur();
-        PairForceArgs pa{};
-        pa.b = b; pa.p = pairs(cp); pa.tb = tables(); pa.dt = springDt;
-        pa.ofx = ofx.p; pa.ofy = ofy.p; pa.ofz = ofz.p; pa.otx = otx.p; pa.oty = oty.p; pa.otz = otz.p;
-        if (fp64) DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<double>, gb, tb, 0, stream, pa);
-        else DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<float>, gb, tb, 0, stream, pa);
+        const PairForceArgs pa = pair_args(springDt);
+        if (with_walls) {
+            prepare_walls();
+            const WallArgs wa = wall_args(wallDt);
+            if (fp64) DEMSPH_LAUNCH(timer, "dem_pair_wall_forces", k_pair_wall_forces<double>, gb, tb, 0, stream, pa, wa);
+            else DEMSPH_LAUNCH(timer, "dem_pair_wall_forces", k_pair_wall_forces<float>, gb, tb, 0, stream, pa, wa);
+        } else if (fp64) {
+            DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<double>, gb, tb, 0, stream, pa);
+        } else {
+            DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<float>, gb, tb, 0, stream, pa);
+        }
         if (nbond > 0) {
             BondArgs ba{};
             ba.nb = nbond; ba.b = b; ba.rank = rank.p; ba.bi = dbi.p; ba.bj = dbj.p; ba.bonded = dbonded.p;
@@ -1446,10 +1506,9 @@ struct demsph_dem {
         }
     }
 
-    void evaluate_walls(double wallDt)
+    // Triangle cells and per-cell candidate lists, when the walls moved.
+    void prepare_walls()
     {
-        const dim3 gb(blocks_for(n, kT)), tb(kT);
-        Balls b = cur();
         {
             if (!tri_cells_valid) {
                 timer.begin("dem_triangle_cells", stream);
@@ -1466,6 +1525,13 @@ struct demsph_dem {
                 timer.end(stream);
                 tri_cells_valid = walls_static;
             }
+        }
+    }
+
+    WallArgs wall_args(double wallDt) const
+    {
+        const Balls b = cur();
+        {
             WallArgs wa{};
             wa.b = b; wa.tb = tables(); wa.dt = wallDt; wa.tg = tg; wa.tcs = tcs.p; wa.tce = tce.p; wa.tsorted = tperm.p;
             wa.tbox = reinterpret_cast<const double2*>(tbox.p);
@@ -1481,9 +1547,17 @@ struct demsph_dem {
             wa.hox = hox.p; wa.hoy = hoy.p; wa.hoz = hoz.p; wa.hov = hov.p;
             wa.ofx = wfx.p; wa.ofy = wfy.p; wa.ofz = wfz.p; wa.otx = wtx.p; wa.oty = wty.p; wa.otz = wtz.p;
             wa.overflow = overflow.p;
-            if (fp64) DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<double>, gb, tb, 0, stream, wa);
-            else DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<float>, gb, tb, 0, stream, wa);
+            return wa;
         }
+    }
+
+    void evaluate_walls(double wallDt)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        prepare_walls();
+        const WallArgs wa = wall_args(wallDt);
+        if (fp64) DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<double>, gb, tb, 0, stream, wa);
+        else DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<float>, gb, tb, 0, stream, wa);
     }
 
     // Per-cell triangle candidate lists (count, scan, one host read of the
@@ -1513,10 +1587,9 @@ struct demsph_dem {
         DEMSPH_HIP(hipGetLastError());
     }
 
-    void gather(bool with_walls)
+    GatherArgs gather_args(bool with_walls) const
     {
-        const dim3 gb(blocks_for(n, kT)), tb(kT);
-        Balls b = cur();
+        const Balls b = cur();
         GatherArgs ga{};
         ga.b = b;
         ga.ofx = ofx.p; ga.ofy = ofy.p; ga.ofz = ofz.p; ga.otx = otx.p; ga.oty = oty.p; ga.otz = otz.p;
@@ -1531,6 +1604,13 @@ struct demsph_dem {
             ga.wfx = wfx.p; ga.wfy = wfy.p; ga.wfz = wfz.p; ga.wtx = wtx.p; ga.wty = wty.p; ga.wtz = wtz.p;
         }
         ga.out = forces();
+        return ga;
+    }
+
+    void gather(bool with_walls)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        const GatherArgs ga = gather_args(with_walls);
         if (fp64) DEMSPH_LAUNCH(timer, "dem_gather", k_gather<double>, gb, tb, 0, stream, ga);
         else DEMSPH_LAUNCH(timer, "dem_gather", k_gather<float>, gb, tb, 0, stream, ga);
     }
@@ -1607,6 +1687,13 @@ struct demsph_dem {
             if (first) DEMSPH_LAUNCH(timer, "dem_clump_half", k_clump_half<true>, dim3(blocks_for(nc, kT)), tb, 0, stream, c);
             else DEMSPH_LAUNCH(timer, "dem_clump_half", k_clump_half<false>, dim3(blocks_for(nc, kT)), tb, 0, stream, c);
         }
+        const BallHalfArgs a = half_args(g, h);
+        if (first) DEMSPH_LAUNCH(timer, "dem_ball_half", k_ball_half<true>, dim3(blocks_for(n, kT)), tb, 0, stream, a);
+        else DEMSPH_LAUNCH(timer, "dem_ball_half", k_ball_half<false>, dim3(blocks_for(n, kT)), tb, 0, stream, a);
+    }
+
+    BallHalfArgs half_args(const double* g, double h) const
+    {
         BallHalfArgs a{};
         a.b = cur(); a.f = forces();
         if (nc > 0) {
@@ -1615,8 +1702,32 @@ struct demsph_dem {
         }
         a.nc = nc;
         a.gx = g[0]; a.gy = g[1]; a.gz = g[2]; a.h = h;
-        if (first) DEMSPH_LAUNCH(timer, "dem_ball_half", k_ball_half<true>, dim3(blocks_for(n, kT)), tb, 0, stream, a);
-        else DEMSPH_LAUNCH(timer, "dem_ball_half", k_ball_half<false>, dim3(blocks_for(n, kT)), tb, 0, stream, a);
+        return a;
+    }
+
+    // Without clumps or hooks, one force evaluation and the half kick that
+    // follows it run as two launches (pairs with walls, gather with kick)
+    // instead of four; each ball's arithmetic is unchanged.
+    bool fused_ok() const { return nc == 0 && hooks.empty() && !hook_fn; }
+
+    void evaluate_half(bool first, const double* g, double h, double dt)
+    {
+        const dim3 gb(blocks_for(n, kT)), tb(kT);
+        const bool walls = nt > 0;
+        evaluate_pairs(h, walls, dt);
+        const GatherArgs ga = gather_args(walls);
+        const BallHalfArgs ha = half_args(g, h);
+        if (fp64) {
+            if (first) DEMSPH_LAUNCH(timer, "dem_gather_half", (k_gather_half<double, true>), gb, tb, 0, stream, ga, ha);
+            else DEMSPH_LAUNCH(timer, "dem_gather_half", (k_gather_half<double, false>), gb, tb, 0, stream, ga, ha);
+        } else {
+            if (first) DEMSPH_LAUNCH(timer, "dem_gather_half", (k_gather_half<float, true>), gb, tb, 0, stream, ga, ha);
+            else DEMSPH_LAUNCH(timer, "dem_gather_half", (k_gather_half<float, false>), gb, tb, 0, stream, ga, ha);
+        }
+        if (walls) {
+            walls_valid = true;
+            if (!walls_static) move_walls(dt);
+        }
     }
 
     void check_overflow()
@@ -1658,10 +1769,15 @@ struct demsph_dem {
     {
         const double h = 0.5 * dt;
         for (int k = 0; k < count; k++) {
-            evaluate(h, dt);
-            half(true, g, h);
-            evaluate(h, dt);
-            half(false, g, h);
+            if (fused_ok()) {
+                evaluate_half(true, g, h, dt);
+                evaluate_half(false, g, h, dt);
+            } else {
+                evaluate(h, dt);
+                half(true, g, h);
+                evaluate(h, dt);
+                half(false, g, h);
+            }
             step_stamp(stamps, stream);
         }
     }
@@ -1722,12 +1838,17 @@ struct demsph_dem {
             iStep++;
             time += dt;
             if (iStep % 5 == 0) neighbor_search();
-            evaluate(h, dt);
-            run_hooks(time - h);
-            half(true, g, h);
-            evaluate(h, dt);
-            run_hooks(time);
-            half(false, g, h);
+            if (fused_ok()) {
+                evaluate_half(true, g, h, dt);
+                evaluate_half(false, g, h, dt);
+            } else {
+                evaluate(h, dt);
+                run_hooks(time - h);
+                half(true, g, h);
+                evaluate(h, dt);
+                run_hooks(time);
+                half(false, g, h);
+            }
             step_stamp(stamps, stream);
             k++;
         }
