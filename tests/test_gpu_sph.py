@@ -30,11 +30,14 @@ def make_gpu(spec, capacity=None):
     return s
 
 
-def test_build_cells_raw_matches_oracle():
+@pytest.mark.parametrize("n", [50000, 100000, 300000])
+def test_build_cells_raw_matches_oracle(n):
+    """Sizes cover the sort's three scan paths: per-tile bases derived in the
+    scatter (up to 32 tiles), the one-workgroup scan of the digit table (up
+    to 16,384 entries), and the three-kernel device scan."""
     import torch
 
     rng = np.random.default_rng(11)
-    n = 50000
     pos = rng.uniform(-0.1, 1.1, (n, 3))  # some points outside the domain exercise the clamps
     g = oracle.grid_set([0, 0, 0], [1.0, 0.7, 0.4], 0.021)
     ng = g.num_grids

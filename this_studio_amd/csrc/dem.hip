@@ -1419,10 +1419,9 @@ struct demsph_dem {
                       cnt.p);
         // the owner CSR goes into the list that becomes current
         const int np = 1 - cp;
-        exclusive_scan(cnt.p, pstart[np].p, n, scan_temp, total.p, stream);
-        // start[n] = total
-        DEMSPH_HIP(hipMemcpyAsync(pstart[np].p + n, total.p, sizeof(int), hipMemcpyDeviceToDevice, stream));
-        DEMSPH_HIP(hipMemcpyAsync(total_h, total.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+        // start[n] = total, written by the scan
+        exclusive_scan(cnt.p, pstart[np].p, n, scan_temp, pstart[np].p + n, stream);
+        DEMSPH_HIP(hipMemcpyAsync(total_h, pstart[np].p + n, sizeof(int), hipMemcpyDeviceToHost, stream));
         DEMSPH_HIP(hipStreamSynchronize(stream));  // resizePair (interaction.h:164-175)
         timer.harvest();
         const int tot = total_h[0];
@@ -1573,9 +1572,8 @@ struct demsph_dem {
         const dim3 gb(blocks_for(tng, kT)), tb(kT);
         hipLaunchKernelGGL(k_cell_candidates<false>, gb, tb, 0, stream, ca);
         DEMSPH_HIP(hipGetLastError());
-        exclusive_scan(tcand_cnt.p, tcand_start.p, (int)tng, tscan_temp, total.p + 1, stream);
-        DEMSPH_HIP(hipMemcpyAsync(tcand_start.p + tng, total.p + 1, sizeof(int), hipMemcpyDeviceToDevice, stream));
-        DEMSPH_HIP(hipMemcpyAsync(total_h + 1, total.p + 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+        exclusive_scan(tcand_cnt.p, tcand_start.p, (int)tng, tscan_temp, tcand_start.p + tng, stream);
+        DEMSPH_HIP(hipMemcpyAsync(total_h + 1, tcand_start.p + tng, sizeof(int), hipMemcpyDeviceToHost, stream));
         DEMSPH_HIP(hipStreamSynchronize(stream));
         const size_t tot = (size_t)total_h[1];
         if (tot > tcand_tri.n || tcand_tri.p == nullptr) {

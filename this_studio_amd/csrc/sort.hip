@@ -236,6 +236,46 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(const int* __restri
     }
 }
 
+// The whole exclusive scan in one workgroup of 1024 threads, for inputs of
+// up to kScanSingleMax values: one launch instead of three.
+constexpr int kScan1Threads = 1024;
+constexpr int kScanSingleMax = 2 * kScan1Threads * kScanItems;
+__global__ __launch_bounds__(kScan1Threads) void k_scan_single(const int* __restrict__ in, int* __restrict__ out,
+                                                               int n, int* __restrict__ total_out)
+{
+    __shared__ int wsum[kScan1Threads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += kScan1Threads * kScanItems) {
+        const int b = base + threadIdx.x * kScanItems;
+        int v[kScanItems];
+        int sum = 0;
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            v[i] = b + i < n ? in[b + i] : 0;
+            sum += v[i];
+        }
+        const int inc = wave_inclusive_sum(sum);
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int wprefix = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kScan1Threads / 64; w++) {
+            if (w < wave) wprefix += wsum[w];
+            tot += wsum[w];
+        }
+        __syncthreads();
+        int run = carry + wprefix + inc - sum;
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            if (b + i < n) out[b + i] = run;
+            run += v[i];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
 __global__ void k_fill_i32(int* p, long long n, int v)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,6 +326,11 @@ void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, 
     if (n <= 0) {
         // A fill kernel rather than a memset node keeps the scan capturable.
         if (total_dev) fill_i32(total_dev, 1, 0, s);
+        return;
+    }
+    if (n <= kScanSingleMax) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScan1Threads), 0, s, in, out, n, total_dev);
+        DEMSPH_HIP(hipGetLastError());
         return;
     }
     const int m = (n + kScanTile - 1) / kScanTile;
