@@ -32,9 +32,9 @@ def make_gpu(spec, capacity=None):
 
 @pytest.mark.parametrize("n", [50000, 100000, 300000])
 def test_build_cells_raw_matches_oracle(n):
-    """Sizes cover the sort's three scan paths: per-tile bases derived in the
-    scatter (up to 32 tiles), the one-workgroup scan of the digit table (up
-    to 16,384 entries), and the three-kernel device scan."""
+    """Sizes cover the sort's two ways to its scatter bases: derived in the
+    scatter from the digit table (up to 32 tiles of 2,048 keys) and a device
+    scan of the table (49 and 147 tiles)."""
     import torch
 
     rng = np.random.default_rng(11)
