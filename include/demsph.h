@@ -122,6 +122,15 @@ int demsph_sph_sizes(demsph_sph* s, size_t* numFluid, size_t* numDummy, size_t* 
  * *count receives the element count; out may be NULL to query it. */
 int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_t capacity, size_t* count);
 
+/* Frame files: the reference's outputWCSPHVTU (WCSPHSolver.h:360-443,
+ * dir/SPH_FFFF.vtu) and outputDummyVTU (:445-520, dir/dummy_FFFF.vtu), the
+ * same ASCII VTU layout, particles in insertion order, values %.10f from
+ * FP64 (density, pressure and dummy normals are stored in FP32).  what:
+ * DEMSPH_VTU_SPH | DEMSPH_VTU_DUMMY; dir is created when missing. */
+#define DEMSPH_VTU_SPH 1
+#define DEMSPH_VTU_DUMMY 2
+int demsph_sph_write_vtu(demsph_sph* s, const char* dir, int what, long frame, long step, double time);
+
 /* Overwrites position / velocity (double, 3n) or densityChange (double, n)
  * by insertion index (density and pressure follow from the EOS). */
 int demsph_sph_set(demsph_sph* s, const char* field, const void* in, size_t count);
@@ -309,6 +318,15 @@ int demsph_dem_sizes(demsph_dem* d, size_t* numBalls, size_t* numClumps, size_t*
  * "wallOrientation" (double 4/wall), "vertexGlobalPosition" (double 3/vertex),
  * "hashValue" "hashIndex" "cellHashStart" "cellHashEnd" (int). */
 int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity, size_t* count);
+
+/* Frame files: the reference's outputBallVTU (DEMSolver.h:908-993,
+ * dir/ball_FFFF.vtu) and outputMeshWallVTU (:995-1098,
+ * dir/triangleWall_FFFF.vtu; nothing without triangles), the same ASCII VTU
+ * layout, balls in insertion order, values %.10f.  what: DEMSPH_VTU_BALL |
+ * DEMSPH_VTU_TRIANGLE_WALL; dir is created when missing. */
+#define DEMSPH_VTU_BALL 1
+#define DEMSPH_VTU_TRIANGLE_WALL 2
+int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time);
 /* Overwrites "position" "velocity" "angularVelocity" (double 3n, insertion
  * order) or "wallVelocity" "wallAngularVelocity" (double 3/wall; the
  * reference's getMeshWallBody() write path). */

@@ -10,8 +10,11 @@
 //  * getBall()'s device pointers (force(), position(), ...) are insertion-
 //    ordered FP64 views that are valid inside addExternalForceTorque(); the
 //    HostCopy() accessors work at any time.
-//  * solve() prints the reference's frame lines but writes no VTU files; a
-//    subclass may override onFrame() to record results.
+//  * solve() writes the ball and triangle-wall frame files into dir as the
+//    reference does (ball_FFFF.vtu, triangleWall_FFFF.vtu, same layout); the
+//    interaction files (ballInteraction, bondedInteraction,
+//    ballTriangleInteraction) are not written.  An empty dir writes nothing.
+//    A subclass may override onFrame() to record results.
 //  * Errors throw std::runtime_error (the reference's debug-build behaviour).
 #pragma once
 
@@ -219,7 +222,6 @@ public:
                const double maximumTime, const size_t numFrame, const std::string dir, const size_t deviceID = 0,
                const size_t maximumThread = 256)
     {
-        (void)dir;
         demsph::check_hip(hipSetDevice((int)deviceID), "hipSetDevice");
         if (timeStep <= 0.) {
             std::printf("failed! Time step is less than 0\n");
@@ -239,6 +241,7 @@ public:
             return;
         }
         std::printf("Initialization Completed.\n");
+        writeFrame(dir, 0, 0, 0.0);
 
         defaultHook_ = false;
         demsph::check(demsph_dem_set_hook(h_, &DEMSolver::hookTrampoline, this));
@@ -257,6 +260,7 @@ public:
                 demsph::check(demsph_dem_clock(h_, &s, &time));
                 iFrame++;
                 std::printf("Frame %zu at Time %g\n", iFrame, time);
+                writeFrame(dir, iFrame, iStep, time);
                 onFrame(iFrame, iStep, time);
                 if (stop_) break;
             }
@@ -266,6 +270,14 @@ public:
     }
 
     demsph_dem* handle() { return h_; }
+
+    // outputBallVTU / outputMeshWallVTU (DEMSolver.h:908-1098) for one frame.
+    void writeFrame(const std::string& dir, const size_t iFrame, const size_t iStep, const double time)
+    {
+        if (dir.empty()) return;
+        demsph::check(demsph_dem_write_vtu(h_, dir.c_str(), DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL, (long)iFrame,
+                                           (long)iStep, time));
+    }
 
 private:
     hipStream_t stream_;

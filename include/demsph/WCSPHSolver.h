@@ -6,9 +6,10 @@
 // getSPH/getDummy :562-564, solve :567-624) and the WCSPH container
 // accessors of DEMSPH/kernel/particle.h:829-835.
 //
-// solve() prints the reference's frame lines but writes no VTU files; a
-// subclass may override onFrame() to record results.  Errors throw
-// std::runtime_error.
+// solve() writes the fluid and dummy frame files into dir as the reference
+// does (SPH_FFFF.vtu, dummy_FFFF.vtu, same layout; an empty dir writes
+// nothing).  A subclass may override onFrame() to record results.  Errors
+// throw std::runtime_error.
 #pragma once
 
 #include <cstdio>
@@ -128,7 +129,6 @@ public:
                const double maximumTime, const size_t numFrame, const std::string dir, const size_t deviceID = 0,
                const size_t maximumThread = 256)
     {
-        (void)dir;
         (void)maximumThread;
         demsph::check_hip(hipSetDevice((int)deviceID), "hipSetDevice");
         if (timeStep <= 0.) {
@@ -147,6 +147,7 @@ public:
         dummy_.hd_.clear();
         demsph::check(demsph_sph_initialize(h_, &minBoundary.x, &maxBoundary.x, &gravity.x));
         std::printf("Initialization Completed.\n");
+        writeFrame(dir, 0, 0, 0.0);
 
         stop_ = false;
         size_t iStep = 0, iFrame = 0;
@@ -155,11 +156,12 @@ public:
             const size_t next = (iStep / frameInterval + 1) * frameInterval;
             const size_t target = next < numStep + 1 ? next : numStep + 1;
             demsph::check(demsph_sph_step(h_, timeStep, (int)(target - iStep)));
-            time += timeStep * (double)(target - iStep);
+            for (size_t k = iStep; k < target; k++) time += timeStep;  // the reference's running sum
             iStep = target;
             if (iStep % frameInterval == 0) {
                 iFrame++;
                 std::printf("Frame %zu at Time %g\n", iFrame, time);
+                writeFrame(dir, iFrame, iStep, time);
                 onFrame(iFrame, iStep, time);
                 if (stop_) break;
             }
@@ -168,6 +170,14 @@ public:
     }
 
     demsph_sph* handle() { return h_; }
+
+    // outputWCSPHVTU / outputDummyVTU (WCSPHSolver.h:360-520) for one frame.
+    void writeFrame(const std::string& dir, const size_t iFrame, const size_t iStep, const double time)
+    {
+        if (dir.empty()) return;
+        demsph::check(demsph_sph_write_vtu(h_, dir.c_str(), DEMSPH_VTU_SPH | DEMSPH_VTU_DUMMY, (long)iFrame, (long)iStep,
+                                           time));
+    }
 
 private:
     hipStream_t stream_;

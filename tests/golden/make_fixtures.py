@@ -7,7 +7,8 @@ as text and stores inputs/outputs as compact .npz/.json fixtures in this
 directory.  It reads /root/reference only when run by hand; tests read the
 fixtures, never the reference.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py          # every fixture
+    python tests/golden/make_fixtures.py vtu      # vtu_frames.json only
 """
 from __future__ import annotations
 
@@ -95,7 +96,39 @@ def ice_cylinder():
     return b0["Points"].shape[0]
 
 
+def vtu_frames():
+    """Size and SHA-256 of the reference's frame-0 VTU files, whose content is
+    the scenes' initial state: a writer in the same layout must reproduce
+    them byte for byte.  Also the array layout (name, type, components) of
+    every file kind, from frame 1."""
+    import hashlib
+    files = ["elasticBounce/ball_0000.vtu", "elasticBounce/triangleWall_0000.vtu", "damBreak/SPH_0000.vtu",
+             "iceCylinderInteraction/ball_0000.vtu", "iceCylinderInteraction/triangleWall_0000.vtu"]
+    out = {"frame0": {}, "layout": {}}
+    for rel in files:
+        b = (REF / rel).read_bytes()
+        out["frame0"][rel] = {"bytes": len(b), "sha256": hashlib.sha256(b).hexdigest(),
+                              "lines": b.count(b"\n")}
+    arr = re.compile(r'<DataArray([^>]*)>')
+    for rel in ["elasticBounce/ball_0001.vtu", "elasticBounce/triangleWall_0001.vtu", "damBreak/SPH_0001.vtu"]:
+        txt = (REF / rel).read_text()
+        rows = []
+        for attrs in arr.findall(txt):
+            name = re.search(r'Name="([^"]+)"', attrs)
+            comp = re.search(r'NumberOfComponents="(\d+)"', attrs)
+            rows.append([name.group(1) if name else "", re.search(r'type="([^"]+)"', attrs).group(1),
+                         int(comp.group(1)) if comp else 1])
+        out["layout"][rel.split("/")[1].rsplit("_", 1)[0]] = rows
+    (OUT / "vtu_frames.json").write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
+    return len(files)
+
+
 if __name__ == "__main__":
-    print("elasticBounce frames:", elastic_bounce())
-    print("damBreak frames:", dam_break())
-    print("iceCylinder balls:", ice_cylinder())
+    import sys
+    if sys.argv[1:] == ["vtu"]:
+        print("VTU frame-0 files:", vtu_frames())
+    else:
+        print("elasticBounce frames:", elastic_bounce())
+        print("damBreak frames:", dam_break())
+        print("iceCylinder balls:", ice_cylinder())
+        print("VTU frame-0 files:", vtu_frames())

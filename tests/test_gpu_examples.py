@@ -3,6 +3,8 @@ DEMSolver / WCSPHSolver facade of include/demsph/) run the reference's four
 application scenes unchanged in structure: their results are compared with
 the reference's own output frames or, bit for bit, with the same scene
 driven from Python through the C ABI."""
+import hashlib
+import json
 import re
 import subprocess
 from pathlib import Path
@@ -19,13 +21,39 @@ BIN = ROOT / "this_studio_amd" / "examples" / "bin"
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
-def run(name, *args, timeout=300):
+def run(name, *args, cwd, timeout=300):
+    """Runs an example program in cwd, where its solve() writes the frame
+    files (dir/ball_FFFF.vtu ... as the reference's apps do)."""
     exe = BIN / name
     if not exe.exists():
         pytest.fail(f"{exe} is missing: build with `python -m this_studio_amd.build`")
-    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=timeout, cwd=cwd)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     return r.stdout
+
+
+VTU = json.loads((GOLDEN / "vtu_frames.json").read_text())
+_ARRAY = re.compile(r'<DataArray([^>]*)>')
+
+
+def check_frame0(root, rel):
+    """A frame-0 file holds the scene's initial state: byte-identical to the
+    reference's own frame 0 (size and SHA-256 recorded in the fixture)."""
+    b = (root / rel).read_bytes()
+    want = VTU["frame0"][rel]
+    assert len(b) == want["bytes"], (rel, len(b), want["bytes"])
+    assert hashlib.sha256(b).hexdigest() == want["sha256"], rel
+
+
+def check_layout(path, kind):
+    """Array names, types and component counts in the reference's order."""
+    rows = []
+    for attrs in _ARRAY.findall(path.read_text()):
+        name = re.search(r'Name="([^"]+)"', attrs)
+        comp = re.search(r'NumberOfComponents="(\d+)"', attrs)
+        rows.append([name.group(1) if name else "", re.search(r'type="([^"]+)"', attrs).group(1),
+                     int(comp.group(1)) if comp else 1])
+    assert rows == VTU["layout"][kind], (path, rows)
 
 
 def frames(stdout):
@@ -34,7 +62,7 @@ def frames(stdout):
 
 def test_elastic_bounce_program_matches_reference_frames(tmp_path):
     dump = tmp_path / "bounce.bin"
-    out = run("elastic_bounce", 2.0, 200, dump)
+    out = run("elastic_bounce", 2.0, 200, dump, cwd=tmp_path)
     assert "Initialization Completed." in out and len(frames(out)) == 200
     got = np.fromfile(dump, dtype=np.float64).reshape(-1, 3)
     ref = np.load(GOLDEN / "elastic_bounce_frames.npz")["frames"]
@@ -42,16 +70,28 @@ def test_elastic_bounce_program_matches_reference_frames(tmp_path):
     np.testing.assert_allclose(got[:, 0], ref[1:, 0], atol=1e-9)  # frame times
     assert np.abs(got[:, 1] - ref[1:, 4]).max() < 1e-6
     assert np.abs(got[:, 2] - ref[1:, 7]).max() < 1e-5
+    # frame files: frame 0 byte-identical to the reference's, every frame in its layout
+    for kind in ("ball", "triangleWall"):
+        check_frame0(tmp_path, f"elasticBounce/{kind}_0000.vtu")
+        check_layout(tmp_path / "elasticBounce" / f"{kind}_0200.vtu", kind)
+    assert len(list((tmp_path / "elasticBounce").glob("ball_*.vtu"))) == 201
 
 
 def test_dam_break_program_matches_reference_frame1(tmp_path):
     dump = tmp_path / "dam.bin"
-    out = run("dam_break", 5.0, 100, dump, 1)
+    out = run("dam_break", 5.0, 100, dump, 1, cwd=tmp_path)
     assert frames(out)[0] == (1, 670)
     pos = np.fromfile(dump, dtype=np.float64).reshape(-1, 3)
     ref = np.load(GOLDEN / "dam_break_frames.npz")
     assert pos.shape == ref["pos1"].shape
     assert np.abs(pos - ref["pos1"]).max() < 5e-8 + 1e-5 * 0.005
+    check_frame0(tmp_path, "damBreak/SPH_0000.vtu")
+    check_layout(tmp_path / "damBreak" / "SPH_0001.vtu", "SPH")
+    # the reference ships no dummy frames: check the dummy file against the
+    # fluid file's points and the dummy layout of outputDummyVTU
+    d = (tmp_path / "damBreak" / "dummy_0001.vtu").read_text()
+    names = [re.search(r'Name="([^"]+)"', a).group(1) if "Name=" in a else "" for a in _ARRAY.findall(d)]
+    assert names == ["TIME", "STEP", "", "connectivity", "offsets", "types", "smoothLength", "velocity", "normal"]
 
 
 def test_bonded_beam_program_equals_python_driven_scene(tmp_path):
@@ -59,7 +99,7 @@ def test_bonded_beam_program_equals_python_driven_scene(tmp_path):
     launchAddGlobalDampingForceTorque) and addInitialCondition bonds,
     against the built-in hook program driven from Python: identical."""
     dump = tmp_path / "beam.bin"
-    out = run("bonded_beam", 0.2, 1, dump)
+    out = run("bonded_beam", 0.2, 1, dump, cwd=tmp_path)
     (iframe, istep), = frames(out)
     pos = np.fromfile(dump, dtype=np.float64).reshape(-1, 3)
     g = ts.DEMSceneGPU(scenes.bonded_beam())
@@ -72,10 +112,12 @@ def test_ice_cylinder_program_equals_python_driven_scene(tmp_path):
     initial pairs, the 2,664-triangle cylinder: identical to the Python-driven
     scene with the built-in buoyancy hook."""
     dump = tmp_path / "ice.bin"
-    out = run("ice_cylinder", 0.004, 2, dump, 1)
+    out = run("ice_cylinder", 0.004, 2, dump, 1, cwd=tmp_path)
     (iframe, istep), = frames(out)
     assert istep >= 400
     pos = np.fromfile(dump, dtype=np.float64).reshape(-1, 3)
     g = ts.DEMSceneGPU(scenes.ice_cylinder())
     g.step(istep)
     np.testing.assert_array_equal(pos, g.get("position"))
+    for kind in ("ball", "triangleWall"):
+        check_frame0(tmp_path, f"iceCylinderInteraction/{kind}_0000.vtu")

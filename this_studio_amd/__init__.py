@@ -84,6 +84,7 @@ def _declare(L: C.CDLL) -> None:
         "demsph_sph_tune_get": [_vp, C.c_char_p, C.POINTER(C.c_int)],
         "demsph_sph_autotune": [_vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(TuneReport)],
         "demsph_sph_autotune_clear_cache": [],
+        "demsph_sph_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -181,6 +182,13 @@ class SPHScene:
         a, b, c = _sz(), _sz(), _sz()
         check(lib().demsph_sph_sizes(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def write_vtu(self, directory, frame: int, step: int, time: float, fluid: bool = True,
+                  dummies: bool = True) -> None:
+        """The reference's SPH_FFFF.vtu / dummy_FFFF.vtu frame files."""
+        what = (1 if fluid else 0) | (2 if dummies else 0)
+        check(lib().demsph_sph_write_vtu(self._h, str(directory).encode(), what, int(frame), int(step),
+                                         float(time)))
 
     def get(self, field: str) -> np.ndarray:
         cnt = _sz()
@@ -316,6 +324,7 @@ def _declare_dem(L):
         "demsph_dem_hook_buoyancy_drag": [_vp, d, d, _dp, _dp, d],
         "demsph_dem_sizes": [_vp] + [C.POINTER(_sz)] * 6,
         "demsph_dem_get": [_vp, C.c_char_p, _vp, _sz, C.POINTER(_sz)],
+        "demsph_dem_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
         "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
         "demsph_dem_set_graphs": [_vp, C.c_int],
         "demsph_dem_graph_captures": [_vp, C.POINTER(C.c_long)],
@@ -483,6 +492,13 @@ class DEMSceneGPU:
         v = BallViews()
         check(self.L.demsph_dem_views(self._h, C.byref(v)))
         return v
+
+    def write_vtu(self, directory, frame: int, step: int, time: float, balls: bool = True,
+                  walls: bool = True) -> None:
+        """The reference's ball_FFFF.vtu / triangleWall_FFFF.vtu frame files."""
+        what = (1 if balls else 0) | (2 if walls else 0)
+        check(self.L.demsph_dem_write_vtu(self._h, str(directory).encode(), what, int(frame), int(step),
+                                          float(time)))
 
     def get(self, field: str) -> np.ndarray:
         cnt = _sz()

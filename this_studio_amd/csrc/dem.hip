@@ -17,6 +17,7 @@
 #include "hook_math.hpp"
 #include "internal.hpp"
 #include "timing.hpp"
+#include "vtu.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -2391,6 +2392,67 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
         if (f == "wallOrientation") return emit(plain4(d->wq0, d->wq1, d->wq2, d->wq3, d->nw), out, capacity, count);
         if (f == "vertexGlobalPosition") return emit(plain3(d->vgx, d->vgy, d->vgz, d->nv), out, capacity, count);
         throw InvalidArg("unknown field '" + f + "'");
+    });
+}
+
+// outputBallVTU (DEMSolver.h:908-993) and outputMeshWallVTU (:995-1098):
+// balls in insertion order with radius, material, clump id, velocity and
+// angular velocity; the walls' vertices at their current global positions
+// and their triangles with wall and material ids.
+extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!dir) throw InvalidArg("null directory");
+        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL)) throw InvalidArg("unknown frame kinds");
+        if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
+        hipStream_t st = d->stream;
+        if (what & DEMSPH_VTU_BALL) {
+            const int cb = d->cb;
+            const size_t n = d->n;
+            const auto orig = fetch(d->borig[cb].p, n, st);
+            const auto vec = [&](const DBuf<double>& x, const DBuf<double>& y, const DBuf<double>& z) {
+                const auto hx = fetch(x.p, n, st), hy = fetch(y.p, n, st), hz = fetch(z.p, n, st);
+                std::vector<double> v(3 * n);
+                for (size_t i = 0; i < n; i++) {
+                    const size_t o = 3 * (size_t)orig[i];
+                    v[o] = hx[i]; v[o + 1] = hy[i]; v[o + 2] = hz[i];
+                }
+                return v;
+            };
+            vtu::Writer w(dir, "ball", frame);
+            w.begin(time, step, n, n);
+            w.points(vec(d->bpx[cb], d->bpy[cb], d->bpz[cb]));
+            w.vertex_cells(n);
+            w.open_data("PointData", "radius");
+            w.reals("radius", d->hb.r);
+            w.ints("materialID", d->hb.mat);
+            w.ints("clumpID", d->hb.clump);
+            w.vectors("velocity", vec(d->bvx[cb], d->bvy[cb], d->bvz[cb]));
+            w.vectors("angularVelocity", vec(d->bwx[cb], d->bwy[cb], d->bwz[cb]));
+            w.close_data("PointData");
+            w.end();
+        }
+        if ((what & DEMSPH_VTU_TRIANGLE_WALL) && d->nt > 0) {
+            const size_t nv = d->nv, nt = d->nt;
+            const auto gx = fetch(d->vgx.p, nv, st), gy = fetch(d->vgy.p, nv, st), gz = fetch(d->vgz.p, nv, st);
+            std::vector<double> xyz(3 * nv);
+            for (size_t i = 0; i < nv; i++) { xyz[3 * i] = gx[i]; xyz[3 * i + 1] = gy[i]; xyz[3 * i + 2] = gz[i]; }
+            std::vector<int> tmat(nt, 0);
+            for (size_t t = 0; t < nt; t++) {
+                const int wl = d->hw.twall[t];
+                if (wl >= 0 && (size_t)wl < d->hw.mat.size()) tmat[t] = d->hw.mat[(size_t)wl];
+            }
+            vtu::Writer w(dir, "triangleWall", frame);
+            w.begin(time, step, nv, nt);
+            w.points(xyz);
+            w.triangle_cells(d->hw.t0, d->hw.t1, d->hw.t2);
+            w.open_data("CellData", "materialID");
+            w.ints("wallID", d->hw.twall);
+            w.ints("materialID", tmat);
+            w.close_data("CellData");
+            w.end();
+        }
     });
 }
 

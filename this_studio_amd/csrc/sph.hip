@@ -29,6 +29,7 @@
 #include "common.hpp"
 #include "internal.hpp"
 #include "timing.hpp"
+#include "vtu.hpp"
 
 #include <algorithm>
 #include <array>
@@ -2410,6 +2411,58 @@ extern "C" int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_
             return emit(v, out, capacity, count);
         }
         throw InvalidArg("unknown field '" + f + "'");
+    });
+}
+
+// outputWCSPHVTU (WCSPHSolver.h:360-443) and outputDummyVTU (:445-520):
+// particles in insertion order with smoothing length, density, pressure and
+// velocity (fluid) or smoothing length, velocity and normal (dummies).
+extern "C" int demsph_sph_write_vtu(demsph_sph* s, const char* dir, int what, long frame, long step, double time)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArg("null scene");
+        s->require_init();
+        if (!dir) throw InvalidArg("null directory");
+        if (what & ~(DEMSPH_VTU_SPH | DEMSPH_VTU_DUMMY)) throw InvalidArg("unknown frame kinds");
+        if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
+        const auto field = [&](const char* name, size_t n) {
+            std::vector<double> v(n);
+            size_t got = 0;
+            if (demsph_sph_get(s, name, v.data(), n, &got) != DEMSPH_OK || got != n)
+                throw std::runtime_error(std::string("reading ") + name + " for a frame failed");
+            return v;
+        };
+        if (what & DEMSPH_VTU_SPH) {
+            const size_t n = s->nf;
+            std::vector<double> h(n);
+            for (size_t i = 0; i < n; i++) h[i] = s->phases[s->fphase_h[i]].h;
+            vtu::Writer w(dir, "SPH", frame);
+            w.begin(time, step, n, n);
+            w.points(field("position", 3 * n));
+            w.vertex_cells(n);
+            w.open_data("PointData", "smoothLength");
+            w.reals("smoothLength", h);
+            w.reals("density", field("density", n));
+            w.reals("pressure", field("pressure", n));
+            w.vectors("velocity", field("velocity", 3 * n));
+            w.close_data("PointData");
+            w.end();
+        }
+        if (what & DEMSPH_VTU_DUMMY) {
+            const size_t m = s->nd;
+            std::vector<double> h(m);
+            for (size_t i = 0; i < m; i++) h[i] = s->phases[s->dphase_h[i]].h;
+            vtu::Writer w(dir, "dummy", frame);
+            w.begin(time, step, m, m);
+            w.points(s->hd.pos);
+            w.vertex_cells(m);
+            w.open_data("PointData", "smoothLength");
+            w.reals("smoothLength", h);
+            w.vectors("velocity", s->hd.vel);
+            w.vectors("normal", m ? field("dummyNormal", 3 * m) : std::vector<double>());
+            w.close_data("PointData");
+            w.end();
+        }
     });
 }
 
