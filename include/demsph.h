@@ -323,10 +323,21 @@ int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity,
  * dir/ball_FFFF.vtu) and outputMeshWallVTU (:995-1098,
  * dir/triangleWall_FFFF.vtu; nothing without triangles), the same ASCII VTU
  * layout, balls in insertion order, values %.10f.  what: DEMSPH_VTU_BALL |
- * DEMSPH_VTU_TRIANGLE_WALL; dir is created when missing. */
+ * DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE; dir is created when
+ * missing. */
 #define DEMSPH_VTU_BALL 1
 #define DEMSPH_VTU_TRIANGLE_WALL 2
+/* outputBallTriangleInteractionVTU (DEMSolver.h:1286-1370,
+ * dir/ballTriangleInteraction_FFFF.vtu): every candidate ball-triangle
+ * contact of the last force evaluation (point, overlap, normal, force,
+ * torque, springs), by ball in insertion order, then in candidate order.
+ * Needs demsph_dem_record_contacts on before that evaluation. */
+#define DEMSPH_VTU_BALL_TRIANGLE 4
 int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time);
+/* Keep each ball-triangle contact's point, normal, force and torque from
+ * every force evaluation for the frame writer (off by default: 96 bytes per
+ * candidate slot and a store per contact). */
+int demsph_dem_record_contacts(demsph_dem* d, int enable);
 /* Overwrites "position" "velocity" "angularVelocity" (double 3n, insertion
  * order) or "wallVelocity" "wallAngularVelocity" (double 3/wall; the
  * reference's getMeshWallBody() write path). */

@@ -10,10 +10,11 @@
 //  * getBall()'s device pointers (force(), position(), ...) are insertion-
 //    ordered FP64 views that are valid inside addExternalForceTorque(); the
 //    HostCopy() accessors work at any time.
-//  * solve() writes the ball and triangle-wall frame files into dir as the
-//    reference does (ball_FFFF.vtu, triangleWall_FFFF.vtu, same layout); the
-//    interaction files (ballInteraction, bondedInteraction,
-//    ballTriangleInteraction) are not written.  An empty dir writes nothing.
+//  * solve() writes the ball, triangle-wall and ball-triangle contact frame
+//    files into dir as the reference does (ball_FFFF.vtu,
+//    triangleWall_FFFF.vtu, ballTriangleInteraction_FFFF.vtu, same layout);
+//    the ball-ball and bond interaction files (ballInteraction,
+//    bondedInteraction) are not written.  An empty dir writes nothing.
 //    A subclass may override onFrame() to record results.
 //  * Errors throw std::runtime_error (the reference's debug-build behaviour).
 #pragma once
@@ -233,6 +234,8 @@ public:
         if (frameInterval < 1) frameInterval = 1;
         ball_.maxThread_ = maximumThread > 0 ? maximumThread : 256;
 
+        // ball-triangle frames report each contact of the last evaluation
+        demsph::check(demsph_dem_record_contacts(h_, dir.empty() ? 0 : 1));
         demsph::check(demsph_dem_initialize(h_, &minBoundary.x, &maxBoundary.x));
         demsph::check(demsph_dem_solve_begin(h_));
         if (addInitialCondition()) demsph::check(demsph_dem_initialize(h_, &minBoundary.x, &maxBoundary.x));
@@ -271,12 +274,14 @@ public:
 
     demsph_dem* handle() { return h_; }
 
-    // outputBallVTU / outputMeshWallVTU (DEMSolver.h:908-1098) for one frame.
+    // outputBallVTU / outputMeshWallVTU / outputBallTriangleInteractionVTU
+    // (DEMSolver.h:908-1098, 1286-1370) for one frame.
     void writeFrame(const std::string& dir, const size_t iFrame, const size_t iStep, const double time)
     {
         if (dir.empty()) return;
-        demsph::check(demsph_dem_write_vtu(h_, dir.c_str(), DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL, (long)iFrame,
-                                           (long)iStep, time));
+        demsph::check(demsph_dem_write_vtu(h_, dir.c_str(),
+                                           DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE,
+                                           (long)iFrame, (long)iStep, time));
     }
 
 private:

@@ -103,14 +103,16 @@ def vtu_frames():
     every file kind, from frame 1."""
     import hashlib
     files = ["elasticBounce/ball_0000.vtu", "elasticBounce/triangleWall_0000.vtu", "damBreak/SPH_0000.vtu",
-             "iceCylinderInteraction/ball_0000.vtu", "iceCylinderInteraction/triangleWall_0000.vtu"]
+             "iceCylinderInteraction/ball_0000.vtu", "iceCylinderInteraction/triangleWall_0000.vtu",
+             "iceCylinderInteraction/ballTriangleInteraction_0000.vtu"]
     out = {"frame0": {}, "layout": {}}
     for rel in files:
         b = (REF / rel).read_bytes()
         out["frame0"][rel] = {"bytes": len(b), "sha256": hashlib.sha256(b).hexdigest(),
                               "lines": b.count(b"\n")}
     arr = re.compile(r'<DataArray([^>]*)>')
-    for rel in ["elasticBounce/ball_0001.vtu", "elasticBounce/triangleWall_0001.vtu", "damBreak/SPH_0001.vtu"]:
+    for rel in ["elasticBounce/ball_0001.vtu", "elasticBounce/triangleWall_0001.vtu", "damBreak/SPH_0001.vtu",
+                "iceCylinderInteraction/ballTriangleInteraction_0001.vtu"]:
         txt = (REF / rel).read_text()
         rows = []
         for attrs in arr.findall(txt):

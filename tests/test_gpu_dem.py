@@ -294,3 +294,54 @@ def test_ice_cylinder_reference_frame_observables():
     assert np.abs(p[:, 2]).max() < 1e-9  # the sheet floats at its buoyant level
     assert (g.get("bondIsBonded") == 0).sum() > 100
     assert g.get("triPointing").size > 0
+
+
+def _vtu_arrays(path):
+    import re
+    out = {}
+    for attrs, body in re.findall(r'<DataArray([^>]*)>(.*?)</DataArray>', path.read_text(), re.S):
+        m = re.search(r'Name="([^"]+)"', attrs)
+        name = m.group(1) if m else "Points"
+        v = np.array(body.split(), dtype=np.float64)
+        if 'NumberOfComponents="3"' in attrs:
+            v = v.reshape(-1, 3)
+        out[name] = v
+    return out
+
+
+def test_ball_triangle_frame_reports_contact_closed_form(tmp_path):
+    """A ball pressed into a flat two-triangle plane by a known overlap, with
+    a linear spring and no damping, gravity or motion.  After the first
+    force evaluation the ballTriangleInteraction frame holds one contact
+    with the closed-form point, normal, overlap and force kn*delta along
+    the normal and zero torque.  Its overlap and springs equal the
+    triOverlap / triSlidingSpring fields."""
+    r, delta, kn = 0.01, 1.0e-4, 2.0e4
+    s = scenes.DEMScene("press")
+    s.add_balls([0.3, -0.2, r - delta], [0, 0, 0], [0, 0, 0], r, 2000.0, 0)
+    verts = np.array([[-1.0, -1.0, 0.0], [1.0, -1.0, 0.0], [1.0, 1.0, 0.0], [-1.0, 1.0, 0.0]])
+    s.add_mesh_wall(verts, [0, 0], [1, 2], [2, 3], [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    s.set_linear_pair(0, 0, kn, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    s.min_bound, s.max_bound = np.array([-1.0, -1.0, -1.0]), np.array([1.0, 1.0, 1.0])
+    s.gravity = np.zeros(3)
+    s.dt = 1e-5
+    g = ts.DEMSceneGPU(s, solve_begin=False)
+    g.record_contacts(True)
+    g.solve_begin()
+    g.write_vtu(tmp_path, 0, 0, 0.0, walls=False, wall_contacts=True)
+    a = _vtu_arrays(tmp_path / "ballTriangleInteraction_0000.vtu")
+    # both triangles are candidates (bounding boxes); the ball touches the
+    # interior of the first only
+    assert a["Points"].shape == (2, 3) and g.get("triPointing").size == 2
+    k = int(np.argmax(a["overlap"]))
+    np.testing.assert_allclose(a["Points"][k], [0.3, -0.2, 0.0], atol=1e-10)
+    np.testing.assert_allclose(a["contact normal"][k], [0.0, 0.0, 1.0], atol=1e-10)
+    assert a["overlap"][k] == pytest.approx(delta, rel=1e-9)
+    # FP32 force arithmetic: 1e-6 relative
+    np.testing.assert_allclose(a["contact force"][k], [0.0, 0.0, kn * delta], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(a["contact torque"], 0.0, atol=1e-12)
+    assert a["overlap"][1 - k] < 0 and np.all(a["contact force"][1 - k] == 0)
+    np.testing.assert_allclose(np.linalg.norm(a["contact normal"], axis=1), 1.0, atol=1e-12)
+    assert np.allclose(np.sort(a["overlap"]), np.sort(g.get("triOverlap")), rtol=0, atol=5e-11)
+    # the ball's total force is the contact's
+    np.testing.assert_allclose(g.get("force")[0], a["contact force"][k], rtol=1e-6, atol=1e-12)

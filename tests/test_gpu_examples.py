@@ -119,5 +119,6 @@ def test_ice_cylinder_program_equals_python_driven_scene(tmp_path):
     g = ts.DEMSceneGPU(scenes.ice_cylinder())
     g.step(istep)
     np.testing.assert_array_equal(pos, g.get("position"))
-    for kind in ("ball", "triangleWall"):
+    for kind in ("ball", "triangleWall", "ballTriangleInteraction"):
         check_frame0(tmp_path, f"iceCylinderInteraction/{kind}_0000.vtu")
+        check_layout(tmp_path / "iceCylinderInteraction" / f"{kind}_0001.vtu", kind)

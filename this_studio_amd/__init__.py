@@ -325,6 +325,7 @@ def _declare_dem(L):
         "demsph_dem_sizes": [_vp] + [C.POINTER(_sz)] * 6,
         "demsph_dem_get": [_vp, C.c_char_p, _vp, _sz, C.POINTER(_sz)],
         "demsph_dem_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
+        "demsph_dem_record_contacts": [_vp, C.c_int],
         "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
         "demsph_dem_set_graphs": [_vp, C.c_int],
         "demsph_dem_graph_captures": [_vp, C.POINTER(C.c_long)],
@@ -493,10 +494,16 @@ class DEMSceneGPU:
         check(self.L.demsph_dem_views(self._h, C.byref(v)))
         return v
 
+    def record_contacts(self, on: bool = True) -> None:
+        """Keep each ball-triangle contact of every evaluation (for the
+        ballTriangleInteraction frame file)."""
+        check(self.L.demsph_dem_record_contacts(self._h, 1 if on else 0))
+
     def write_vtu(self, directory, frame: int, step: int, time: float, balls: bool = True,
-                  walls: bool = True) -> None:
-        """The reference's ball_FFFF.vtu / triangleWall_FFFF.vtu frame files."""
-        what = (1 if balls else 0) | (2 if walls else 0)
+                  walls: bool = True, wall_contacts: bool = False) -> None:
+        """The reference's ball_FFFF.vtu / triangleWall_FFFF.vtu /
+        ballTriangleInteraction_FFFF.vtu frame files."""
+        what = (1 if balls else 0) | (2 if walls else 0) | (4 if wall_contacts else 0)
         check(self.L.demsph_dem_write_vtu(self._h, str(directory).encode(), what, int(frame), int(step),
                                           float(time)))
 

@@ -444,6 +444,10 @@ struct WallArgs {
     int *htri, *hcancel;
     double *hsx, *hsy, *hsz, *hrx, *hry, *hrz, *hox, *hoy, *hoz;
     double *hov;
+    // optional record of every candidate contact of the evaluation, 12
+    // doubles per history entry: point, normal, force, torque (the torque
+    // at the contact, without the lever term)
+    double* rec;
     void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // wall force/torque per slot (precision T)
     int* overflow;
 };
@@ -567,6 +571,11 @@ __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
                              F, Tq, Fs);
             Fw = Fw + F;
             Tw = Tw + Tq + cross(cvt<T>(rc - ri), Fs);
+            if (a.rec) {
+                double* r = a.rec + ((size_t)c * N + o) * 12;
+                r[6] = F.x; r[7] = F.y; r[8] = F.z;
+                r[9] = Tq.x; r[10] = Tq.y; r[11] = Tq.z;
+            }
         }
     }
     // new history
@@ -579,6 +588,14 @@ __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
         a.hrx[h] = er[c].x; a.hry[h] = er[c].y; a.hrz[h] = er[c].z;
         a.hox[h] = et[c].x; a.hoy[h] = et[c].y; a.hoz[h] = et[c].z;
         a.hov[h] = rad - len(ri - cp[c]);
+        if (a.rec) {
+            double* r = a.rec + h * 12;
+            const D3 n = normalize(ri - cp[c]);
+            r[0] = cp[c].x; r[1] = cp[c].y; r[2] = cp[c].z;
+            r[3] = n.x; r[4] = n.y; r[5] = n.z;
+            if (cancel[c] || im < 1.e-20)
+                for (int k = 6; k < 12; k++) r[k] = 0.0;
+        }
     }
     ((T*)a.ofx)[i] = Fw.x; ((T*)a.ofy)[i] = Fw.y; ((T*)a.ofz)[i] = Fw.z;
     ((T*)a.otx)[i] = Tw.x; ((T*)a.oty)[i] = Tw.y; ((T*)a.otz)[i] = Tw.z;
@@ -1043,6 +1060,8 @@ struct demsph_dem {
     bool walls_static = true;
     void* tsort_temp = nullptr;
     int KW = 16;
+    bool record_contacts = false;  // keep every wall contact of the last evaluation (frame files)
+    DBuf<double> crec;
     DBuf<int> hcnt, htri, hcancel;
     DBuf<double> hsx, hsy, hsz, hrx, hry, hrz, hox, hoy, hoz, hov;
     DBuf<double> wfx, wfy, wfz, wtx, wty, wtz;  // hold float or double
@@ -1379,6 +1398,7 @@ struct demsph_dem {
             htri.alloc(hn); hcancel.alloc(hn);
             for (auto* d : {&hsx, &hsy, &hsz, &hrx, &hry, &hrz, &hox, &hoy, &hoz, &hov}) d->alloc(hn);
             for (auto* d : {&wfx, &wfy, &wfz, &wtx, &wty, &wtz}) d->alloc(n);
+            if (record_contacts) crec.alloc(12 * hn);
         }
         overflow.alloc(1);
         DEMSPH_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), stream));
@@ -1542,6 +1562,7 @@ struct demsph_dem {
             wa.wpx = wpx.p; wa.wpy = wpy.p; wa.wpz = wpz.p; wa.wvx = wvx.p; wa.wvy = wvy.p; wa.wvz = wvz.p;
             wa.wwx = wwx.p; wa.wwy = wwy.p; wa.wwz = wwz.p; wa.wmat = wmat.p;
             wa.KW = KW; wa.nballs = n;
+            wa.rec = record_contacts ? crec.p : nullptr;
             wa.hcnt = hcnt.p; wa.htri = htri.p; wa.hcancel = hcancel.p;
             wa.hsx = hsx.p; wa.hsy = hsy.p; wa.hsz = hsz.p; wa.hrx = hrx.p; wa.hry = hry.p; wa.hrz = hrz.p;
             wa.hox = hox.p; wa.hoy = hoy.p; wa.hoz = hoz.p; wa.hov = hov.p;
@@ -2399,12 +2420,23 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
 // balls in insertion order with radius, material, clump id, velocity and
 // angular velocity; the walls' vertices at their current global positions
 // and their triangles with wall and material ids.
+extern "C" int demsph_dem_record_contacts(demsph_dem* d, int enable)
+{
+    DEM_GUARD({
+        if (d->stream) DEMSPH_HIP(hipStreamSynchronize(d->stream));
+        d->drop_graphs();  // captured evaluations hold the record pointer
+        d->record_contacts = enable != 0;
+        if (d->record_contacts && d->initialized && d->nt > 0) d->crec.alloc(12 * (size_t)d->KW * d->n);
+    });
+}
+
 extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time)
 {
     DEM_GUARD({
         d->require_init();
         if (!dir) throw InvalidArg("null directory");
-        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL)) throw InvalidArg("unknown frame kinds");
+        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE))
+            throw InvalidArg("unknown frame kinds");
         if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
         hipStream_t st = d->stream;
         if (what & DEMSPH_VTU_BALL) {
@@ -2451,6 +2483,54 @@ extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, lo
             w.ints("wallID", d->hw.twall);
             w.ints("materialID", tmat);
             w.close_data("CellData");
+            w.end();
+        }
+        if ((what & DEMSPH_VTU_BALL_TRIANGLE) && d->nt > 0) {
+            // outputBallTriangleInteractionVTU (DEMSolver.h:1286-1370): every
+            // candidate contact of the last evaluation, by ball in insertion
+            // order and then in the candidate walk's order
+            if (!d->record_contacts)
+                throw InvalidArg("ball-triangle frames need demsph_dem_record_contacts before the evaluation");
+            const size_t n = d->n, hn = (size_t)d->KW * n;
+            std::vector<int> cnt;
+            std::vector<double> rec, sx, sy, sz, rx, ry, rz, ox, oy, oz, ov;
+            if (d->nt > 0) {
+                cnt = fetch(d->hcnt.p, n, st);
+                rec = fetch(d->crec.p, 12 * hn, st);
+                sx = fetch(d->hsx.p, hn, st); sy = fetch(d->hsy.p, hn, st); sz = fetch(d->hsz.p, hn, st);
+                rx = fetch(d->hrx.p, hn, st); ry = fetch(d->hry.p, hn, st); rz = fetch(d->hrz.p, hn, st);
+                ox = fetch(d->hox.p, hn, st); oy = fetch(d->hoy.p, hn, st); oz = fetch(d->hoz.p, hn, st);
+                ov = fetch(d->hov.p, hn, st);
+            }
+            std::vector<double> pt, nr, fc, tc, ss, sr, stq, overlap;
+            for (size_t o = 0; o < cnt.size(); o++) {
+                for (int c = 0; c < cnt[o] && c < d->KW; c++) {
+                    const size_t h = (size_t)c * n + o;
+                    const double* r = &rec[12 * h];
+                    pt.insert(pt.end(), r, r + 3);
+                    nr.insert(nr.end(), r + 3, r + 6);
+                    fc.insert(fc.end(), r + 6, r + 9);
+                    tc.insert(tc.end(), r + 9, r + 12);
+                    overlap.push_back(ov[h]);
+                    ss.insert(ss.end(), {sx[h], sy[h], sz[h]});
+                    sr.insert(sr.end(), {rx[h], ry[h], rz[h]});
+                    stq.insert(stq.end(), {ox[h], oy[h], oz[h]});
+                }
+            }
+            const size_t m = overlap.size();
+            vtu::Writer w(dir, "ballTriangleInteraction", frame);
+            w.begin(time, step, m, m);
+            w.points(pt);
+            w.vertex_cells(m);
+            w.open_data("PointData", "overlap");
+            w.reals("overlap", overlap);
+            w.vectors("contact normal", nr);
+            w.vectors("contact force", fc);
+            w.vectors("contact torque", tc);
+            w.vectors("sliding spring", ss);
+            w.vectors("rolling spring", sr);
+            w.vectors("torsion spring", stq);
+            w.close_data("PointData");
             w.end();
         }
     });
