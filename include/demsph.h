@@ -333,10 +333,15 @@ int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity,
  * torque, springs), by ball in insertion order, then in candidate order.
  * Needs demsph_dem_record_contacts on before that evaluation. */
 #define DEMSPH_VTU_BALL_TRIANGLE 4
+/* outputBallInteractionVTU (DEMSolver.h:1100-1185,
+ * dir/ballInteraction_FFFF.vtu): every ball pair of the current list,
+ * same fields, by pointed ball in insertion order, then in list order.
+ * Needs demsph_dem_record_contacts on before the evaluation. */
+#define DEMSPH_VTU_BALL_INTERACTION 8
 int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time);
-/* Keep each ball-triangle contact's point, normal, force and torque from
- * every force evaluation for the frame writer (off by default: 96 bytes per
- * candidate slot and a store per contact). */
+/* Keep each ball pair's and ball-triangle contact's point, normal, force
+ * and torque from every force evaluation for the frame writer (off by
+ * default: 96 bytes per pair or candidate slot and a store per entry). */
 int demsph_dem_record_contacts(demsph_dem* d, int enable);
 /* Overwrites "position" "velocity" "angularVelocity" (double 3n, insertion
  * order) or "wallVelocity" "wallAngularVelocity" (double 3/wall; the

@@ -345,3 +345,38 @@ def test_ball_triangle_frame_reports_contact_closed_form(tmp_path):
     assert np.allclose(np.sort(a["overlap"]), np.sort(g.get("triOverlap")), rtol=0, atol=5e-11)
     # the ball's total force is the contact's
     np.testing.assert_allclose(g.get("force")[0], a["contact force"][k], rtol=1e-6, atol=1e-12)
+
+
+def test_ball_pair_frame_reports_contact_closed_form(tmp_path):
+    """Two equal balls overlapping by delta on a linear spring, at rest, no
+    gravity.  The ballInteraction frame after the first evaluation holds
+    the pair: contact point at the midpoint, normal along the centre line,
+    overlap delta, force kn*delta along the normal on the pointed ball, zero
+    torque.  Its entries follow the pair getters' order."""
+    r, delta, kn = 0.01, 2.0e-4, 5.0e3
+    s = scenes.DEMScene("pair")
+    s.add_balls([[0.1, 0.2, 0.3], [0.1 + 2 * r - delta, 0.2, 0.3]], [0, 0, 0], [0, 0, 0], r, 2000.0, 0)
+    s.set_linear_pair(0, 0, kn, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    s.min_bound, s.max_bound = np.array([-1.0, -1.0, -1.0]), np.array([1.0, 1.0, 1.0])
+    s.gravity = np.zeros(3)
+    s.dt = 1e-5
+    g = ts.DEMSceneGPU(s, solve_begin=False)
+    g.record_contacts(True)
+    g.solve_begin()
+    g.write_vtu(tmp_path, 0, 0, 0.0, walls=False, pairs=True)
+    a = _vtu_arrays(tmp_path / "ballInteraction_0000.vtu")
+    assert a["Points"].shape == (1, 3)
+    assert g.get("objectPointed").size == 1
+    np.testing.assert_allclose(a["Points"][0], [0.1 + r - 0.5 * delta, 0.2, 0.3], atol=1e-12)
+    nrm = a["contact normal"][0]
+    np.testing.assert_allclose(np.abs(nrm), [1.0, 0.0, 0.0], atol=1e-12)
+    assert a["overlap"][0] == pytest.approx(delta, rel=1e-9)
+    f = a["contact force"][0]
+    assert float(f @ nrm) == pytest.approx(kn * delta, rel=1e-6)
+    np.testing.assert_allclose(np.cross(f, nrm), 0.0, atol=1e-9)
+    np.testing.assert_allclose(a["contact torque"], 0.0, atol=1e-12)
+    # frame values carry 10 decimals
+    np.testing.assert_allclose(a["overlap"], g.get("overlap"), rtol=0, atol=5e-11)
+    # the pointed ball's force is the contact force
+    pointed = int(g.get("objectPointed")[0])
+    np.testing.assert_allclose(g.get("force")[pointed], f, rtol=1e-6, atol=1e-12)

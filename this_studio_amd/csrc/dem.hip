@@ -209,6 +209,9 @@ struct PairForceArgs {
     Pairs p;
     Tables tb;
     double dt;
+    // optional record of every pair of the evaluation, 12 doubles per pair
+    // slot: contact point, normal, force, torque (without the lever term)
+    double* rec;
     void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner-side sums per slot (precision T)
 };
 
@@ -257,6 +260,13 @@ __device__ __forceinline__ void pair_forces_ball(const PairForceArgs& a, int i)
         const V3<T> Tj = Tq + cross(lj, Fs);
         ((T*)a.p.Fx)[p] = F.x; ((T*)a.p.Fy)[p] = F.y; ((T*)a.p.Fz)[p] = F.z;
         ((T*)a.p.Rx)[p] = Tj.x; ((T*)a.p.Ry)[p] = Tj.y; ((T*)a.p.Rz)[p] = Tj.z;
+        if (a.rec) {
+            double* r = a.rec + (size_t)p * 12;
+            r[0] = rc.x; r[1] = rc.y; r[2] = rc.z;
+            r[3] = n.x; r[4] = n.y; r[5] = n.z;
+            r[6] = F.x; r[7] = F.y; r[8] = F.z;
+            r[9] = Tq.x; r[10] = Tq.y; r[11] = Tq.z;
+        }
         Fo = Fo + F;
         To = To + Ti;
     }
@@ -1060,8 +1070,8 @@ struct demsph_dem {
     bool walls_static = true;
     void* tsort_temp = nullptr;
     int KW = 16;
-    bool record_contacts = false;  // keep every wall contact of the last evaluation (frame files)
-    DBuf<double> crec;
+    bool record_contacts = false;  // keep every contact of the last evaluation (frame files)
+    DBuf<double> crec, prec;       // wall contacts per history slot, ball pairs per pair slot
     DBuf<int> hcnt, htri, hcancel;
     DBuf<double> hsx, hsy, hsz, hrx, hry, hrz, hox, hoy, hoz, hov;
     DBuf<double> wfx, wfy, wfz, wtx, wty, wtz;  // hold float or double
@@ -1086,15 +1096,18 @@ struct demsph_dem {
     // Captured graphs of the four steps between two pair rebuilds.  The
     // rebuild flips the ball and pair buffers (cb, cp) and may reallocate the
     // pair arrays, so a graph is keyed by what its launches bake in.
+    // Every buffer a capture bakes in that can be reallocated between
+    // captures is part of the key (the contact records are reallocated with
+    // either pair list).
     struct GraphKey {
         double dt, gx, gy, gz;
         int cb, cp, nbond, fp64;
-        const void *pairs, *rev, *cand, *stamp;
+        const void *pairs, *rev, *cand, *stamp, *prec, *crec;
         bool operator==(const GraphKey& o) const
         {
             return dt == o.dt && gx == o.gx && gy == o.gy && gz == o.gz && cb == o.cb && cp == o.cp &&
                    nbond == o.nbond && fp64 == o.fp64 && pairs == o.pairs && rev == o.rev && cand == o.cand &&
-                   stamp == o.stamp;
+                   stamp == o.stamp && prec == o.prec && crec == o.crec;
         }
     };
     struct GraphEntry {
@@ -1164,6 +1177,7 @@ struct demsph_dem {
         pFx[which].alloc(c); pFy[which].alloc(c); pFz[which].alloc(c);
         pRx[which].alloc(c); pRy[which].alloc(c); pRz[which].alloc(c);
         pcap[which] = c;
+        if (record_contacts) prec.alloc(12 * (size_t)std::max(pcap[0], pcap[1]));
     }
 
     // contactModelParameters::buildFromTables (contactKernel.cu:6-104)
@@ -1493,6 +1507,7 @@ struct demsph_dem {
     {
         PairForceArgs pa{};
         pa.b = cur(); pa.p = pairs(cp); pa.tb = tables(); pa.dt = springDt;
+        pa.rec = record_contacts ? prec.p : nullptr;
         pa.ofx = ofx.p; pa.ofy = ofy.p; pa.ofz = ofz.p; pa.otx = otx.p; pa.oty = oty.p; pa.otz = otz.p;
         return pa;
     }
@@ -1808,6 +1823,8 @@ struct demsph_dem {
         key.dt = dt; key.gx = g[0]; key.gy = g[1]; key.gz = g[2];
         key.cb = cb; key.cp = cp; key.nbond = nbond; key.fp64 = fp64 ? 1 : 0;
         key.pairs = ppartner[cp].p; key.rev = rpair.p; key.cand = tcand_tri.p; key.stamp = stamps.buf;
+        key.prec = record_contacts ? prec.p : nullptr;
+        key.crec = record_contacts ? crec.p : nullptr;
         for (auto& e : graphs)
             if (e.key == key) return e.exec;
         hipGraph_t gr = nullptr;
@@ -2427,6 +2444,8 @@ extern "C" int demsph_dem_record_contacts(demsph_dem* d, int enable)
         d->drop_graphs();  // captured evaluations hold the record pointer
         d->record_contacts = enable != 0;
         if (d->record_contacts && d->initialized && d->nt > 0) d->crec.alloc(12 * (size_t)d->KW * d->n);
+        if (d->record_contacts && d->initialized)
+            d->prec.alloc(12 * (size_t)std::max(1, std::max(d->pcap[0], d->pcap[1])));
     });
 }
 
@@ -2435,10 +2454,68 @@ extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, lo
     DEM_GUARD({
         d->require_init();
         if (!dir) throw InvalidArg("null directory");
-        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE))
-            throw InvalidArg("unknown frame kinds");
         if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
         hipStream_t st = d->stream;
+        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE | DEMSPH_VTU_BALL_INTERACTION))
+            throw InvalidArg("unknown frame kinds");
+        // one interaction frame: point, overlap, normal, force, torque and
+        // the three springs per entry (DEMSolver.h:1100-1185, 1286-1370)
+        const auto interactions = [&](const char* stem, const std::vector<double>& pt, const std::vector<double>& ov,
+                                      const std::vector<double>& nr, const std::vector<double>& fc,
+                                      const std::vector<double>& tc, const std::vector<double>& ss,
+                                      const std::vector<double>& sr, const std::vector<double>& stq) {
+            const size_t m = ov.size();
+            vtu::Writer w(dir, stem, frame);
+            w.begin(time, step, m, m);
+            w.points(pt);
+            w.vertex_cells(m);
+            w.open_data("PointData", "overlap");
+            w.reals("overlap", ov);
+            w.vectors("contact normal", nr);
+            w.vectors("contact force", fc);
+            w.vectors("contact torque", tc);
+            w.vectors("sliding spring", ss);
+            w.vectors("rolling spring", sr);
+            w.vectors("torsion spring", stq);
+            w.close_data("PointData");
+            w.end();
+        };
+        if (what & DEMSPH_VTU_BALL_INTERACTION) {
+            // outputBallInteractionVTU: every pair of the current list, by
+            // pointed ball in insertion order, then in the list's order
+            if (!d->record_contacts)
+                throw InvalidArg("ball interaction frames need demsph_dem_record_contacts before the evaluation");
+            const size_t n = d->n;
+            std::vector<double> pt, ov, nr, fc, tc, ss, sr, stq;
+            if (d->has_list && d->npairs > 0) {
+                const int c = d->cp;
+                const size_t np = (size_t)d->npairs;
+                const auto orig = fetch(d->borig[d->cb].p, n, st);
+                const auto start = fetch(d->pstart[c].p, n + 1, st);
+                const auto rec = fetch(d->prec.p, 12 * np, st);
+                const auto o_v = fetch(d->pov[c].p, np, st);
+                const auto sx = fetch(d->psx[c].p, np, st), sy = fetch(d->psy[c].p, np, st), sz = fetch(d->psz[c].p, np, st);
+                const auto rx = fetch(d->prx[c].p, np, st), ry = fetch(d->pry[c].p, np, st), rz = fetch(d->prz[c].p, np, st);
+                const auto ox = fetch(d->pox[c].p, np, st), oy = fetch(d->poy[c].p, np, st), oz = fetch(d->poz[c].p, np, st);
+                std::vector<int> rankh(n);
+                for (size_t i = 0; i < n; i++) rankh[orig[i]] = (int)i;
+                for (size_t o = 0; o < n; o++) {
+                    const int i = rankh[o];
+                    for (int p = start[i]; p < start[i + 1]; p++) {
+                        const double* r = &rec[12 * (size_t)p];
+                        pt.insert(pt.end(), r, r + 3);
+                        nr.insert(nr.end(), r + 3, r + 6);
+                        fc.insert(fc.end(), r + 6, r + 9);
+                        tc.insert(tc.end(), r + 9, r + 12);
+                        ov.push_back(o_v[p]);
+                        ss.insert(ss.end(), {sx[p], sy[p], sz[p]});
+                        sr.insert(sr.end(), {rx[p], ry[p], rz[p]});
+                        stq.insert(stq.end(), {ox[p], oy[p], oz[p]});
+                    }
+                }
+            }
+            interactions("ballInteraction", pt, ov, nr, fc, tc, ss, sr, stq);
+        }
         if (what & DEMSPH_VTU_BALL) {
             const int cb = d->cb;
             const size_t n = d->n;
@@ -2517,21 +2594,7 @@ extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, lo
                     stq.insert(stq.end(), {ox[h], oy[h], oz[h]});
                 }
             }
-            const size_t m = overlap.size();
-            vtu::Writer w(dir, "ballTriangleInteraction", frame);
-            w.begin(time, step, m, m);
-            w.points(pt);
-            w.vertex_cells(m);
-            w.open_data("PointData", "overlap");
-            w.reals("overlap", overlap);
-            w.vectors("contact normal", nr);
-            w.vectors("contact force", fc);
-            w.vectors("contact torque", tc);
-            w.vectors("sliding spring", ss);
-            w.vectors("rolling spring", sr);
-            w.vectors("torsion spring", stq);
-            w.close_data("PointData");
-            w.end();
+            interactions("ballTriangleInteraction", pt, overlap, nr, fc, tc, ss, sr, stq);
         }
     });
 }
