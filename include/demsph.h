@@ -338,6 +338,10 @@ int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity,
  * same fields, by pointed ball in insertion order, then in list order.
  * Needs demsph_dem_record_contacts on before the evaluation. */
 #define DEMSPH_VTU_BALL_INTERACTION 8
+/* outputBondedInteractionVTU (DEMSolver.h:1186-1285,
+ * dir/bondedInteraction_FFFF.vtu): every bond in bond-list order with its
+ * point, state, forces, torques and maximum stresses. */
+#define DEMSPH_VTU_BONDED_INTERACTION 16
 int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time);
 /* Keep each ball pair's and ball-triangle contact's point, normal, force
  * and torque from every force evaluation for the frame writer (off by

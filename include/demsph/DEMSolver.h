@@ -10,11 +10,9 @@
 //  * getBall()'s device pointers (force(), position(), ...) are insertion-
 //    ordered FP64 views that are valid inside addExternalForceTorque(); the
 //    HostCopy() accessors work at any time.
-//  * solve() writes the ball, triangle-wall, ball-pair and ball-triangle
-//    frame files into dir as the reference does (ball_FFFF.vtu,
-//    triangleWall_FFFF.vtu, ballInteraction_FFFF.vtu,
-//    ballTriangleInteraction_FFFF.vtu, same layout); the bond file
-//    (bondedInteraction) is not written.  An empty dir writes nothing.
+//  * solve() writes the reference's frame files into dir, in its layout:
+//    ball, triangleWall (with walls), ballInteraction, bondedInteraction and
+//    ballTriangleInteraction (with walls).  An empty dir writes nothing.
 //    A subclass may override onFrame() to record results.
 //  * Errors throw std::runtime_error (the reference's debug-build behaviour).
 #pragma once
@@ -275,13 +273,14 @@ public:
     demsph_dem* handle() { return h_; }
 
     // outputBallVTU / outputMeshWallVTU / outputBallInteractionVTU /
-    // outputBallTriangleInteractionVTU (DEMSolver.h:908-1185, 1286-1370).
+    // outputBondedInteractionVTU / outputBallTriangleInteractionVTU
+    // (DEMSolver.h:908-1370) for one frame.
     void writeFrame(const std::string& dir, const size_t iFrame, const size_t iStep, const double time)
     {
         if (dir.empty()) return;
         demsph::check(demsph_dem_write_vtu(h_, dir.c_str(),
                                            DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE |
-                                               DEMSPH_VTU_BALL_INTERACTION,
+                                               DEMSPH_VTU_BALL_INTERACTION | DEMSPH_VTU_BONDED_INTERACTION,
                                            (long)iFrame, (long)iStep, time));
     }
 

@@ -105,6 +105,25 @@ def test_bonded_beam_program_equals_python_driven_scene(tmp_path):
     g = ts.DEMSceneGPU(scenes.bonded_beam())
     g.step(istep)
     np.testing.assert_array_equal(pos, g.get("position"))
+    # the bond frame (outputBondedInteractionVTU's arrays, in its order) holds
+    # the same bonds as the Python-driven scene, to the 10 printed decimals
+    path = tmp_path / "bondedParticleBeam" / f"bondedInteraction_{iframe:04d}.vtu"
+    arrays = {}
+    names = []
+    for attrs, body in re.findall(r'<DataArray([^>]*)>(.*?)</DataArray>', path.read_text(), re.S):
+        m = re.search(r'Name="([^"]+)"', attrs)
+        names.append(m.group(1) if m else "")
+        v = np.array(body.split(), dtype=np.float64)
+        arrays[names[-1] or "Points"] = v.reshape(-1, 3) if 'NumberOfComponents="3"' in attrs else v
+    assert names == ["TIME", "STEP", "", "connectivity", "offsets", "types", "is bonded", "normal force",
+                     "torsion torque", "maximum normal stress", "maximum shear stress", "contact normal",
+                     "shearing force", "bending torque"]
+    for name, field in (("Points", "bondPoint"), ("is bonded", "bondIsBonded"), ("normal force", "bondNormalForce"),
+                        ("torsion torque", "bondTorsionTorque"), ("contact normal", "bondNormal"),
+                        ("shearing force", "bondShearForce"), ("bending torque", "bondBendingTorque")):
+        want = np.asarray(g.get(field), dtype=np.float64).reshape(arrays[name].shape)
+        np.testing.assert_allclose(arrays[name], want, rtol=1e-12, atol=5.1e-11, err_msg=name)
+    assert (tmp_path / "bondedParticleBeam" / "ballInteraction_0000.vtu").exists()
 
 
 def test_ice_cylinder_program_equals_python_driven_scene(tmp_path):

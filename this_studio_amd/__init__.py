@@ -500,12 +500,14 @@ class DEMSceneGPU:
         check(self.L.demsph_dem_record_contacts(self._h, 1 if on else 0))
 
     def write_vtu(self, directory, frame: int, step: int, time: float, balls: bool = True,
-                  walls: bool = True, wall_contacts: bool = False, pairs: bool = False) -> None:
+                  walls: bool = True, wall_contacts: bool = False, pairs: bool = False,
+                  bonds: bool = False) -> None:
         """The reference's ball_FFFF.vtu / triangleWall_FFFF.vtu /
         ballTriangleInteraction_FFFF.vtu / ballInteraction_FFFF.vtu frame
         files (the contact files need record_contacts before the
         evaluation they report)."""
-        what = (1 if balls else 0) | (2 if walls else 0) | (4 if wall_contacts else 0) | (8 if pairs else 0)
+        what = ((1 if balls else 0) | (2 if walls else 0) | (4 if wall_contacts else 0) | (8 if pairs else 0)
+                | (16 if bonds else 0))
         check(self.L.demsph_dem_write_vtu(self._h, str(directory).encode(), what, int(frame), int(step),
                                           float(time)))
 

@@ -2456,8 +2456,35 @@ extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, lo
         if (!dir) throw InvalidArg("null directory");
         if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
         hipStream_t st = d->stream;
-        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE | DEMSPH_VTU_BALL_INTERACTION))
+        if (what & ~(DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE | DEMSPH_VTU_BALL_INTERACTION |
+                     DEMSPH_VTU_BONDED_INTERACTION))
             throw InvalidArg("unknown frame kinds");
+        if (what & DEMSPH_VTU_BONDED_INTERACTION) {
+            // outputBondedInteractionVTU (DEMSolver.h:1186-1285): every bond in
+            // the bond list's order with its point, state and loads
+            const size_t nb = d->nbond;
+            const auto v3 = [&](const DBuf<double>& x, const DBuf<double>& y, const DBuf<double>& z) {
+                const auto hx = fetch(x.p, nb, st), hy = fetch(y.p, nb, st), hz = fetch(z.p, nb, st);
+                std::vector<double> v(3 * nb);
+                for (size_t i = 0; i < nb; i++) { v[3 * i] = hx[i]; v[3 * i + 1] = hy[i]; v[3 * i + 2] = hz[i]; }
+                return v;
+            };
+            vtu::Writer w(dir, "bondedInteraction", frame);
+            w.begin(time, step, nb, nb);
+            w.points(v3(d->dbpx, d->dbpy, d->dbpz));
+            w.vertex_cells(nb);
+            w.open_data("PointData", "overlap");  // the reference's name for the active scalars
+            w.ints("is bonded", fetch(d->dbonded.p, nb, st));
+            w.reals("normal force", fetch(d->dbFn.p, nb, st));
+            w.reals("torsion torque", fetch(d->dbTt.p, nb, st));
+            w.reals("maximum normal stress", fetch(d->dbsig.p, nb, st));
+            w.reals("maximum shear stress", fetch(d->dbtau.p, nb, st));
+            w.vectors("contact normal", v3(d->dbnx, d->dbny, d->dbnz));
+            w.vectors("shearing force", v3(d->dbsx, d->dbsy, d->dbsz));
+            w.vectors("bending torque", v3(d->dbtx, d->dbty, d->dbtz));
+            w.close_data("PointData");
+            w.end();
+        }
         // one interaction frame: point, overlap, normal, force, torque and
         // the three springs per entry (DEMSolver.h:1100-1185, 1286-1370)
         const auto interactions = [&](const char* stem, const std::vector<double>& pt, const std::vector<double>& ov,
