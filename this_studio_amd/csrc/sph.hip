@@ -967,6 +967,11 @@ struct RevArgs {
     int* rfluid;
     int rcap;  // entries rfluid holds; a list overflow (reported by the fluid side) may exceed it
     double hf; // the fluid's smoothing length when it has one phase
+    // FP32 prefilter of the fluid positions (null: every test in FP64), as
+    // in the list walks: decisions outside the band are exact
+    const float4* Pf;
+    float band;
+    double R;
 };
 
 // Reverse index of the fluid -> ball lists: for ball j, the fluid slots i
@@ -986,6 +991,9 @@ __global__ __launch_bounds__(256) void k_fs_rev_walk(RevArgs a)
     const double bx = a.SP[j].x, by = a.SP[j].y, bz = a.SP[j].z;
     const Stencil st = make_stencil(g, bx, by, bz);
     int off = COUNT ? 0 : a.rstart[j];
+    const float4 bf = pos_f32(g, a.R, bx, by, bz);
+    const float c2 = (float)(4.0 * a.hf * a.hf);
+    const float band2 = bf.w != 0.f ? INFINITY : a.band * c2;
     for (int k = 0; k < 9; k++) {
         const int dz = k / 3 - 1, dy = k % 3 - 1;
         if (dz < st.loz || dz > st.hiz || dy < st.loy || dy > st.hiy) continue;
@@ -1002,7 +1010,18 @@ __global__ __launch_bounds__(256) void k_fs_rev_walk(RevArgs a)
         for (int base = s; base < e; base += 64) {
             const int i = base + lane;
             bool hit = false;
-            if (i < e) {
+            if (UNIF && a.Pf) {
+                if (i < e) {
+                    const float4 q = a.Pf[i];
+                    const float fx = q.x - bf.x, fy = q.y - bf.y, fz = q.z - bf.z;
+                    const float d2 = fx * fx + fy * fy + fz * fz;
+                    hit = d2 < c2;
+                    if ((q.w != 0.f) | (fabsf(d2 - c2) <= band2)) {
+                        const FluidPos p = a.P[i];
+                        hit = within_cut(2.0 * a.hf, p.x - bx, p.y - by, p.z - bz);
+                    }
+                }
+            } else if (i < e) {
                 const FluidPos p = a.P[i];
                 const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
                 hit = within_cut(2.0 * h, p.x - bx, p.y - by, p.z - bz);
@@ -2777,6 +2796,10 @@ struct demsph_couple {
         ra.ns = ns; ra.grid = f->grid; ra.cells = f->cells; ra.P = f->P[f->cP]; ra.V = f->V[f->cV];
         ra.phases = f->d_phases; ra.SP = SP; ra.rcnt = rcnt; ra.rstart = rstart; ra.rfluid = rfluid; ra.rcap = rcap();
         ra.hf = f->phf.h;
+        // the fluid's FP32 copy is current when the search wrote it
+        ra.Pf = f->prefilter ? f->Pf : nullptr;
+        ra.band = demsph_sph::kPrefilterBand;
+        ra.R = f->pref_R;
         const dim3 wb(blocks_for(ns, kBallsPerBlock)), wt(64 * kBallsPerBlock);
         // one fluid phase: every fluid particle has the same support
         const bool unif = f->fluid_uniform();
