@@ -17,6 +17,11 @@ Prints one JSON line:
   cpu_baseline = the FP64 oracle on the host cores over a bounded sample
   dem_collapse = the 4,096-ball DEM granular collapse, ball-steps per second,
                with its own oracle baseline
+  serving_mix  = the batched-serving job stream, plain vs batched (and
+               batched with autotuning on admission): throughput, p50/p99
+  autotune     = the autotuner's trials on the dam break against the rules
+  cantilever_clumps, ice_cylinder_coupled = the bonded-beam-with-clumps and
+               coupled ice-cylinder workloads
 """
 from __future__ import annotations
 
