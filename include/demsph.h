@@ -130,6 +130,15 @@ int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_t capacity,
 #define DEMSPH_VTU_SPH 1
 #define DEMSPH_VTU_DUMMY 2
 int demsph_sph_write_vtu(demsph_sph* s, const char* dir, int what, long frame, long step, double time);
+/* The same without waiting for the fluid frame: the scene's arrays are
+ * copied in stream order to pinned host memory and a background thread
+ * writes the file while the scene keeps stepping (no counterpart in the
+ * reference, whose writers block).  Dummy frames are written before the
+ * call returns.  At most two frames wait for the thread; a third call
+ * waits for a slot.  demsph_frames_wait() waits for every pending frame
+ * and reports the first failure since its last call. */
+int demsph_sph_write_vtu_async(demsph_sph* s, const char* dir, int what, long frame, long step, double time);
+int demsph_frames_wait(void);
 
 /* Overwrites position / velocity (double, 3n) or densityChange (double, n)
  * by insertion index (density and pressure follow from the EOS). */
@@ -343,6 +352,10 @@ int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_t capacity,
  * point, state, forces, torques and maximum stresses. */
 #define DEMSPH_VTU_BONDED_INTERACTION 16
 int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time);
+/* The same with the ball frame written by the background frame thread (as
+ * demsph_sph_write_vtu_async); the other kinds are written before the call
+ * returns. */
+int demsph_dem_write_vtu_async(demsph_dem* d, const char* dir, int what, long frame, long step, double time);
 /* Keep each ball pair's and ball-triangle contact's point, normal, force
  * and torque from every force evaluation for the frame writer (off by
  * default: 96 bytes per pair or candidate slot and a store per entry). */

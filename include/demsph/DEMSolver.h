@@ -268,6 +268,7 @@ public:
         }
         demsph::check(demsph_dem_set_hook(h_, nullptr, nullptr));
         demsph::check(demsph_dem_synchronize(h_));
+        demsph::check(demsph_frames_wait());  // ball frames are written in the background
     }
 
     demsph_dem* handle() { return h_; }
@@ -278,7 +279,7 @@ public:
     void writeFrame(const std::string& dir, const size_t iFrame, const size_t iStep, const double time)
     {
         if (dir.empty()) return;
-        demsph::check(demsph_dem_write_vtu(h_, dir.c_str(),
+        demsph::check(demsph_dem_write_vtu_async(h_, dir.c_str(),
                                            DEMSPH_VTU_BALL | DEMSPH_VTU_TRIANGLE_WALL | DEMSPH_VTU_BALL_TRIANGLE |
                                                DEMSPH_VTU_BALL_INTERACTION | DEMSPH_VTU_BONDED_INTERACTION,
                                            (long)iFrame, (long)iStep, time));

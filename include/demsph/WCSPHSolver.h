@@ -167,6 +167,7 @@ public:
             }
         }
         demsph::check(demsph_sph_synchronize(h_));
+        demsph::check(demsph_frames_wait());  // fluid frames are written in the background
     }
 
     demsph_sph* handle() { return h_; }
@@ -175,7 +176,7 @@ public:
     void writeFrame(const std::string& dir, const size_t iFrame, const size_t iStep, const double time)
     {
         if (dir.empty()) return;
-        demsph::check(demsph_sph_write_vtu(h_, dir.c_str(), DEMSPH_VTU_SPH | DEMSPH_VTU_DUMMY, (long)iFrame, (long)iStep,
+        demsph::check(demsph_sph_write_vtu_async(h_, dir.c_str(), DEMSPH_VTU_SPH | DEMSPH_VTU_DUMMY, (long)iFrame, (long)iStep,
                                            time));
     }
 

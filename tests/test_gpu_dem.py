@@ -380,3 +380,15 @@ def test_ball_pair_frame_reports_contact_closed_form(tmp_path):
     # the pointed ball's force is the contact force
     pointed = int(g.get("objectPointed")[0])
     np.testing.assert_allclose(g.get("force")[pointed], f, rtol=1e-6, atol=1e-12)
+
+
+def test_background_ball_frame_is_a_snapshot(tmp_path):
+    """The ball frame written in the background equals the blocking write of
+    the same state while the scene keeps stepping."""
+    g = ts.DEMSceneGPU(scenes.granular_collapse())
+    g.step(20)
+    g.write_vtu(tmp_path / "now", 1, 20, 0.0002, walls=False)
+    g.write_vtu(tmp_path / "bg", 1, 20, 0.0002, walls=False, background=True)
+    g.step(50)
+    ts.frames_wait()
+    assert (tmp_path / "now" / "ball_0001.vtu").read_bytes() == (tmp_path / "bg" / "ball_0001.vtu").read_bytes()

@@ -345,3 +345,20 @@ def test_full_size_dam_break_cells_and_counts():
                              oracle.d(spec.fluid_pos), oracle.d(h), oracle.i(si), oracle.i(cs), oracle.i(ce), 1,
                              oracle.i(ocnt))
     assert np.array_equal(cnt, ocnt)
+
+
+def test_background_frame_is_a_snapshot(small, tmp_path):
+    """A fluid frame written in the background equals, byte for byte, the
+    blocking write of the same state, although the scene steps on before
+    the frame thread writes it."""
+    s = make_gpu(small)
+    s.step(small.dt, 10)
+    s.write_vtu(tmp_path / "now", 1, 10, 0.5, dummies=False)
+    s.write_vtu(tmp_path / "bg", 1, 10, 0.5, dummies=False, background=True)
+    s.step(small.dt, 30)
+    ts.frames_wait()
+    a = (tmp_path / "now" / "SPH_0001.vtu").read_bytes()
+    b = (tmp_path / "bg" / "SPH_0001.vtu").read_bytes()
+    assert a == b
+    s.write_vtu(tmp_path / "later", 2, 40, 1.0, dummies=False)
+    assert (tmp_path / "later" / "SPH_0002.vtu").read_bytes() != a

@@ -85,11 +85,19 @@ def _declare(L: C.CDLL) -> None:
         "demsph_sph_autotune": [_vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(TuneReport)],
         "demsph_sph_autotune_clear_cache": [],
         "demsph_sph_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
+        "demsph_sph_write_vtu_async": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
+        "demsph_frames_wait": [],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+
+
+def frames_wait() -> None:
+    """Waits for every frame written in the background; raises on the first
+    failure since the last call."""
+    check(lib().demsph_frames_wait())
 
 
 def check(status: int) -> None:
@@ -184,11 +192,13 @@ class SPHScene:
         return a.value, b.value, c.value
 
     def write_vtu(self, directory, frame: int, step: int, time: float, fluid: bool = True,
-                  dummies: bool = True) -> None:
-        """The reference's SPH_FFFF.vtu / dummy_FFFF.vtu frame files."""
+                  dummies: bool = True, background: bool = False) -> None:
+        """The reference's SPH_FFFF.vtu / dummy_FFFF.vtu frame files.  With
+        background=True the fluid frame is snapshotted in stream order and
+        written by the frame thread (frames_wait() waits for it)."""
         what = (1 if fluid else 0) | (2 if dummies else 0)
-        check(lib().demsph_sph_write_vtu(self._h, str(directory).encode(), what, int(frame), int(step),
-                                         float(time)))
+        fn = lib().demsph_sph_write_vtu_async if background else lib().demsph_sph_write_vtu
+        check(fn(self._h, str(directory).encode(), what, int(frame), int(step), float(time)))
 
     def get(self, field: str) -> np.ndarray:
         cnt = _sz()
@@ -325,6 +335,7 @@ def _declare_dem(L):
         "demsph_dem_sizes": [_vp] + [C.POINTER(_sz)] * 6,
         "demsph_dem_get": [_vp, C.c_char_p, _vp, _sz, C.POINTER(_sz)],
         "demsph_dem_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
+        "demsph_dem_write_vtu_async": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
         "demsph_dem_record_contacts": [_vp, C.c_int],
         "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
         "demsph_dem_set_graphs": [_vp, C.c_int],
@@ -501,15 +512,15 @@ class DEMSceneGPU:
 
     def write_vtu(self, directory, frame: int, step: int, time: float, balls: bool = True,
                   walls: bool = True, wall_contacts: bool = False, pairs: bool = False,
-                  bonds: bool = False) -> None:
+                  bonds: bool = False, background: bool = False) -> None:
         """The reference's ball_FFFF.vtu / triangleWall_FFFF.vtu /
         ballTriangleInteraction_FFFF.vtu / ballInteraction_FFFF.vtu frame
         files (the contact files need record_contacts before the
         evaluation they report)."""
         what = ((1 if balls else 0) | (2 if walls else 0) | (4 if wall_contacts else 0) | (8 if pairs else 0)
                 | (16 if bonds else 0))
-        check(self.L.demsph_dem_write_vtu(self._h, str(directory).encode(), what, int(frame), int(step),
-                                          float(time)))
+        fn = self.L.demsph_dem_write_vtu_async if background else self.L.demsph_dem_write_vtu
+        check(fn(self._h, str(directory).encode(), what, int(frame), int(step), float(time)))
 
     def get(self, field: str) -> np.ndarray:
         cnt = _sz()

@@ -7,6 +7,7 @@
 #include <mutex>
 #include <string>
 #include <stdexcept>
+#include <vector>
 
 #include "../../include/demsph.h"
 
@@ -109,6 +110,49 @@ inline void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind ki
     DEMSPH_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
     DEMSPH_HIP(hipStreamSynchronize(s));
 }
+
+// Pinned host copies of device arrays taken in stream order; complete once
+// wait() returns.  Freed when the owner goes away (on any thread).
+struct PinnedSnap {
+    std::vector<void*> bufs;
+    hipEvent_t ev = nullptr;
+    int device = 0;
+    PinnedSnap() = default;
+    PinnedSnap(const PinnedSnap&) = delete;
+    PinnedSnap& operator=(const PinnedSnap&) = delete;
+    ~PinnedSnap()
+    {
+        for (void* b : bufs) host_free(b);
+        if (ev) {
+            CaptureGuard g(capture_lock());
+            (void)hipEventDestroy(ev);
+        }
+    }
+    template <class T>
+    const T* take(const T* dev, size_t n, hipStream_t s)
+    {
+        void* p = nullptr;
+        DEMSPH_HIP(host_malloc(&p, (n ? n : 1) * sizeof(T)));
+        bufs.push_back(p);
+        if (n) DEMSPH_HIP(hipMemcpyAsync(p, dev, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        return static_cast<const T*>(p);
+    }
+    // call after the last take()
+    void mark(hipStream_t s)
+    {
+        DEMSPH_HIP(hipGetDevice(&device));
+        {
+            CaptureGuard g(capture_lock());
+            DEMSPH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
+        DEMSPH_HIP(hipEventRecord(ev, s));
+    }
+    void wait() const
+    {
+        DEMSPH_HIP(hipSetDevice(device));
+        DEMSPH_HIP(hipEventSynchronize(ev));
+    }
+};
 
 // ---------------------------------------------------------------- memory
 template <class T>

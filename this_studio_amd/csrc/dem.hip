@@ -19,6 +19,8 @@
 #include "timing.hpp"
 #include "vtu.hpp"
 
+#include <memory>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -2628,6 +2630,63 @@ extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, lo
             }
             interactions("ballTriangleInteraction", pt, overlap, nr, fc, tc, ss, sr, stq);
         }
+    });
+}
+
+// demsph_dem_write_vtu without waiting for the ball frame: the ball arrays
+// are copied to pinned host memory in stream order and the frame thread
+// writes the file while the scene keeps stepping.  The other kinds are
+// written before returning.
+extern "C" int demsph_dem_write_vtu_async(demsph_dem* d, const char* dir, int what, long frame, long step,
+                                          double time)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!dir) throw InvalidArg("null directory");
+        if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
+        const int rest = what & ~DEMSPH_VTU_BALL;
+        if (rest) {
+            const int rc = demsph_dem_write_vtu(d, dir, rest, frame, step, time);
+            if (rc != DEMSPH_OK) throw std::runtime_error(demsph_last_error());
+        }
+        if (!(what & DEMSPH_VTU_BALL)) return;
+        const int cb = d->cb;
+        const size_t n = d->n;
+        hipStream_t st = d->stream;
+        auto snap = std::make_shared<PinnedSnap>();
+        const double* a[9] = {snap->take(d->bpx[cb].p, n, st), snap->take(d->bpy[cb].p, n, st),
+                              snap->take(d->bpz[cb].p, n, st), snap->take(d->bvx[cb].p, n, st),
+                              snap->take(d->bvy[cb].p, n, st), snap->take(d->bvz[cb].p, n, st),
+                              snap->take(d->bwx[cb].p, n, st), snap->take(d->bwy[cb].p, n, st),
+                              snap->take(d->bwz[cb].p, n, st)};
+        const int* orig = snap->take(d->borig[cb].p, n, st);
+        snap->mark(st);
+        vtu::post([snap, a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3], a4 = a[4], a5 = a[5], a6 = a[6], a7 = a[7],
+                   a8 = a[8], orig, r = d->hb.r, mat = d->hb.mat, clump = d->hb.clump, dd = std::string(dir), n, frame,
+                   step, time] {
+            snap->wait();
+            const double* c[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+            std::vector<double> v[3];
+            for (auto& x : v) x.resize(3 * n);
+            for (size_t i = 0; i < n; i++) {
+                const size_t o = 3 * (size_t)orig[i];
+                for (int k = 0; k < 3; k++) {
+                    v[k][o] = c[3 * k][i]; v[k][o + 1] = c[3 * k + 1][i]; v[k][o + 2] = c[3 * k + 2][i];
+                }
+            }
+            vtu::Writer w(dd, "ball", frame);
+            w.begin(time, step, n, n);
+            w.points(v[0]);
+            w.vertex_cells(n);
+            w.open_data("PointData", "radius");
+            w.reals("radius", r);
+            w.ints("materialID", mat);
+            w.ints("clumpID", clump);
+            w.vectors("velocity", v[1]);
+            w.vectors("angularVelocity", v[2]);
+            w.close_data("PointData");
+            w.end();
+        });
     });
 }
 

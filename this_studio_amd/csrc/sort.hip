@@ -8,11 +8,15 @@
 // stably inside a wavefront, and LDS prefix sums across the wavefronts of a
 // workgroup.  Only the bits that hold hashes in [0, numGrids] are sorted.
 #include "common.hpp"
+#include "vtu.hpp"
 
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -429,6 +433,78 @@ std::recursive_mutex& capture_lock()
     static std::recursive_mutex m;
     return m;
 }
+
+// ------------------------------------------------------------ frame thread
+namespace vtu {
+namespace {
+struct FrameQueue {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::function<void()>> jobs;
+    int running = 0;
+    bool started = false;
+    std::string error;
+};
+// Never destroyed: the detached writer thread may outlive static teardown.
+FrameQueue& frame_queue()
+{
+    static FrameQueue* q = new FrameQueue();
+    return *q;
+}
+void frame_thread()
+{
+    FrameQueue& q = frame_queue();
+    for (;;) {
+        std::function<void()> job;
+        {
+            std::unique_lock<std::mutex> lk(q.mu);
+            q.cv.wait(lk, [&] { return !q.jobs.empty(); });
+            job = std::move(q.jobs.front());
+            q.jobs.pop_front();
+            q.running = 1;
+        }
+        std::string err;
+        try {
+            job();
+        } catch (const std::exception& e) {
+            err = e.what();
+        } catch (...) {
+            err = "frame output failed";
+        }
+        job = nullptr;  // releases the job's snapshot before it is reported done
+        {
+            std::lock_guard<std::mutex> lk(q.mu);
+            q.running = 0;
+            if (!err.empty() && q.error.empty()) q.error = err;
+        }
+        q.cv.notify_all();
+    }
+}
+}  // namespace
+
+void post(std::function<void()> job)
+{
+    FrameQueue& q = frame_queue();
+    std::unique_lock<std::mutex> lk(q.mu);
+    if (!q.started) {
+        std::thread(frame_thread).detach();
+        q.started = true;
+    }
+    q.cv.wait(lk, [&] { return q.jobs.size() + (size_t)q.running < 2; });
+    q.jobs.push_back(std::move(job));
+    q.cv.notify_all();
+}
+
+std::string drain()
+{
+    FrameQueue& q = frame_queue();
+    std::unique_lock<std::mutex> lk(q.mu);
+    q.cv.wait(lk, [&] { return q.jobs.empty() && q.running == 0; });
+    std::string e;
+    e.swap(q.error);
+    return e;
+}
+}  // namespace vtu
 
 // ------------------------------------------------------------ block cache
 namespace {

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -2482,6 +2483,68 @@ extern "C" int demsph_sph_write_vtu(demsph_sph* s, const char* dir, int what, lo
             w.close_data("PointData");
             w.end();
         }
+    });
+}
+
+// demsph_sph_write_vtu without waiting: the fluid arrays are copied to
+// pinned host memory in stream order, and the frame thread restores
+// insertion order and writes the file while the scene keeps stepping.  The
+// dummy frame (static particles) is written before returning.
+extern "C" int demsph_sph_write_vtu_async(demsph_sph* s, const char* dir, int what, long frame, long step,
+                                          double time)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArg("null scene");
+        s->require_init();
+        if (!dir) throw InvalidArg("null directory");
+        if (what & ~(DEMSPH_VTU_SPH | DEMSPH_VTU_DUMMY)) throw InvalidArg("unknown frame kinds");
+        if (frame < 0 || step < 0) throw InvalidArg("negative frame or step");
+        if (what & DEMSPH_VTU_DUMMY) {
+            const int rc = demsph_sph_write_vtu(s, dir, DEMSPH_VTU_DUMMY, frame, step, time);
+            if (rc != DEMSPH_OK) throw std::runtime_error(demsph_last_error());
+        }
+        if (!(what & DEMSPH_VTU_SPH)) return;
+        const size_t n = s->nf;
+        hipStream_t st = s->stream;
+        auto snap = std::make_shared<PinnedSnap>();
+        const FluidPos* P = snap->take(s->P[s->cP], n, st);
+        const double* vx = snap->take(s->vx[s->cV], n, st);
+        const double* vy = snap->take(s->vy[s->cV], n, st);
+        const double* vz = snap->take(s->vz[s->cV], n, st);
+        const int* orig = snap->take(s->orig[s->cO], n, st);
+        snap->mark(st);
+        std::vector<double> h(n);
+        for (size_t i = 0; i < n; i++) h[i] = s->phases[s->fphase_h[i]].h;
+        vtu::post([snap, P, vx, vy, vz, orig, h = std::move(h), d = std::string(dir), n, frame, step, time] {
+            snap->wait();
+            std::vector<double> pos(3 * n), vel(3 * n), rho(n), pr(n);
+            for (size_t i = 0; i < n; i++) {
+                const size_t o = (size_t)orig[i];
+                pos[3 * o] = P[i].x; pos[3 * o + 1] = P[i].y; pos[3 * o + 2] = P[i].z;
+                vel[3 * o] = vx[i]; vel[3 * o + 1] = vy[i]; vel[3 * o + 2] = vz[i];
+                rho[o] = P[i].rho;
+                pr[o] = P[i].p;
+            }
+            vtu::Writer w(d, "SPH", frame);
+            w.begin(time, step, n, n);
+            w.points(pos);
+            w.vertex_cells(n);
+            w.open_data("PointData", "smoothLength");
+            w.reals("smoothLength", h);
+            w.reals("density", rho);
+            w.reals("pressure", pr);
+            w.vectors("velocity", vel);
+            w.close_data("PointData");
+            w.end();
+        });
+    });
+}
+
+extern "C" int demsph_frames_wait(void)
+{
+    return guarded([&] {
+        const std::string e = vtu::drain();
+        if (!e.empty()) throw std::runtime_error("frame output: " + e);
     });
 }
 

@@ -8,6 +8,7 @@
 
 #include <cerrno>
 #include <cstdio>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -142,6 +143,14 @@ private:
     std::FILE* f_ = nullptr;
     std::string path_;
 };
+
+// Background frame output: posted jobs run in order on one process-wide
+// writer thread.  post() waits while two jobs are queued or running, which
+// bounds the pinned snapshots held for them; drain() waits until every
+// posted job has finished and returns the first error message since the
+// last drain (empty if none).
+void post(std::function<void()> job);
+std::string drain();
 
 }  // namespace vtu
 }  // namespace demsph
