@@ -140,6 +140,15 @@ int demsph_sph_write_vtu(demsph_sph* s, const char* dir, int what, long frame, l
 int demsph_sph_write_vtu_async(demsph_sph* s, const char* dir, int what, long frame, long step, double time);
 int demsph_frames_wait(void);
 
+/* Scene state to host memory and back (no counterpart in the reference; for
+ * evicting a served job and resuming it): the fluid's dynamic state in its
+ * current cell-sorted order.  A scene staged with the same particles and
+ * initialised on the same domain continues from an imported state bit for
+ * bit as the exporting scene does.  *bytes receives the size; out may be
+ * NULL to query it. */
+int demsph_sph_export_state(demsph_sph* s, void* out, size_t capacity, size_t* bytes);
+int demsph_sph_import_state(demsph_sph* s, const void* in, size_t bytes);
+
 /* Overwrites position / velocity (double, 3n) or densityChange (double, n)
  * by insertion index (density and pressure follow from the EOS). */
 int demsph_sph_set(demsph_sph* s, const char* field, const void* in, size_t count);
@@ -360,6 +369,16 @@ int demsph_dem_write_vtu_async(demsph_dem* d, const char* dir, int what, long fr
  * and torque from every force evaluation for the frame writer (off by
  * default: 96 bytes per pair or candidate slot and a store per entry). */
 int demsph_dem_record_contacts(demsph_dem* d, int enable);
+
+/* DEM scene state to host memory and back (no counterpart in the reference;
+ * for evicting a served job and resuming it): ball state in
+ * its current slot order with forces, the current pair list with its
+ * contact history, clump and wall poses, wall-contact history, bond state,
+ * step counter and time.  The importing scene must be staged with the same
+ * balls, clumps, walls and bonds, initialised, and of the same precision. */
+int demsph_dem_export_state(demsph_dem* d, void* out, size_t capacity, size_t* bytes);
+int demsph_dem_import_state(demsph_dem* d, const void* in, size_t bytes);
+
 /* Overwrites "position" "velocity" "angularVelocity" (double 3n, insertion
  * order) or "wallVelocity" "wallAngularVelocity" (double 3/wall; the
  * reference's getMeshWallBody() write path). */

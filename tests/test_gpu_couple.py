@@ -176,3 +176,26 @@ def test_balls_in_two_fluid_phases_match_oracle():
     assert np.abs(Fg - Fo).max() < 1e-4 * np.abs(Fo).max()
     assert np.abs(gd.get("position") - od.get("pos")).max() < 1e-9
     assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5
+
+
+def test_coupled_job_resumes_from_exported_scenes():
+    """A coupled job is the pair of its scenes: both exported after 7 coupled
+    steps and imported into a newly built pair continue bit for bit as the
+    uninterrupted job (the coupling's own buffers hold only the last
+    evaluation's diagnostics)."""
+    spec, fluid, dem = falling_balls()
+    c = ts.CoupledScene(fluid, dem)
+    c.step(spec.dt, 7, spec.gravity)
+    fs, ds = fluid.export_state(), dem.export_state()
+    spec2, fluid2, dem2 = falling_balls()
+    fluid2.import_state(fs)
+    dem2.import_state(ds)
+    c2 = ts.CoupledScene(fluid2, dem2)
+    c.step(spec.dt, 12, spec.gravity)
+    c2.step(spec2.dt, 12, spec2.gravity)
+    for f in ("position", "velocity", "density", "densityChange"):
+        assert np.array_equal(fluid.get(f), fluid2.get(f)), f
+    for f in ("position", "velocity", "angularVelocity"):
+        assert np.array_equal(dem.get(f), dem2.get(f)), f
+    assert np.array_equal(c.get("couplingForce"), c2.get("couplingForce"))
+    assert dem.clock() == dem2.clock()

@@ -362,3 +362,31 @@ def test_background_frame_is_a_snapshot(small, tmp_path):
     assert a == b
     s.write_vtu(tmp_path / "later", 2, 40, 1.0, dummies=False)
     assert (tmp_path / "later" / "SPH_0002.vtu").read_bytes() != a
+
+
+def test_exported_state_resumes_bit_for_bit(small):
+    """A scene evicted to host memory and restored into a freshly built scene
+    of the same particles continues exactly as the scene that kept running,
+    with graphs on and off; a state of other particle counts is refused."""
+    a = make_gpu(small)
+    a.step(small.dt, 13)
+    state = a.export_state()
+    for graphs in (True, False):
+        b = make_gpu(small)
+        b.set_graphs(graphs)
+        b.step(small.dt, 3)  # the state replaces whatever b held
+        b.import_state(state)
+        ref = make_gpu(small)
+        ref.step(small.dt, 13)
+        assert ref.export_state() == state
+        ref.step(small.dt, 21)
+        b.step(small.dt, 21)
+        for f in ("position", "velocity", "density", "pressure", "densityChange"):
+            assert np.array_equal(ref.get(f), b.get(f)), (graphs, f)
+    with pytest.raises(ts.DemsphError):
+        b.import_state(state[:-4])
+    other = scenes.small_dam_break()
+    assert other.fluid_vel is None and other.fluid_c0 is None
+    other.fluid_pos = other.fluid_pos[:-1]
+    with pytest.raises(ts.DemsphError):
+        make_gpu(other).import_state(state)

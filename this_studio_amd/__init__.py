@@ -87,6 +87,8 @@ def _declare(L: C.CDLL) -> None:
         "demsph_sph_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
         "demsph_sph_write_vtu_async": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
         "demsph_frames_wait": [],
+        "demsph_sph_export_state": [_vp, _vp, _sz, C.POINTER(_sz)],
+        "demsph_sph_import_state": [_vp, _vp, _sz],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -190,6 +192,18 @@ class SPHScene:
         a, b, c = _sz(), _sz(), _sz()
         check(lib().demsph_sph_sizes(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def export_state(self) -> bytes:
+        """The fluid's dynamic state (current cell-sorted order) as bytes."""
+        n = _sz()
+        check(lib().demsph_sph_export_state(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().demsph_sph_export_state(self._h, buf, n.value, C.byref(n)))
+        return buf.raw
+
+    def import_state(self, state: bytes) -> None:
+        """Continues from a state exported by a scene of the same particles."""
+        check(lib().demsph_sph_import_state(self._h, state, len(state)))
 
     def write_vtu(self, directory, frame: int, step: int, time: float, fluid: bool = True,
                   dummies: bool = True, background: bool = False) -> None:
@@ -337,6 +351,8 @@ def _declare_dem(L):
         "demsph_dem_write_vtu": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
         "demsph_dem_write_vtu_async": [_vp, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_double],
         "demsph_dem_record_contacts": [_vp, C.c_int],
+        "demsph_dem_export_state": [_vp, _vp, _sz, C.POINTER(_sz)],
+        "demsph_dem_import_state": [_vp, _vp, _sz],
         "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
         "demsph_dem_set_graphs": [_vp, C.c_int],
         "demsph_dem_graph_captures": [_vp, C.POINTER(C.c_long)],
@@ -509,6 +525,18 @@ class DEMSceneGPU:
         """Keep each ball-triangle contact of every evaluation (for the
         ballTriangleInteraction frame file)."""
         check(self.L.demsph_dem_record_contacts(self._h, 1 if on else 0))
+
+    def export_state(self) -> bytes:
+        """Ball, contact-history, clump, wall and bond state, step and time."""
+        n = _sz()
+        check(self.L.demsph_dem_export_state(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(self.L.demsph_dem_export_state(self._h, buf, n.value, C.byref(n)))
+        return buf.raw
+
+    def import_state(self, state: bytes) -> None:
+        """Continues from a state exported by a scene staged the same way."""
+        check(self.L.demsph_dem_import_state(self._h, state, len(state)))
 
     def write_vtu(self, directory, frame: int, step: int, time: float, balls: bool = True,
                   walls: bool = True, wall_contacts: bool = False, pairs: bool = False,

@@ -2456,6 +2456,154 @@ extern "C" int demsph_dem_record_contacts(demsph_dem* d, int enable)
     });
 }
 
+// Dynamic DEM state to host memory and back: the balls in their current
+// slot order with ranks and forces, the current pair list with its contact
+// history and reverse index, clump and wall poses, wall-contact history and
+// bond state, plus the step counter and time.  Static data (materials,
+// triangles, bond pairs, clump inertia, hooks) comes from the importing
+// scene's own staging; the triangle cells are rebuilt from the imported
+// vertices at the next evaluation.
+namespace {
+constexpr uint32_t kDemStateMagic = 0x4d454444u;  // "DDEM"
+struct DemStateHeader {
+    uint32_t magic, version;
+    int32_t n, nc, nw, nv, nt, nbond, KW, fp64;
+    int32_t npairs, has_list, walls_static, walls_valid;
+    uint64_t iStep;
+    double time;
+};
+
+// Every array of the state in one fixed order, as (device pointer, bytes).
+template <class F>
+void dem_state_arrays(demsph_dem* d, F&& io)
+{
+    const size_t n = d->n, nc = d->nc, nw = d->nw, nv = d->nv, nt = d->nt, nb = d->nbond_dev;
+    const size_t np = d->has_list ? (size_t)d->npairs : 0, hn = (size_t)d->KW * n;
+    const int cb = d->cb, cp = d->cp;
+    for (auto* b : {&d->bpx[cb], &d->bpy[cb], &d->bpz[cb], &d->bvx[cb], &d->bvy[cb], &d->bvz[cb], &d->bwx[cb],
+                    &d->bwy[cb], &d->bwz[cb], &d->br[cb], &d->bim[cb], &d->fx, &d->fy, &d->fz, &d->tx, &d->ty, &d->tz})
+        io(b->p, n * 8);
+    for (auto* b : {&d->bmat[cb], &d->bclump[cb], &d->borig[cb], &d->rank}) io(b->p, n * 4);
+    if (d->has_list) {
+        io(d->pstart[cp].p, (n + 1) * 4);
+        io(d->ppartner[cp].p, np * 4);
+        for (auto* b : {&d->psx[cp], &d->psy[cp], &d->psz[cp], &d->prx[cp], &d->pry[cp], &d->prz[cp], &d->pox[cp],
+                        &d->poy[cp], &d->poz[cp], &d->pov[cp], &d->pFx[cp], &d->pFy[cp], &d->pFz[cp], &d->pRx[cp],
+                        &d->pRy[cp], &d->pRz[cp]})
+            io(b->p, np * 8);
+        io(d->rstart.p, (n + 1) * 4);
+        io(d->rpair.p, np * 4);
+    }
+    if (nc > 0)
+        for (auto* b : {&d->cpx, &d->cpy, &d->cpz, &d->cvx, &d->cvy, &d->cvz, &d->cwx, &d->cwy, &d->cwz, &d->cq0,
+                        &d->cq1, &d->cq2, &d->cq3, &d->cFx, &d->cFy, &d->cFz, &d->cTx, &d->cTy, &d->cTz})
+            io(b->p, nc * 8);
+    if (nt > 0) {
+        for (auto* b : {&d->wpx, &d->wpy, &d->wpz, &d->wvx, &d->wvy, &d->wvz, &d->wwx, &d->wwy, &d->wwz, &d->wq0,
+                        &d->wq1, &d->wq2, &d->wq3})
+            io(b->p, nw * 8);
+        for (auto* b : {&d->vgx, &d->vgy, &d->vgz}) io(b->p, nv * 8);
+        io(d->tcc.p, 3 * nt * 8);
+        io(d->hcnt.p, n * 4);
+        io(d->htri.p, hn * 4);
+        io(d->hcancel.p, hn * 4);
+        for (auto* b : {&d->hsx, &d->hsy, &d->hsz, &d->hrx, &d->hry, &d->hrz, &d->hox, &d->hoy, &d->hoz, &d->hov})
+            io(b->p, hn * 8);
+        for (auto* b : {&d->wfx, &d->wfy, &d->wfz, &d->wtx, &d->wty, &d->wtz}) io(b->p, n * 8);
+    }
+    if (nb > 0) {
+        for (auto* b : {&d->dbpx, &d->dbpy, &d->dbpz, &d->dbnx, &d->dbny, &d->dbnz, &d->dbsx, &d->dbsy, &d->dbsz,
+                        &d->dbtx, &d->dbty, &d->dbtz, &d->dbFn, &d->dbTt, &d->dbsig, &d->dbtau, &d->dbFx, &d->dbFy,
+                        &d->dbFz, &d->dbTix, &d->dbTiy, &d->dbTiz, &d->dbTjx, &d->dbTjy, &d->dbTjz})
+            io(b->p, nb * 8);
+        io(d->dbonded.p, nb * 4);
+    }
+}
+
+DemStateHeader dem_state_header(const demsph_dem* d)
+{
+    DemStateHeader h{};
+    h.magic = kDemStateMagic; h.version = 1u;
+    h.n = d->n; h.nc = d->nc; h.nw = d->nw; h.nv = d->nv; h.nt = d->nt; h.nbond = d->nbond_dev;
+    h.KW = d->KW; h.fp64 = d->fp64 ? 1 : 0;
+    h.npairs = d->has_list ? d->npairs : 0;
+    h.has_list = d->has_list ? 1 : 0;
+    h.walls_static = d->walls_static ? 1 : 0;
+    h.walls_valid = d->walls_valid ? 1 : 0;
+    h.iStep = d->iStep; h.time = d->time;
+    return h;
+}
+}  // namespace
+
+extern "C" int demsph_dem_export_state(demsph_dem* d, void* out, size_t capacity, size_t* bytes)
+{
+    DEM_GUARD({
+        d->require_init();
+        const DemStateHeader h = dem_state_header(d);
+        size_t total = sizeof h;
+        dem_state_arrays(d, [&](const void*, size_t b) { total += b; });
+        if (bytes) *bytes = total;
+        if (!out) return;
+        if (capacity < total) throw InvalidArg("state buffer too small");
+        char* o = static_cast<char*>(out);
+        std::memcpy(o, &h, sizeof h);
+        o += sizeof h;
+        dem_state_arrays(d, [&](const void* dev, size_t b) {
+            if (b) DEMSPH_HIP(hipMemcpyAsync(o, dev, b, hipMemcpyDeviceToHost, d->stream));
+            o += b;
+        });
+        DEMSPH_HIP(hipStreamSynchronize(d->stream));
+    });
+}
+
+extern "C" int demsph_dem_import_state(demsph_dem* d, const void* in, size_t bytes)
+{
+    DEM_GUARD({
+        if (!in) throw InvalidArg("null state");
+        d->require_init();
+        DemStateHeader h;
+        if (bytes < sizeof h) throw InvalidArg("state too short");
+        std::memcpy(&h, in, sizeof h);
+        if (h.magic != kDemStateMagic || h.version != 1u) throw InvalidArg("not a DEM scene state");
+        if (h.n != d->n || h.nc != d->nc || h.nw != d->nw || h.nv != d->nv || h.nt != d->nt ||
+            h.nbond != d->nbond_dev || h.KW != d->KW || h.fp64 != (d->fp64 ? 1 : 0))
+            throw InvalidArg("the state belongs to a scene of other counts or precision");
+        if (h.npairs < 0 || (h.has_list == 0 && h.npairs != 0)) throw InvalidArg("corrupt DEM state header");
+        // the size the header implies, computed before anything is resized
+        const size_t np = (size_t)h.npairs;
+        size_t total = sizeof h;
+        {
+            const bool had = d->has_list;
+            const int had_np = d->npairs;
+            d->has_list = h.has_list != 0;
+            d->npairs = h.npairs;
+            dem_state_arrays(d, [&](const void*, size_t b) { total += b; });
+            d->has_list = had;
+            d->npairs = had_np;
+        }
+        if (bytes != total) throw InvalidArg("state size does not match the scene");
+        DEMSPH_HIP(hipStreamSynchronize(d->stream));
+        d->drop_graphs();
+        if (h.has_list) {
+            if ((size_t)d->pcap[d->cp] < np) d->alloc_pairs(d->cp, h.npairs + h.npairs / 4);
+            if (d->rpair.n < np) d->rpair.alloc(std::max<size_t>(np, (size_t)d->pcap[d->cp]));
+        }
+        d->has_list = h.has_list != 0;
+        d->npairs = h.npairs;
+        d->iStep = h.iStep;
+        d->time = h.time;
+        d->walls_static = h.walls_static != 0;
+        d->walls_valid = h.walls_valid != 0;
+        d->tri_cells_valid = false;
+        const char* p = static_cast<const char*>(in) + sizeof h;
+        dem_state_arrays(d, [&](void* dev, size_t b) {
+            if (b) DEMSPH_HIP(hipMemcpyAsync(dev, p, b, hipMemcpyHostToDevice, d->stream));
+            p += b;
+        });
+        DEMSPH_HIP(hipStreamSynchronize(d->stream));
+    });
+}
+
 extern "C" int demsph_dem_write_vtu(demsph_dem* d, const char* dir, int what, long frame, long step, double time)
 {
     DEM_GUARD({

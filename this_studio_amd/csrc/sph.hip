@@ -2540,6 +2540,84 @@ extern "C" int demsph_sph_write_vtu_async(demsph_sph* s, const char* dir, int wh
     });
 }
 
+// Dynamic fluid state to host memory and back, in the current cell-sorted
+// order (the stable sort of the next step depends on it): positions with
+// density and pressure, velocities (FP32 record and FP64), density change,
+// and the slot <-> insertion maps.  Header: magic, version, nf, nd,
+// numGrids (the importing scene must be staged with the same particles).
+namespace {
+constexpr uint32_t kStateMagic = 0x48505344u;  // "DSPH"
+struct StateHeader {
+    uint32_t magic, version;
+    uint64_t nf, nd, numGrids;
+};
+size_t state_bytes(size_t n)
+{
+    return sizeof(StateHeader) + n * (sizeof(FluidPos) + sizeof(float4) + 3 * sizeof(double) + sizeof(float) +
+                                      2 * sizeof(int));
+}
+}  // namespace
+
+extern "C" int demsph_sph_export_state(demsph_sph* s, void* out, size_t capacity, size_t* bytes)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArg("null scene");
+        s->require_init();
+        const size_t n = s->nf, total = state_bytes(n);
+        if (bytes) *bytes = total;
+        if (!out) return;
+        if (capacity < total) throw InvalidArg("state buffer too small");
+        hipStream_t st = s->stream;
+        char* o = static_cast<char*>(out);
+        const StateHeader h{kStateMagic, 1u, (uint64_t)n, (uint64_t)s->nd, (uint64_t)s->numGrids};
+        std::memcpy(o, &h, sizeof h);
+        o += sizeof h;
+        const auto get = [&](const void* dev, size_t b) {
+            if (b) DEMSPH_HIP(hipMemcpyAsync(o, dev, b, hipMemcpyDeviceToHost, st));
+            o += b;
+        };
+        get(s->P[s->cP], n * sizeof(FluidPos));
+        get(s->V[s->cV], n * sizeof(float4));
+        get(s->vx[s->cV], n * 8); get(s->vy[s->cV], n * 8); get(s->vz[s->cV], n * 8);
+        get(s->drho[s->cD], n * 4);
+        get(s->orig[s->cO], n * 4);
+        get(s->rank, n * 4);
+        DEMSPH_HIP(hipStreamSynchronize(st));
+    });
+}
+
+extern "C" int demsph_sph_import_state(demsph_sph* s, const void* in, size_t bytes)
+{
+    return guarded([&] {
+        if (!s || !in) throw InvalidArg("null argument");
+        s->require_init();
+        StateHeader h;
+        if (bytes < sizeof h) throw InvalidArg("state too short");
+        std::memcpy(&h, in, sizeof h);
+        if (h.magic != kStateMagic || h.version != 1u) throw InvalidArg("not a fluid scene state");
+        if (h.nf != (uint64_t)s->nf || h.nd != (uint64_t)s->nd || h.numGrids != (uint64_t)s->numGrids)
+            throw InvalidArg("the state belongs to a scene of other particle counts or grid");
+        const size_t n = s->nf;
+        if (bytes != state_bytes(n)) throw InvalidArg("state size does not match the scene");
+        hipStream_t st = s->stream;
+        DEMSPH_HIP(hipStreamSynchronize(st));
+        const char* p = static_cast<const char*>(in) + sizeof h;
+        const auto put = [&](void* dev, size_t b) {
+            if (b) DEMSPH_HIP(hipMemcpyAsync(dev, p, b, hipMemcpyHostToDevice, st));
+            p += b;
+        };
+        put(s->P[s->cP], n * sizeof(FluidPos));
+        put(s->V[s->cV], n * sizeof(float4));
+        put(s->vx[s->cV], n * 8); put(s->vy[s->cV], n * 8); put(s->vz[s->cV], n * 8);
+        put(s->drho[s->cD], n * 4);
+        put(s->orig[s->cO], n * 4);
+        put(s->rank, n * 4);
+        DEMSPH_HIP(hipStreamSynchronize(st));
+        s->lists_valid = false;  // the lists belong to the replaced state
+        s->cells_track_skey = false;
+    });
+}
+
 extern "C" int demsph_frames_wait(void)
 {
     return guarded([&] {
