@@ -20,6 +20,8 @@ Prints one JSON line:
   serving_mix  = the batched-serving job stream, plain vs batched (and
                batched with autotuning on admission): throughput, p50/p99
   autotune     = the autotuner's trials on the dam break against the rules
+  state_io     = the dam break's state exported to host memory and imported
+               back (the cost of evicting and resuming a served job)
   cantilever_clumps, ice_cylinder_coupled = the bonded-beam-with-clumps and
                coupled ice-cylinder workloads
 """
@@ -302,6 +304,8 @@ def main() -> int:
                     help="also serve the batched path with fluid scenes autotuned on admission")
     ap.add_argument("--no-autotune-report", action="store_true",
                     help="skip the autotuner trials on the dam break after the timed steps")
+    ap.add_argument("--no-state-io", action="store_true",
+                    help="skip timing the scene state export and import after the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate HBM counters")
     args = ap.parse_args()
@@ -409,6 +413,26 @@ def main() -> int:
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
         ts.SPHScene.clear_autotune_cache()
 
+    # Eviction cost of the same scene: its state to pageable host memory and
+    # back (a serving loop evicting and resuming the job), best of 3 each.
+    state_io = None
+    if not args.no_state_io:
+        t_exp, t_imp = [], []
+        for _ in range(3):
+            t = time.perf_counter()
+            blob = s.export_state()
+            t_exp.append(time.perf_counter() - t)
+        for _ in range(3):
+            t = time.perf_counter()
+            s.import_state(blob)
+            t_imp.append(time.perf_counter() - t)
+        state_io = {"bytes": len(blob), "bytes_per_particle": round(len(blob) / nf, 2),
+                    "export_ms": round(min(t_exp) * 1e3, 3), "import_ms": round(min(t_imp) * 1e3, 3),
+                    "export_GBps": round(len(blob) / min(t_exp) / 1e9, 2),
+                    "import_GBps": round(len(blob) / min(t_imp) / 1e9, 2),
+                    "roundtrip_equal": s.export_state() == blob}
+        print(f"[bench] state export/import: {state_io}", file=sys.stderr, flush=True)
+
     # whole-step algorithmic HBM rate (sum over the step's kernels)
     step_bytes = sum(algorithmic_bytes(k, nf, pairs_ff, pairs_fd) * timing[k][0] for k in timing) / args.steps
     cpu = None
@@ -470,6 +494,7 @@ def main() -> int:
         "dem_collapse": dem,
         "serving_mix": mix,
         "autotune": tune,
+        "state_io": state_io,
         "cantilever_clumps": beam,
         "ice_cylinder_coupled": ice,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
