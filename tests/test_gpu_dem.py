@@ -398,29 +398,35 @@ _STATE_FIELDS = ("position", "velocity", "angularVelocity", "force", "torque", "
                  "torsionSpring", "overlap", "objectPointed", "objectPointing")
 
 
-@pytest.mark.parametrize("name", ["collapse", "cantilever_clumps"])
+@pytest.mark.parametrize("name", ["collapse", "collapse_fp64", "cantilever_clumps"])
 def test_exported_state_resumes_bit_for_bit(name):
     """A DEM scene exported between pair rebuilds and imported into a freshly
     built scene of the same staging continues bit for bit as the scene that
-    kept running: ball and wall contact history on the collapse (box walls),
-    bond state, clumps and the time-ramped hook load on the cantilever."""
-    if name == "collapse":
+    kept running: ball and wall contact history on the collapse (box walls,
+    FP32 and FP64 contact arithmetic), bond state, clumps and the time-ramped
+    hook load on the cantilever.  The importer has either stepped already or
+    (collapse) never run solve_begin; a scene of the other precision refuses
+    the state."""
+    fp64 = name == "collapse_fp64"
+    if name.startswith("collapse"):
         make, extra = scenes.granular_collapse, ("triPointed", "triSlidingSpring", "triOverlap")
     else:
         def make():
             return scenes.cantilever_with_clumps(nx=8, ny=3, nz=3, n_clumps=12, ramp_steps=50)
         extra = ("clumpPosition", "clumpOrientation", "clumpAngularVelocity", "bondIsBonded", "bondNormalForce",
                  "bondShearForce", "bondBendingTorque", "bondTorsionTorque")
-    a = ts.DEMSceneGPU(make())
+    a = ts.DEMSceneGPU(make(), fp64=fp64)
     a.step(23)  # 23 % 5 != 0: mid-way between rebuilds
     state = a.export_state()
     for graphs in (True, False):
-        b = ts.DEMSceneGPU(make())
+        # the cantilever's bonds come from solve_begin's initial pair list
+        b = ts.DEMSceneGPU(make(), fp64=fp64, solve_begin=graphs or name == "cantilever_clumps")
         b.set_graphs(graphs)
-        b.step(7)  # the state replaces whatever b held
+        if graphs:
+            b.step(7)  # the state replaces whatever b held
         b.import_state(state)
         assert b.clock() == a.clock()
-        ref = ts.DEMSceneGPU(make())
+        ref = ts.DEMSceneGPU(make(), fp64=fp64)
         ref.step(23)
         ref.step(41)
         b.step(41)
@@ -431,3 +437,5 @@ def test_exported_state_resumes_bit_for_bit(name):
         b.import_state(state[:-8])
     with pytest.raises(ts.DemsphError):
         ts.DEMSceneGPU(scenes.bonded_beam()).import_state(state)
+    with pytest.raises(ts.DemsphError):
+        ts.DEMSceneGPU(make(), fp64=not fp64).import_state(state)
