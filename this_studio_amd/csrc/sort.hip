@@ -92,20 +92,47 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     for (int d = threadIdx.x; d < BINS; d += kSortThreads) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
+// Per-digit exclusive scan over the tiles, one wave per digit:
+// offs[d][t] = sum of hist[d][t'] over t' < t, dtot[d] = digit d's total.
+// Replaces a device-wide scan of the whole digit-major table (three
+// launches); the scatter adds the digit bases itself.
+template <int BINS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_tile_scan(const uint32_t* __restrict__ hist, int nblocks,
+                                                                  uint32_t* __restrict__ offs,
+                                                                  uint32_t* __restrict__ dtot)
+{
+    const int lane = threadIdx.x & 63;
+    const int d = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (d >= BINS) return;  // wave-uniform
+    const uint32_t* h = hist + (size_t)d * nblocks;
+    uint32_t* o = offs + (size_t)d * nblocks;
+    int carry = 0;
+    for (int t0 = 0; t0 < nblocks; t0 += 64) {
+        const int t = t0 + lane;
+        const int v = t < nblocks ? (int)h[t] : 0;
+        const int inc = wave_inclusive_sum(v);
+        if (t < nblocks) o[t] = (uint32_t)(carry + inc - v);
+        carry += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) dtot[d] = (uint32_t)carry;
+}
+
 // Stable scatter.  Tile element order is wave-major, then item, then lane,
 // which is the global index order inside the tile.
 //
 // FUSED (few tiles): `table` is the digit-major histogram itself and each
 // workgroup derives its scatter bases from it (thread d sums digit d over
 // all tiles and over the tiles before its own, then a workgroup scan over
-// the digits), which replaces the device-wide scan's three launches.
-// Otherwise `table` holds the scanned offsets.
+// the digits).  Otherwise `table` holds the per-digit tile offsets of
+// k_radix_tile_scan and `dtot` the digit totals, whose workgroup scan gives
+// the digit bases.
 template <bool FUSED, int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ keys_in,
                                                                 const int* __restrict__ vals_in,
                                                                 uint32_t* __restrict__ keys_out,
                                                                 int* __restrict__ vals_out, int n, int shift,
-                                                                const uint32_t* __restrict__ table, int nblocks)
+                                                                const uint32_t* __restrict__ table,
+                                                                const uint32_t* __restrict__ dtot, int nblocks)
 {
     constexpr int BINS = 1 << BITS;
     constexpr int DPT = BINS / kSortThreads;  // digits per thread in the fused bases: 1 or 2
@@ -139,7 +166,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
             run += tot[k];
         }
     } else {
-        for (int d = threadIdx.x; d < BINS; d += kSortThreads) block_base[d] = table[(size_t)d * nblocks + blockIdx.x];
+        uint32_t tot[DPT];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < DPT; k++) {
+            tot[k] = dtot[DPT * threadIdx.x + k];
+            sum += tot[k];
+        }
+        int total;
+        uint32_t run = (uint32_t)block_exclusive((int)sum, total);
+#pragma unroll
+        for (int k = 0; k < DPT; k++) {
+            const int d = DPT * threadIdx.x + k;
+            block_base[d] = run + table[(size_t)d * nblocks + blockIdx.x];
+            run += tot[k];
+        }
     }
     __syncthreads();
 
@@ -351,7 +392,7 @@ size_t radix_sort_temp_bytes(int n)
 {
     const int nb = sort_blocks(n > 0 ? n : 1);
     const size_t hist = sizeof(uint32_t) * (size_t)kMaxBins * nb;
-    return hist * 2 + scan_temp_bytes(kMaxBins * nb) + 256;
+    return hist * 2 + sizeof(uint32_t) * kMaxBins + 256;
 }
 
 namespace {
@@ -363,7 +404,7 @@ void radix_passes(const uint32_t* keys_in, const int* vals_in, uint32_t* keys_ou
     const int nb = sort_blocks(n);
     uint32_t* hist = static_cast<uint32_t*>(temp);
     uint32_t* offs = hist + (size_t)kMaxBins * nb;
-    void* scan_tmp = offs + (size_t)kMaxBins * nb;
+    uint32_t* dtot = offs + (size_t)kMaxBins * nb;  // kMaxBins digit totals
     // Ping-pong so that the last pass writes keys_out / vals_out.
     const uint32_t* ksrc = keys_in;
     const int* vsrc = vals_in;
@@ -375,12 +416,12 @@ void radix_passes(const uint32_t* keys_in, const int* vals_in, uint32_t* keys_ou
         hipLaunchKernelGGL(k_radix_hist<BITS>, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
         if (nb <= kFusedScatterTiles) {
             hipLaunchKernelGGL((k_radix_scatter<true, BITS>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst,
-                               vdst, n, shift, hist, nb);
+                               vdst, n, shift, hist, nullptr, nb);
         } else {
-            exclusive_scan(reinterpret_cast<const int*>(hist), reinterpret_cast<int*>(offs), BINS * nb, scan_tmp,
-                           nullptr, s);
+            hipLaunchKernelGGL(k_radix_tile_scan<BINS>, dim3(BINS / kWaves), dim3(kSortThreads), 0, s, hist, nb, offs,
+                               dtot);
             hipLaunchKernelGGL((k_radix_scatter<false, BITS>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, kdst,
-                               vdst, n, shift, offs, nb);
+                               vdst, n, shift, offs, dtot, nb);
         }
         DEMSPH_HIP(hipGetLastError());
         ksrc = kdst;
