@@ -41,9 +41,17 @@ def random_scene(n=3000, seed=7, box=0.4):
     return s
 
 
-@pytest.mark.parametrize("name", ["random", "collapse"])
+@pytest.mark.parametrize("name", ["random", "collapse", "random_40k"])
 def test_neighbor_search_bit_exact(name):
-    s = random_scene() if name == "random" else scenes.granular_collapse()
+    """Hash, cells and pair lists against the oracle, bit for bit.  Up to
+    32,768 balls the pair walk runs 8 lanes per ball, above it one (the
+    40k-ball scene)."""
+    if name == "random":
+        s = random_scene()
+    elif name == "random_40k":
+        s = random_scene(n=40000, seed=11, box=0.8)
+    else:
+        s = scenes.granular_collapse()
     o = oracle.OracleDEM(s, solve_begin=False)
     g = ts.DEMSceneGPU(s, solve_begin=False)
     o.neighbor_search()

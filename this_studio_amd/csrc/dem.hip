@@ -95,41 +95,84 @@ __global__ __launch_bounds__(kT) void k_reorder(ReorderArgs a)
 // index of A below that of B, not both fixed, not one clump, within
 // 1.1 (rA + rB).  Visit order = the reference's write order.
 template <class F>
-__device__ __forceinline__ void walk_pairs(const Balls& b, const GridGeom& g, const int* cs, const int* ce, int A,
-                                           F&& visit)
+__device__ __forceinline__ void walk_pair_rows(const Balls& b, const GridGeom& g, const int* cs, const int* ce, int A,
+                                               int q0, int q1, F&& visit)
 {
     const double xa = b.px[A], ya = b.py[A], za = b.pz[A], ra = b.r[A];
     const int oa = b.orig[A];
     const bool fa = b.im[A] == 0.0;
     const int ca = b.clump[A];
     const Stencil st = make_stencil(g, xa, ya, za);
-    for (int zz = st.loz; zz <= st.hiz; zz++)
-        for (int yy = st.loy; yy <= st.hiy; yy++) {
-            const int row = (st.cz + zz) * g.ny * g.nx + (st.cy + yy) * g.nx + st.cx;
-            for (int xx = st.lox; xx <= st.hix; xx++) {
-                const int hb = row + xx;
-                const int s0 = cs[hb];
-                if (s0 == -1) continue;
-                const int s1 = ce[hb];
-                for (int j = s0; j < s1; j++) {
-                    if (oa >= b.orig[j]) continue;
-                    if (fa && b.im[j] == 0.0) continue;
-                    const int cb = b.clump[j];
-                    if (cb >= 0 && ca == cb) continue;
-                    const double cut = 1.1 * (ra + b.r[j]);
-                    if (within_cut(cut, xa - b.px[j], ya - b.py[j], za - b.pz[j])) visit(j);
-                }
+    const int nyr = st.hiy - st.loy + 1;
+    const int nrows = (st.hiz - st.loz + 1) * nyr;
+    const int qe = min(q1, nrows);
+    for (int q = q0; q < qe; q++) {
+        const int zz = st.loz + q / nyr, yy = st.loy + q % nyr;
+        const int row = (st.cz + zz) * g.ny * g.nx + (st.cy + yy) * g.nx + st.cx;
+        for (int xx = st.lox; xx <= st.hix; xx++) {
+            const int hb = row + xx;
+            const int s0 = cs[hb];
+            if (s0 == -1) continue;
+            const int s1 = ce[hb];
+            for (int j = s0; j < s1; j++) {
+                if (oa >= b.orig[j]) continue;
+                if (fa && b.im[j] == 0.0) continue;
+                const int cb = b.clump[j];
+                if (cb >= 0 && ca == cb) continue;
+                const double cut = 1.1 * (ra + b.r[j]);
+                if (within_cut(cut, xa - b.px[j], ya - b.py[j], za - b.pz[j])) visit(j);
             }
         }
+    }
 }
 
+// The pair walk of ball A split over a group of L lanes (L = 1 or 8): with
+// 8, lane q walks stencil row q ((z, y) in the serial walk's order, x cells
+// and slots within a row in order) and the last lane also row 8, so the
+// rows of a group, taken in lane order, give the serial walk's visit order.
+// Up to kPairLanesMaxBalls balls a scene uses 8 lanes per ball (measured on
+// MI355X: C1's 4,096 balls 48 -> 44 us per step); larger scenes have threads
+// enough and use 1 (8 lanes cost C3's 112,768 balls 163 -> 168 us).
+constexpr int kPairLanesMaxBalls = 32768;
+template <int L>
+__device__ __forceinline__ void pair_lane_rows(int q, int& q0, int& q1)
+{
+    q0 = L == 1 ? 0 : q;
+    q1 = (L == 1 || q == L - 1) ? 9 : q + 1;
+}
+template <int L>
+__device__ __forceinline__ int group_sum(int v)
+{
+#pragma unroll
+    for (int m = L / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, L);
+    return v;
+}
+template <int L>
+__device__ __forceinline__ int group_exclusive(int v, int q)
+{
+    int inc = v;
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+        const int t = __shfl_up(inc, d, L);
+        if (q >= d) inc += t;
+    }
+    return inc - v;
+}
+
+// L lanes per ball (whole groups exit together, so the group shuffles see
+// only their own group's lanes).
+template <int L>
 __global__ __launch_bounds__(kT) void k_count_pairs(Balls b, GridGeom g, const int* cs, const int* ce, int* cnt)
 {
-    const int i = xcd_block() * kT + threadIdx.x;
+    const int t = xcd_block() * kT + threadIdx.x;
+    const int i = t / L, q = t % L;
     if (i >= b.n) return;
+    int q0, q1;
+    pair_lane_rows<L>(q, q0, q1);
     int c = 0;
-    walk_pairs(b, g, cs, ce, i, [&](int) { c++; });
-    cnt[i] = c;
+    walk_pair_rows(b, g, cs, ce, i, q0, q1, [&](int) { c++; });
+    c = group_sum<L>(c);
+    if (q == 0) cnt[i] = c;
 }
 
 struct WritePairsArgs {
@@ -145,11 +188,21 @@ struct WritePairsArgs {
 // writeBallInteractionsKernel (ballNeighborSearchKernel.cu:61-151) with the
 // history carried by insertion indices: a new pair (A,B) takes the springs
 // of the old pair (A,B) if it existed, else zero.
+template <int L>
 __global__ __launch_bounds__(kT) void k_write_pairs(WritePairsArgs a)
 {
-    const int i = xcd_block() * kT + threadIdx.x;
+    const int t = xcd_block() * kT + threadIdx.x;
+    const int i = t / L, q = t % L;
     if (i >= a.b.n) return;
+    int q0, q1;
+    pair_lane_rows<L>(q, q0, q1);
+    // this lane's rows start after the pairs of the group's earlier lanes
     int w = a.cur.start[i];
+    if (L > 1) {
+        int c = 0;
+        walk_pair_rows(a.b, a.g, a.cs, a.ce, i, q0, q1, [&](int) { c++; });
+        w += group_exclusive<L>(c, q);
+    }
     const int oa = a.b.orig[i];
     int os0 = 0, os1 = 0;
     if (a.has_old) {
@@ -157,7 +210,7 @@ __global__ __launch_bounds__(kT) void k_write_pairs(WritePairsArgs a)
         os0 = a.old.start[s];
         os1 = a.old.start[s + 1];
     }
-    walk_pairs(a.b, a.g, a.cs, a.ce, i, [&](int j) {
+    walk_pair_rows(a.b, a.g, a.cs, a.ce, i, q0, q1, [&](int j) {
         a.cur.partner[w] = j;
         double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const int ob = a.b.orig[j];
@@ -1457,8 +1510,14 @@ struct demsph_dem {
         const int old_cb = cb;
         cb = 1 - cb;
         Balls nb = cur();
-        DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs, gb, tb, 0, stream, nb, bg, cellStart.p, cellEnd.p,
-                      cnt.p);
+        const bool lanes8 = n <= kPairLanesMaxBalls;
+        const dim3 gl(blocks_for((long long)n * (lanes8 ? 8 : 1), kT));
+        if (lanes8)
+            DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs<8>, gl, tb, 0, stream, nb, bg, cellStart.p,
+                          cellEnd.p, cnt.p);
+        else
+            DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs<1>, gl, tb, 0, stream, nb, bg, cellStart.p,
+                          cellEnd.p, cnt.p);
         // the owner CSR goes into the list that becomes current
         const int np = 1 - cp;
         // start[n] = total, written by the scan
@@ -1476,7 +1535,8 @@ struct demsph_dem {
         wa.old_orig = borig[old_cb].p;
         wa.rank_old = rank_old.p;
         wa.has_old = has_list ? 1 : 0;
-        DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs, gb, tb, 0, stream, wa);
+        if (lanes8) DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<8>, gl, tb, 0, stream, wa);
+        else DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<1>, gl, tb, 0, stream, wa);
         cp = np;
         has_list = true;
         npairs = tot;
