@@ -66,6 +66,8 @@ struct ReorderArgs {
     const uint32_t* skey;
     const int* perm;
     int* rank;
+    int* rank_old;  // insertion index -> slot before this reorder
+    int *rcnt, *rcursor;  // reverse-index counters, cleared here for the build after the pair write
     long long numGrids;
     int *cellStart, *cellEnd;
 };
@@ -76,7 +78,11 @@ __global__ __launch_bounds__(kT) void k_reorder(ReorderArgs a)
     if (k >= a.src.n) return;
     const int o = a.perm[k];
     const int s = a.rank[o];
+    a.rank_old[o] = s;
     a.rank[o] = k;
+    a.rcnt[k] = 0;
+    a.rcursor[k] = 0;
+    if (k == 0) a.rcnt[a.src.n] = 0;
     a.dst.orig[k] = o;
     a.dst.px[k] = a.src.px[s]; a.dst.py[k] = a.src.py[s]; a.dst.pz[k] = a.src.pz[s];
     a.dst.vx[k] = a.src.vx[s]; a.dst.vy[k] = a.src.vy[s]; a.dst.vz[k] = a.src.vz[s];
@@ -1500,11 +1506,11 @@ struct demsph_dem {
         fill_i32(cellStart.p, bng, -1, stream);
         fill_i32(cellEnd.p, bng, -1, stream);
         timer.end(stream);
-        DEMSPH_HIP(hipMemcpyAsync(rank_old.p, rank.p, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
         ReorderArgs ra{};
         ra.src = b;
         ra.dst = balls(1 - cb);
         ra.skey = skey.p; ra.perm = perm.p; ra.rank = rank.p; ra.numGrids = bng;
+        ra.rank_old = rank_old.p; ra.rcnt = rcnt.p; ra.rcursor = rcursor.p;
         ra.cellStart = cellStart.p; ra.cellEnd = cellEnd.p;
         DEMSPH_LAUNCH(timer, "dem_reorder", k_reorder, gb, tb, 0, stream, ra);
         const int old_cb = cb;
@@ -1542,8 +1548,7 @@ struct demsph_dem {
         npairs = tot;
         // reverse index by partner (DEMSolver.h:59-78, reset every build)
         timer.begin("dem_reverse_index", stream);
-        DEMSPH_HIP(hipMemsetAsync(rcnt.p, 0, (n + 1) * sizeof(int), stream));
-        DEMSPH_HIP(hipMemsetAsync(rcursor.p, 0, n * sizeof(int), stream));
+        // rcnt and rcursor were cleared by k_reorder
         hipLaunchKernelGGL(k_rev_count, gb, tb, 0, stream, n, pstart[cp].p, ppartner[cp].p, rcnt.p);
         exclusive_scan(rcnt.p, rstart.p, n + 1, scan_temp, nullptr, stream);
         hipLaunchKernelGGL(k_rev_fill, gb, tb, 0, stream, n, pstart[cp].p, ppartner[cp].p, rstart.p, rcursor.p,
