@@ -272,6 +272,8 @@ void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, 
 
 // Fill n ints with v.
 void fill_i32(int* p, long long n, int v, hipStream_t s);
+// Fills p[0, n) and q[0, n) with v in one launch.
+void fill2_i32(int* p, int* q, long long n, int v, hipStream_t s);
 // p[i] = i.
 void iota_i32(int* p, int n, hipStream_t s);
 // key[orig[s]] = cell hash of SoA point s (orig == nullptr: key[s]).

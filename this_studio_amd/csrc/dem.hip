@@ -1503,8 +1503,7 @@ struct demsph_dem {
         timer.begin("dem_hash_sort", stream);
         hash_soa_by_orig(n, bg, b.px, b.py, b.pz, b.orig, key.p, stream);
         radix_sort_pairs(key.p, nullptr, skey.p, perm.p, ktmp.p, vtmp.p, sort_temp, n, key_bits(bng), stream);
-        fill_i32(cellStart.p, bng, -1, stream);
-        fill_i32(cellEnd.p, bng, -1, stream);
+        fill2_i32(cellStart.p, cellEnd.p, bng, -1, stream);
         timer.end(stream);
         ReorderArgs ra{};
         ra.src = b;
@@ -1622,8 +1621,7 @@ struct demsph_dem {
                 hash_aos(tcc.p, nt, tg, tkey.p, stream);
                 radix_sort_pairs(tkey.p, nullptr, tskey.p, tperm.p, tktmp.p, tvtmp.p, tsort_temp, nt, key_bits(tng),
                                  stream);
-                fill_i32(tcs.p, tng, -1, stream);
-                fill_i32(tce.p, tng, -1, stream);
+                fill2_i32(tcs.p, tce.p, tng, -1, stream);
                 cell_ranges(tskey.p, nt, tng, tcs.p, tce.p, stream);
                 hipLaunchKernelGGL(k_tri_boxes, dim3(blocks_for(nt, 256)), dim3(256), 0, stream, nt, tperm.p, dt0.p,
                                    dt1.p, dt2.p, vgx.p, vgy.p, vgz.p, reinterpret_cast<double2*>(tbox.p));

@@ -329,6 +329,15 @@ __global__ void k_fill_i32(int* p, long long n, int v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill2_i32(int* p, int* q, long long n, int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+        q[i] = v;
+    }
+}
+
 __global__ void k_step_stamp(unsigned long long* buf, int* counter, int cap)
 {
     if (threadIdx.x != 0) return;
@@ -702,6 +711,13 @@ void fill_i32(int* p, long long n, int v, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+    DEMSPH_HIP(hipGetLastError());
+}
+
+void fill2_i32(int* p, int* q, long long n, int v, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill2_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, q, n, v);
     DEMSPH_HIP(hipGetLastError());
 }
 
