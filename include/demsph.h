@@ -392,6 +392,12 @@ int demsph_dem_set(demsph_dem* d, const char* field, const void* in, size_t coun
  * the captures. */
 int demsph_dem_set_graphs(demsph_dem* d, int enable);
 int demsph_dem_graph_captures(demsph_dem* d, long* count);
+/* DEM kernel tuning knob (no counterpart in the reference): "pairLanes",
+ * lanes per ball in the pair-list build: 0 (default: 8 up to 32,768 balls,
+ * else 1), 1, 2, 4 or 8.  Every setting builds the same pair lists in the
+ * same order.  demsph_dem_tune_get reads the value in effect. */
+int demsph_dem_tune(demsph_dem* d, const char* knob, int value);
+int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value);
 int demsph_dem_enable_timing(demsph_dem* d, int enable);
 int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long* launches, double* totalMs);
 

@@ -447,3 +447,22 @@ def test_exported_state_resumes_bit_for_bit(name):
         ts.DEMSceneGPU(scenes.bonded_beam()).import_state(state)
     with pytest.raises(ts.DemsphError):
         ts.DEMSceneGPU(make(), fp64=not fp64).import_state(state)
+
+
+@pytest.mark.parametrize("name", ["collapse", "random_40k"])
+def test_pair_lane_settings_build_identical_lists(name):
+    """Every "pairLanes" setting builds the oracle's pair lists in the
+    oracle's order; the default depends on the ball count."""
+    s = scenes.granular_collapse() if name == "collapse" else random_scene(n=40000, seed=11, box=0.8)
+    o = oracle.OracleDEM(s, solve_begin=False)
+    o.neighbor_search()
+    g = ts.DEMSceneGPU(s, solve_begin=False)
+    assert g.tuned("pairLanes") == (8 if name == "collapse" else 1)
+    for lanes in (1, 2, 4, 8):
+        g.tune("pairLanes", lanes)
+        assert g.tuned("pairLanes") == lanes
+        g.neighbor_search()
+        assert np.array_equal(g.get("objectPointed"), o.get("pointed")), lanes
+        assert np.array_equal(g.get("objectPointing"), o.get("pointing")), lanes
+    with pytest.raises(ts.DemsphError):
+        g.tune("pairLanes", 3)

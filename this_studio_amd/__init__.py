@@ -355,6 +355,8 @@ def _declare_dem(L):
         "demsph_dem_import_state": [_vp, _vp, _sz],
         "demsph_dem_set": [_vp, C.c_char_p, _vp, _sz],
         "demsph_dem_set_graphs": [_vp, C.c_int],
+        "demsph_dem_tune": [_vp, C.c_char_p, C.c_int],
+        "demsph_dem_tune_get": [_vp, C.c_char_p, C.POINTER(C.c_int)],
         "demsph_dem_graph_captures": [_vp, C.POINTER(C.c_long)],
         "demsph_dem_enable_timing": [_vp, C.c_int],
         "demsph_dem_timing": [_vp, C.c_int, C.c_char_p, _sz, C.POINTER(C.c_long), C.POINTER(C.c_double)],
@@ -570,6 +572,15 @@ class DEMSceneGPU:
         """Replay captured four-step graphs between pair rebuilds (on by
         default; needs a scene stream other than the null stream)."""
         check(self.L.demsph_dem_set_graphs(self._h, 1 if on else 0))
+
+    def tune(self, knob: str, value: int) -> None:
+        """Kernel tuning knob ("pairLanes": 0 automatic, 1, 2, 4, 8)."""
+        check(self.L.demsph_dem_tune(self._h, knob.encode(), int(value)))
+
+    def tuned(self, knob: str) -> int:
+        v = C.c_int()
+        check(self.L.demsph_dem_tune_get(self._h, knob.encode(), C.byref(v)))
+        return v.value
 
     def graph_captures(self) -> int:
         n = C.c_long()

@@ -132,19 +132,22 @@ __device__ __forceinline__ void walk_pair_rows(const Balls& b, const GridGeom& g
     }
 }
 
-// The pair walk of ball A split over a group of L lanes (L = 1 or 8): with
-// 8, lane q walks stencil row q ((z, y) in the serial walk's order, x cells
-// and slots within a row in order) and the last lane also row 8, so the
-// rows of a group, taken in lane order, give the serial walk's visit order.
+// The pair walk of ball A split over a group of L lanes (L = 1 or 8): each
+// lane walks a run of stencil rows ((z, y) in the serial walk's order, x
+// cells and slots within a row in order), so the rows of a group, taken in
+// lane order, give the serial walk's visit order.
 // Up to kPairLanesMaxBalls balls a scene uses 8 lanes per ball (measured on
-// MI355X: C1's 4,096 balls 48 -> 44 us per step); larger scenes have threads
-// enough and use 1 (8 lanes cost C3's 112,768 balls 163 -> 168 us).
+// MI355X: C1's 4,096 balls 48 -> 44 us per step; the rebuild kernels 47 ->
+// 22 us); larger scenes have threads enough and use 1 (C3's 112,768 balls,
+// one process: 168 us per step with 1 lane, 170 with 2, 171 with 4, 173
+// with 8).  Lane q walks rows [9q/L, 9(q+1)/L).  "pairLanes" overrides it.
 constexpr int kPairLanesMaxBalls = 32768;
+inline int pair_walk_lanes(int n, int set) { return set > 0 ? set : (n <= kPairLanesMaxBalls ? 8 : 1); }
 template <int L>
 __device__ __forceinline__ void pair_lane_rows(int q, int& q0, int& q1)
 {
-    q0 = L == 1 ? 0 : q;
-    q1 = (L == 1 || q == L - 1) ? 9 : q + 1;
+    q0 = 9 * q / L;
+    q1 = 9 * (q + 1) / L;
 }
 template <int L>
 __device__ __forceinline__ int group_sum(int v)
@@ -1136,6 +1139,7 @@ struct demsph_dem {
     bool walls_static = true;
     void* tsort_temp = nullptr;
     int KW = 16;
+    int pair_lanes_set = 0;  // lanes per ball in the pair-list build; 0: by ball count
     bool record_contacts = false;  // keep every contact of the last evaluation (frame files)
     DBuf<double> crec, prec;       // wall contacts per history slot, ball pairs per pair slot
     DBuf<int> hcnt, htri, hcancel;
@@ -1515,10 +1519,16 @@ struct demsph_dem {
         const int old_cb = cb;
         cb = 1 - cb;
         Balls nb = cur();
-        const bool lanes8 = n <= kPairLanesMaxBalls;
-        const dim3 gl(blocks_for((long long)n * (lanes8 ? 8 : 1), kT));
-        if (lanes8)
+        const int lanes = pair_walk_lanes(n, pair_lanes_set);
+        const dim3 gl(blocks_for((long long)n * lanes, kT));
+        if (lanes == 8)
             DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs<8>, gl, tb, 0, stream, nb, bg, cellStart.p,
+                          cellEnd.p, cnt.p);
+        else if (lanes == 4)
+            DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs<4>, gl, tb, 0, stream, nb, bg, cellStart.p,
+                          cellEnd.p, cnt.p);
+        else if (lanes == 2)
+            DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs<2>, gl, tb, 0, stream, nb, bg, cellStart.p,
                           cellEnd.p, cnt.p);
         else
             DEMSPH_LAUNCH(timer, "dem_count_pairs", k_count_pairs<1>, gl, tb, 0, stream, nb, bg, cellStart.p,
@@ -1540,7 +1550,9 @@ struct demsph_dem {
         wa.old_orig = borig[old_cb].p;
         wa.rank_old = rank_old.p;
         wa.has_old = has_list ? 1 : 0;
-        if (lanes8) DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<8>, gl, tb, 0, stream, wa);
+        if (lanes == 8) DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<8>, gl, tb, 0, stream, wa);
+        else if (lanes == 4) DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<4>, gl, tb, 0, stream, wa);
+        else if (lanes == 2) DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<2>, gl, tb, 0, stream, wa);
         else DEMSPH_LAUNCH(timer, "dem_write_pairs", k_write_pairs<1>, gl, tb, 0, stream, wa);
         cp = np;
         has_list = true;
@@ -2951,6 +2963,28 @@ extern "C" int demsph_dem_set(demsph_dem* d, const char* field, const void* in, 
             return;
         }
         throw InvalidArg("unknown field '" + f + "'");
+    });
+}
+
+extern "C" int demsph_dem_tune(demsph_dem* d, const char* knob, int value)
+{
+    DEM_GUARD({
+        if (!knob) throw InvalidArg("null knob");
+        const std::string k(knob);
+        if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+            throw InvalidArg("pairLanes must be 0 (by ball count), 1, 2, 4 or 8");
+        d->pair_lanes_set = value;
+    });
+}
+
+extern "C" int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value)
+{
+    DEM_GUARD({
+        if (!knob || !value) throw InvalidArg("null argument");
+        const std::string k(knob);
+        if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
+        *value = pair_walk_lanes(d->initialized ? d->n : (int)d->hb.size(), d->pair_lanes_set);
     });
 }
 
