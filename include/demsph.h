@@ -398,6 +398,11 @@ int demsph_dem_graph_captures(demsph_dem* d, long* count);
  * same order.  demsph_dem_tune_get reads the value in effect. */
 int demsph_dem_tune(demsph_dem* d, const char* knob, int value);
 int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value);
+/* Times each "pairLanes" setting (1, 2, 4, 8) over trialSteps >= 5 steps of
+ * gravity g and step dt from the scene's current state, keeps the fastest
+ * and restores the state (no counterpart in the reference).  The scene must
+ * have run solve_begin; *lanes (may be NULL) receives the choice. */
+int demsph_dem_autotune(demsph_dem* d, const double* g, double dt, int trialSteps, int* lanes);
 int demsph_dem_enable_timing(demsph_dem* d, int enable);
 int demsph_dem_timing(demsph_dem* d, int index, char* name, size_t nameCap, long* launches, double* totalMs);
 

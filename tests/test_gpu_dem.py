@@ -466,3 +466,25 @@ def test_pair_lane_settings_build_identical_lists(name):
         assert np.array_equal(g.get("objectPointing"), o.get("pointing")), lanes
     with pytest.raises(ts.DemsphError):
         g.tune("pairLanes", 3)
+
+
+def test_autotune_restores_state_and_keeps_results():
+    """The DEM autotuner times every "pairLanes" setting from a checkpoint
+    taken with the state export, restores the state byte for byte, and the
+    tuned scene continues bit for bit as an untuned one."""
+    s = scenes.granular_collapse()
+    a = ts.DEMSceneGPU(s)
+    b = ts.DEMSceneGPU(s)
+    a.step(12)
+    b.step(12)
+    before = a.export_state()
+    lanes = a.autotune(trial_steps=10)
+    assert lanes in (1, 2, 4, 8) and a.tuned("pairLanes") == lanes
+    assert a.export_state() == before
+    a.step(30)
+    b.step(30)
+    for f in ("position", "velocity", "angularVelocity", "slidingSpring", "objectPointed"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    assert a.clock() == b.clock()
+    with pytest.raises(ts.DemsphError):
+        a.autotune(trial_steps=3)

@@ -357,6 +357,7 @@ def _declare_dem(L):
         "demsph_dem_set_graphs": [_vp, C.c_int],
         "demsph_dem_tune": [_vp, C.c_char_p, C.c_int],
         "demsph_dem_tune_get": [_vp, C.c_char_p, C.POINTER(C.c_int)],
+        "demsph_dem_autotune": [_vp, _dp, C.c_double, C.c_int, C.POINTER(C.c_int)],
         "demsph_dem_graph_captures": [_vp, C.POINTER(C.c_long)],
         "demsph_dem_enable_timing": [_vp, C.c_int],
         "demsph_dem_timing": [_vp, C.c_int, C.c_char_p, _sz, C.POINTER(C.c_long), C.POINTER(C.c_double)],
@@ -580,6 +581,16 @@ class DEMSceneGPU:
     def tuned(self, knob: str) -> int:
         v = C.c_int()
         check(self.L.demsph_dem_tune_get(self._h, knob.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, trial_steps: int = 20, dt: float | None = None, gravity=None) -> int:
+        """Times each "pairLanes" setting from the current state, keeps the
+        fastest and restores the state; returns the lanes chosen."""
+        s = self.scene
+        dt = s.dt if dt is None else dt
+        g = _f64(s.gravity if gravity is None else gravity, 3)
+        v = C.c_int()
+        check(self.L.demsph_dem_autotune(self._h, _d(g), float(dt), int(trial_steps), C.byref(v)))
         return v.value
 
     def graph_captures(self) -> int:

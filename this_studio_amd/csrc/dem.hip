@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstring>
 #include <set>
 #include <utility>
@@ -2985,6 +2986,45 @@ extern "C" int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value)
         const std::string k(knob);
         if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
         *value = pair_walk_lanes(d->initialized ? d->n : (int)d->hb.size(), d->pair_lanes_set);
+    });
+}
+
+// Times every "pairLanes" setting over the same steps from the same state
+// (the scene state export serves as the checkpoint), keeps the fastest and
+// restores the state, so the scene continues as if it had not been tuned.
+extern "C" int demsph_dem_autotune(demsph_dem* d, const double* g, double dt, int trialSteps, int* lanes)
+{
+    DEM_GUARD({
+        d->require_init();
+        if (!d->has_list) throw InvalidArg("solve_begin() must precede autotune");
+        if (!g || !(dt > 0.0)) throw InvalidArg("gravity and a positive time step are required");
+        if (trialSteps < 5) throw InvalidArg("trials need at least 5 steps (one pair rebuild)");
+        const auto ok = [](int rc) {
+            if (rc != DEMSPH_OK) throw InvalidArg(std::string("autotune: ") + demsph_last_error());
+        };
+        size_t bytes = 0;
+        ok(demsph_dem_export_state(d, nullptr, 0, &bytes));
+        std::vector<char> ck(bytes);
+        ok(demsph_dem_export_state(d, ck.data(), ck.size(), &bytes));
+        int best = 0;
+        double best_s = 0.0;
+        for (int L : {1, 2, 4, 8}) {
+            ok(demsph_dem_import_state(d, ck.data(), ck.size()));
+            d->pair_lanes_set = L;
+            d->step(g, dt, 5);  // warm-up: graph capture, first rebuild
+            DEMSPH_HIP(hipStreamSynchronize(d->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            d->step(g, dt, trialSteps);
+            DEMSPH_HIP(hipStreamSynchronize(d->stream));
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (best == 0 || sec < best_s) {
+                best = L;
+                best_s = sec;
+            }
+        }
+        ok(demsph_dem_import_state(d, ck.data(), ck.size()));
+        d->pair_lanes_set = best;
+        if (lanes) *lanes = best;
     });
 }
 
