@@ -41,12 +41,21 @@
  *                         test_linear_collision_restitution_closed_form
  *   Hertz-Mindlin         test_hertz_collision_matches_closed_form
  *   Coulomb cap, rolling  test_sliding_ball_coulomb_cap_and_rolling_transition
- *   parallel bond         test_bonded_beam_tip_deflection_is_euler_bernoulli
+ *   parallel bond         test_bonded_beam_tip_deflection_is_euler_bernoulli (elastic part)
+ *   bond breakage (sigma > sigma_t, tau > C - mu sigma)
+ *                         test_bond_breaks_at_closed_form_evaluation (tension, cohesion with
+ *                         friction, bending, torsion, shear: the breaking evaluation in closed form)
  *   o_sphere_triangle     test_closest_point_on_triangle_brute_force
  *   mesh walls            test_cylinder_mesh_matches_reference_wall_frame (reference wall frame)
- *   clumps                test_rigid_clump_torque_free_rotation
+ *   wall motion (position, orientation, vertices, circumcentres, D3)
+ *                         test_moving_wall_kinematics_closed_form
+ *   clumps                test_rigid_clump_torque_free_rotation,
+ *                         test_clump_force_sum_closed_form (also D1 against the intended sum),
+ *                         test_clump_torque_sum_closed_form
  *   hooks                 test_buoyancy_hook_is_archimedes, test_global_damping_hook_closed_form,
  *                         constant force through the beam's Euler-Bernoulli deflection
+ *   scene builders        test_full_size_cantilever_keeps_every_lattice_bond,
+ *                         test_full_size_ice_sheet_keeps_every_touching_bond (tests/test_scenes.py)
  * Parity unpinned: none of the functions above.  Clump-on-wall contact
  * reuses the pinned ball-wall and clump code paths and has no pin of its own.
  */

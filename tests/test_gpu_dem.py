@@ -488,3 +488,127 @@ def test_autotune_restores_state_and_keeps_results():
     assert a.clock() == b.clock()
     with pytest.raises(ts.DemsphError):
         a.autotune(trial_steps=3)
+
+
+@pytest.mark.parametrize("case", list(scenes.BOND_LOAD_CASES))
+def test_bond_breaks_at_closed_form_evaluation(case):
+    """The GPU bond kernel under constant relative rates (tension, tension
+    against cohesion with friction, bending, torsion, shear): the governing
+    stress grows by inc per evaluation and the bond breaks at the same
+    evaluation as the closed form and the oracle (ParallelBondedContact,
+    contactKernel.cuh:4-62).  Bond state and stresses are FP64 on the GPU."""
+    s, inc, kb, gov = scenes.bonded_pair_load(case)
+    g = ts.DEMSceneGPU(s)
+    o = oracle.OracleDEM(s)
+    assert g.get("bondPointed").tolist() == [0] and g.get("bondPointing").tolist() == [1]
+    tol = 2e-5 if case == "shear" else 1e-9
+    field = "bondMaxNormalStress" if gov == "sigma" else "bondMaxShearStress"
+    for step in range(1, kb // 2 + 3):
+        g.step(1)
+        o.step(1)
+        k = 2 * step
+        val = g.get(field)[0]
+        assert g.get("bondIsBonded")[0] == o.get("bond_isBonded")[0] == (1 if k < kb else 0), (case, step)
+        assert val == pytest.approx(min(k, kb) * inc, rel=tol)
+        assert val == pytest.approx(o.get("bond_maxNormalStress" if gov == "sigma" else "bond_maxShearStress")[0],
+                                    rel=1e-9)
+    g.close()
+    o.close()
+
+
+def test_bonded_strip_breaks_like_oracle():
+    """Progressive fracture: a bonded 24 x 4 strip, clamped at one end and
+    pulled at the other, loses bonds from step ~800 on.  In FP64 the GPU
+    breaks the same bonds at the same steps as the oracle and the strip's
+    pieces stay on the oracle's trajectories."""
+    s = scenes.bonded_strip_pull(force=150.0)
+    g = ts.DEMSceneGPU(s, fp64=True)
+    o = oracle.OracleDEM(s)
+    nb = g.get("bondIsBonded").size
+    broken = 0
+    for blk in range(12):
+        g.step(100)
+        o.step(100)
+        gb, ob = g.get("bondIsBonded"), o.get("bond_isBonded")
+        assert np.array_equal(gb, ob), (blk + 1) * 100
+        broken = nb - int(gb.sum())
+        assert np.abs(g.get("position") - o.get("pos")).max() < 1e-9
+    assert broken >= 8, broken
+    g.close()
+    o.close()
+
+
+def test_moving_wall_kinematics_closed_form():
+    """Wall motion on the GPU (k_wall_move, k_vertex_update, k_circumcenters;
+    wallIntegrationKernel.cu:3-53, a full dt at each of the two evaluations
+    of a step, DEMSolver.h:556-564) against the closed-form pose."""
+    from test_oracle_dem import moving_wall_closed_form
+
+    s = scenes.moving_plate(ball_on_plate=False)
+    g = ts.DEMSceneGPU(s)
+    v, w, N = np.array([0.05, -0.02, 0.01]), np.array([10.0, -20.0, 40.0]), 500
+    g.set("wallVelocity", v)
+    g.set("wallAngularVelocity", w)
+    g.step(N)
+    local = s.objects[1][1]["verts"]
+    p, q, verts = moving_wall_closed_form(v, w, s.dt, N, local)
+    np.testing.assert_allclose(g.get("wallPosition").reshape(-1), p, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(g.get("wallOrientation").reshape(-1), q, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g.get("vertexGlobalPosition").reshape(-1, 3), verts, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(g.get("triangleCircumcenter").reshape(-1, 3), [p, p], rtol=0, atol=1e-13)
+    g.close()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_ball_on_moving_plate_matches_oracle(fp64):
+    """A ball resting on a plate that translates and spins about its normal:
+    friction drags the ball (the wall's rigid velocity at the contact point
+    enters the relative velocity, contactKernel.cu:420-533), and the plate's
+    triangles are re-hashed every evaluation.  GPU against the FP64 oracle
+    after 3,000 steps."""
+    s = scenes.moving_plate(ball_on_plate=True)
+    g = ts.DEMSceneGPU(s, fp64=fp64)
+    o = oracle.OracleDEM(s)
+    v, w = np.array([0.2, 0.0, 0.0]), np.array([0.0, 0.0, 3.0])
+    for obj in (g, o):
+        obj.set("wallVelocity" if obj is g else "wall_vel", v)
+        obj.set("wallAngularVelocity" if obj is g else "wall_angVel", w)
+    g.step(3000)
+    o.step(3000)
+    gp, op = g.get("position")[0], o.get("pos")[0]
+    # the ball was dragged along: it moved about a millimetre in x and y
+    assert np.linalg.norm(op[:2] - [0.03, 0.02]) > 5e-4
+    d = 0.02
+    assert np.abs(gp - op).max() / d < (1e-10 if fp64 else 1e-5)
+    assert np.abs(g.get("velocity") - o.get("vel")).max() < (1e-8 if fp64 else 1e-4)
+    np.testing.assert_allclose(g.get("wallPosition"), o.get("wall_pos").reshape(-1, 3), rtol=0, atol=1e-14)
+    g.close()
+    o.close()
+
+
+def test_clump_sums_closed_form():
+    """The clump force and torque sums (sumClumpForceTorqueKernel,
+    ballIntegrationKernel.cu:3-28, intended form) on the GPU: four equal
+    pebble forces translate the clump at 4f/M exactly; an opposed pair of
+    pebble forces turns it at omega = dt T / I after one step."""
+    s = scenes.pushed_clump("force", f=2.0)
+    N, dt, M = 200, s.dt, 0.05
+    g = ts.DEMSceneGPU(s)
+    g.step(N)
+    acc = 4 * 2.0 / M
+    assert g.get("clumpVelocity")[0] == pytest.approx([acc * N * dt, 0, 0], rel=1e-12, abs=1e-15)
+    assert g.get("clumpPosition")[0] == pytest.approx([acc * (N * dt) ** 2 / 2, 0, 0], rel=1e-12, abs=1e-15)
+    assert np.abs(g.get("clumpAngularVelocity")).max() < 1e-12
+    g.close()
+    s = scenes.pushed_clump("torque", f=2.0)
+    g = ts.DEMSceneGPU(s)
+    o = oracle.OracleDEM(s)
+    g.step(1)
+    o.step(1)
+    pts = s.objects[0][1]["points"]
+    T = np.cross(pts[0] - pts[1], [0.0, 0.0, 2.0])
+    np.testing.assert_allclose(g.get("clumpAngularVelocity")[0], dt * T / 2e-5, rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(g.get("clumpAngularVelocity")[0], o.get("clump_angVel")[0], rtol=1e-9, atol=1e-15)
+    assert not g.get("clumpVelocity").any()
+    g.close()
+    o.close()

@@ -288,3 +288,123 @@ def test_global_damping_hook_closed_form():
     v = fz / m * dt / 2 + (2 * N - 1) * fz * (1 - c) / m * dt / 2
     assert o.get("vel")[0, 2] == pytest.approx(v, rel=1e-12)
     assert o.get("force")[0, 2] == pytest.approx(fz * (1 - c), rel=1e-12)
+
+
+def _bond_stresses(o):
+    return o.get("bond_maxNormalStress")[0], o.get("bond_maxShearStress")[0]
+
+
+@pytest.mark.parametrize("case", list(scenes.BOND_LOAD_CASES))
+def test_bond_breaks_at_closed_form_evaluation(case):
+    """ParallelBondedContact (contactKernel.cuh:4-62) under a constant
+    relative rate: the governing stress grows by inc per evaluation (two per
+    step, dt/2 each) and the bond breaks at the first evaluation whose
+    stress exceeds the threshold: sigma > sigma_t, or tau > C - mu sigma.
+    The stresses keep their values of the breaking evaluation; the next
+    evaluation zeroes the forces (contactKernel.cu:577-584).  Swapping sigma
+    and tau, dropping mu sigma, or using I for J or A changes the step."""
+    s, inc, kb, gov = scenes.bonded_pair_load(case)
+    o = oracle.OracleDEM(s)
+    assert o.get("bond_pointed").tolist() == [0] and o.get("bond_pointing").tolist() == [1]
+    tol = 2e-5 if case == "shear" else 1e-9
+    for step in range(1, kb // 2 + 3):
+        o.step(1)
+        k = 2 * step
+        sig, tau = _bond_stresses(o)
+        g, other = (sig, tau) if gov == "sigma" else (tau, sig)
+        if k < kb:
+            assert o.get("bond_isBonded")[0] == 1, (case, step)
+            assert g == pytest.approx(k * inc, rel=tol)
+        else:
+            assert o.get("bond_isBonded")[0] == 0, (case, step)
+            assert g == pytest.approx(kb * inc, rel=tol)
+            assert o.get("bond_normalForce")[0] == 0.0 and not o.get("bond_shearForce").any()
+        if case != "shear":
+            assert other == 0.0
+    o.close()
+
+
+def _rotation(axis, angle):
+    k = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+def moving_wall_closed_form(v, w, dt, steps, local):
+    """Pose of a wall moving at fixed v and omega from the origin at rest
+    orientation: wallPositionIntegrationKernel / wallOrientationIntegrationKernel
+    (wallIntegrationKernel.cu:3-23) run with the full dt at each of the two
+    force evaluations of a step (DEMSolver.h:556-564), so 2 N updates.  Each
+    orientation update q <- normalize(q + q (0, omega dt / 2)) is a rotation
+    by 2 atan(|omega| dt / 2) about omega; about one fixed axis they add up.
+    Returns (position, quaternion, global vertices)."""
+    m = 2 * steps
+    p = m * dt * np.asarray(v, float)
+    ang = m * 2.0 * math.atan(np.linalg.norm(w) * dt / 2.0)
+    k = np.asarray(w, float) / np.linalg.norm(w)
+    q = np.concatenate([[math.cos(ang / 2)], math.sin(ang / 2) * k])
+    return p, q, p + local @ _rotation(k, ang).T
+
+
+def test_moving_wall_kinematics_closed_form():
+    """A wall translating and rotating at set rates lands at the closed-form
+    pose, vertex positions (updateWallVertexGlobalPosition,
+    wallIntegrationKernel.cu:25-37) and circumcentres (the plate's two right
+    triangles have theirs at the plate centre, the wall position)."""
+    s = scenes.moving_plate(ball_on_plate=False)
+    o = oracle.OracleDEM(s)
+    v, w, N = np.array([0.05, -0.02, 0.01]), np.array([10.0, -20.0, 40.0]), 500
+    o.set("wall_vel", v)
+    o.set("wall_angVel", w)
+    o.step(N)
+    local = s.objects[1][1]["verts"]
+    p, q, verts = moving_wall_closed_form(v, w, s.dt, N, local)
+    np.testing.assert_allclose(o.get("wall_pos").reshape(-1), p, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(o.get("wall_q").reshape(-1), q, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(o.get("vertex_global").reshape(-1, 3), verts, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(o.get("tri_circumcenter").reshape(-1, 3), [p, p], rtol=0, atol=1e-13)
+    # a full dt per evaluation (D3): the wall has moved 2 N dt v, not N dt v
+    assert abs(o.get("wall_pos").reshape(-1)[0] - N * s.dt * v[0]) > 1e-4
+    o.close()
+
+
+def test_clump_force_sum_closed_form():
+    """sumClumpForceTorqueKernel (ballIntegrationKernel.cu:3-28), intended
+    form: the clump force is the sum of its pebbles' forces.  Four equal hook
+    forces f on a symmetric clump give zero torque and a constant
+    acceleration 4f/M, which kick-drift-kick integrates exactly.  The
+    reference's defective read of the clump array (D1, policy_d1 = 1) never
+    moves the clump: its pebble 0 reads the clump force of the previous
+    evaluation, zero at the start, and the other pebbles read nothing."""
+    s = scenes.pushed_clump("force", f=2.0)
+    N, dt, M = 200, s.dt, 0.05
+    o = oracle.OracleDEM(s)
+    o.step(N)
+    acc = 4 * 2.0 / M
+    assert o.get("clump_vel")[0] == pytest.approx([acc * N * dt, 0, 0], rel=1e-12, abs=1e-15)
+    assert o.get("clump_pos")[0] == pytest.approx([acc * (N * dt) ** 2 / 2, 0, 0], rel=1e-12, abs=1e-15)
+    assert np.abs(o.get("clump_angVel")).max() < 1e-12
+    assert o.get("clump_force")[0] == pytest.approx([8.0, 0, 0], rel=1e-14)
+    # the pebbles move with the clump
+    pts = s.objects[0][1]["points"]
+    np.testing.assert_allclose(o.get("pos"), pts + [acc * (N * dt) ** 2 / 2, 0, 0], rtol=0, atol=1e-15)
+    o.close()
+    d1 = oracle.OracleDEM(s, policy_d1=1)
+    d1.step(N)
+    assert not d1.get("clump_vel").any()
+    d1.close()
+
+
+def test_clump_torque_sum_closed_form():
+    """The clump torque is the sum of pebble torques plus (x_p - x_c) x F_p:
+    +f z on pebble 0 and -f z on pebble 1 give no net force and the torque
+    (x_0 - x_1) x f z, so after one step omega = dt T / I for the isotropic
+    inertia, to O(|omega| dt) from the pebbles' turn within the step."""
+    s = scenes.pushed_clump("torque", f=2.0)
+    o = oracle.OracleDEM(s)
+    o.step(1)
+    pts = s.objects[0][1]["points"]
+    T = np.cross(pts[0] - pts[1], [0.0, 0.0, 2.0])
+    np.testing.assert_allclose(o.get("clump_angVel")[0], s.dt * T / 2e-5, rtol=1e-6, atol=1e-12)
+    assert not o.get("clump_vel").any()
+    o.close()

@@ -374,7 +374,7 @@ DEM_INT_FIELDS = {"objectPointed", "objectPointing", "triPointed", "triPointing"
 DEM_VEC_FIELDS = {"position", "velocity", "angularVelocity", "force", "torque", "slidingSpring", "rollingSpring",
                   "torsionSpring", "triSlidingSpring", "triRollingSpring", "triTorsionSpring", "bondShearForce", "bondBendingTorque", "bondPoint",
                   "bondNormal", "clumpPosition", "clumpVelocity", "clumpAngularVelocity", "wallPosition",
-                  "vertexGlobalPosition"}
+                  "vertexGlobalPosition", "triangleCircumcenter"}
 
 
 def _i32(a):

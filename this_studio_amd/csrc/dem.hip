@@ -2500,6 +2500,8 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
         if (f == "bondIsBonded") return emit(fetch(d->dbonded.p, nb, st), out, capacity, count);
         if (f == "bondNormalForce") return emit(fetch(d->dbFn.p, nb, st), out, capacity, count);
         if (f == "bondTorsionTorque") return emit(fetch(d->dbTt.p, nb, st), out, capacity, count);
+        if (f == "bondMaxNormalStress") return emit(fetch(d->dbsig.p, nb, st), out, capacity, count);
+        if (f == "bondMaxShearStress") return emit(fetch(d->dbtau.p, nb, st), out, capacity, count);
         if (f == "bondShearForce") return emit(plain3(d->dbsx, d->dbsy, d->dbsz, nb), out, capacity, count);
         if (f == "bondBendingTorque") return emit(plain3(d->dbtx, d->dbty, d->dbtz, nb), out, capacity, count);
         if (f == "bondPoint") return emit(plain3(d->dbpx, d->dbpy, d->dbpz, nb), out, capacity, count);
@@ -2512,6 +2514,7 @@ extern "C" int demsph_dem_get(demsph_dem* d, const char* field, void* out, size_
         if (f == "wallPosition") return emit(plain3(d->wpx, d->wpy, d->wpz, d->nw), out, capacity, count);
         if (f == "wallOrientation") return emit(plain4(d->wq0, d->wq1, d->wq2, d->wq3, d->nw), out, capacity, count);
         if (f == "vertexGlobalPosition") return emit(plain3(d->vgx, d->vgy, d->vgz, d->nv), out, capacity, count);
+        if (f == "triangleCircumcenter") return emit(fetch(d->tcc.p, 3 * (size_t)d->nt, st), out, capacity, count);
         throw InvalidArg("unknown field '" + f + "'");
     });
 }

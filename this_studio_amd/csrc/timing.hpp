@@ -29,20 +29,38 @@ public:
         if (!enabled || pending_.empty()) return;
         DEMSPH_HIP(hipEventRecord(pending_.back().b, s));
     }
-    // Converts recorded event pairs into totals; the stream must be idle.
+    // Converts recorded event pairs into totals.  Each event is waited on
+    // before it is read: a stream synchronisation alone has been seen to
+    // leave an event's timestamp not yet readable ("device not ready") while
+    // other host threads share the hardware queues.  A pair that still cannot
+    // be read is counted in dropped() and its events are destroyed, not
+    // pooled; timing never fails the work it measures, and the pending list
+    // is taken over before the loop so that no event is both pending and
+    // pooled.
     void harvest()
     {
-        for (auto& r : pending_) {
+        std::vector<Rec> recs;
+        recs.swap(pending_);
+        for (auto& r : recs) {
             float ms = 0.f;
-            DEMSPH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-            auto& t = totals_[r.name];
-            t.first += 1;
-            t.second += ms;
-            pool_.push_back(r.a);
-            pool_.push_back(r.b);
+            hipError_t e = hipEventSynchronize(r.b);
+            if (e == hipSuccess) e = hipEventSynchronize(r.a);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+            if (e == hipSuccess) {
+                auto& t = totals_[r.name];
+                t.first += 1;
+                t.second += ms;
+                pool_.push_back(r.a);
+                pool_.push_back(r.b);
+            } else {
+                (void)hipGetLastError();
+                dropped_++;
+                (void)hipEventDestroy(r.a);
+                (void)hipEventDestroy(r.b);
+            }
         }
-        pending_.clear();
     }
+    long dropped() const { return dropped_; }
     void reset()
     {
         totals_.clear();
@@ -78,6 +96,7 @@ private:
     }
     std::vector<Rec> pending_;
     std::vector<hipEvent_t> pool_;
+    long dropped_ = 0;
     std::map<std::string, std::pair<long, double>> totals_;
 };
 

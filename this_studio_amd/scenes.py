@@ -455,6 +455,21 @@ def granular_collapse(n_side: int = 16, seed: int = 20261202) -> DEMScene:
 # ---------------------------------------------------------------------------
 # Coupled DEM-SPH scenes
 # ---------------------------------------------------------------------------
+def half_cell_bounds(min_bound, max_bound, cell: float) -> tuple[np.ndarray, np.ndarray]:
+    """Grows max_bound so that every axis spans a whole number m of cells
+    plus one half.  The grid then takes gridSize = m and the cell size
+    length / m = cell (1 + 1/(2m)) (boundary.h:85-91), strictly above `cell`
+    by far more than rounding: two balls at most `cell` apart along an axis
+    land at most one cell apart there, wherever the lattice sits relative to
+    the cell faces.  Without the margin, a lattice of spacing equal to the
+    cell size can put touching neighbours two cells apart through the
+    rounding of (x - min) / cellSize, out of each other's 27-cell stencil."""
+    mn = np.asarray(min_bound, dtype=np.float64)
+    mx = np.asarray(max_bound, dtype=np.float64)
+    m = np.maximum(1.0, np.ceil((mx - mn) / cell - 0.5))
+    return mn.copy(), mn + (m + 0.5) * cell
+
+
 def lattice(nx: int, ny: int, nz: int, dx: float, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
     """Simple cubic lattice, x fastest."""
     z, y, x = np.meshgrid(np.arange(nz) * dx, np.arange(ny) * dx, np.arange(nx) * dx, indexing="ij")
@@ -669,8 +684,8 @@ def cantilever_with_clumps(nx: int = 128, ny: int = 16, nz: int = 16, r: float =
     s.hooks = [("const", f, ramp_steps * dt)]
     s.bonds_from_initial_pairs = True
     top = z0 + (n_clumps // (cx * cy) + 1) * pitch
-    s.min_bound = np.array([-4.0 * r, -4.0 * r, -0.5])
-    s.max_bound = np.array([nx * 2.0 * r + 4.0 * r, ny * 2.0 * r + 4.0 * r, top])
+    s.min_bound, s.max_bound = half_cell_bounds([-4.0 * r, -4.0 * r, -0.5],
+                                                [nx * 2.0 * r + 4.0 * r, ny * 2.0 * r + 4.0 * r, top], 2.0 * r)
     s.gravity = np.array([0.0, 0.0, -9.81])
     s.dt = dt
     return s
@@ -696,7 +711,8 @@ def ice_cylinder_coupled(nx: int = 500, ny: int = 200, nz: int = 10, dx: float =
     * The cylinder is the only DEM wall: the triangle grid takes its cell from the largest triangle
       (DEMSolver.h:432-449), so a tank box of a few large triangles would make every ball test every facet.
 
-    dt = 5e-6 s.  Returns (fluid spec, DEM scene), sharing the fluid's bounds.
+    dt = 5e-6 s.  Returns (fluid spec, DEM scene); the DEM grid spans the fluid's bounds (see
+    half_cell_bounds).
     """
     g, rho0, dt = 9.81, 1000.0, 5e-6
     Lx, Ly = nx * dx, ny * dx
@@ -746,7 +762,10 @@ def ice_cylinder_coupled(nx: int = 500, ny: int = 200, nz: int = 10, dx: float =
                                   caps=False)
     d.add_mesh_wall(v, t0, t1, t2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 1)
     d.bonds_from_initial_pairs = True
-    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
+    # the ball grid spans the tank with half a cell to spare per axis, so the
+    # hex layers' touching neighbours never round two cells apart
+    d.min_bound, d.max_bound = half_cell_bounds(spec.min_bound, spec.max_bound, 2.0 * r)
+    d.gravity = spec.gravity
     d.dt = dt
     return spec, d
 
@@ -774,3 +793,139 @@ def two_phase_tank(eos: int = 0) -> SPHSceneSpec:
     spec.fluid_nu = np.concatenate([np.full(na, 1e-6), np.full(nb, 1e-3)])
     spec.fluid_c0 = np.full(na + nb, c0)
     return spec
+
+
+# ---------------------------------------------------------------------------
+# Closed-form scenes for bond breakage, moving walls and clump force sums
+# ---------------------------------------------------------------------------
+# Rates of the bonded-pair load cases: per force evaluation (spring time
+# dt/2) the governing stress grows by inc = 5e4 Pa; the break evaluation is
+# the first k with k * inc above the threshold.
+BOND_LOAD_CASES = {
+    # name: (v0 of ball 0, omega0 of ball 0, sigma_t / inc, C / inc, mu, governing stress)
+    "tension": ((0.1, 0.0, 0.0), (0.0, 0.0, 0.0), 10.5, 0.0, 0.0, "sigma"),
+    "cohesion": ((0.1, 0.0, 0.0), (0.0, 0.0, 0.0), 0.0, 25.0, 2.0, "sigma"),
+    "bending": ((0.0, 0.0, 0.0), (0.0, 0.0, 10.0), 8.5, 0.0, 0.0, "sigma"),
+    "torsion": ((0.0, 0.0, 0.0), (26.0, 0.0, 0.0), 0.0, 10.5, 0.0, "tau"),
+    "shear": ((0.0, 0.26, 0.0), (0.0, 0.0, 0.0), 0.0, 6.5, 0.0, "tau"),
+}
+
+
+def bonded_pair_load(case: str, dt: float = 1e-5):
+    """Two touching bonded balls driven at constant relative rates until the
+    bond breaks (ParallelBondedContact, contactKernel.cuh:4-62).
+
+    * Balls of r = 1 cm at x = +-r, ball 0 at +r, so the bond normal is +x.
+      Ball 0 moves with (v0, omega0), ball 1 with (-v0, -omega0).  A density of
+      1e15 kg/m^3 keeps the rates fixed: the bond changes them by about 1e-11
+      relative over the test.  No gravity, no contact rows (zero contact force,
+      the reference's Hertz defaults), bonds from the initial pairs.
+    * Bond: E = 1e9 Pa, gamma = 1 (R_b = r, A = pi r^2, I = pi r^4 / 4, J = 2 I),
+      kn/ks = 2.6, so kn'' = E / 2r = 5e10 Pa/m and ks'' = kn'' / 2.6.
+    * Each force evaluation integrates the bond over dt/2, so the governing
+      stress after k evaluations is k * inc with inc = 5e4 Pa:
+        tension   sigma = 2 v (dt/2) kn''           (normal separation)
+        cohesion  the same motion with sigma_t = 0: tau = 0 exceeds C - mu sigma
+                  once sigma > C / mu
+        bending   sigma = 2 w (dt/2) kn'' R_b       (omega about z, across the bond)
+        torsion   tau = 2 w (dt/2) ks'' R_b         (omega about the bond normal)
+        shear     tau = 2 v (dt/2) ks''             (sliding along y; the normal turns
+                  by about 1e-3 rad, a 1e-6 relative change)
+
+    Returns (scene, inc, k_break, governing "sigma" or "tau")."""
+    v0, w0, st, c, mu, gov = BOND_LOAD_CASES[case]
+    r, E, knks = 0.01, 1e9, 2.6
+    h = 0.5 * dt
+    kn = E / (2 * r)
+    ks = kn / knks
+    inc = {"tension": 2 * v0[0] * h * kn, "cohesion": 2 * v0[0] * h * kn, "bending": 2 * w0[2] * h * kn * r,
+           "torsion": 2 * w0[0] * h * ks * r, "shear": 2 * v0[1] * h * ks}[case]
+    thresh = st if st > 0 else c / mu if mu > 0 else c
+    k_break = int(math.floor(thresh)) + 1
+    s = DEMScene(f"bond_{case}")
+    s.set_bonded_pair(0, 0, 1.0, E, knks, st * inc, c * inc, mu)
+    s.add_balls([[r, 0.0, 0.0], [-r, 0.0, 0.0]], [v0, [-x for x in v0]], [w0, [-x for x in w0]], r, 1e15, 0)
+    s.bonds_from_initial_pairs = True
+    s.min_bound, s.max_bound = -0.1 * np.ones(3), 0.1 * np.ones(3)
+    s.gravity = np.zeros(3)
+    s.dt = dt
+    return s, inc, k_break, gov
+
+
+def bonded_strip_pull(nx: int = 24, ny: int = 4, r: float = 0.005, force: float = 40.0, dt: float = 2e-6):
+    """A bonded nx x ny strip of touching balls in the z = 0 plane, its x = 0
+    column fixed, pulled at its free end by a constant force (split over the
+    end column) in +x and -y, so that bonds break progressively near the
+    root under combined tension, bending and shear.  E = 1e8 Pa, kn/ks = 2.6,
+    sigma_t = C = 2e6 Pa, mu = 0.3; linear contact rows (kn = 1e5 N/m)
+    between the balls.  No gravity.  The GPU and the oracle should break the
+    same bonds at the same steps in FP64."""
+    s = DEMScene("bonded_strip_pull")
+    s.set_bonded_pair(0, 0, 1.0, 1e8, 2.6, 2e6, 2e6, 0.3)
+    s.set_linear_pair(0, 0, 1e5, 1e5 / 2.6, 0.0, 0.0, 0.1, 0.1, 0.0, 0.0, 0.3, 0.0, 0.0)
+    pts = lattice(nx, ny, 1, 2.0 * r)
+    fixed = pts[:, 0] < 0.5 * r
+    s.add_fixed_balls(pts[fixed], r, 0)
+    s.add_balls(pts[~fixed], [0, 0, 0], [0, 0, 0], r, 2500.0, 0)
+    n = pts.shape[0]
+    f = np.zeros((n, 3))
+    nfix = int(fixed.sum())
+    end = np.flatnonzero(pts[~fixed][:, 0] > (nx - 1.5) * 2.0 * r)
+    f[nfix + end] = np.array([force, -0.5 * force, 0.0]) / end.size
+    s.hooks = [("const", f, 0.0)]
+    s.bonds_from_initial_pairs = True
+    s.min_bound, s.max_bound = half_cell_bounds([-4.0 * r, -0.5, -4.0 * r], [0.5, 0.5, 4.0 * r], 2.0 * r)
+    s.gravity = np.zeros(3)
+    s.dt = dt
+    return s
+
+
+def moving_plate(ball_on_plate: bool = True, half: float = 0.1, r: float = 0.01):
+    """A square plate of two triangles (side 2 half, in the z = 0 plane of
+    its wall frame, wall position at the origin), with one ball.  The wall's
+    velocity and angular velocity are set after initialisation (addMeshWall
+    stores zeros, DEMSolver.h:1901-1911).  With ball_on_plate the ball rests
+    on the plate under gravity with Coulomb friction (linear contact), and
+    the moving plate drags it; otherwise the ball floats 0.5 m above it in
+    zero gravity and the wall moves freely."""
+    s = DEMScene("moving_plate")
+    v = np.array([[-half, -half, 0.0], [half, -half, 0.0], [half, half, 0.0], [-half, half, 0.0]])
+    s.set_linear_pair(0, 0, 2e4, 2e4 / 2.6, 0.0, 0.0, 0.2, 0.2, 0.0, 0.0, 0.5, 0.0, 0.0)
+    s.set_linear_pair(0, 1, 2e4, 2e4 / 2.6, 0.0, 0.0, 0.2, 0.2, 0.0, 0.0, 0.5, 0.0, 0.0)
+    z = r - 2e-5 if ball_on_plate else 0.5
+    s.add_balls([[0.03, 0.02, z]], [0, 0, 0], [0, 0, 0], r, 2500.0, 0)
+    s.add_mesh_wall(v, [0, 0], [1, 2], [2, 3], [0, 0, 0], [0, 0, 0], [0, 0, 0], 1)
+    s.min_bound, s.max_bound = np.array([-0.4, -0.4, -0.2]), np.array([0.4, 0.4, 0.8])
+    s.gravity = np.array([0.0, 0.0, -9.81]) if ball_on_plate else np.zeros(3)
+    s.dt = 1e-5
+    return s
+
+
+def pushed_clump(mode: str = "force", f: float = 2.0):
+    """One rigid tetrahedral clump of four pebbles (r = 4 mm, centres 6 mm
+    from the clump centre, mass 0.05 kg, isotropic inertia 2e-5 kg m^2) in
+    zero gravity, pushed by constant hook forces on its pebbles
+    (sumClumpForceTorqueKernel, ballIntegrationKernel.cu:3-28, sums them).
+
+    * mode "force": f along +x on every pebble.  The pebbles sit symmetrically
+      about the clump centre, so the torque is zero and the clump translates:
+      after N steps v = N dt 4f / M and x = (4f / M) (N dt)^2 / 2 (KDK is exact
+      for a constant force).
+    * mode "torque": +f z on pebble 0 and -f z on pebble 1: zero net force and
+      torque T = (p0 - p1) x f z, so omega = 2 (dt/2) T / I after one step, to
+      O(|omega| dt)."""
+    s = DEMScene(f"clump_{mode}")
+    a = 0.006 / math.sqrt(3.0)
+    pts = np.array([[a, a, a], [a, -a, -a], [-a, a, -a], [-a, -a, a]])
+    s.set_linear_pair(0, 0, 1e3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0)
+    s.add_clump(pts, np.full(4, 0.004), [0.0, 0.0, 0.0], [0, 0, 0], [0, 0, 0], 0.05, [2e-5, 2e-5, 2e-5, 0, 0, 0], 0)
+    fo = np.zeros((4, 3))
+    if mode == "force":
+        fo[:, 0] = f
+    else:
+        fo[0, 2], fo[1, 2] = f, -f
+    s.hooks = [("const", fo, 0.0)]
+    s.min_bound, s.max_bound = -np.ones(3), np.ones(3)
+    s.gravity = np.zeros(3)
+    s.dt = 1e-5
+    return s
