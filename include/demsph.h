@@ -53,6 +53,23 @@ int demsph_build_cells(const double* position, int* hashIndex, int* hashValue,
                        const double* cellSize, const int* gridSize,
                        size_t numGrids, size_t numObject, void* stream);
 
+/* demsph_build_cells without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as demsph_build_cells;
+ * the sort scratch is a fixed array inside the library (about 28 MiB per
+ * GPU, held until the process ends), so numObject is limited to
+ * demsph_build_cells_noalloc_max_objects() (2,097,152) and larger inputs
+ * return DEMSPH_ERR_CAPACITY.  The caller passes every input and output.
+ * All calls on one GPU share that scratch: calls that overlap in time on
+ * one GPU (other streams or threads) race and must be serialised. */
+int demsph_build_cells_noalloc(const double* position, int* hashIndex, int* hashValue,
+                               int* cellHashStart, int* cellHashEnd,
+                               const double* minBound, const double* maxBound,
+                               const double* cellSize, const int* gridSize,
+                               size_t numGrids, size_t numObject, void* stream);
+size_t demsph_build_cells_noalloc_max_objects(void);
+
 /* spatialGrid::set (kernel/boundary.h:70-94): computes gridSize and
  * cellSize from the domain and the one-dimensional cell size.  Returns
  * DEMSPH_ERR_INVALID where the reference silently ignores the call. */

@@ -21,7 +21,7 @@ def declared_symbols():
     for h in HEADERS:
         txt = h.read_text()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        names |= set(re.findall(r"^\s*(?:int|const char\*|void)\s+(demsph_\w+|launch\w+)\s*\(", txt, flags=re.M))
+        names |= set(re.findall(r"^\s*(?:int|size_t|const char\*|void)\s+(demsph_\w+|launch\w+)\s*\(", txt, flags=re.M))
     return sorted(names)
 
 
@@ -68,3 +68,13 @@ def test_scene_host_validation_without_gpu():
     s.add_fluid(np.zeros((2, 3)), None, [10.0, -1.0], 0.01, 1000.0, 1e-3)  # c <= 0 is skipped (WCSPHSolver.h:526)
     assert s.sizes()[0] == 1
     s.close()
+
+
+def test_noalloc_build_cells_refuses_oversized_input_on_the_host():
+    """The capacity check runs before any device call, so it needs no GPU."""
+    cap = ts.build_cells_noalloc_max_objects()
+    assert cap == 1 << 21
+    fake = 0x1000  # never dereferenced: the call is refused first
+    with pytest.raises(ts.DemsphError, match="scratch capacity"):
+        ts.build_cells_noalloc(fake, fake, fake, fake, fake, [0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5], [2, 2, 2], 8,
+                               cap + 1)

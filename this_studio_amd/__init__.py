@@ -60,6 +60,7 @@ def _declare(L: C.CDLL) -> None:
     L.demsph_version.restype = C.c_char_p
     sig = {
         "demsph_build_cells": [_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, _ip, _sz, _sz, _vp],
+        "demsph_build_cells_noalloc": [_vp, _vp, _vp, _vp, _vp, _dp, _dp, _dp, _ip, _sz, _sz, _vp],
         "demsph_grid_geometry": [_dp, _dp, C.c_double, _dp, _ip],
         "demsph_sph_create": [C.POINTER(_vp), _vp],
         "demsph_sph_destroy": [_vp],
@@ -94,6 +95,8 @@ def _declare(L: C.CDLL) -> None:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.demsph_build_cells_noalloc_max_objects.argtypes = []
+    L.demsph_build_cells_noalloc_max_objects.restype = C.c_size_t
 
 
 def frames_wait() -> None:
@@ -290,6 +293,23 @@ def build_cells(pos_dev_ptr: int, hash_index_ptr: int, hash_value_ptr: int, cell
     check(lib().demsph_build_cells(_vp(pos_dev_ptr), _vp(hash_index_ptr), _vp(hash_value_ptr), _vp(cell_start_ptr),
                                    _vp(cell_end_ptr), _d(mn), _d(mx), _d(cs), gs.ctypes.data_as(_ip), num_grids, n,
                                    _vp(stream) if stream else None))
+
+
+def build_cells_noalloc(pos_dev_ptr: int, hash_index_ptr: int, hash_value_ptr: int, cell_start_ptr: int,
+                        cell_end_ptr: int, min_bound, max_bound, cell_size, grid_size, num_grids: int, n: int,
+                        stream: int | None = None) -> None:
+    """demsph_build_cells_noalloc: build_cells with its sort scratch in a fixed
+    array inside the library (no device allocation; at most
+    build_cells_noalloc_max_objects() objects; overlapping calls on one GPU race)."""
+    mn, mx, cs = (_f64(x, 3) for x in (min_bound, max_bound, cell_size))
+    gs = np.ascontiguousarray(np.asarray(grid_size, dtype=np.int32))
+    check(lib().demsph_build_cells_noalloc(_vp(pos_dev_ptr), _vp(hash_index_ptr), _vp(hash_value_ptr),
+                                           _vp(cell_start_ptr), _vp(cell_end_ptr), _d(mn), _d(mx), _d(cs),
+                                           gs.ctypes.data_as(_ip), num_grids, n, _vp(stream) if stream else None))
+
+
+def build_cells_noalloc_max_objects() -> int:
+    return int(lib().demsph_build_cells_noalloc_max_objects())
 
 
 def grid_geometry(min_bound, max_bound, cell_one_dim: float):

@@ -397,6 +397,7 @@ void exclusive_scan(const int* in, int* out, int n, void* temp, int* total_dev, 
     DEMSPH_HIP(hipGetLastError());
 }
 
+// The fixed scratch of demsph_build_cells_noalloc repeats this layout.
 size_t radix_sort_temp_bytes(int n)
 {
     const int nb = sort_blocks(n > 0 ? n : 1);
@@ -848,6 +849,87 @@ extern "C" int demsph_build_cells(const double* position, int* hashIndex, int* h
         hash_aos(position, n, g, g_raw.keys, s);
         radix_sort_pairs(g_raw.keys, nullptr, reinterpret_cast<uint32_t*>(hashValue), hashIndex, g_raw.keys_tmp,
                          g_raw.vals_tmp, g_raw.temp, n, key_bits((long long)numGrids), s);
+        cell_ranges(reinterpret_cast<const uint32_t*>(hashValue), n, (long long)numGrids, cellHashStart, cellHashEnd, s);
+    });
+}
+
+// ----------------------------------------------------------------- cells without allocation
+// demsph_build_cells_noalloc runs the same kernels as demsph_build_cells with
+// its sort scratch (hash keys, the ping-pong keys and values, the digit
+// table) in one file-scope device array instead of memory the library
+// allocates.  Every kernel writes the scratch it later reads (the histogram
+// kernel writes every digit of every tile), so nothing is cleared between
+// calls.  Calls that overlap on one GPU share the array and race.
+namespace {
+constexpr size_t kNoAllocMaxObjects = size_t(1) << 21;  // 2,097,152 objects
+constexpr size_t kNoAllocKeyBytes = kNoAllocMaxObjects * sizeof(uint32_t);
+// radix_sort_temp_bytes(kNoAllocMaxObjects): two digit tables of 512 bins
+// x 1,024 tiles, the digit totals and a pad.
+constexpr size_t kNoAllocTempBytes =
+    2 * sizeof(uint32_t) * (size_t)kMaxBins * ((kNoAllocMaxObjects + kTile - 1) / kTile) + sizeof(uint32_t) * kMaxBins +
+    256;
+constexpr size_t kNoAllocBytes = 3 * kNoAllocKeyBytes + kNoAllocTempBytes;  // about 28 MiB per GPU
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocScratch {
+    std::mutex mu;
+    std::map<int, unsigned char*> addr;  // device ordinal -> address of `scratch` there
+};
+NoAllocScratch g_noalloc;
+
+// Address of `scratch` on the current device, looked up once per device.
+unsigned char* noalloc_scratch()
+{
+    int dev = 0;
+    DEMSPH_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc.mu);
+    auto it = g_noalloc.addr.find(dev);
+    if (it != g_noalloc.addr.end()) return it->second;
+    void* p = nullptr;
+    DEMSPH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned a null scratch address");
+    g_noalloc.addr[dev] = static_cast<unsigned char*>(p);
+    return static_cast<unsigned char*>(p);
+}
+}  // namespace
+
+extern "C" size_t demsph_build_cells_noalloc_max_objects(void) { return kNoAllocMaxObjects; }
+
+extern "C" int demsph_build_cells_noalloc(const double* position, int* hashIndex, int* hashValue, int* cellHashStart,
+                                          int* cellHashEnd, const double* minBound, const double* maxBound,
+                                          const double* cellSize, const int* gridSize, size_t numGrids,
+                                          size_t numObject, void* stream)
+{
+    return guarded([&] {
+        if (!minBound || !maxBound || !cellSize || !gridSize) throw InvalidArg("null grid argument");
+        if (numObject > 0 && (!position || !hashIndex || !hashValue)) throw InvalidArg("null particle array");
+        if (numGrids > 0 && (!cellHashStart || !cellHashEnd)) throw InvalidArg("null cell array");
+        if (numGrids > (size_t)INT32_MAX) throw InvalidArg("numGrids exceeds int32 range");
+        if (numObject > kNoAllocMaxObjects)
+            throw CapacityError("demsph_build_cells_noalloc: numObject " + std::to_string(numObject) +
+                                " exceeds the fixed scratch capacity of " + std::to_string(kNoAllocMaxObjects) +
+                                " objects (use demsph_build_cells)");
+        unsigned char* base = noalloc_scratch();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        GridGeom g;
+        g.minx = minBound[0]; g.miny = minBound[1]; g.minz = minBound[2];
+        g.maxx = maxBound[0]; g.maxy = maxBound[1]; g.maxz = maxBound[2];
+        g.cx = cellSize[0]; g.cy = cellSize[1]; g.cz = cellSize[2];
+        g.nx = gridSize[0]; g.ny = gridSize[1]; g.nz = gridSize[2];
+        fill_i32(cellHashStart, (long long)numGrids, -1, s);
+        fill_i32(cellHashEnd, (long long)numGrids, -1, s);
+        const int n = (int)numObject;
+        if (n == 0) return;
+        uint32_t* keys = reinterpret_cast<uint32_t*>(base);
+        uint32_t* keys_tmp = reinterpret_cast<uint32_t*>(base + kNoAllocKeyBytes);
+        int* vals_tmp = reinterpret_cast<int*>(base + 2 * kNoAllocKeyBytes);
+        void* temp = base + 3 * kNoAllocKeyBytes;
+        hash_aos(position, n, g, keys, s);
+        radix_sort_pairs(keys, nullptr, reinterpret_cast<uint32_t*>(hashValue), hashIndex, keys_tmp, vals_tmp, temp, n,
+                         key_bits((long long)numGrids), s);
         cell_ranges(reinterpret_cast<const uint32_t*>(hashValue), n, (long long)numGrids, cellHashStart, cellHashEnd, s);
     });
 }
