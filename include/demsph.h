@@ -275,6 +275,18 @@ int demsph_dem_add_mesh_wall(demsph_dem* d, size_t numVertices, const double* ve
 int demsph_dem_add_bonded_interactions(demsph_dem* d, size_t n, const int* objectPointed,
                                        const int* objectPointing, size_t numPositions, const double* ballPosition);
 
+/* setPeriodicBoundary(X, Y, Z) (DEMSolver.h:1944-1959, ghost passes
+ * :279-426, neighborSearchKernel.cu:29-77), before initialize.  Intended
+ * semantics, not the reference's seven ghost passes (which list some
+ * contacts twice and miss others): on a periodic axis the 27-cell stencil
+ * wraps around the domain, each contact with a periodic image is listed
+ * once in the ball-pair list with its image code (field "pairShift": (sx+1)
+ * + 3 (sy+1) + 9 (sz+1), s = -1, 0, +1 periods; 13 = no image), bonds join
+ * nearest images, and balls that are not clump pebbles are wrapped into
+ * [min, max) at every pair rebuild.  A periodic axis needs at least three
+ * ball cells (DEMSPH_ERR_INVALID at initialize otherwise). */
+int demsph_dem_set_periodic_boundary(demsph_dem* d, int x, int y, int z);
+
 /* Contact force arithmetic: 0 = FP32 forces with FP64 overlap, contact
  * geometry and spring history (default); 1 = FP64 throughout. */
 int demsph_dem_set_precision(demsph_dem* d, int fp64Forces);

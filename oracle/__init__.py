@@ -234,6 +234,8 @@ def _declare_dem(L):
     L.o_dem_hook_damping.argtypes = [_vp, C.c_double]
     L.o_dem_hook_buoyancy.argtypes = [_vp, C.c_double, C.c_double, _dp, _dp, C.c_double]
     L.o_dem_set_policy.argtypes = [_vp, C.c_int]
+    L.o_dem_set_periodic.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.o_dem_set_periodic.restype = None
     L.o_dem_get.argtypes = [_vp, C.c_char_p, _vp]
     L.o_dem_get.restype = C.c_long
     L.o_dem_set.argtypes = [_vp, C.c_char_p, _vp]
@@ -246,7 +248,7 @@ def _declare_dem(L):
     _DEM_DECLARED = True
 
 
-DEM_INT = {"pointed", "pointing", "neighborCount", "tri_pointed", "tri_pointing", "tri_cancel", "bond_pointed",
+DEM_INT = {"pointed", "pointing", "pairShift", "neighborCount", "tri_pointed", "tri_pointing", "tri_cancel", "bond_pointed",
            "bond_pointing", "bond_isBonded", "hash", "index", "cellStart", "cellEnd", "tri_index"}
 DEM_VEC = {"pos", "vel", "angVel", "force", "torque", "sliding", "rolling", "torsion", "contactForce",
            "contactTorque", "contactPoint", "tri_point", "tri_normal", "tri_force", "tri_sliding", "bond_shearForce",
@@ -300,6 +302,8 @@ class OracleDEM:
                 t0, t1, t2 = (np.ascontiguousarray(o[k], dtype=np.int32) for k in ("t0", "t1", "t2"))
                 L.o_dem_add_mesh_wall(self.h, o["verts"].shape[0], d(o["verts"]), t0.size, i(t0), i(t1), i(t2),
                                       d(o["pos"]), d(o["vel"]), d(o["w"]), o["mat"])
+        per = getattr(scene, "periodic", (False, False, False))
+        L.o_dem_set_periodic(self.h, int(per[0]), int(per[1]), int(per[2]))
         if L.o_dem_initialize(self.h, d(_a(scene.min_bound)), d(_a(scene.max_bound))) != 0:
             raise ValueError("oracle DEM initialize failed")
         for h in scene.hooks:
@@ -397,3 +401,75 @@ class OracleCoupled:
         out = np.empty(n, dtype=np.float64)
         self.L.o_couple_get(self.fluid.h, name.encode(), d(out))
         return out if name == "ballPressure" else out.reshape(-1, 3)
+
+
+def reference_periodic_pairs(pos, radius, inv_mass, clump, g: Grid, flag) -> list:
+    """The (pointed, pointing) pairs of one of the reference's periodic ghost
+    passes, restated for small inputs to document what that scheme lists
+    (this oracle's periodic boundaries follow the intended rule instead).
+
+    addDummyContactForceTorque (DEMSolver.h:279-426) for one directionFlag:
+    buildDummyPosition shifts a ball by +L on each flagged axis where it lies
+    within one cell of the minimum face (neighborSearchKernel.cu:59-77);
+    calculateDummyHash hashes a shifted position only if it is out of the
+    domain, clamped as calculateHash does, and gives every other one -1
+    (:29-56); the stable (hash, index) sort and cell ranges follow
+    (buildHashStartEnd.cu:14-76); countBallInteractionsKernel then walks each
+    ball's stencil from its shifted position over those cells with the
+    half-list rule A < B (ballNeighborSearchKernel.cu:6-59)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    mn = np.array([g.minx, g.miny, g.minz])
+    mx = np.array([g.maxx, g.maxy, g.maxz])
+    cs = np.array([g.cx, g.cy, g.cz])
+    ncell = (g.nx, g.ny, g.nz)
+    dom = mx - mn
+    dummy = pos.copy()
+    for a in range(3):
+        if flag[a]:
+            m = pos[:, a] - mn[a] < cs[a]
+            dummy[m, a] += dom[a]
+    hashes = np.full(n, -1, dtype=np.int64)
+    for k in range(n):
+        p = dummy[k].copy()
+        out = False
+        for a in range(3):
+            if p[a] < mn[a]:
+                out, p[a] = True, mn[a]
+            elif p[a] >= mx[a]:
+                out, p[a] = True, mx[a] - 0.5 * cs[a]
+        if out:
+            c = [int((p[a] - mn[a]) / cs[a]) for a in range(3)]
+            hashes[k] = c[2] * ncell[1] * ncell[0] + c[1] * ncell[0] + c[0]
+    order = sorted(range(n), key=lambda k: (hashes[k], k))
+    cells: dict = {}
+    for k in order:
+        if hashes[k] >= 0:
+            cells.setdefault(int(hashes[k]), []).append(k)
+    pairs = []
+    for A in range(n):
+        c, lo, hi = [], [], []
+        for a in range(3):
+            v = int((dummy[A, a] - mn[a]) / cs[a])
+            l, h = -1, 1
+            if v <= 0:
+                v, l = 0, 0
+            if v >= ncell[a] - 1:
+                v, h = ncell[a] - 1, 0
+            c.append(v); lo.append(l); hi.append(h)
+        for zz in range(lo[2], hi[2] + 1):
+            for yy in range(lo[1], hi[1] + 1):
+                for xx in range(lo[0], hi[0] + 1):
+                    hb = (c[2] + zz) * ncell[1] * ncell[0] + (c[1] + yy) * ncell[0] + (c[0] + xx)
+                    for B in cells.get(hb, ()):
+                        if A >= B:
+                            continue
+                        if inv_mass[A] == 0 and inv_mass[B] == 0:
+                            continue
+                        if clump[B] >= 0 and clump[A] == clump[B]:
+                            continue
+                        cut = 1.1 * (radius[A] + radius[B])
+                        r = dummy[A] - dummy[B]
+                        if cut * cut - (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) >= 0.0:
+                            pairs.append((A, B))
+    return pairs

@@ -54,6 +54,10 @@
  *                         test_clump_torque_sum_closed_form
  *   hooks                 test_buoyancy_hook_is_archimedes, test_global_damping_hook_closed_form,
  *                         constant force through the beam's Euler-Bernoulli deflection
+ *   periodic boundaries (o_dem_set_periodic: wrapped stencil, images, wrap, nearest-image bonds)
+ *                         test_periodic_pairs_match_minimum_image_brute_force,
+ *                         test_periodic_sheet_is_translation_invariant,
+ *                         test_rigid_translation_through_the_face (tests/test_oracle_periodic.py)
  *   scene builders        test_full_size_cantilever_keeps_every_lattice_bond,
  *                         test_full_size_ice_sheet_keeps_every_touching_bond (tests/test_scenes.py)
  * Parity unpinned: none of the functions above.  Clump-on-wall contact

@@ -52,6 +52,7 @@ typedef struct {
     long n, cap;          /* numActivated_, device capacity */
     long nold;
     int *pointed, *pointing, *hashIndex, *cancel;
+    int* shift;           /* periodic image code of the pointing object per pair */
     double *sl, *ro, *to; /* springs */
     double *F, *T, *P, *N, *ov;
     int *opointed, *ohashIndex, *ocancel;
@@ -95,6 +96,9 @@ struct o_dem {
     size_t iStep;
     double time;
     int policyD1;
+    /* periodic axes (setPeriodicBoundary, DEMSolver.h:1944-1959) and periods */
+    int per[3];
+    double Lp[3];
     /* hook program */
     int nhook;
     struct { int kind; double c, rho, cd, level, ramp; o_vec3 g, vf; double* f; long nf; } hook[8];
@@ -104,7 +108,7 @@ o_dem* o_dem_create(void) { return (o_dem*)calloc(1, sizeof(o_dem)); }
 
 static void inter_free(inter_t* I)
 {
-    void* p[] = {I->pointed, I->pointing, I->hashIndex, I->cancel, I->sl, I->ro, I->to, I->F, I->T, I->P, I->N, I->ov,
+    void* p[] = {I->pointed, I->pointing, I->hashIndex, I->cancel, I->shift, I->sl, I->ro, I->to, I->F, I->T, I->P, I->N, I->ov,
                  I->opointed, I->ohashIndex, I->ocancel, I->osl, I->oro, I->oto, I->count, I->prefix, I->istart, I->iend};
     for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); k++) free(p[k]);
     memset(I, 0, sizeof(*I));
@@ -325,6 +329,10 @@ int o_dem_add_mesh_wall(o_dem* d, int nv, const double* verts, int nt, const int
 
 /* bondedInteraction::add (DEMSolver.h:93-134): dedup of (min,max) pairs,
  * bond point = midpoint, normal = normalize(p[a]-p[b]), isBonded = 1. */
+static int nearest_image(const o_dem* d, o_vec3 pi, o_vec3 pj);
+static o_vec3 shift_point(const o_dem* d, o_vec3 p, int code);
+static int any_periodic(const o_dem* d);
+
 void o_dem_add_bonds(o_dem* d, int n, const int* ob0, const int* ob1, int npos, const double* pos)
 {
     for (int k = 0; k < n; k++) {
@@ -339,6 +347,8 @@ void o_dem_add_bonds(o_dem* d, int n, const int* ob0, const int* ob1, int npos, 
         if (found) continue;
         ipush(&d->bpointed, a); ipush(&d->bpointing, b); ipush(&d->bonded, 1);
         o_vec3 pa = vld(pos, a), pb = vld(pos, b);
+        /* a bond across a periodic face joins a to the nearest image of b */
+        if (any_periodic(d) && d->bgValid) pb = shift_point(d, pb, nearest_image(d, pa, pb));
         dpush3(&d->bpoint, vmul(vadd(pa, pb), 0.5));
         dpush3(&d->bnormal, vnormalize(vsub(pa, pb)));
         dpush3(&d->bshear, v3(0, 0, 0)); dpush3(&d->bbend, v3(0, 0, 0));
@@ -355,6 +365,7 @@ static void inter_alloc(inter_t* I, long cap, int npointed, int npointing)
     I->pointing = (int*)realloc(I->pointing, sizeof(int) * c);
     I->hashIndex = (int*)realloc(I->hashIndex, sizeof(int) * c);
     I->cancel = (int*)realloc(I->cancel, sizeof(int) * c);
+    I->shift = (int*)realloc(I->shift, sizeof(int) * c);
     I->sl = (double*)realloc(I->sl, sizeof(double) * 3 * c);
     I->ro = (double*)realloc(I->ro, sizeof(double) * 3 * c);
     I->to = (double*)realloc(I->to, sizeof(double) * 3 * c);
@@ -431,6 +442,67 @@ static void inter_build_pointing(inter_t* I)
     free(key); free(sk);
 }
 
+/* ------------------------------------------------------------- periodic boundaries
+ * Intended semantics of setPeriodicBoundary (DEMSolver.h:1944-1959,
+ * :279-426, neighborSearchKernel.cu:29-77): on a periodic axis the 27-cell
+ * stencil wraps around the domain, a ball in the first cell layer meets the
+ * balls of the last layer through their periodic images, and every contact
+ * is listed once (half list by insertion index, as the ball-ball list).
+ * The reference's seven ghost passes list some contacts twice and miss
+ * others (see reference_periodic_pairs in oracle/__init__.py); this oracle
+ * follows the intended rule.  A pair stores the image of its pointing ball
+ * as a code (sx+1) + 3 (sy+1) + 9 (sz+1), s = -1, 0, +1 periods per axis;
+ * 13 = the ball itself.  Balls that are not clump pebbles are wrapped back
+ * into [min, max) on periodic axes at every neighbour search. */
+#define O_NOSHIFT 13
+static double shifted(double x, int s, double L) { return s == 0 ? x : (s > 0 ? x + L : x - L); }
+static o_vec3 shift_point(const o_dem* d, o_vec3 p, int code)
+{
+    if (code == O_NOSHIFT) return p;
+    return v3(shifted(p.x, code % 3 - 1, d->Lp[0]), shifted(p.y, code / 3 % 3 - 1, d->Lp[1]),
+              shifted(p.z, code / 9 - 1, d->Lp[2]));
+}
+static o_vec3 image_pos(const o_dem* d, int j, int code) { return shift_point(d, vld(d->pos.p, j), code); }
+/* Image of ball j nearest to ball i on the periodic axes (bonds). */
+static int nearest_image(const o_dem* d, o_vec3 pi, o_vec3 pj)
+{
+    double dd[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+    int s[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        if (!d->per[a]) continue;
+        if (dd[a] > 0.5 * d->Lp[a]) s[a] = 1;
+        else if (dd[a] < -0.5 * d->Lp[a]) s[a] = -1;
+    }
+    return (s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1);
+}
+static int any_periodic(const o_dem* d) { return d->per[0] || d->per[1] || d->per[2]; }
+
+void o_dem_set_periodic(o_dem* d, int x, int y, int z)
+{
+    d->per[0] = x != 0; d->per[1] = y != 0; d->per[2] = z != 0;
+}
+
+/* A position just below the maximum face can round into the cell past the
+ * last one, (x - min) / cellSize >= n; such a ball takes its image just
+ * below the minimum face instead (which hashes into cell 0), so every
+ * wrapped ball hashes inside the grid. */
+static void wrap_positions(o_dem* d)
+{
+    const double lo[3] = {d->bg.minx, d->bg.miny, d->bg.minz}, hi[3] = {d->bg.maxx, d->bg.maxy, d->bg.maxz};
+    const double cs[3] = {d->bg.cx, d->bg.cy, d->bg.cz};
+    const int nc[3] = {d->bg.nx, d->bg.ny, d->bg.nz};
+    for (long i = 0; i < d->radius.n; i++) {
+        if (d->clump.p[i] >= 0) continue;
+        for (int a = 0; a < 3; a++) {
+            if (!d->per[a]) continue;
+            double* x = &d->pos.p[3 * i + a];
+            if (*x >= hi[a]) *x -= d->Lp[a];
+            else if (*x < lo[a]) *x += d->Lp[a];
+            if ((int)((*x - lo[a]) / cs[a]) >= nc[a]) *x -= d->Lp[a];
+        }
+    }
+}
+
 /* ------------------------------------------------------------- init */
 static void grid_arrays(int** h, int** idx, int** sh, int** cs, int** ce, long n, long ng)
 {
@@ -461,6 +533,12 @@ int o_dem_initialize(o_dem* d, const double* minb, const double* maxb)
     }
     d->tgValid = (o_grid_set(&d->tg, minb, maxb, c) == 0) || d->tgValid;
     d->bng = d->bgValid ? (long)d->bg.nx * d->bg.ny * d->bg.nz : 0;
+    if (d->bgValid) {
+        d->Lp[0] = d->bg.maxx - d->bg.minx; d->Lp[1] = d->bg.maxy - d->bg.miny; d->Lp[2] = d->bg.maxz - d->bg.minz;
+        const int ncell[3] = {d->bg.nx, d->bg.ny, d->bg.nz};
+        for (int a = 0; a < 3; a++)
+            if (d->per[a] && ncell[a] < 3) return -2; /* a wrapped stencil needs three distinct cells */
+    }
     d->tng = d->tgValid ? (long)d->tg.nx * d->tg.ny * d->tg.nz : 0;
     grid_arrays(&d->bhash, &d->bindex, &d->bshash, &d->bcs, &d->bce, nb, d->bng);
     grid_arrays(&d->thash, &d->tindex, &d->tshash, &d->tcs, &d->tce, nt, d->tng);
@@ -477,13 +555,22 @@ static int ball_walk(o_dem* d, int A, int write, long base)
     const o_grid* g = &d->bg;
     int c[3], lo[3], hi[3];
     o_stencil(g, &d->pos.p[3 * (long)A], c, lo, hi);
+    const int ncell[3] = {g->nx, g->ny, g->nz};
+    for (int a = 0; a < 3; a++)
+        if (d->per[a]) { lo[a] = -1; hi[a] = 1; }
     o_vec3 pa = vld(d->pos.p, A);
     double ra = d->radius.p[A];
     int count = 0;
     for (int zz = lo[2]; zz <= hi[2]; zz++)
         for (int yy = lo[1]; yy <= hi[1]; yy++)
             for (int xx = lo[0]; xx <= hi[0]; xx++) {
-                int hb = o_hash_cell(g, c[0] + xx, c[1] + yy, c[2] + zz);
+                int cc[3] = {c[0] + xx, c[1] + yy, c[2] + zz}, sh[3] = {0, 0, 0};
+                for (int a = 0; a < 3; a++) {
+                    if (cc[a] < 0) { cc[a] += ncell[a]; sh[a] = -1; }
+                    else if (cc[a] >= ncell[a]) { cc[a] -= ncell[a]; sh[a] = 1; }
+                }
+                const int code = (sh[0] + 1) + 3 * (sh[1] + 1) + 9 * (sh[2] + 1);
+                int hb = o_hash_cell(g, cc[0], cc[1], cc[2]);
                 int s = d->bcs[hb];
                 if (s == -1) continue;
                 int e = d->bce[hb];
@@ -493,12 +580,13 @@ static int ball_walk(o_dem* d, int A, int write, long base)
                     if (d->invMass.p[A] == 0 && d->invMass.p[B] == 0) continue;
                     if (d->clump.p[B] >= 0 && d->clump.p[A] == d->clump.p[B]) continue;
                     double cut = 1.1 * (ra + d->radius.p[B]);
-                    o_vec3 r = vsub(pa, vld(d->pos.p, B));
+                    o_vec3 r = vsub(pa, image_pos(d, B, code));
                     if ((cut * cut - vdot(r, r)) >= 0.) {
                         if (write) {
                             long w = base + count;
                             d->bb.pointed[w] = A;
                             d->bb.pointing[w] = B;
+                            d->bb.shift[w] = code;
                             carry(&d->bb, A, B, w);
                         }
                         count++;
@@ -513,6 +601,7 @@ void o_dem_neighbor_search(o_dem* d)
 {
     long nb = d->radius.n;
     if (nb == 0) return;
+    if (any_periodic(d)) wrap_positions(d);
     o_hash(&d->bg, d->pos.p, (int)nb, d->bhash);
     o_sort_by_hash(d->bhash, (int)nb, (int)d->bng, d->bshash, d->bindex);
     o_cell_ranges(d->bshash, (int)nb, (int)d->bng, d->bcs, d->bce);
@@ -663,7 +752,7 @@ void o_dem_contact(o_dem* d, double dt)
 #pragma omp parallel for schedule(static)
     for (long k = 0; k < I->n; k++) {
         int i = I->pointed[k], j = I->pointing[k];
-        o_vec3 ri = vld(d->pos.p, i), rj = vld(d->pos.p, j);
+        o_vec3 ri = vld(d->pos.p, i), rj = image_pos(d, j, I->shift[k]);
         double ai = d->radius.p[i], aj = d->radius.p[j];
         o_vec3 rij = vsub(ri, rj);
         o_vec3 n = vnormalize(rij);
@@ -691,6 +780,7 @@ void o_dem_contact(o_dem* d, double dt)
         }
         int i = d->bpointed.p[b], j = d->bpointing.p[b];
         o_vec3 ri = vld(d->pos.p, i), rj = vld(d->pos.p, j);
+        if (any_periodic(d)) rj = image_pos(d, j, nearest_image(d, ri, rj));
         double ai = d->radius.p[i], aj = d->radius.p[j];
         o_vec3 n0 = vld(d->bnormal.p, b);
         o_vec3 n = vnormalize(vsub(ri, rj));
@@ -749,9 +839,11 @@ void o_dem_contact(o_dem* d, double dt)
             for (int k = I->istart[i]; k < I->iend[i]; k++) {
                 int k1 = I->hashIndex[k];
                 o_vec3 Fc = vld(I->F, k1);
+                /* lever arm about the image of i that the pair met */
+                o_vec3 rim = I->shift[k1] == O_NOSHIFT ? ri : image_pos(d, (int)i, I->shift[k1]);
                 F = vsub(F, Fc);
                 T = vsub(T, vld(I->T, k1));
-                T = vsub(T, vcross(vsub(vld(I->P, k1), ri), Fc));
+                T = vsub(T, vcross(vsub(vld(I->P, k1), rim), Fc));
             }
         vst(d->force.p, i, vadd(vld(d->force.p, i), F));
         vst(d->torque.p, i, vadd(vld(d->torque.p, i), T));
@@ -1272,6 +1364,7 @@ long o_dem_get(o_dem* d, const char* name, void* out)
     GD("force", d->force.p, 3 * nb) GD("torque", d->torque.p, 3 * nb) GD("radius", d->radius.p, nb)
     GD("invMass", d->invMass.p, nb)
     GI("pointed", d->bb.pointed, d->bb.n) GI("pointing", d->bb.pointing, d->bb.n)
+    GI("pairShift", d->bb.shift, d->bb.n)
     GD("sliding", d->bb.sl, 3 * d->bb.n) GD("rolling", d->bb.ro, 3 * d->bb.n) GD("torsion", d->bb.to, 3 * d->bb.n)
     GD("overlap", d->bb.ov, d->bb.n) GD("contactForce", d->bb.F, 3 * d->bb.n)
     GD("contactTorque", d->bb.T, 3 * d->bb.n) GD("contactPoint", d->bb.P, 3 * d->bb.n)

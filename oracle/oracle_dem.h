@@ -42,7 +42,13 @@ int o_dem_add_mesh_wall(o_dem* d, int nv, const double* verts, int nt, const int
 /* bondedInteraction::add (DEMSolver.h:93-134). */
 void o_dem_add_bonds(o_dem* d, int n, const int* ob0, const int* ob1, int npos, const double* pos);
 
-/* initialize() (DEMSolver.h:245-276). */
+/* setPeriodicBoundary(X, Y, Z) (DEMSolver.h:1944-1959), intended semantics:
+ * wrapped stencil, periodic images in the ball-pair list, nearest images for
+ * bonds, balls (not clump pebbles) wrapped into the domain at each neighbour
+ * search.  Call before o_dem_initialize. */
+void o_dem_set_periodic(o_dem* d, int x, int y, int z);
+/* initialize() (DEMSolver.h:245-276).  Returns -2 when a periodic axis has
+ * fewer than three cells. */
 int o_dem_initialize(o_dem* d, const double* minb, const double* maxb);
 /* neighborSearch() (DEMSolver.h:698-777). */
 void o_dem_neighbor_search(o_dem* d);

@@ -475,12 +475,16 @@ def test_autotune_restores_state_and_keeps_results():
     s = scenes.granular_collapse()
     a = ts.DEMSceneGPU(s)
     b = ts.DEMSceneGPU(s)
+    a.enable_timing(True)
     a.step(12)
     b.step(12)
+    timed_before = a.timing()
     before = a.export_state()
     lanes = a.autotune(trial_steps=10)
     assert lanes in (1, 2, 4, 8) and a.tuned("pairLanes") == lanes
     assert a.export_state() == before
+    assert a.timing() == timed_before  # the trials' own timings are not kept
+    a.enable_timing(False)
     a.step(30)
     b.step(30)
     for f in ("position", "velocity", "angularVelocity", "slidingSpring", "objectPointed"):
@@ -612,3 +616,87 @@ def test_clump_sums_closed_form():
     assert not g.get("clumpVelocity").any()
     g.close()
     o.close()
+
+
+def _periodic_box(per, seed):
+    rng = np.random.default_rng(seed)
+    s = scenes.DEMScene("periodic_box")
+    box = 0.2
+    pos = rng.uniform(0.0, box, (3000, 3))
+    pos[:20, 0] = box + rng.uniform(0.0, 0.004, 20)  # outside the faces: wrapped at the first search
+    pos[20:40, 1] = -rng.uniform(1e-6, 0.004, 20)
+    rad = rng.uniform(0.004, 0.006, 3000)
+    s.add_balls(pos, rng.normal(scale=0.05, size=(3000, 3)), np.zeros(3), rad, 1000.0, 0)
+    s.set_hertzian_pair(0, 0, 5e6, 1e6, 0.5, 0.1, 0.1, 0.3, 0.05, 0.05)
+    s.set_periodic_boundary(*per)
+    s.min_bound, s.max_bound, s.gravity, s.dt = np.zeros(3), box * np.ones(3), np.zeros(3), 1e-5
+    return s
+
+
+@pytest.mark.parametrize("per,lanes", [((True, True, True), 8), ((True, True, True), 1), ((False, True, False), 4),
+                                       ((True, False, True), 2)])
+def test_periodic_lists_and_steps_match_oracle(per, lanes):
+    """Periodic boundaries (setPeriodicBoundary, DEMSolver.h:1944-1959, with
+    the wrapped stencil): wrapped positions, pair lists with their image
+    codes bit for bit against the oracle, with the pair walk on 1 to 8
+    lanes per ball; then 100 steps (20 rebuilds) of a dense random box."""
+    s = _periodic_box(per, seed=17)
+    o = oracle.OracleDEM(s, solve_begin=False)
+    g = ts.DEMSceneGPU(s, solve_begin=False)
+    g.tune("pairLanes", lanes)
+    o.neighbor_search()
+    g.neighbor_search()
+    assert np.array_equal(g.get("position"), o.get("pos"))  # the wrap is exact
+    for f, of in (("objectPointed", "pointed"), ("objectPointing", "pointing"), ("pairShift", "pairShift")):
+        assert np.array_equal(g.get(f), o.get(of)), f
+    assert (g.get("pairShift") != 13).sum() > 50
+    # a dense random box starts with large overlaps: compare the trajectory
+    # with FP64 forces, where rounding differences stay small
+    o2 = oracle.OracleDEM(s)
+    g2 = ts.DEMSceneGPU(s, fp64=True)
+    g2.tune("pairLanes", lanes)
+    o2.step(100)
+    g2.step(100)
+    assert np.array_equal(g2.get("objectPointing"), o2.get("pointing"))
+    assert np.array_equal(g2.get("pairShift"), o2.get("pairShift"))
+    err = np.abs(g2.get("position") - o2.get("pos")).max() / 0.008
+    assert err < 1e-10, err
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_periodic_ice_sheet_matches_oracle(fp64):
+    """The y-periodic bonded ice sheet (rows of iceCylinderInteraction.cpp on
+    its snapped periodic cell), rolled by 6 rows so that a row starts just
+    below the maximum face: bonds across the face from the initial pairs,
+    400 steps of waves that break bonds, against the FP64 oracle."""
+    s = scenes.periodic_ice_sheet(roll_rows=6, gap=1e-3)
+    o = oracle.OracleDEM(s)
+    g = ts.DEMSceneGPU(s, fp64=fp64)
+    assert np.array_equal(g.get("bondPointed"), o.get("bond_pointed"))
+    assert np.array_equal(g.get("bondPointing"), o.get("bond_pointing"))
+    assert np.array_equal(g.get("pairShift"), o.get("pairShift"))
+    o.step(400)
+    g.step(400)
+    err = np.abs(g.get("position") - o.get("pos")).max() / 0.2
+    assert err < (1e-12 if fp64 else 1e-6), err
+    assert np.array_equal(g.get("bondIsBonded"), o.get("bond_isBonded"))
+    assert (o.get("bond_isBonded") == 0).sum() > 10
+
+
+def test_periodic_sheet_translates_rigidly_through_the_face():
+    """A bonded periodic sheet moving at 50 m/s along y crosses the face
+    (positions wrapped into the cell at each rebuild) with every bond intact
+    and no force beyond rounding."""
+    s = scenes.periodic_ice_sheet(uniform_velocity=[0.0, 50.0, 0.0])
+    g = ts.DEMSceneGPU(s)
+    p0 = g.get("position").copy()
+    nb = g.get("bondIsBonded").size
+    g.step(1500)
+    ly = s.max_bound[1] - s.min_bound[1]
+    p = g.get("position")
+    d = p[:, 1] - (p0[:, 1] + 1500 * s.dt * 50.0)
+    d -= ly * np.round(d / ly)
+    assert np.abs(d).max() < 1e-9
+    assert (p[:, 1] < ly + 5 * 50.0 * s.dt).all() and (p[:, 1] > -5 * 50.0 * s.dt).all()
+    assert g.get("bondIsBonded").sum() == nb
+    assert np.abs(g.get("velocity") - [0.0, 50.0, 0.0]).max() < 1e-9

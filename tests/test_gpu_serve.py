@@ -85,6 +85,26 @@ def test_autotuned_serving_keeps_results():
     ts.SPHScene.clear_autotune_cache()
 
 
+def test_autotune_on_admission_with_eight_slots():
+    """Autotuning on admission while 8 slot threads share the hardware
+    queues (the trial timings read HIP events then): no job fails, and every
+    job equals its plain run bit for bit."""
+    ts.SPHScene.clear_autotune_cache()
+    kinds = ("wcsph", "dem", "wcsph", "coupled")
+    jobs = [MixJob(kinds[k % 4], 2500 + 700 * k, 30 + (k % 3) * 6, 0.0, 300 + k) for k in range(16)]
+    print("[serve] 16 jobs, 8 slots, autotune on admission", flush=True)
+    srv, ids = serving.serve(jobs, slots=8, graphs=True, block=16, autotune="exact")
+    st = srv.stats()
+    assert st["jobsFailed"] == 0 and st["jobsDone"] == len(jobs), [srv.info(i)["error"] for i in ids]
+    plain, pids = serving.serve(jobs, slots=1, graphs=False)
+    for job, i, k in zip(jobs, ids, pids):
+        for f in _fields(job.kind):
+            assert np.array_equal(srv.result(i, f), plain.result(k, f)), (job, f)
+    srv.close()
+    plain.close()
+    ts.SPHScene.clear_autotune_cache()
+
+
 def test_served_jobs_match_oracle():
     """A WCSPH and a DEM job served together against the FP64 oracle."""
     jobs = [MixJob("wcsph", 2000, 40, 0.0, 21), MixJob("dem", 2000, 200, 0.0, 22)]

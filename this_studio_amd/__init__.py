@@ -350,6 +350,7 @@ def _declare_dem(L):
         "demsph_dem_add_mesh_wall": [_vp, _sz, _dp, _sz, _ip, _ip, _ip, _dp, _dp, _dp, C.c_int],
         "demsph_dem_add_bonded_interactions": [_vp, _sz, _ip, _ip, _sz, _dp],
         "demsph_dem_set_precision": [_vp, C.c_int],
+        "demsph_dem_set_periodic_boundary": [_vp, C.c_int, C.c_int, C.c_int],
         "demsph_dem_initialize": [_vp, _dp, _dp],
         "demsph_dem_neighbor_search": [_vp],
         "demsph_dem_contact": [_vp, d],
@@ -386,10 +387,12 @@ def _declare_dem(L):
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.demsph_build_cells_noalloc_max_objects.argtypes = []
+    L.demsph_build_cells_noalloc_max_objects.restype = C.c_size_t
     _DEM_DECLARED = True
 
 
-DEM_INT_FIELDS = {"objectPointed", "objectPointing", "triPointed", "triPointing", "triCancel", "bondPointed",
+DEM_INT_FIELDS = {"objectPointed", "objectPointing", "pairShift", "triPointed", "triPointing", "triCancel", "bondPointed",
                   "bondPointing", "bondIsBonded", "hashValue", "hashIndex", "cellHashStart", "cellHashEnd"}
 DEM_VEC_FIELDS = {"position", "velocity", "angularVelocity", "force", "torque", "slidingSpring", "rollingSpring",
                   "torsionSpring", "triSlidingSpring", "triRollingSpring", "triTorsionSpring", "bondShearForce", "bondBendingTorque", "bondPoint",
@@ -428,6 +431,9 @@ class DEMSceneGPU:
 
     def _replay(self, s, initialize: bool = True):
         L, h = self.L, self._h
+        per = getattr(s, "periodic", (False, False, False))
+        if any(per):
+            check(L.demsph_dem_set_periodic_boundary(h, int(per[0]), int(per[1]), int(per[2])))
         for kind, a, b, p in s.materials:
             fn = {"linear": L.demsph_dem_set_linear_pair, "hertz": L.demsph_dem_set_hertzian_pair,
                   "bonded": L.demsph_dem_set_bonded_pair}[kind]
@@ -714,6 +720,8 @@ def _declare_server(L):
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    L.demsph_build_cells_noalloc_max_objects.argtypes = []
+    L.demsph_build_cells_noalloc_max_objects.restype = C.c_size_t
     _SERVER_DECLARED = True
 
 

@@ -172,6 +172,11 @@ class DEMScene:
         self.hooks = []          # ("const", f[n,3], ramp) | ("damping", c) | ("buoyancy", rho, cd, g, vf, level)
         self.bonds_from_initial_pairs = False
         self.explicit_bonds = None
+        self.periodic = (False, False, False)
+
+    def set_periodic_boundary(self, x: bool, y: bool, z: bool):
+        """setPeriodicBoundary (DEMSolver.h:1944-1959)."""
+        self.periodic = (bool(x), bool(y), bool(z))
 
     # material rows ------------------------------------------------------
     def set_linear_pair(self, a, b, kn, ks, kr, kt, dn, ds, dr, dt, mus, mur, mut):
@@ -368,6 +373,62 @@ def ice_cylinder() -> DEMScene:
     s.min_bound = np.array([-cmax[0], cmin[1], -3.0])
     s.max_bound = np.array([cmax[0], cmax[1], 3.0])
     s.gravity = np.array([0.0, 0.0, -9.81])
+    s.dt = tc / 50.0
+    return s
+
+
+def periodic_ice_sheet(nx: int = 24, ny: int = 20, roll_rows: int = 0, speed: float = 0.3,
+                       uniform_velocity=None, bonded: bool = True, gap: float = 0.0) -> DEMScene:
+    """The ice sheet of iceCylinderInteraction.cpp (hex rows of r = 0.1 m
+    balls at 900 kg/m^3, bonds from the initial pairs breaking at 0.5 MPa,
+    linear contacts, dt = t_c / 50) on its own y-periodic cell: the rows of
+    generateHexPointCloud with the y axis snapped to ny rows (ny even, so the
+    row pattern continues across the face) and setPeriodicBoundary(false,
+    true, false) on y in [0, ny dy).  No walls, no gravity.
+
+    Velocities are periodic in y: v = speed (sin k y, 0.7 cos k y + 0.3,
+    0.2 sin 2 k y), k = 2 pi / Ly, evaluated at each ball's lattice row, or
+    uniform_velocity for every ball.  roll_rows (even) moves every ball by
+    that many rows along y, wrapped into the cell, with the velocity it had:
+    the same periodic system translated, balls in the same insertion order.
+    gap > 0 spaces the lattice at 2 r (1 + gap): touching balls (gap 0, as
+    the application) start at zero overlap, where the rounding of the
+    overlap decides whether a damped contact acts, so such a sheet follows
+    the rounding of its positions chaotically."""
+    if ny % 2 or roll_rows % 2:
+        raise ValueError("ny and roll_rows must be even (hex row parity)")
+    tc, r, den = 0.0002, 0.1, 900.0
+    m = r ** 3 * math.pi * 4.0 / 3.0 * den
+    kn = 0.5 * m * math.pi ** 2 / tc / tc
+    E = kn * 2.0 * r / (r * r * math.pi)
+    sp = 2 * r * (1.0 + gap)
+    dy = sp * math.sqrt(3.0) * 0.5
+    s = DEMScene("periodic_ice_sheet")
+    if bonded:
+        s.set_bonded_pair(0, 0, 1.0, E, 2.6, 0.5e6, 0.5e6, 0.1)
+        s.bonds_from_initial_pairs = True
+    s.set_linear_pair(0, 0, kn, kn / 2.6, 0.0, 0.0, 0.1, 0.1, 0.0, 0.0, 0.1, 0.0, 0.0)
+    pts, cmin, cmax = hex_sheet([0.0, 0.0, 0.0], [sp * nx, ny * dy, 0.0], sp, False, True, with_cell=True)
+    ly = cmax[1] - cmin[1]
+    k = 2.0 * math.pi / ly
+    y0 = pts[:, 1].copy()
+    if uniform_velocity is not None:
+        vel = np.broadcast_to(np.asarray(uniform_velocity, dtype=np.float64), pts.shape).copy()
+    else:
+        vel = speed * np.stack([np.sin(k * y0), 0.7 * np.cos(k * y0) + 0.3, 0.2 * np.sin(2 * k * y0)], axis=1)
+    if roll_rows:
+        y = y0 + roll_rows * dy
+        y = np.where(y >= cmax[1], y - ly, y)
+        pts = pts.copy()
+        pts[:, 1] = y
+    s.add_balls(pts, vel, [0.0, 0.0, 0.0], r, den, 0)
+    s.set_periodic_boundary(False, True, False)
+    # x and z: cells strictly above 2r (the lattice never sits on cell faces);
+    # y: the snapped period, whose cells are above 2r since dy < 2r
+    lo, hi = half_cell_bounds([-1.0, cmin[1], -1.0], [sp * nx + 1.0, cmax[1], 1.0], sp)
+    s.min_bound = lo
+    s.max_bound = np.array([hi[0], cmax[1], hi[2]])
+    s.gravity = np.zeros(3)
     s.dt = tc / 50.0
     return s
 
