@@ -310,7 +310,13 @@ int demsph_dem_second_half(demsph_dem* d, const double* gravity, double halfTime
 int demsph_dem_solve_begin(demsph_dem* d);
 /* nsteps iterations of loop_ball / loop_ball_wall (DEMSolver.h:1376-1405,
  * 1452-1485): rebuild every 5th step, two force evaluations per step,
- * external forces (hook) at t - dt/2 and t, kick-drift-kick. */
+ * external forces (hook) at t - dt/2 and t, kick-drift-kick.  The pair
+ * lists of rebuilds inside the call are sized on the device (the reference
+ * reads the total back at every rebuild, interaction.h:164-175): a list that
+ * outgrows its capacity is truncated, and the call goes back to a device
+ * checkpoint taken at its start and runs again with both lists grown, with
+ * the same results.  With a host hook (demsph_dem_set_hook) the rebuilds
+ * read the total back instead, so the hook sees each step once. */
 int demsph_dem_step(demsph_dem* d, const double* gravity, double timeStep, int nsteps);
 int demsph_dem_synchronize(demsph_dem* d);
 /* Steps taken since solve_begin, and the simulated time. */
@@ -413,18 +419,22 @@ int demsph_dem_import_state(demsph_dem* d, const void* in, size_t bytes);
  * reference's getMeshWallBody() write path). */
 int demsph_dem_set(demsph_dem* d, const char* field, const void* in, size_t count);
 
-/* Step graphs (no counterpart in the reference): demsph_dem_step replays the
- * four steps between two pair rebuilds as one captured hipGraph when the
- * scene runs on a stream of its own (not the null stream), has no hooks, its
- * walls are static and timing is off; rebuild steps stay eager.  On by
- * default; bit-identical to eager steps.  demsph_dem_graph_captures counts
- * the captures. */
+/* Step graphs (no counterpart in the reference): demsph_dem_step replays
+ * each whole cycle of a pair rebuild and its five steps as one captured
+ * hipGraph (and the four steps after a rebuild as another, at the start of
+ * a call) when the scene runs on a stream of its own (not the null stream),
+ * has no hooks, its walls are static and timing is off.  On by default;
+ * bit-identical to eager steps.  demsph_dem_graph_captures counts the
+ * captures. */
 int demsph_dem_set_graphs(demsph_dem* d, int enable);
 int demsph_dem_graph_captures(demsph_dem* d, long* count);
 /* DEM kernel tuning knob (no counterpart in the reference): "pairLanes",
  * lanes per ball in the pair-list build: 0 (default: 8 up to 32,768 balls,
  * else 1), 1, 2, 4 or 8.  Every setting builds the same pair lists in the
- * same order.  demsph_dem_tune_get reads the value in effect. */
+ * same order.  "pairCapacity": entries of each of the two pair lists,
+ * set between initialize and solve_begin (they grow on demand from there).
+ * demsph_dem_tune_get reads the value in effect, and "pairReplays", the
+ * step calls run again after a pair list outgrew its capacity. */
 int demsph_dem_tune(demsph_dem* d, const char* knob, int value);
 int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value);
 /* Times each "pairLanes" setting (1, 2, 4, 8) over trialSteps >= 5 steps of

@@ -700,3 +700,74 @@ def test_periodic_sheet_translates_rigidly_through_the_face():
     assert (p[:, 1] < ly + 5 * 50.0 * s.dt).all() and (p[:, 1] > -5 * 50.0 * s.dt).all()
     assert g.get("bondIsBonded").sum() == nb
     assert np.abs(g.get("velocity") - [0.0, 50.0, 0.0]).max() < 1e-9
+
+
+def test_collapse_history_matches_oracle():
+    """The 4,096-ball granular collapse over its full 10,000 steps against the
+    FP64 oracle's history (tests/golden/collapse_history.npz, written by
+    tests/golden/make_collapse_history.py), every 100 steps: translational
+    kinetic energy within 0.2 % of its peak, rotational within 3 % of its
+    peak (rolling starts with frictional contacts, sensitive to FP32 force
+    rounding), mean and top ball height within 10 um."""
+    ref = np.load(GOLDEN / "collapse_history.npz")["history"]
+    s = scenes.granular_collapse()
+    g = ts.DEMSceneGPU(s)
+    r, rho = 0.005, 2500.0
+    m = 4.0 / 3.0 * math.pi * r ** 3 * rho
+    rows = []
+    for k in range(ref.shape[0]):
+        if k:
+            g.step(100)
+        v = g.get("velocity")
+        w = g.get("angularVelocity")
+        z = g.get("position")[:, 2]
+        rows.append([0.5 * m * (v * v).sum(), 0.5 * 0.4 * m * r * r * (w * w).sum(), z.mean(), z.max()])
+    h = np.array(rows)
+    assert np.abs(h[:, 0] - ref[:, 1]).max() < 2e-3 * ref[:, 1].max()
+    assert np.abs(h[:, 1] - ref[:, 2]).max() < 3e-2 * ref[:, 2].max()
+    assert np.abs(h[:, 2] - ref[:, 3]).max() < 1e-5
+    assert np.abs(h[:, 3] - ref[:, 4]).max() < 1e-5
+    assert ref[-1, 1] > 0.9  # the collapse is under way at the end of the run
+
+
+def test_pair_lists_grown_on_device_replay_bit_for_bit():
+    """Pair lists built inside step blocks are sized on the device: a list
+    that outgrows its capacity is truncated and noted, and the block is
+    replayed from its device checkpoint with grown lists (no host read per
+    rebuild; whole rebuild cycles replay one graph).  Lists that start at a
+    quarter of what the scene needs replay and then match a scene with ample
+    lists bit for bit, with graphs and with eager launches, over blocks that
+    start on and off the rebuild cycle."""
+    import torch
+
+    st = torch.cuda.Stream()
+    s = scenes.granular_collapse()
+    a = ts.DEMSceneGPU(s, stream=st.cuda_stream)
+    need = a.get("objectPointed").size
+    small = []
+    for graphs in (True, False):
+        b = ts.DEMSceneGPU(s, solve_begin=False, stream=st.cuda_stream)
+        b.set_graphs(graphs)
+        b.tune("pairCapacity", max(need // 4, 1))
+        # solve_begin's own search reads the total back and grows the list
+        # it writes; the other list keeps the small capacity, so the first
+        # rebuild inside a step block overflows
+        b.solve_begin()
+        assert b.get("objectPointed").size == need
+        small.append(b)
+    for n in (23, 40, 7, 30):
+        for g in [a] + small:
+            g.step(n)
+        for b in small:
+            for f in ("position", "velocity", "angularVelocity", "slidingSpring", "objectPointed",
+                      "objectPointing"):
+                assert np.array_equal(a.get(f), b.get(f)), (n, f)
+            assert a.clock() == b.clock()
+    assert a.tuned("pairReplays") == 0
+    for b in small:
+        assert b.tuned("pairReplays") >= 1
+        assert b.tuned("pairCapacity") >= a.get("objectPointed").size
+    assert a.graph_captures() >= 2 and small[0].graph_captures() >= 2  # rebuild cycles replay as graphs
+    assert small[1].graph_captures() == 0
+    for g in [a] + small:
+        g.close()

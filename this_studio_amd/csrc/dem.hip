@@ -52,6 +52,7 @@ struct Forces {
 
 struct Pairs {
     int* start;    // n + 1 (owner CSR)
+    int cap;       // entries allocated; a list longer than cap is truncated (and replayed)
     int* partner;  // cap
     uint8_t* shift;  // cap, periodic image code of the partner (periodic scenes only, else null)
     double *sx, *sy, *sz, *rx, *ry, *rz, *ox, *oy, *oz;  // sliding, rolling, torsion springs
@@ -301,10 +302,14 @@ __global__ __launch_bounds__(kT) void k_write_pairs(WritePairsArgs a)
     int os0 = 0, os1 = 0;
     if (a.has_old) {
         const int s = a.rank_old[oa];
-        os0 = a.old.start[s];
-        os1 = a.old.start[s + 1];
+        os0 = min(a.old.start[s], a.old.cap);
+        os1 = min(a.old.start[s + 1], a.old.cap);
     }
     walk_pair_rows<PER>(a.b, a.g, a.cs, a.ce, i, q0, q1, a.per, [&](int j, int code) {
+        if (w >= a.cur.cap) {  // over capacity: counted, not stored; the block is replayed
+            w++;
+            return;
+        }
         a.cur.partner[w] = j;
         if (PER) a.cur.shift[w] = (uint8_t)code;
         double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -326,18 +331,20 @@ __global__ __launch_bounds__(kT) void k_write_pairs(WritePairsArgs a)
 
 // Reverse index: pairs grouped by partner, ascending pair id within each
 // partner (the reference sorts pairs by pointing object, DEMSolver.h:59-78).
-__global__ __launch_bounds__(kT) void k_rev_count(int n, const int* start, const int* partner, int* rcnt)
+__global__ __launch_bounds__(kT) void k_rev_count(int n, const int* start, int cap, const int* partner, int* rcnt)
 {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
-    for (int p = start[i]; p < start[i + 1]; p++) atomicAdd(&rcnt[partner[p]], 1);
+    const int p1 = min(start[i + 1], cap);
+    for (int p = start[i]; p < p1; p++) atomicAdd(&rcnt[partner[p]], 1);
 }
-__global__ __launch_bounds__(kT) void k_rev_fill(int n, const int* start, const int* partner, const int* rstart,
-                                                 int* rcursor, int* rpair)
+__global__ __launch_bounds__(kT) void k_rev_fill(int n, const int* start, int cap, const int* partner,
+                                                 const int* rstart, int* rcursor, int* rpair)
 {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
-    for (int p = start[i]; p < start[i + 1]; p++) {
+    const int p1 = min(start[i + 1], cap);
+    for (int p = start[i]; p < p1; p++) {
         const int j = partner[p];
         rpair[rstart[j] + atomicAdd(&rcursor[j], 1)] = p;
     }
@@ -356,6 +363,39 @@ __global__ __launch_bounds__(kT) void k_rev_sort(int n, const int* rstart, int* 
         }
         rpair[b + 1] = v;
     }
+}
+
+// A pair list longer than its capacity: the needed length, for the replay.
+__global__ void k_pair_capacity(const int* total, int cap, int* over)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *total > cap) atomicMax(over, *total);
+}
+
+// One launch copies every segment of a table (the device checkpoint of a
+// block of steps): segment blockIdx.y, 4-byte words, grid-stride in x.
+struct CopySeg {
+    const void* src;
+    void* dst;
+    unsigned long long bytes;
+};
+constexpr int kMaxCopySegs = 160;  // the table travels as a kernel argument (< 4 KB)
+struct CopyTable {
+    CopySeg s[kMaxCopySegs];
+    int n;
+};
+__global__ __launch_bounds__(256) void k_multi_copy(CopyTable t)
+{
+    const int seg = blockIdx.y;
+    if (seg >= t.n) return;
+    const CopySeg c = t.s[seg];
+    const size_t words = c.bytes / 4;
+    const uint32_t* src = static_cast<const uint32_t*>(c.src);
+    uint32_t* dst = static_cast<uint32_t*>(c.dst);
+    for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < words; w += (size_t)gridDim.x * 256) dst[w] = src[w];
+    // byte arrays (periodic image codes) may end in a partial word
+    if (blockIdx.x == 0 && threadIdx.x < (c.bytes & 3))
+        static_cast<unsigned char*>(c.dst)[4 * words + threadIdx.x] =
+            static_cast<const unsigned char*>(c.src)[4 * words + threadIdx.x];
 }
 
 // ----------------------------------------------------------------- forces
@@ -383,7 +423,7 @@ __device__ __forceinline__ void pair_forces_ball(const PairForceArgs& a, int i)
     const double ai = a.b.r[i], imi = a.b.im[i];
     const int mi = a.b.mat[i];
     V3<T> Fo{0, 0, 0}, To{0, 0, 0};
-    const int p0 = a.p.start[i], p1 = a.p.start[i + 1];
+    const int p0 = a.p.start[i], p1 = min(a.p.start[i + 1], a.p.cap);
     for (int p = p0; p < p1; p++) {
         const int j = a.p.partner[p];
         const D3 rj = a.p.shift ? shift_point(ld3(a.b.px, a.b.py, a.b.pz, j), a.p.shift[p], a.per)
@@ -1158,6 +1198,13 @@ struct DBuf {
 
 }  // namespace
 
+struct demsph_dem;
+namespace {
+// Device checkpoint of a block of steps (defined after the state arrays).
+void dem_block_save(demsph_dem* d);
+void dem_block_restore(demsph_dem* d, int grow_pairs);
+}  // namespace
+
 struct demsph_dem {
     hipStream_t stream = nullptr;
     bool initialized = false;
@@ -1243,6 +1290,18 @@ struct demsph_dem {
     DBuf<double> wfx, wfy, wfz, wtx, wty, wtz;  // hold float or double
     DBuf<int> overflow;
     int* overflow_h = nullptr;
+    // Pair lists built inside step blocks: the longest list needed beyond
+    // capacity (0: none), and the device checkpoint the block replays from.
+    DBuf<int> pair_over;
+    int* block_h = nullptr;  // pinned: [0] pair_over, [1] pairs of the current list
+    DBuf<unsigned char> ckbuf;
+    long pair_replays = 0;  // blocks replayed with grown pair lists
+    struct HostCk {
+        size_t iStep;
+        double time;
+        int cb, cp, npairs;
+        bool has_list, walls_valid, tri_cells_valid, walls_static;
+    } hck{};
     bool walls_valid = false;  // wall forces of the last triangle evaluation are held
     // hooks
     struct HookOp {
@@ -1265,15 +1324,18 @@ struct demsph_dem {
     // Every buffer a capture bakes in that can be reallocated between
     // captures is part of the key (the contact records are reallocated with
     // either pair list).
+    // kind 4: the four steps after a rebuild; kind 5: a rebuild and the
+    // five steps of its cycle (lists sized on the device).
     struct GraphKey {
         double dt, gx, gy, gz;
-        int cb, cp, nbond, fp64;
-        const void *pairs, *rev, *cand, *stamp, *prec, *crec;
+        int kind, cb, cp, nbond, fp64, cap0, cap1;
+        const void *pairs, *pairs1, *rev, *cand, *stamp, *prec, *crec;
         bool operator==(const GraphKey& o) const
         {
-            return dt == o.dt && gx == o.gx && gy == o.gy && gz == o.gz && cb == o.cb && cp == o.cp &&
-                   nbond == o.nbond && fp64 == o.fp64 && pairs == o.pairs && rev == o.rev && cand == o.cand &&
-                   stamp == o.stamp && prec == o.prec && crec == o.crec;
+            return dt == o.dt && gx == o.gx && gy == o.gy && gz == o.gz && kind == o.kind && cb == o.cb &&
+                   cp == o.cp && nbond == o.nbond && fp64 == o.fp64 && cap0 == o.cap0 && cap1 == o.cap1 &&
+                   pairs == o.pairs && pairs1 == o.pairs1 && rev == o.rev && cand == o.cand && stamp == o.stamp &&
+                   prec == o.prec && crec == o.crec;
         }
     };
     struct GraphEntry {
@@ -1302,6 +1364,7 @@ struct demsph_dem {
         dev_free(scan_temp);
         host_free(total_h);
         host_free(overflow_h);
+        host_free(block_h);
         for (auto* b : hook_bufs) delete b;
     }
 
@@ -1321,7 +1384,7 @@ struct demsph_dem {
     Pairs pairs(int which) const
     {
         Pairs p;
-        p.start = pstart[which].p; p.partner = ppartner[which].p;
+        p.start = pstart[which].p; p.partner = ppartner[which].p; p.cap = pcap[which];
         p.sx = psx[which].p; p.sy = psy[which].p; p.sz = psz[which].p;
         p.rx = prx[which].p; p.ry = pry[which].p; p.rz = prz[which].p;
         p.ox = pox[which].p; p.oy = poy[which].p; p.oz = poz[which].p;
@@ -1592,6 +1655,9 @@ struct demsph_dem {
         overflow.alloc(1);
         DEMSPH_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), stream));
         DEMSPH_HIP(host_malloc((void**)&overflow_h, sizeof(int)));
+        pair_over.alloc(1);
+        fill_i32(pair_over.p, 1, 0, stream);
+        DEMSPH_HIP(host_malloc((void**)&block_h, 2 * sizeof(int)));
         nbond_dev = 0;
         upload_bonds();
         DEMSPH_HIP(hipStreamSynchronize(stream));
@@ -1642,7 +1708,12 @@ struct demsph_dem {
     }
 
     // neighborSearch (DEMSolver.h:698-777)
-    void neighbor_search()
+    // sync_resize: read the pair total back and grow the lists before writing
+    // them (resizePair, interaction.h:164-175), as the API's neighbour search
+    // does.  Otherwise (inside step blocks) nothing is read back: a list
+    // longer than its capacity is truncated and noted in pair_over, and the
+    // block is replayed from its checkpoint with larger lists.
+    void neighbor_search(bool sync_resize = true)
     {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         Balls b = cur();
@@ -1669,12 +1740,18 @@ struct demsph_dem {
         const int np = 1 - cp;
         // start[n] = total, written by the scan
         exclusive_scan(cnt.p, pstart[np].p, n, scan_temp, pstart[np].p + n, stream);
-        DEMSPH_HIP(hipMemcpyAsync(total_h, pstart[np].p + n, sizeof(int), hipMemcpyDeviceToHost, stream));
-        DEMSPH_HIP(hipStreamSynchronize(stream));  // resizePair (interaction.h:164-175)
-        timer.harvest();
-        const int tot = total_h[0];
-        if (tot > pcap[np]) alloc_pairs(np, tot + tot / 4);
-        if (tot > (int)rpair.n) rpair.alloc(pcap[np]);
+        int tot = -1;  // on the device only
+        if (sync_resize) {
+            DEMSPH_HIP(hipMemcpyAsync(total_h, pstart[np].p + n, sizeof(int), hipMemcpyDeviceToHost, stream));
+            DEMSPH_HIP(hipStreamSynchronize(stream));  // resizePair (interaction.h:164-175)
+            timer.harvest();
+            tot = total_h[0];
+            if (tot > pcap[np]) alloc_pairs(np, tot + tot / 4);
+            if (tot > (int)rpair.n) rpair.alloc(std::max(pcap[0], pcap[1]));
+        } else {
+            hipLaunchKernelGGL(k_pair_capacity, dim3(1), dim3(64), 0, stream, pstart[np].p + n, pcap[np], pair_over.p);
+            DEMSPH_HIP(hipGetLastError());
+        }
         WritePairsArgs wa{};
         wa.b = nb; wa.g = bg; wa.cs = cellStart.p; wa.ce = cellEnd.p;
         wa.cur = pairs(np);
@@ -1690,10 +1767,10 @@ struct demsph_dem {
         // reverse index by partner (DEMSolver.h:59-78, reset every build)
         timer.begin("dem_reverse_index", stream);
         // rcnt and rcursor were cleared by k_reorder
-        hipLaunchKernelGGL(k_rev_count, gb, tb, 0, stream, n, pstart[cp].p, ppartner[cp].p, rcnt.p);
+        hipLaunchKernelGGL(k_rev_count, gb, tb, 0, stream, n, pstart[cp].p, pcap[cp], ppartner[cp].p, rcnt.p);
         exclusive_scan(rcnt.p, rstart.p, n + 1, scan_temp, nullptr, stream);
-        hipLaunchKernelGGL(k_rev_fill, gb, tb, 0, stream, n, pstart[cp].p, ppartner[cp].p, rstart.p, rcursor.p,
-                           rpair.p);
+        hipLaunchKernelGGL(k_rev_fill, gb, tb, 0, stream, n, pstart[cp].p, pcap[cp], ppartner[cp].p, rstart.p,
+                           rcursor.p, rpair.p);
         hipLaunchKernelGGL(k_rev_sort, gb, tb, 0, stream, n, rstart.p, rpair.p);
         DEMSPH_HIP(hipGetLastError());
         timer.end(stream);
@@ -2031,28 +2108,39 @@ struct demsph_dem {
         }
     }
 
-    hipGraphExec_t quad_graph(const double* g, double dt)
+    // kind 4: the four steps after a rebuild.  kind 5: a rebuild with lists
+    // sized on the device and the five steps of its cycle; capturing runs
+    // the host side of the rebuild (buffer flips), which is put back: the
+    // caller flips after launching.
+    hipGraphExec_t cycle_graph(int kind, const double* g, double dt)
     {
         GraphKey key{};
         key.dt = dt; key.gx = g[0]; key.gy = g[1]; key.gz = g[2];
-        key.cb = cb; key.cp = cp; key.nbond = nbond; key.fp64 = fp64 ? 1 : 0;
-        key.pairs = ppartner[cp].p; key.rev = rpair.p; key.cand = tcand_tri.p; key.stamp = stamps.buf;
+        key.kind = kind; key.cb = cb; key.cp = cp; key.nbond = nbond; key.fp64 = fp64 ? 1 : 0;
+        key.cap0 = pcap[0]; key.cap1 = pcap[1];
+        key.pairs = ppartner[cp].p; key.pairs1 = ppartner[1 - cp].p; key.rev = rpair.p; key.cand = tcand_tri.p;
+        key.stamp = stamps.buf;
         key.prec = record_contacts ? prec.p : nullptr;
         key.crec = record_contacts ? crec.p : nullptr;
         for (auto& e : graphs)
             if (e.key == key) return e.exec;
+        const int cb0 = cb, cp0 = cp, np0 = npairs;
+        const bool has0 = has_list;
         hipGraph_t gr = nullptr;
         // No other thread may allocate, free or synchronise the device
         // until the graph is instantiated (see capture_lock).
         CaptureGuard capture_guard(capture_lock());
         DEMSPH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
-            plain_steps(g, dt, 4);
+            if (kind == 5) neighbor_search(false);
+            plain_steps(g, dt, kind);
         } catch (...) {
             (void)hipStreamEndCapture(stream, &gr);
             if (gr) (void)hipGraphDestroy(gr);
+            cb = cb0; cp = cp0; npairs = np0; has_list = has0;
             throw;
         }
+        cb = cb0; cp = cp0; npairs = np0; has_list = has0;
         DEMSPH_HIP(hipStreamEndCapture(stream, &gr));
         hipGraphExec_t ex = nullptr;
         const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
@@ -2068,17 +2156,30 @@ struct demsph_dem {
         return ex;
     }
 
-    // loop_ball / loop_ball_wall (DEMSolver.h:1376-1405, 1452-1485)
-    void step(const double* g, double dt, int nsteps)
+    // One block of loop_ball / loop_ball_wall (DEMSolver.h:1376-1405,
+    // 1452-1485) with the pair lists sized on the device: whole cycles of a
+    // rebuild and five steps replay one graph, the four steps after a
+    // rebuild another, other steps run eagerly.
+    void run_block(const double* g, double dt, int nsteps)
     {
-        require_init();
-        if (!has_list) throw InvalidArg("solve_begin() (or neighbor_search) must precede step()");
-        if (!(dt > 0)) throw InvalidArg("time step must be positive");
         const double h = 0.5 * dt;
         for (int k = 0; k < nsteps;) {
+            if (graphs_ok() && iStep % 5 == 4 && nsteps - k >= 5) {
+                DEMSPH_HIP(hipGraphLaunch(cycle_graph(5, g, dt), stream));
+                for (int q = 0; q < 5; q++) {
+                    iStep++;
+                    time += dt;
+                }
+                cb = 1 - cb;
+                cp = 1 - cp;
+                has_list = true;
+                npairs = -1;
+                k += 5;
+                continue;
+            }
             // right after a rebuild (or at the start) the next four steps rebuild nothing
-            if (iStep % 5 == 0 && nsteps - k >= 4 && graphs_ok()) {
-                DEMSPH_HIP(hipGraphLaunch(quad_graph(g, dt), stream));
+            if (graphs_ok() && iStep % 5 == 0 && nsteps - k >= 4) {
+                DEMSPH_HIP(hipGraphLaunch(cycle_graph(4, g, dt), stream));
                 for (int q = 0; q < 4; q++) {
                     iStep++;
                     time += dt;
@@ -2088,7 +2189,12 @@ struct demsph_dem {
             }
             iStep++;
             time += dt;
-            if (iStep % 5 == 0) neighbor_search();
+            if (iStep % 5 == 0) {
+                // a host callback must not see its steps twice: with one,
+                // rebuilds size their lists on the host and nothing replays
+                neighbor_search(hook_fn != nullptr);
+                if (!hook_fn) npairs = -1;
+            }
             if (fused_ok()) {
                 evaluate_half(true, g, h, dt);
                 evaluate_half(false, g, h, dt);
@@ -2103,7 +2209,40 @@ struct demsph_dem {
             step_stamp(stamps, stream);
             k++;
         }
-        check_overflow();
+    }
+
+    // nsteps steps with one host synchronisation at the end.  A block that
+    // rebuilds the pair lists saves a device checkpoint first; a list that
+    // outgrew its capacity sends the block back to the checkpoint with
+    // lists grown to fit, and it runs again.
+    void step(const double* g, double dt, int nsteps)
+    {
+        require_init();
+        if (!has_list) throw InvalidArg("solve_begin() (or neighbor_search) must precede step()");
+        if (!(dt > 0)) throw InvalidArg("time step must be positive");
+        if (nsteps <= 0) return;
+        const bool rebuilds = !hook_fn && nsteps >= 5 - (int)(iStep % 5);
+        for (int attempt = 0; attempt < 8; attempt++) {
+            if (rebuilds) {
+                dem_block_save(this);
+                fill_i32(pair_over.p, 1, 0, stream);
+            }
+            run_block(g, dt, nsteps);
+            DEMSPH_HIP(hipMemcpyAsync(block_h, pair_over.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+            DEMSPH_HIP(hipMemcpyAsync(block_h + 1, pstart[cp].p + n, sizeof(int), hipMemcpyDeviceToHost, stream));
+            DEMSPH_HIP(hipStreamSynchronize(stream));
+            const int need = rebuilds ? block_h[0] : 0;
+            if (need == 0) {
+                npairs = block_h[1];
+                check_overflow();
+                return;
+            }
+            // back to the checkpoint with both lists grown to fit
+            pair_replays++;
+            drop_graphs();
+            dem_block_restore(this, need + need / 4 + 64);
+        }
+        throw std::runtime_error("pair list capacity did not converge");
     }
 };
 
@@ -2743,6 +2882,61 @@ void dem_state_arrays(demsph_dem* d, F&& io)
     }
 }
 
+// The block checkpoint: every array of the state export copied to one
+// device buffer (segments 256-byte aligned) by k_multi_copy launches, with
+// the host-side values that select and size them.
+void dem_block_copy(demsph_dem* d, bool save)
+{
+    std::vector<std::pair<void*, size_t>> segs;
+    size_t total = 0;
+    dem_state_arrays(d, [&](void* p, size_t b) {
+        if (!b) return;
+        segs.push_back({p, b});
+        total += (b + 255) & ~size_t(255);
+    });
+    if (save && d->ckbuf.n < total) d->ckbuf.alloc(total + total / 4);
+    unsigned char* ck = d->ckbuf.p;
+    size_t off = 0;
+    for (size_t s0 = 0; s0 < segs.size(); s0 += kMaxCopySegs) {
+        CopyTable t{};
+        size_t most = 0;
+        t.n = (int)std::min<size_t>(kMaxCopySegs, segs.size() - s0);
+        for (int k = 0; k < t.n; k++) {
+            const auto& sg = segs[s0 + k];
+            unsigned char* c = ck + off;
+            t.s[k].src = save ? static_cast<const void*>(sg.first) : static_cast<const void*>(c);
+            t.s[k].dst = save ? static_cast<void*>(c) : sg.first;
+            t.s[k].bytes = sg.second;
+            most = std::max(most, sg.second);
+            off += (sg.second + 255) & ~size_t(255);
+        }
+        const int gx = (int)std::min<size_t>(64, std::max<size_t>(1, (most / 4 + 255) / 256));
+        hipLaunchKernelGGL(k_multi_copy, dim3(gx, t.n), dim3(256), 0, d->stream, t);
+        DEMSPH_HIP(hipGetLastError());
+    }
+}
+
+void dem_block_save(demsph_dem* d)
+{
+    if (d->npairs < 0) throw std::logic_error("block checkpoint with an unknown pair count");
+    d->hck = {d->iStep, d->time, d->cb, d->cp, d->npairs, d->has_list, d->walls_valid, d->tri_cells_valid,
+              d->walls_static};
+    dem_block_copy(d, true);
+}
+
+void dem_block_restore(demsph_dem* d, int grow_pairs)
+{
+    const auto& h = d->hck;
+    d->iStep = h.iStep; d->time = h.time; d->cb = h.cb; d->cp = h.cp; d->npairs = h.npairs;
+    d->has_list = h.has_list; d->walls_valid = h.walls_valid; d->tri_cells_valid = h.tri_cells_valid;
+    d->walls_static = h.walls_static;
+    // all work using the old lists is done (the caller synchronised)
+    for (int w = 0; w < 2; w++)
+        if (d->pcap[w] < grow_pairs) d->alloc_pairs(w, grow_pairs);
+    if ((int)d->rpair.n < std::max(d->pcap[0], d->pcap[1])) d->rpair.alloc(std::max(d->pcap[0], d->pcap[1]));
+    dem_block_copy(d, false);
+}
+
 DemStateHeader dem_state_header(const demsph_dem* d)
 {
     DemStateHeader h{};
@@ -3128,6 +3322,14 @@ extern "C" int demsph_dem_tune(demsph_dem* d, const char* knob, int value)
     DEM_GUARD({
         if (!knob) throw InvalidArg("null knob");
         const std::string k(knob);
+        if (k == "pairCapacity") {
+            // entries of each pair list; lists grow on demand from there
+            if (!d->initialized || d->has_list) throw InvalidArg("pairCapacity is set between initialize and solve_begin");
+            if (value < 1) throw InvalidArg("pairCapacity must be positive");
+            for (int w = 0; w < 2; w++) d->alloc_pairs(w, value);
+            d->rpair.alloc(value);
+            return;
+        }
         if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
             throw InvalidArg("pairLanes must be 0 (by ball count), 1, 2, 4 or 8");
@@ -3140,6 +3342,14 @@ extern "C" int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value)
     DEM_GUARD({
         if (!knob || !value) throw InvalidArg("null argument");
         const std::string k(knob);
+        if (k == "pairCapacity") {
+            *value = d->pcap[d->cp];
+            return;
+        }
+        if (k == "pairReplays") {
+            *value = (int)d->pair_replays;
+            return;
+        }
         if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
         *value = pair_walk_lanes(d->initialized ? d->n : (int)d->hb.size(), d->pair_lanes_set);
     });
