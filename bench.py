@@ -63,26 +63,78 @@ def algorithmic_bytes(kernel: str, nf: int, pairs_ff: int, pairs_fd: int) -> flo
     return 0.0
 
 
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
-# this workload (HBM bytes = 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE, from a
-# plain 256 MiB copy), as recorded in profiles/wcsph_dam_break_256k_kernels.md.
-MEASURED_HBM_BYTES = {
-    "sph_lists": (20.8 + 88.3) * 1e6,
-    "sph_momentum_first": (124.1 + 18.4) * 1e6,
-    "sph_momentum_second": (123.5 + 10.2) * 1e6,
-    "sph_continuity": (118.1 + 9.2) * 1e6,
-    "sph_reorder": (24.4 + 26.0) * 1e6,
-}
+PROFILE = ROOT / "profiles" / "wcsph_dam_break_256k_kernels.md"
+# profile table rows are named by kernel; the bench names the launches
+PROFILE_KERNEL = {"sph_lists": "k_build_lists", "sph_momentum_first": "k_momentum<true",
+                  "sph_momentum_second": "k_momentum<false", "sph_continuity": "k_continuity",
+                  "sph_reorder": "k_reorder"}
 
 
-# What bounds each step kernel on this workload (hardware counters, kernel trace).
+def profiled_traffic(name: str):
+    """HBM bytes per launch of a step kernel, read from the committed rocprofv3
+    summary (its FETCH_SIZE / WRITE_SIZE columns, already scaled by the copy
+    calibration): (bytes, source) or (None, reason)."""
+    key = PROFILE_KERNEL.get(name)
+    if key is None or not PROFILE.exists():
+        return None, "no profile row"
+    for line in PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0].startswith(key):
+            try:
+                return (float(cells[4]) + float(cells[5])) * 1e6, f"{PROFILE.name}: {cells[0]}"
+            except ValueError:
+                return None, "unreadable profile row"
+    return None, "no profile row"
+
+
+# What bounds each step kernel on this workload, from the counters in the
+# same profile (SQ_WAIT_ANY / SQ_WAVE_CYCLES, VALU and LDS instructions).
 LIMITER = {
-    "sph_lists": "instruction issue: ~420 candidate cut tests per particle (SQ counters), not HBM",
+    "sph_lists": "instruction issue over ~420 candidate cut tests per particle, not HBM",
     "sph_momentum_first": "latency of dependent neighbour gathers",
     "sph_momentum_second": "latency of dependent neighbour gathers",
     "sph_continuity": "latency of dependent neighbour gathers",
     "sph_reorder": "HBM",
 }
+
+
+def step_model_bytes(nf: int, nd: int) -> float:
+    """Compulsory HBM bytes of one WCSPH step per fluid particle: three pair
+    passes over the fluid state (40 B each), the writes of v twice, x and
+    drho (52 B), hash + sort + ranges (44 B) and the dummy state read by the
+    three passes (108 B per dummy, dummies hashed once)."""
+    return 3 * 40 + 52 + 44 + 108 * (nd / nf)
+
+
+def dem_model_bytes(p: float, w: float, b: float) -> float:
+    """Compulsory HBM bytes of one DEM ball-step for p ball-pair, w
+    ball-triangle and b bond entries per ball: ball state 60 B, pair ids and
+    two-way history 104 B per pair, wall contacts 100 B, bonds 112 B, all
+    read in each of the two evaluations, the v, w and x writes (72 B), and
+    the rebuild every 5th step (44 + 104 p)."""
+    return 2 * (60 + 104 * p + 100 * w + 112 * b) + 72 + (44 + 104 * p) / 5
+
+
+def dem_roofline(g, n: int, ms_per_step: float) -> dict:
+    p = g.get("objectPointed").size / n
+    w = int((g.get("triCancel") == 0).sum()) / n if g.sizes()[3] > 0 else 0.0
+    b = g.get("bondPointed").size / n
+    per = dem_model_bytes(p, w, b)
+    achieved = per * n / (ms_per_step * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+            "model": "whole step, compulsory bytes per ball-step 2(60 + 104p + 100w + 112b) + 72 + (44 + 104p)/5",
+            "p": round(p, 3), "w": round(w, 3), "b": round(b, 3), "bytes_per_ball_step": round(per, 1)}
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def algorithmic_flops(kernel: str, pairs_ff: int, pairs_fd: int) -> float:
@@ -95,23 +147,37 @@ def algorithmic_flops(kernel: str, pairs_ff: int, pairs_fd: int) -> float:
     return 0.0
 
 
-def cpu_baseline(spec, budget_s: float = 15.0) -> dict:
+def timed_oracle(step_once, budget_s: float, max_steps: int):
+    """Steps until budget_s seconds or max_steps steps; returns (steps, s)."""
+    k, t1 = 0, time.perf_counter()
+    while True:
+        step_once()
+        k += 1
+        el = time.perf_counter() - t1
+        if el >= budget_s or k >= max_steps:
+            return k, el
+
+
+def cpu_baseline(spec, budget_s: float = 10.0) -> dict:
+    """The FP64 oracle on the host cores: 10 warm-up steps, then 100 steps or
+    budget_s seconds, whichever comes first, on every core; then budget_s
+    seconds on one thread from the same state.  Setup and I/O excluded."""
     import oracle
 
     t0 = time.perf_counter()
     o = oracle.OracleSPH(spec)
-    o.step(spec.dt, 1)  # warm-up (first allocation of the lists)
-    steps = 0
-    t1 = time.perf_counter()
-    while True:
+    L = oracle.lib()
+    threads = int(L.o_num_threads())
+    for _ in range(10):
         o.step(spec.dt, 1)
-        steps += 1
-        el = time.perf_counter() - t1
-        if el >= budget_s or steps >= 50:
-            break
-    threads = int(oracle.lib().o_num_threads())
-    # The GPU path over the same 1 + steps steps from the same state: error
-    # against the FP64 oracle at full size.
+    t1 = time.perf_counter()
+    steps, el = timed_oracle(lambda: o.step(spec.dt, 1), budget_s, 100)
+    L.o_set_num_threads(1)
+    s1, e1 = timed_oracle(lambda: o.step(spec.dt, 1), budget_s, 100)
+    L.o_set_num_threads(threads)
+    total = 10 + steps + s1
+    # The GPU path over the same steps from the same state: error against
+    # the FP64 oracle at full size.
     import this_studio_amd as ts
 
     g = ts.SPHScene()
@@ -119,16 +185,19 @@ def cpu_baseline(spec, budget_s: float = 15.0) -> dict:
     g.add_fluid(spec.fluid_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
     g.add_dummy(spec.dummy_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
     g.initialize(spec.min_bound, spec.max_bound, spec.gravity)
-    g.step(spec.dt, 1 + steps)
-    parity = {"steps": 1 + steps,
+    g.step(spec.dt, total)
+    parity = {"steps": total,
               "max_dx_over_spacing": float(abs(g.get("position") - o.get("pos")).max() / spec.dx),
               "max_dv_m_per_s": float(abs(g.get("velocity") - o.get("vel")).max()),
               "max_drho_kg_per_m3": float(abs(g.get("density") - o.get("rho")).max())}
     g.close()
     o.close()
     return {"value": spec.nf * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (C, OpenMP) on the same {spec.nf}-fluid dam break, {steps} steps after "
-                      f"1 warm-up step ({el:.1f} s; setup {t1 - t0:.1f} s excluded)",
+            "cpu": cpu_model(), "logical_cpus": os.cpu_count(),
+            "sample": f"FP64 oracle (C, OpenMP, {threads} threads) on the same {spec.nf}-fluid dam break: "
+                      f"{steps} steps ({el:.1f} s) after 10 warm-up steps; setup {t1 - t0:.1f} s excluded",
+            "single_thread": {"value": spec.nf * s1 / e1, "unit": "particle-steps/s", "cores": 1,
+                              "sample": f"the next {s1} steps on one thread ({e1:.1f} s)"},
             "gpu_vs_oracle": parity}
 
 
@@ -151,22 +220,27 @@ def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     n = g.sizes()[0]
     out = {"workload": "dem_granular_collapse", "balls": n, "triangles": g.sizes()[3], "steps": steps,
            "value": round(n * steps / el, 1), "unit": "ball-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
-           "precision": "fp32 contact forces, fp64 positions/velocities/springs"}
+           "precision": "fp32 contact forces, fp64 positions/velocities/springs",
+           "roofline": dem_roofline(g, n, el / steps * 1e3)}
     if with_cpu:
         import oracle
 
         o = oracle.OracleDEM(s)
-        o.step(5)
-        k, t1 = 0, time.perf_counter()
-        while True:
-            o.step(10)
-            k += 10
-            ce = time.perf_counter() - t1
-            if ce >= cpu_budget or k >= 2000:
-                break
-        out["cpu_baseline"] = {"value": round(n * k / ce, 1), "unit": "ball-steps/s",
-                               "cores": int(oracle.lib().o_num_threads()), "kind": "port",
-                               "sample": f"FP64 oracle (C, OpenMP) on the same collapse, {k} steps ({ce:.1f} s)"}
+        L = oracle.lib()
+        threads = int(L.o_num_threads())
+        o.step(10)
+        k, ce = timed_oracle(lambda: o.step(10), cpu_budget, 200)
+        k *= 10
+        L.o_set_num_threads(1)
+        k1, c1 = timed_oracle(lambda: o.step(10), cpu_budget, 200)
+        k1 *= 10
+        L.o_set_num_threads(threads)
+        out["cpu_baseline"] = {"value": round(n * k / ce, 1), "unit": "ball-steps/s", "cores": threads,
+                               "kind": "port", "cpu": cpu_model(),
+                               "sample": f"FP64 oracle (C, OpenMP, {threads} threads) on the same collapse, "
+                                         f"{k} steps ({ce:.1f} s) after 10 warm-up steps",
+                               "single_thread": {"value": round(n * k1 / c1, 1), "unit": "ball-steps/s",
+                                                 "cores": 1, "sample": f"the next {k1} steps ({c1:.1f} s)"}}
     return out
 
 
@@ -191,10 +265,11 @@ def cantilever_clumps(steps: int) -> dict:
     n, nc = g.sizes()[0], g.sizes()[1]
     nbond = int(g.get("bondPointed").size)
     tip = float(g.get("position")[:, 2].min())
+    roof = dem_roofline(g, n, el / steps * 1e3)
     g.close()
     return {"workload": "dem_cantilever_clumps", "balls": n, "clumps": nc, "bonds": nbond, "steps": steps,
             "value": round(n * steps / el, 1), "unit": "ball-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
-            "lowest_ball_z_m": tip,
+            "lowest_ball_z_m": tip, "roofline": roof,
             "precision": "fp32 contact and bond forces, fp64 positions/velocities/springs/bond state"}
 
 
@@ -227,6 +302,13 @@ def ice_cylinder_coupled(steps: int) -> dict:
            "dt": spec.dt, "value": round((nf + nb) * steps / el, 1), "unit": "particle-steps/s (fluid + balls)",
            "ms_per_step": round(el / steps * 1e3, 4),
            "precision": "fp32 pair and contact forces, fp64 positions/velocities/springs"}
+    # the fluid half against the WCSPH step model, the ice half against the DEM one
+    fluid_b = step_model_bytes(nf, nd) * nf
+    dem_b = dem_roofline(g, nb, 1.0)["bytes_per_ball_step"] * nb
+    achieved = (fluid_b + dem_b) / (el / steps) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                       "model": "WCSPH bytes per fluid particle-step + DEM bytes per ball-step, whole coupled step"}
     c.close()
     g.close()
     f.close()
@@ -286,6 +368,33 @@ def serving_mix(count: int, scale: float, slots: int, autotune: str | None = Non
     return out
 
 
+def full_dam_break(spec) -> dict:
+    """The whole configured run: the dam break from rest to t = 0.5 s
+    (ceil(0.5 / dt) steps) on a fresh scene, as served (graph replays),
+    with the front position and column height at the end."""
+    import math
+
+    import this_studio_amd as ts
+
+    steps = int(math.ceil(0.5 / spec.dt))
+    g = ts.SPHScene()
+    g.set_eos(spec.eos, spec.gamma)
+    g.add_fluid(spec.fluid_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
+    g.add_dummy(spec.dummy_pos, None, spec.c0, spec.dx, spec.rho0, spec.nu)
+    g.initialize(spec.min_bound, spec.max_bound, spec.gravity)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(spec.dt, steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    p = g.get("position")
+    out = {"t_end_s": steps * spec.dt, "steps": steps, "wall_s": round(el, 3),
+           "ms_per_step": round(el / steps * 1e3, 4), "value": round(spec.nf * steps / el, 1),
+           "unit": "particle-steps/s", "front_x_m": float(p[:, 0].max()), "max_z_m": float(p[:, 2].max())}
+    g.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,7 +403,10 @@ def main() -> int:
     ap.add_argument("--dx", type=float, default=0.005)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--dem-steps", type=int, default=2000, help="steps of the DEM collapse leg (0: skip it)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=10.0,
+                    help="seconds of each CPU-baseline sample (all cores, then one thread)")
+    ap.add_argument("--full-run", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the whole 0.5 s dam break from rest")
     ap.add_argument("--mix-scenes", type=int, default=48, help="scenes of the serving-mix leg (0: skip it)")
     ap.add_argument("--beam-steps", type=int, default=400, help="steps of the cantilever-with-clumps leg (0: skip it)")
     ap.add_argument("--ice-steps", type=int, default=40, help="steps of the coupled ice-cylinder leg (0: skip it)")
@@ -394,9 +506,10 @@ def main() -> int:
     d_avg_s = d_total / d_launches * 1e-3
     d_bytes = algorithmic_bytes(dom, nf, pairs_ff, pairs_fd)
     achieved = d_bytes / d_avg_s / 1e9
+    traffic, traffic_src = profiled_traffic(dom)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                "traffic": MEASURED_HBM_BYTES.get(dom),
+                "traffic": traffic, "traffic_source": traffic_src,
                 "avg_launch_us": round(d_avg_s * 1e6, 2), "algorithmic_bytes": int(d_bytes),
                 "limiter": LIMITER.get(dom)}
     fl = algorithmic_flops(dom, pairs_ff, pairs_fd)
@@ -435,6 +548,19 @@ def main() -> int:
 
     # whole-step algorithmic HBM rate (sum over the step's kernels)
     step_bytes = sum(algorithmic_bytes(k, nf, pairs_ff, pairs_fd) * timing[k][0] for k in timing) / args.steps
+    # whole step against the compulsory-traffic model of one fluid
+    # particle-step, at the served step time
+    per_particle = step_model_bytes(nf, spec.nd)
+    step_achieved = per_particle * nf / (el / args.steps) / 1e9
+    step_roofline = {"bound": "hbm", "achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(step_achieved / HBM_PEAK_GBPS, 4), "bytes_per_particle_step": round(per_particle, 1),
+                     "model": "3*40 + 52 + 44 + 108*Nd/Nf compulsory bytes per fluid particle-step",
+                     "measured_traffic_per_step": None}
+    full = None
+    if args.full_run:
+        full = full_dam_break(spec)
+        print(f"[bench] full 0.5 s dam break: {full['steps']} steps, {full['ms_per_step']:.4f} ms/step, "
+              f"{full['wall_s']:.2f} s", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -490,6 +616,8 @@ def main() -> int:
                    "step_hbm_GBps": round(step_bytes / (el / args.steps) / 1e9, 1),
                    "precision": "fp32 pair arithmetic, fp64 position/velocity, fp32 drho"},
         "roofline": roofline,
+        "step_roofline": step_roofline,
+        "full_run": full,
         "cpu_baseline": cpu,
         "dem_collapse": dem,
         "serving_mix": mix,

@@ -75,6 +75,8 @@ def _declare(L):
     L.o_sph_set.restype = C.c_int
     L.o_sph_grid.argtypes = [_vp, C.POINTER(Grid)]
     L.o_num_threads.restype = C.c_int
+    L.o_set_num_threads.argtypes = [C.c_int]
+    L.o_set_num_threads.restype = None
 
 
 def d(a):

@@ -150,6 +150,7 @@ int o_sph_set(o_sph* s, const char* name, const void* in);
 void o_sph_grid(o_sph* s, o_grid* out);
 /* OpenMP threads the oracle runs with. */
 int o_num_threads(void);
+void o_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

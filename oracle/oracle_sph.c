@@ -663,6 +663,17 @@ int o_num_threads(void)
 #endif
 }
 
+/* Sets the threads of the oracle's parallel regions (the CPU baseline's
+ * one-thread run). */
+void o_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------- coupling */
 static void couple_load(o_sph* s, o_dem* d)
 {
