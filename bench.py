@@ -87,15 +87,28 @@ def profiled_traffic(name: str):
     return None, "no profile row"
 
 
-# What bounds each step kernel on this workload, from the counters in the
-# same profile (SQ_WAIT_ANY / SQ_WAVE_CYCLES, VALU and LDS instructions).
-LIMITER = {
-    "sph_lists": "instruction issue over ~420 candidate cut tests per particle, not HBM",
-    "sph_momentum_first": "latency of dependent neighbour gathers",
-    "sph_momentum_second": "latency of dependent neighbour gathers",
-    "sph_continuity": "latency of dependent neighbour gathers",
-    "sph_reorder": "HBM",
-}
+def profiled_limiter(name: str, avg_us: float):
+    """What bounds a step kernel, from the committed profile's counter table
+    (SQ_WAVES, SQ_INSTS_VALU, SQ_WAIT_ANY / SQ_WAVE_CYCLES per dispatch) and
+    this run's launch time: the SIMD time that VALU issue needs (a wave64
+    instruction holds its SIMD 4 cycles; 1,024 SIMDs at 2.4 GHz) against the
+    launch, and the share of wave cycles spent waiting."""
+    key = PROFILE_KERNEL.get(name)
+    if key is None or not PROFILE.exists() or not avg_us:
+        return None
+    for line in PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 8 and cells[0].startswith(key):
+            try:
+                waves, valu, wait = float(cells[1]), float(cells[2]), float(cells[6])
+            except ValueError:
+                return None
+            busy = waves * valu * 4 / (1024 * avg_us * 1e-6 * 2.4e9)
+            what = ("VALU issue" if busy >= 0.6 else "load latency" if wait >= 50 else
+                    "VALU issue and load latency together")
+            return {"bound_by": what, "valu_busy_est": round(busy, 2), "wait_frac_of_wave_cycles": wait / 100,
+                    "valu_per_wave": valu, "source": f"{PROFILE.name}: {cells[0]}"}
+    return None
 
 
 def step_model_bytes(nf: int, nd: int) -> float:
@@ -203,15 +216,15 @@ def cpu_baseline(spec, budget_s: float = 10.0) -> dict:
 
 def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     """The DEM granular collapse (4,096 Hertz-Mindlin spheres in a 0.3 m box
-    of 12 triangles, dt = 1e-5 s): ball-steps per second over `steps` steps
-    after solve() setup and 50 warm-up steps, state resident; the same scene
-    through the FP64 oracle beside it."""
+    of 12 triangles, dt = 1e-5 s): ball-steps per second over the first
+    `steps` steps after solve() setup (10,000: the whole configured run, the
+    spheres landing on the floor after ~3,200), state resident; the same
+    scene through the FP64 oracle beside it."""
     import this_studio_amd as ts
     from this_studio_amd import scenes
 
     s = scenes.granular_collapse()
     g = ts.DEMSceneGPU(s)
-    g.step(50)
     g.synchronize()
     t0 = time.perf_counter()
     g.step(steps)
@@ -248,15 +261,14 @@ def cantilever_clumps(steps: int) -> dict:
     """The bonded cantilever with falling clumps: a 128 x 16 x 16 lattice of
     32,768 bonded 2 mm spheres, clamped at x = 0, with a 10 N tip load ramped by
     the hook, and 20,000 tetrahedral 4-sphere clumps above it (112,768 balls in
-    all), dt = 5e-7 s.  Ball-steps per second over `steps` steps after solve()
-    setup (bonds from the initial pairs) and 20 warm-up steps, state
-    resident."""
+    all), dt = 5e-7 s.  Ball-steps per second over the first `steps` steps
+    after solve() setup (bonds from the initial pairs; 10,000 steps: the whole
+    load ramp), state resident."""
     import this_studio_amd as ts
     from this_studio_amd import scenes
 
     s = scenes.cantilever_with_clumps()
     g = ts.DEMSceneGPU(s)
-    g.step(20)
     g.synchronize()
     t0 = time.perf_counter()
     g.step(steps)
@@ -402,14 +414,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dx", type=float, default=0.005)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--dem-steps", type=int, default=2000, help="steps of the DEM collapse leg (0: skip it)")
+    ap.add_argument("--dem-steps", type=int, default=10000, help="steps of the DEM collapse leg (0: skip it)")
     ap.add_argument("--cpu-budget", type=float, default=10.0,
                     help="seconds of each CPU-baseline sample (all cores, then one thread)")
     ap.add_argument("--full-run", action=argparse.BooleanOptionalAction, default=True,
                     help="also time the whole 0.5 s dam break from rest")
     ap.add_argument("--mix-scenes", type=int, default=48, help="scenes of the serving-mix leg (0: skip it)")
-    ap.add_argument("--beam-steps", type=int, default=400, help="steps of the cantilever-with-clumps leg (0: skip it)")
-    ap.add_argument("--ice-steps", type=int, default=40, help="steps of the coupled ice-cylinder leg (0: skip it)")
+    ap.add_argument("--beam-steps", type=int, default=10000,
+                    help="steps of the cantilever-with-clumps leg (0: skip it)")
+    ap.add_argument("--ice-steps", type=int, default=200, help="steps of the coupled ice-cylinder leg (0: skip it)")
     ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
     ap.add_argument("--mix-slots", type=int, default=8, help="concurrent jobs of the batched server")
     ap.add_argument("--mix-autotune", choices=("off", "exact", "pairs"), default="exact",
@@ -511,7 +524,7 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                 "traffic": traffic, "traffic_source": traffic_src,
                 "avg_launch_us": round(d_avg_s * 1e6, 2), "algorithmic_bytes": int(d_bytes),
-                "limiter": LIMITER.get(dom)}
+                "limiter": profiled_limiter(dom, d_avg_s * 1e6)}
     fl = algorithmic_flops(dom, pairs_ff, pairs_fd)
     if fl:
         roofline["fp32_valu_tflops"] = round(fl / d_avg_s / 1e12, 2)

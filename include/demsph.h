@@ -433,6 +433,9 @@ int demsph_dem_graph_captures(demsph_dem* d, long* count);
  * else 1), 1, 2, 4 or 8.  Every setting builds the same pair lists in the
  * same order.  "pairCapacity": entries of each of the two pair lists,
  * set between initialize and solve_begin (they grow on demand from there).
+ * "wallSlots": 1 (default) lets scenes of up to 32,768 balls evaluate
+ * ball-wall contacts of cells with at most four candidate triangles in
+ * registers, 0 keeps every ball on the general path (same results).
  * demsph_dem_tune_get reads the value in effect, and "pairReplays", the
  * step calls run again after a pair list outgrew its capacity. */
 int demsph_dem_tune(demsph_dem* d, const char* knob, int value);

@@ -771,3 +771,26 @@ def test_pair_lists_grown_on_device_replay_bit_for_bit():
     assert small[1].graph_captures() == 0
     for g in [a] + small:
         g.close()
+
+
+def test_wall_register_slots_equal_general_wall_path():
+    """Small walled scenes take ball-wall contacts through register slots
+    (cells with up to four candidate triangles); the general candidate path
+    must give the same wall histories and trajectories bit for bit, in FP32
+    and FP64."""
+    for fp64 in (False, True):
+        s = scenes.granular_collapse()
+        a = ts.DEMSceneGPU(s, fp64=fp64)
+        b = ts.DEMSceneGPU(s, fp64=fp64, solve_begin=False)
+        b.tune("wallSlots", 0)
+        b.solve_begin()
+        assert a.tuned("wallSlots") == 1 and b.tuned("wallSlots") == 0
+        for n in (3, 597, 3000):  # the lowest layer lands after ~3,200 steps
+            a.step(n)
+            b.step(n)
+            for f in ("position", "velocity", "angularVelocity", "slidingSpring", "triPointing", "triCancel",
+                      "triSlidingSpring", "triOverlap"):
+                assert np.array_equal(a.get(f), b.get(f)), (fp64, n, f)
+        assert (a.get("triCancel") == 0).sum() > 50  # the lowest layer reaches the floor
+        a.close()
+        b.close()

@@ -54,9 +54,11 @@ struct Pairs {
     int* start;    // n + 1 (owner CSR)
     int cap;       // entries allocated; a list longer than cap is truncated (and replayed)
     int* partner;  // cap
+    int* owner;    // cap, the owner slot of each entry (written with the list)
     uint8_t* shift;  // cap, periodic image code of the partner (periodic scenes only, else null)
     double *sx, *sy, *sz, *rx, *ry, *rz, *ox, *oy, *oz;  // sliding, rolling, torsion springs
     void *Fx, *Fy, *Fz, *Rx, *Ry, *Rz;                   // pair force, partner-side torque (precision T)
+    void *Ix, *Iy, *Iz;                                  // owner-side torque (precision T)
     double* ov;                                           // overlap
 };
 
@@ -311,6 +313,7 @@ __global__ __launch_bounds__(kT) void k_write_pairs(WritePairsArgs a)
             return;
         }
         a.cur.partner[w] = j;
+        a.cur.owner[w] = i;
         if (PER) a.cur.shift[w] = (uint8_t)code;
         double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         const int ob = a.b.orig[j];
@@ -407,75 +410,63 @@ struct PairForceArgs {
     // optional record of every pair of the evaluation, 12 doubles per pair
     // slot: contact point, normal, force, torque (without the lever term)
     double* rec;
-    void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner-side sums per slot (precision T)
     Periodic per;  // used when p.shift is set
 };
 
 // updateBallContactKernel + calBallContactForceTorqueKernel
-// (contactKernel.cu:135-259), one thread per owner ball over its pairs, and
-// the owner half of sumObjectPointedForceTorque (:657-681).
+// (contactKernel.cu:135-259), one thread per pair entry p of the owner CSR
+// (entries past the list's end or capacity return).  Each entry leaves the
+// pair force, the owner-side and the partner-side torque; the gather sums
+// them per ball in list order (sumObjectPointedForceTorque,
+// sumObjectPointingForceTorqueFromInteraction, :657-714).
 template <class T>
-__device__ __forceinline__ void pair_forces_ball(const PairForceArgs& a, int i)
+__device__ __forceinline__ void pair_force_one(const PairForceArgs& a, int p)
 {
+    const int i = a.p.owner[p];
     const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i);
     const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i);
     const D3 wi = ld3(a.b.wx, a.b.wy, a.b.wz, i);
     const double ai = a.b.r[i], imi = a.b.im[i];
     const int mi = a.b.mat[i];
-    V3<T> Fo{0, 0, 0}, To{0, 0, 0};
-    const int p0 = a.p.start[i], p1 = min(a.p.start[i + 1], a.p.cap);
-    for (int p = p0; p < p1; p++) {
-        const int j = a.p.partner[p];
-        const D3 rj = a.p.shift ? shift_point(ld3(a.b.px, a.b.py, a.b.pz, j), a.p.shift[p], a.per)
-                                : ld3(a.b.px, a.b.py, a.b.pz, j);
-        const double aj = a.b.r[j];
-        const D3 rij = ri - rj;
-        const D3 n = normalize(rij);
-        const double delta = ai + aj - len(rij);
-        const D3 rc = rj + n * (aj - 0.5 * delta);
-        a.p.ov[p] = delta;
-        D3 es{a.p.sx[p], a.p.sy[p], a.p.sz[p]}, er{a.p.rx[p], a.p.ry[p], a.p.rz[p]}, et{a.p.ox[p], a.p.oy[p], a.p.oz[p]};
-        V3<T> F{0, 0, 0}, Tq{0, 0, 0}, Fs{0, 0, 0};
-        if (delta > 0) {
-            const double R = ai * aj / (ai + aj);
-            const double m = 1. / (imi + a.b.im[j]);
-            const D3 vj = ld3(a.b.vx, a.b.vy, a.b.vz, j), wj = ld3(a.b.wx, a.b.wy, a.b.wz, j);
-            const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
-            const int ip = pair_index(mi, a.b.mat[j], a.tb.nMat, a.tb.cap);
-            contact_model<T>(a.tb, ip, m, R, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es, er, et, F, Tq,
-                             Fs);
-        } else {
-            es = er = et = D3{0, 0, 0};
-        }
-        a.p.sx[p] = es.x; a.p.sy[p] = es.y; a.p.sz[p] = es.z;
-        a.p.rx[p] = er.x; a.p.ry[p] = er.y; a.p.rz[p] = er.z;
-        a.p.ox[p] = et.x; a.p.oy[p] = et.y; a.p.oz[p] = et.z;
-        // T_c + (r_c - r) x F_c (contactKernel.cu:676, :707-708), with the
-        // normal part of F_c (parallel to the lever arm) left out
-        const V3<T> li = cvt<T>(rc - ri), lj = cvt<T>(rc - rj);
-        const V3<T> Ti = Tq + cross(li, Fs);
-        const V3<T> Tj = Tq + cross(lj, Fs);
-        ((T*)a.p.Fx)[p] = F.x; ((T*)a.p.Fy)[p] = F.y; ((T*)a.p.Fz)[p] = F.z;
-        ((T*)a.p.Rx)[p] = Tj.x; ((T*)a.p.Ry)[p] = Tj.y; ((T*)a.p.Rz)[p] = Tj.z;
-        if (a.rec) {
-            double* r = a.rec + (size_t)p * 12;
-            r[0] = rc.x; r[1] = rc.y; r[2] = rc.z;
-            r[3] = n.x; r[4] = n.y; r[5] = n.z;
-            r[6] = F.x; r[7] = F.y; r[8] = F.z;
-            r[9] = Tq.x; r[10] = Tq.y; r[11] = Tq.z;
-        }
-        Fo = Fo + F;
-        To = To + Ti;
+    const int j = a.p.partner[p];
+    const D3 rj = a.p.shift ? shift_point(ld3(a.b.px, a.b.py, a.b.pz, j), a.p.shift[p], a.per)
+                            : ld3(a.b.px, a.b.py, a.b.pz, j);
+    const double aj = a.b.r[j];
+    const D3 rij = ri - rj;
+    const D3 n = normalize(rij);
+    const double delta = ai + aj - len(rij);
+    const D3 rc = rj + n * (aj - 0.5 * delta);
+    a.p.ov[p] = delta;
+    D3 es{a.p.sx[p], a.p.sy[p], a.p.sz[p]}, er{a.p.rx[p], a.p.ry[p], a.p.rz[p]}, et{a.p.ox[p], a.p.oy[p], a.p.oz[p]};
+    V3<T> F{0, 0, 0}, Tq{0, 0, 0}, Fs{0, 0, 0};
+    if (delta > 0) {
+        const double R = ai * aj / (ai + aj);
+        const double m = 1. / (imi + a.b.im[j]);
+        const D3 vj = ld3(a.b.vx, a.b.vy, a.b.vz, j), wj = ld3(a.b.wx, a.b.wy, a.b.wz, j);
+        const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
+        const int ip = pair_index(mi, a.b.mat[j], a.tb.nMat, a.tb.cap);
+        contact_model<T>(a.tb, ip, m, R, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es, er, et, F, Tq, Fs);
+    } else {
+        es = er = et = D3{0, 0, 0};
     }
-    ((T*)a.ofx)[i] = Fo.x; ((T*)a.ofy)[i] = Fo.y; ((T*)a.ofz)[i] = Fo.z;
-    ((T*)a.otx)[i] = To.x; ((T*)a.oty)[i] = To.y; ((T*)a.otz)[i] = To.z;
-}
-template <class T>
-__global__ __launch_bounds__(kT) void k_pair_forces(PairForceArgs a)
-{
-    const int i = xcd_block() * kT + threadIdx.x;
-    if (i >= a.b.n) return;
-    pair_forces_ball<T>(a, i);
+    a.p.sx[p] = es.x; a.p.sy[p] = es.y; a.p.sz[p] = es.z;
+    a.p.rx[p] = er.x; a.p.ry[p] = er.y; a.p.rz[p] = er.z;
+    a.p.ox[p] = et.x; a.p.oy[p] = et.y; a.p.oz[p] = et.z;
+    // T_c + (r_c - r) x F_c (contactKernel.cu:676, :707-708), with the
+    // normal part of F_c (parallel to the lever arm) left out
+    const V3<T> li = cvt<T>(rc - ri), lj = cvt<T>(rc - rj);
+    const V3<T> Ti = Tq + cross(li, Fs);
+    const V3<T> Tj = Tq + cross(lj, Fs);
+    ((T*)a.p.Fx)[p] = F.x; ((T*)a.p.Fy)[p] = F.y; ((T*)a.p.Fz)[p] = F.z;
+    ((T*)a.p.Ix)[p] = Ti.x; ((T*)a.p.Iy)[p] = Ti.y; ((T*)a.p.Iz)[p] = Ti.z;
+    ((T*)a.p.Rx)[p] = Tj.x; ((T*)a.p.Ry)[p] = Tj.y; ((T*)a.p.Rz)[p] = Tj.z;
+    if (a.rec) {
+        double* r = a.rec + (size_t)p * 12;
+        r[0] = rc.x; r[1] = rc.y; r[2] = rc.z;
+        r[3] = n.x; r[4] = n.y; r[5] = n.z;
+        r[6] = F.x; r[7] = F.y; r[8] = F.z;
+        r[9] = Tq.x; r[10] = Tq.y; r[11] = Tq.z;
+    }
 }
 
 struct BondArgs {
@@ -496,10 +487,8 @@ struct BondArgs {
 // goes to both balls through the per-ball gather (no atomics); the lever
 // arm of the shear force is taken about each ball centre.
 template <class T>
-__global__ __launch_bounds__(kT) void k_bond_forces(BondArgs a)
+__device__ __forceinline__ void bond_forces_one(const BondArgs& a, int k)
 {
-    const int k = blockIdx.x * kT + threadIdx.x;
-    if (k >= a.nb) return;
     if (a.bonded[k] == 0) {
         a.bFn[k] = 0; a.bTt[k] = 0;
         a.bsx[k] = a.bsy[k] = a.bsz[k] = 0;
@@ -668,7 +657,157 @@ struct WallArgs {
 // calBallWallContactForceTorqueKernel (:420-533), fused per ball.  The
 // candidate list, its spring history and cancel flags live in registers
 // and are written back as the new history.
-template <class T>
+// wall_contacts_ball for a ball whose cell lists at most NS candidate
+// triangles: every per-candidate value sits in a register slot indexed at
+// compile time (slot u = the cell's candidate u), so nothing goes through
+// scratch memory.  A slot is live when its triangle box is within reach
+// and capacity remains; live slots, in order, are the general path's
+// compacted candidates, so every result and its order are the same.
+template <class T, int NS>
+__device__ __forceinline__ void wall_contacts_slots(const WallArgs& a, int i, int o, const D3& ri, double rad,
+                                                    int c0, int c1)
+{
+    const int N = a.nballs;
+    const int cap = min(16, a.KW);
+    const double r2 = rad * rad;
+    bool live[NS];
+    int tri[NS], cidx[NS];
+    int cnt = 0, hits = 0;
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int j = c0 + u;
+        bool in = false;
+        if (j < c1) {
+            const double2* q = a.cbox + 3 * (size_t)j;
+            in = box_within(ri, r2, q[0], q[1], q[2]);
+        }
+        live[u] = in && hits < cap;
+        hits += in ? 1 : 0;
+        tri[u] = live[u] ? a.ctri[j] : -1;
+        cidx[u] = cnt;
+        cnt += live[u] ? 1 : 0;
+    }
+    if (hits > cap) atomicMax(a.overflow, hits);
+    D3 es[NS], er[NS], et[NS], cp[NS];
+    int type[NS], cancel[NS];
+    const int oldc = a.hcnt[o];
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        es[u] = er[u] = et[u] = D3{0, 0, 0};
+        cp[u] = D3{0, 0, 0};
+        type[u] = ST_NONE;
+        cancel[u] = 0;
+        if (!live[u]) continue;
+        for (int q = 0; q < oldc; q++) {
+            const size_t h = (size_t)q * N + o;
+            if (a.htri[h] == tri[u]) {
+                es[u] = D3{a.hsx[h], a.hsy[h], a.hsz[h]};
+                er[u] = D3{a.hrx[h], a.hry[h], a.hrz[h]};
+                et[u] = D3{a.hox[h], a.hoy[h], a.hoz[h]};
+                break;
+            }
+        }
+        const int t = tri[u];
+        const D3 p0 = ld3(a.vgx, a.vgy, a.vgz, a.t0[t]), p1 = ld3(a.vgx, a.vgy, a.vgz, a.t1[t]),
+                 p2 = ld3(a.vgx, a.vgy, a.vgz, a.t2[t]);
+        type[u] = classify(ri, rad, p0, p1, p2, cp[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        if (!live[u]) continue;
+        if (type[u] == ST_NONE) {
+            es[u] = er[u] = et[u] = D3{0, 0, 0};
+            cancel[u] = 1;
+            continue;
+        }
+        if (type[u] == ST_FACE) continue;
+        bool done = false;
+#pragma unroll
+        for (int u1 = 0; u1 < NS; u1++) {
+            if (u1 == u || done || !live[u1]) continue;
+            const int t1 = type[u1];
+            if (t1 == ST_NONE) continue;
+            bool hit = false;
+            if (t1 == ST_FACE) {
+                const int t = tri[u1];
+                const D3 q0 = ld3(a.vgx, a.vgy, a.vgz, a.t0[t]), q1 = ld3(a.vgx, a.vgy, a.vgz, a.t1[t]),
+                         q2 = ld3(a.vgx, a.vgy, a.vgz, a.t2[t]);
+                const D3 e01 = cross(cp[u] - q0, q1 - q0), e12 = cross(cp[u] - q1, q2 - q1),
+                         e02 = cross(cp[u] - q0, q2 - q0);
+                hit = dot(e01, e01) < 1.e-20 || dot(e12, e12) < 1.e-20 || dot(e02, e02) < 1.e-20;
+            } else {
+                const D3 d = cp[u] - cp[u1];
+                const bool same = dot(d, d) < 1.e-20;
+                if (t1 == ST_EDGE) hit = (type[u] == t1) ? (u1 < u && same) : same;
+                else hit = (type[u] == t1) && u1 < u && same;
+            }
+            if (hit) {
+                es[u] = es[u1]; er[u] = er[u1]; et[u] = et[u1];
+                cancel[u] = 1;
+                done = true;
+            }
+        }
+    }
+    V3<T> Fw{0, 0, 0}, Tw{0, 0, 0};
+    const double im = a.b.im[i];
+    if (!(im < 1.e-20)) {
+        const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i), wi = ld3(a.b.wx, a.b.wy, a.b.wz, i);
+        const double m = 1. / im;
+#pragma unroll
+        for (int u = 0; u < NS; u++) {
+            if (!live[u] || cancel[u]) continue;
+            const int w = a.twall[tri[u]];
+            const D3 rc = cp[u];
+            const D3 n = normalize(ri - rc);
+            const double delta = rad - len(ri - rc);
+            const D3 rj = ld3(a.wpx, a.wpy, a.wpz, w), vj = ld3(a.wvx, a.wvy, a.wvz, w), wj = ld3(a.wwx, a.wwy, a.wwz, w);
+            const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
+            const int ip = pair_index(a.b.mat[i], a.wmat[w], a.tb.nMat, a.tb.cap);
+            V3<T> F{0, 0, 0}, Tq{0, 0, 0}, Fs{0, 0, 0};
+            contact_model<T>(a.tb, ip, m, rad, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es[u], er[u], et[u],
+                             F, Tq, Fs);
+            Fw = Fw + F;
+            Tw = Tw + Tq + cross(cvt<T>(rc - ri), Fs);
+            if (a.rec) {
+                double* r = a.rec + ((size_t)cidx[u] * N + o) * 12;
+                r[6] = F.x; r[7] = F.y; r[8] = F.z;
+                r[9] = Tq.x; r[10] = Tq.y; r[11] = Tq.z;
+            }
+        }
+    }
+    a.hcnt[o] = cnt;
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        if (!live[u]) continue;
+        const size_t h = (size_t)cidx[u] * N + o;
+        a.htri[h] = tri[u];
+        a.hcancel[h] = cancel[u];
+        a.hsx[h] = es[u].x; a.hsy[h] = es[u].y; a.hsz[h] = es[u].z;
+        a.hrx[h] = er[u].x; a.hry[h] = er[u].y; a.hrz[h] = er[u].z;
+        a.hox[h] = et[u].x; a.hoy[h] = et[u].y; a.hoz[h] = et[u].z;
+        a.hov[h] = rad - len(ri - cp[u]);
+        if (a.rec) {
+            double* r = a.rec + h * 12;
+            const D3 n = normalize(ri - cp[u]);
+            r[0] = cp[u].x; r[1] = cp[u].y; r[2] = cp[u].z;
+            r[3] = n.x; r[4] = n.y; r[5] = n.z;
+            if (cancel[u] || im < 1.e-20)
+                for (int k = 6; k < 12; k++) r[k] = 0.0;
+        }
+    }
+    ((T*)a.ofx)[i] = Fw.x; ((T*)a.ofy)[i] = Fw.y; ((T*)a.ofz)[i] = Fw.z;
+    ((T*)a.otx)[i] = Tw.x; ((T*)a.oty)[i] = Tw.y; ((T*)a.otz)[i] = Tw.z;
+}
+
+// Slots of the register path (the cells of a box of walls list 2 to 6
+// triangles: two per face).
+constexpr int kWallSlots = 4;
+// Scenes up to this many balls put at most one wave of pair work and one of
+// wall work on each SIMD, so the register path's lower occupancy costs
+// nothing there.
+constexpr int kWallSlotBalls = 32768;
+
+template <class T, bool SLOTS = false>
 __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
 {
     constexpr int KMAX = 16;
@@ -685,6 +824,10 @@ __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
     const int cell = (st.cz * a.tg.ny + st.cy) * a.tg.nx + st.cx;
     const double r2 = rad * rad;
     const int c0 = a.cstart[cell], c1 = a.cstart[cell + 1];
+    if (SLOTS && c1 - c0 <= kWallSlots) {
+        wall_contacts_slots<T, kWallSlots>(a, i, o, ri, rad, c0, c1);
+        return;
+    }
     for (int j0 = c0; j0 < c1; j0 += 4) {
         double2 bx[4], by[4], bz[4];
 #pragma unroll
@@ -822,7 +965,9 @@ __global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
 // ----------------------------------------------------------------- gather
 struct GatherArgs {
     Balls b;
-    const void *ofx, *ofy, *ofz, *otx, *oty, *otz;  // owner sums
+    const int *pstart;                            // owner CSR of the pairs
+    int pcap;
+    const void *pIx, *pIy, *pIz;                  // owner-side pair torques
     const void *pFx, *pFy, *pFz, *pRx, *pRy, *pRz;  // pair results
     const int *rstart, *rpair;
     const int *istart, *ibond;                    // bond incidence by insertion index (id*2 + side)
@@ -839,8 +984,15 @@ __device__ __forceinline__ T at(const void* p, int i) { return ((const T*)p)[i];
 template <class T>
 __device__ __forceinline__ void gather_ball(const GatherArgs& a, int j)
 {
-    T fx = at<T>(a.ofx, j), fy = at<T>(a.ofy, j), fz = at<T>(a.ofz, j);
-    T tx = at<T>(a.otx, j), ty = at<T>(a.oty, j), tz = at<T>(a.otz, j);
+    // owner side: the sum over the ball's own entries, in list order
+    V3<T> Fo{0, 0, 0}, To{0, 0, 0};
+    const int p1 = min(a.pstart[j + 1], a.pcap);
+    for (int p = a.pstart[j]; p < p1; p++) {
+        Fo = Fo + V3<T>{at<T>(a.pFx, p), at<T>(a.pFy, p), at<T>(a.pFz, p)};
+        To = To + V3<T>{at<T>(a.pIx, p), at<T>(a.pIy, p), at<T>(a.pIz, p)};
+    }
+    T fx = Fo.x, fy = Fo.y, fz = Fo.z;
+    T tx = To.x, ty = To.y, tz = To.z;
     for (int q = a.rstart[j]; q < a.rstart[j + 1]; q++) {
         const int p = a.rpair[q];
         fx -= at<T>(a.pFx, p); fy -= at<T>(a.pFy, p); fz -= at<T>(a.pFz, p);
@@ -979,13 +1131,33 @@ __global__ __launch_bounds__(kT) void k_ball_half(BallHalfArgs a)
 // The two per-ball loops of a force evaluation as one launch: contacts over
 // the ball's owned pairs, then its wall contacts.  Neither reads what the
 // other writes.
-template <class T>
-__global__ __launch_bounds__(kT) void k_pair_wall_forces(PairForceArgs pa, WallArgs wa)
+// One launch for the force terms of an evaluation, in three block ranges:
+// ball pairs, ball-wall contacts and bonds.  The three write disjoint
+// outputs, so their latency chains
+// overlap on the device instead of running back to back.  WALLS: 0 none,
+// 1 the general candidate path, 2 register slots for cells of up to
+// kWallSlots candidates (more registers, fewer waves per SIMD: used for
+// scenes too small to fill the device).
+//
+// Each range is padded to a multiple of the XCD count and every XCD takes an
+// eighth of each range (consecutive blocks of it, for L2 reuse): blocks are
+// dispatched to XCDs round-robin, so hardware block b runs on XCD b % 8 as
+// the (b / 8)-th of that XCD's blocks.  Ranges laid end to end instead would
+// crowd the heavier pair blocks onto a few XCDs.
+template <class T, int WALLS, bool BONDS>
+__global__ __launch_bounds__(kT) void k_forces(PairForceArgs pa, WallArgs wa, BondArgs ba, int qp, int qw, int qb)
 {
-    const int i = xcd_block() * kT + threadIdx.x;
-    if (i >= pa.b.n) return;
-    pair_forces_ball<T>(pa, i);
-    wall_contacts_ball<T>(wa, i);
+    const int xcd = blockIdx.x % kNumXcd, pos = blockIdx.x / kNumXcd;
+    if (pos < qp) {
+        const int p = (xcd * qp + pos) * kT + threadIdx.x;
+        if (p < min(pa.p.start[pa.b.n], pa.p.cap)) pair_force_one<T>(pa, p);
+    } else if (WALLS && pos < qp + qw) {
+        const int i = (xcd * qw + pos - qp) * kT + threadIdx.x;
+        if (i < wa.b.n) wall_contacts_ball<T, WALLS == 2>(wa, i);
+    } else if (BONDS && pos < qp + qw + qb) {
+        const int k = (xcd * qb + pos - qp - qw) * kT + threadIdx.x;
+        if (k < ba.nb) bond_forces_one<T>(ba, k);
+    }
 }
 
 // A ball's force and torque gather and its half kick (with the drift in the
@@ -1221,6 +1393,7 @@ struct demsph_dem {
     long long bng = 0, tng = 0;
     bool bgValid = false, tgValid = false;
     int nMat = 0, cap = 0;
+    int wall_slots_set = 1;  // register-slot wall path for small scenes (tuning knob "wallSlots")
     KernelTimer timer;
     StepStamps stamps;  // step-end timestamps (batched serving)
 
@@ -1237,9 +1410,10 @@ struct demsph_dem {
     DBuf<int> cellStart, cellEnd;
     // pairs: two lists, cp is the live one
     int pcap[2] = {0, 0};
-    DBuf<int> pstart[2], ppartner[2];
+    DBuf<int> pstart[2], ppartner[2], powner[2];
     DBuf<double> psx[2], psy[2], psz[2], prx[2], pry[2], prz[2], pox[2], poy[2], poz[2], pov[2];
     DBuf<double> pFx[2], pFy[2], pFz[2], pRx[2], pRy[2], pRz[2];  // hold float or double
+    DBuf<double> pIx[2], pIy[2], pIz[2];                            // hold float or double
     int cp = 0;
     bool has_list = false;
     int npairs = 0;
@@ -1251,7 +1425,6 @@ struct demsph_dem {
     void* scan_temp = nullptr;
     DBuf<int> total;  // device int
     int* total_h = nullptr;
-    DBuf<double> ofx, ofy, ofz, otx, oty, otz;  // hold float or double
     // material tables
     DBuf<double> tH, tL, tB;
     // bonds
@@ -1384,12 +1557,13 @@ struct demsph_dem {
     Pairs pairs(int which) const
     {
         Pairs p;
-        p.start = pstart[which].p; p.partner = ppartner[which].p; p.cap = pcap[which];
+        p.start = pstart[which].p; p.partner = ppartner[which].p; p.owner = powner[which].p; p.cap = pcap[which];
         p.sx = psx[which].p; p.sy = psy[which].p; p.sz = psz[which].p;
         p.rx = prx[which].p; p.ry = pry[which].p; p.rz = prz[which].p;
         p.ox = pox[which].p; p.oy = poy[which].p; p.oz = poz[which].p;
         p.Fx = pFx[which].p; p.Fy = pFy[which].p; p.Fz = pFz[which].p;
         p.Rx = pRx[which].p; p.Ry = pRy[which].p; p.Rz = pRz[which].p;
+        p.Ix = pIx[which].p; p.Iy = pIy[which].p; p.Iz = pIz[which].p;
         p.ov = pov[which].p;
         p.shift = periodic ? pshift[which].p : nullptr;
         return p;
@@ -1400,12 +1574,14 @@ struct demsph_dem {
     {
         c = std::max(c, 1);
         ppartner[which].alloc(c);
+        powner[which].alloc(c);
         psx[which].alloc(c); psy[which].alloc(c); psz[which].alloc(c);
         prx[which].alloc(c); pry[which].alloc(c); prz[which].alloc(c);
         pox[which].alloc(c); poy[which].alloc(c); poz[which].alloc(c);
         pov[which].alloc(c);
         pFx[which].alloc(c); pFy[which].alloc(c); pFz[which].alloc(c);
         pRx[which].alloc(c); pRy[which].alloc(c); pRz[which].alloc(c);
+        pIx[which].alloc(c); pIy[which].alloc(c); pIz[which].alloc(c);
         if (periodic) pshift[which].alloc(c);
         pcap[which] = c;
         if (record_contacts) prec.alloc(12 * (size_t)std::max(pcap[0], pcap[1]));
@@ -1586,7 +1762,6 @@ struct demsph_dem {
         iota_i32(borig[0].p, n, stream);
         iota_i32(rank.p, n, stream);
         for (auto* d : {&fx, &fy, &fz, &tx, &ty, &tz}) d->alloc(n);
-        for (auto* d : {&ofx, &ofy, &ofz, &otx, &oty, &otz}) d->alloc(n);
         key.alloc(n); skey.alloc(n); ktmp.alloc(n); perm.alloc(n); vtmp.alloc(n);
         DEMSPH_HIP(dev_malloc(&sort_temp, radix_sort_temp_bytes(n)));
         cellStart.alloc(bng); cellEnd.alloc(bng);
@@ -1784,8 +1959,8 @@ struct demsph_dem {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         Balls b = cur();
         const bool walls = do_walls && nt > 0;
-        if (do_pairs) evaluate_pairs(springDt);
-        if (walls) evaluate_walls(wallDt);
+        if (do_pairs) evaluate_pairs(springDt, walls, wallDt);
+        else if (walls) evaluate_walls(wallDt);
         gather(nt > 0 && (walls || walls_valid));
         if (walls) {
             walls_valid = true;
@@ -1798,39 +1973,63 @@ struct demsph_dem {
         PairForceArgs pa{};
         pa.b = cur(); pa.p = pairs(cp); pa.tb = tables(); pa.dt = springDt;
         pa.rec = record_contacts ? prec.p : nullptr;
-        pa.ofx = ofx.p; pa.ofy = ofy.p; pa.ofz = ofz.p; pa.otx = otx.p; pa.oty = oty.p; pa.otz = otz.p;
         pa.per = per;
         return pa;
     }
 
+    BondArgs bond_args(double springDt) const
+    {
+        BondArgs ba{};
+        ba.nb = nbond; ba.b = cur(); ba.rank = rank.p; ba.bi = dbi.p; ba.bj = dbj.p; ba.bonded = dbonded.p;
+        ba.bpx = dbpx.p; ba.bpy = dbpy.p; ba.bpz = dbpz.p; ba.bnx = dbnx.p; ba.bny = dbny.p; ba.bnz = dbnz.p;
+        ba.bsx = dbsx.p; ba.bsy = dbsy.p; ba.bsz = dbsz.p; ba.btx = dbtx.p; ba.bty = dbty.p; ba.btz = dbtz.p;
+        ba.bFn = dbFn.p; ba.bTt = dbTt.p; ba.bsig = dbsig.p; ba.btau = dbtau.p;
+        ba.Fx = dbFx.p; ba.Fy = dbFy.p; ba.Fz = dbFz.p;
+        ba.Tix = dbTix.p; ba.Tiy = dbTiy.p; ba.Tiz = dbTiz.p; ba.Tjx = dbTjx.p; ba.Tjy = dbTjy.p; ba.Tjz = dbTjz.p;
+        ba.tb = tables(); ba.dt = springDt;
+        ba.periodic = periodic ? 1 : 0; ba.per = per;
+        return ba;
+    }
+
+    template <class T, int WALLS>
+    void forces_launch(bool bonds, const PairForceArgs& pa, const WallArgs& wa, const BondArgs& ba, int nbp, int nbw,
+                       int nbb)
+    {
+        // blocks per XCD of each range (k_forces)
+        const int qp = (nbp + kNumXcd - 1) / kNumXcd, qw = (nbw + kNumXcd - 1) / kNumXcd,
+                  qb = (nbb + kNumXcd - 1) / kNumXcd;
+        const dim3 gb(kNumXcd * (qp + qw + qb)), tb(kT);
+        if (bonds) DEMSPH_LAUNCH(timer, "dem_forces", (k_forces<T, WALLS, true>), gb, tb, 0, stream, pa, wa, ba, qp, qw, qb);
+        else DEMSPH_LAUNCH(timer, "dem_forces", (k_forces<T, WALLS, false>), gb, tb, 0, stream, pa, wa, ba, qp, qw, qb);
+    }
+    template <class T>
+    void forces_as(int walls, bool bonds, const PairForceArgs& pa, const WallArgs& wa, const BondArgs& ba, int nbp,
+                   int nbw, int nbb)
+    {
+        if (walls == 2) forces_launch<T, 2>(bonds, pa, wa, ba, nbp, nbw, nbb);
+        else if (walls == 1) forces_launch<T, 1>(bonds, pa, wa, ba, nbp, nbw, nbb);
+        else forces_launch<T, 0>(bonds, pa, wa, ba, nbp, nbw, nbb);
+    }
+
+    // Ball-pair forces, and with them the bond forces and (with_walls) the
+    // ball-wall contacts, in one launch.
     void evaluate_pairs(double springDt, bool with_walls = false, double wallDt = 0.0)
     {
-        const dim3 gb(blocks_for(n, kT)), tb(kT);
-        Balls b = cur();
+        const int nbp = blocks_for(pcap[cp], kT);  // one thread per list entry
         const PairForceArgs pa = pair_args(springDt);
+        WallArgs wa{};
+        int walls = 0, nbw = 0;
         if (with_walls) {
             prepare_walls();
-            const WallArgs wa = wall_args(wallDt);
-            if (fp64) DEMSPH_LAUNCH(timer, "dem_pair_wall_forces", k_pair_wall_forces<double>, gb, tb, 0, stream, pa, wa);
-            else DEMSPH_LAUNCH(timer, "dem_pair_wall_forces", k_pair_wall_forces<float>, gb, tb, 0, stream, pa, wa);
-        } else if (fp64) {
-            DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<double>, gb, tb, 0, stream, pa);
-        } else {
-            DEMSPH_LAUNCH(timer, "dem_pair_forces", k_pair_forces<float>, gb, tb, 0, stream, pa);
+            wa = wall_args(wallDt);
+            walls = wall_slots_set && n <= kWallSlotBalls ? 2 : 1;
+            nbw = blocks_for(n, kT);
         }
-        if (nbond > 0) {
-            BondArgs ba{};
-            ba.nb = nbond; ba.b = b; ba.rank = rank.p; ba.bi = dbi.p; ba.bj = dbj.p; ba.bonded = dbonded.p;
-            ba.bpx = dbpx.p; ba.bpy = dbpy.p; ba.bpz = dbpz.p; ba.bnx = dbnx.p; ba.bny = dbny.p; ba.bnz = dbnz.p;
-            ba.bsx = dbsx.p; ba.bsy = dbsy.p; ba.bsz = dbsz.p; ba.btx = dbtx.p; ba.bty = dbty.p; ba.btz = dbtz.p;
-            ba.bFn = dbFn.p; ba.bTt = dbTt.p; ba.bsig = dbsig.p; ba.btau = dbtau.p;
-            ba.Fx = dbFx.p; ba.Fy = dbFy.p; ba.Fz = dbFz.p;
-            ba.Tix = dbTix.p; ba.Tiy = dbTiy.p; ba.Tiz = dbTiz.p; ba.Tjx = dbTjx.p; ba.Tjy = dbTjy.p; ba.Tjz = dbTjz.p;
-            ba.tb = tables(); ba.dt = springDt;
-            ba.periodic = periodic ? 1 : 0; ba.per = per;
-            if (fp64) DEMSPH_LAUNCH(timer, "dem_bond_forces", k_bond_forces<double>, dim3(blocks_for(nbond, kT)), tb, 0, stream, ba);
-            else DEMSPH_LAUNCH(timer, "dem_bond_forces", k_bond_forces<float>, dim3(blocks_for(nbond, kT)), tb, 0, stream, ba);
-        }
+        BondArgs ba{};
+        const int nbb = nbond > 0 ? blocks_for(nbond, kT) : 0;
+        if (nbb > 0) ba = bond_args(springDt);
+        if (fp64) forces_as<double>(walls, nbb > 0, pa, wa, ba, nbp, nbw, nbb);
+        else forces_as<float>(walls, nbb > 0, pa, wa, ba, nbp, nbw, nbb);
     }
 
     // Triangle cells and per-cell candidate lists, when the walls moved.
@@ -1918,7 +2117,8 @@ struct demsph_dem {
         const Balls b = cur();
         GatherArgs ga{};
         ga.b = b;
-        ga.ofx = ofx.p; ga.ofy = ofy.p; ga.ofz = ofz.p; ga.otx = otx.p; ga.oty = oty.p; ga.otz = otz.p;
+        ga.pstart = pstart[cp].p; ga.pcap = pcap[cp];
+        ga.pIx = pIx[cp].p; ga.pIy = pIy[cp].p; ga.pIz = pIz[cp].p;
         ga.pFx = pFx[cp].p; ga.pFy = pFy[cp].p; ga.pFz = pFz[cp].p; ga.pRx = pRx[cp].p; ga.pRy = pRy[cp].p; ga.pRz = pRz[cp].p;
         ga.rstart = rstart.p; ga.rpair = rpair.p;
         if (nbond > 0) {
@@ -2848,10 +3048,11 @@ void dem_state_arrays(demsph_dem* d, F&& io)
     if (d->has_list) {
         io(d->pstart[cp].p, (n + 1) * 4);
         io(d->ppartner[cp].p, np * 4);
+        io(d->powner[cp].p, np * 4);
         if (d->periodic) io(d->pshift[cp].p, np);
         for (auto* b : {&d->psx[cp], &d->psy[cp], &d->psz[cp], &d->prx[cp], &d->pry[cp], &d->prz[cp], &d->pox[cp],
                         &d->poy[cp], &d->poz[cp], &d->pov[cp], &d->pFx[cp], &d->pFy[cp], &d->pFz[cp], &d->pRx[cp],
-                        &d->pRy[cp], &d->pRz[cp]})
+                        &d->pRy[cp], &d->pRz[cp], &d->pIx[cp], &d->pIy[cp], &d->pIz[cp]})
             io(b->p, np * 8);
         io(d->rstart.p, (n + 1) * 4);
         io(d->rpair.p, np * 4);
@@ -2940,7 +3141,7 @@ void dem_block_restore(demsph_dem* d, int grow_pairs)
 DemStateHeader dem_state_header(const demsph_dem* d)
 {
     DemStateHeader h{};
-    h.magic = kDemStateMagic; h.version = 1u;
+    h.magic = kDemStateMagic; h.version = 2u;
     h.n = d->n; h.nc = d->nc; h.nw = d->nw; h.nv = d->nv; h.nt = d->nt; h.nbond = d->nbond_dev;
     h.KW = d->KW; h.fp64 = d->fp64 ? 1 : 0;
     h.npairs = d->has_list ? d->npairs : 0;
@@ -2981,7 +3182,7 @@ extern "C" int demsph_dem_import_state(demsph_dem* d, const void* in, size_t byt
         DemStateHeader h;
         if (bytes < sizeof h) throw InvalidArg("state too short");
         std::memcpy(&h, in, sizeof h);
-        if (h.magic != kDemStateMagic || h.version != 1u) throw InvalidArg("not a DEM scene state");
+        if (h.magic != kDemStateMagic || h.version != 2u) throw InvalidArg("not a DEM scene state (or of another layout version)");
         if (h.n != d->n || h.nc != d->nc || h.nw != d->nw || h.nv != d->nv || h.nt != d->nt ||
             h.nbond != d->nbond_dev || h.KW != d->KW || h.fp64 != (d->fp64 ? 1 : 0))
             throw InvalidArg("the state belongs to a scene of other counts or precision");
@@ -3330,6 +3531,12 @@ extern "C" int demsph_dem_tune(demsph_dem* d, const char* knob, int value)
             d->rpair.alloc(value);
             return;
         }
+        if (k == "wallSlots") {
+            if (value != 0 && value != 1) throw InvalidArg("wallSlots must be 0 or 1");
+            d->wall_slots_set = value;
+            d->drop_graphs();
+            return;
+        }
         if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
             throw InvalidArg("pairLanes must be 0 (by ball count), 1, 2, 4 or 8");
@@ -3348,6 +3555,10 @@ extern "C" int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value)
         }
         if (k == "pairReplays") {
             *value = (int)d->pair_replays;
+            return;
+        }
+        if (k == "wallSlots") {
+            *value = d->wall_slots_set;
             return;
         }
         if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");
