@@ -794,3 +794,73 @@ def test_wall_register_slots_equal_general_wall_path():
         assert (a.get("triCancel") == 0).sum() > 50  # the lowest layer reaches the floor
         a.close()
         b.close()
+
+
+def _lattice_beam_static_tip(nx, ny, nz, r=0.002, E=1e9, F=10.0, ks_ratio=2.6, rho=2500.0, dt=5e-7):
+    """Static tip deflection of the simple-cubic bonded beam under a tip load
+    F, from beam theory with the lattice's own stiffnesses:
+    * each chain of bonds along x is a spring k = E pi r / 2 per spacing 2r
+      (parallel bond of radius r, normal stiffness E / 2r per area);
+    * bending: EI = k (2r)^3 ny nz (nz^2 - 1) / 12 from the chains, plus each
+      bond's own bending stiffness E pi r^4 / 4;
+    * shear (Timoshenko, kappa = 5/6): G = (E pi / 4) / ks_ratio over the
+      section (ny 2r) x (nz 2r);
+    * span L = (nx - 1) 2r from the clamped layer to the loaded layer.
+    Also the first bending mode's period in steps."""
+    k = E * math.pi * r / 2
+    L = (nx - 1) * 2 * r
+    EI = k * (2 * r) ** 3 * ny * nz * (nz * nz - 1) / 12 + ny * nz * E * math.pi * r ** 4 / 4
+    G = (E * math.pi / 4) / ks_ratio
+    A = (ny * 2 * r) * (nz * 2 * r)
+    static = F * L ** 3 / (3 * EI) + F * L / (5 / 6 * G * A)
+    mu = rho * 4 / 3 * math.pi * r ** 3 * ny * nz / (2 * r)
+    w1 = 1.875104 ** 2 * math.sqrt(EI / (mu * L ** 4))
+    return static, int(round(2 * math.pi / w1 / dt))
+
+
+def test_full_size_cantilever_tip_deflection_is_beam_theory():
+    """The 128 x 16 x 16 bonded cantilever (32,768 balls, 93,472 bonds; no
+    clumps and no gravity, so the tip load alone bends it): after the
+    10,000-step ramp the undamped beam oscillates about its static shape,
+    and the tip deflection averaged over two periods of the first bending
+    mode is the beam-theory value within 2 %.  (The same average on the
+    oracle: 32 x 4 x 4 within 0.3 %, 48 x 6 x 6 within 1.1 %.)"""
+    s = scenes.cantilever_with_clumps(n_clumps=0)
+    s.gravity = np.zeros(3)
+    g = ts.DEMSceneGPU(s)
+    assert g.get("bondPointed").size == 93472
+    static, T1 = _lattice_beam_static_tip(128, 16, 16)
+    p0 = g.get("position")
+    tip = p0[:, 0] > (128 - 1.5) * 2 * 0.002
+    assert tip.sum() == 256
+    g.step(10000 + T1 // 2)
+    every = T1 // 64
+    defl = []
+    for _ in range(128):
+        g.step(every)
+        defl.append(-(g.get("position")[tip, 2] - p0[tip, 2]).mean())
+    mean = float(np.mean(defl))
+    assert abs(mean - static) / static < 0.02, (mean, static, T1)
+    assert g.get("bondIsBonded").sum() == 93472  # the bonds never break (zero strengths)
+    g.close()
+
+
+def test_full_size_cantilever_matches_oracle():
+    """The same full-size beam with its clumps and gravity, GPU in FP64
+    against the FP64 oracle over the first 100 steps (the clumps fall, the
+    load ramps): bonds bit-exact, positions and velocities within 1e-8 of
+    their largest magnitude."""
+    s = scenes.cantilever_with_clumps()
+    g = ts.DEMSceneGPU(s, fp64=True)
+    o = oracle.OracleDEM(s)
+    assert np.array_equal(g.get("bondPointed"), o.get("bond_pointed"))
+    assert np.array_equal(g.get("bondPointing"), o.get("bond_pointing"))
+    g.step(100)
+    o.step(100)
+    for f, of in (("position", "pos"), ("velocity", "vel")):
+        a, b = g.get(f), o.get(of)
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 1e-8 * scale, (f, np.abs(a - b).max(), scale)
+    v = o.get("vel")
+    assert np.abs(v).max() > 1e-4  # the clumps and the beam are moving
+    g.close()
