@@ -327,7 +327,8 @@ def ice_cylinder_coupled(steps: int) -> dict:
     return out
 
 
-def serving_mix(count: int, scale: float, slots: int, autotune: str | None = None) -> dict:
+def serving_mix(count: int, scale: float, slots: int, autotune: str | None = None,
+                runs_wanted=("plain", "batched", "batched_autotuned"), hash_file: str | None = None) -> dict:
     """The batched-serving job stream (256 scenes of 2k-200k particles,
     log-uniform; 50/30/20 DEM/WCSPH/coupled; 1k-20k steps; Poisson arrivals
     of mean 50 ms) scaled to finish in seconds: `count` scenes, step counts
@@ -338,7 +339,12 @@ def serving_mix(count: int, scale: float, slots: int, autotune: str | None = Non
     alone, so equal results in the batched run show that the mix changes
     nothing.  With `autotune`, a third run serves the batched path with
     every fluid scene autotuned on admission (the tuning inside the wall
-    time; "exact" keeps results bit-identical)."""
+    time; "exact" keeps results bit-identical).  runs_wanted picks runs (the
+    full-size stream takes minutes per run); each job's final positions are
+    also hashed (SHA-256), and hash_file keeps the hashes so that runs made
+    in separate processes can be compared."""
+    import hashlib
+
     import numpy as np
 
     import this_studio_amd as ts
@@ -354,10 +360,19 @@ def serving_mix(count: int, scale: float, slots: int, autotune: str | None = Non
     if autotune:
         runs.append(("batched_autotuned", slots, True, autotune))
         ts.SPHScene.clear_autotune_cache()
+    runs = [r for r in runs if r[0] in runs_wanted]
+    hashes = {}
     for name, sl, graphs, tune in runs:
         print(f"[bench] serving mix: {name} ({sl} slot(s), graphs={graphs}, autotune={tune}) ...", file=sys.stderr,
               flush=True)
-        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs, autotune=tune)
+        every = max(1, count // 16)
+
+        def progress(done, total, t, name=name):
+            if done % every == 0 or done == total:
+                print(f"[bench] serving mix: {name}: {done}/{total} jobs finished, {t:.1f} s", file=sys.stderr,
+                      flush=True)
+
+        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs, autotune=tune, progress=progress)
         st = srv.stats()
         out[name] = {"slots": sl, "graphs": graphs, "autotune": tune, "wall_s": round(st["wall"], 4),
                      "particle_steps_per_s": round(st["particleStepsPerSecond"], 1),
@@ -369,14 +384,21 @@ def serving_mix(count: int, scale: float, slots: int, autotune: str | None = Non
             out[name]["errors"] = errs[:5]
         results[name] = [(srv.result(i, "position") if j.kind != "dem" else srv.result(i, "ballPosition"))
                          if not srv.info(i)["status"] else None for j, i in zip(jobs, ids)]
+        hashes[name] = [hashlib.sha256(r.tobytes()).hexdigest() if r is not None else None for r in results[name]]
         srv.close()
-    out["speedup"] = round(out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"], 3)
-    if autotune:
+    if "plain" in out and "batched" in out:
+        out["speedup"] = round(out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"], 3)
+    if "plain" in out and "batched_autotuned" in out:
         out["speedup_autotuned"] = round(out["batched_autotuned"]["particle_steps_per_s"]
                                          / out["plain"]["particle_steps_per_s"], 3)
-    out["results_equal_alone_and_mixed"] = all(
-        all(r[k] is not None and np.array_equal(r[k], results["plain"][k]) for r in results.values())
-        for k in range(len(jobs)))
+    if len(results) > 1:
+        ref = next(iter(results.values()))
+        out["results_equal_alone_and_mixed"] = all(
+            all(r[k] is not None and np.array_equal(r[k], ref[k]) for r in results.values())
+            for k in range(len(jobs)))
+    if hash_file:
+        Path(hash_file).write_text(json.dumps({"scenes": count, "scale": scale, "runs": hashes,
+                                               "summary": {k: out[k] for k in hashes}}))
     return out
 
 
@@ -425,6 +447,10 @@ def main() -> int:
     ap.add_argument("--ice-steps", type=int, default=200, help="steps of the coupled ice-cylinder leg (0: skip it)")
     ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
     ap.add_argument("--mix-slots", type=int, default=8, help="concurrent jobs of the batched server")
+    ap.add_argument("--mix-runs", default="plain,batched,batched_autotuned",
+                    help="comma-separated serving runs to make (plain, batched, batched_autotuned)")
+    ap.add_argument("--mix-hashes", default=None, metavar="FILE",
+                    help="write each run's per-job result hashes and summary to FILE")
     ap.add_argument("--mix-autotune", choices=("off", "exact", "pairs"), default="exact",
                     help="also serve the batched path with fluid scenes autotuned on admission")
     ap.add_argument("--no-autotune-report", action="store_true",
@@ -598,7 +624,9 @@ def main() -> int:
     mix = None
     if args.mix_scenes > 0:
         mix = serving_mix(args.mix_scenes, args.mix_scale, args.mix_slots,
-                          None if args.mix_autotune == "off" else args.mix_autotune)
+                          None if args.mix_autotune == "off" else args.mix_autotune,
+                          tuple(args.mix_runs.split(",")), args.mix_hashes)
+    if mix is not None and "speedup" in mix:
         print(f"[bench] serving mix: plain {mix['plain']['particle_steps_per_s']:.3e} particle-steps/s "
               f"p50 {mix['plain']['p50_step_ms']:.3f} ms p99 {mix['plain']['p99_step_ms']:.3f} ms | batched "
               f"{mix['batched']['particle_steps_per_s']:.3e} p50 {mix['batched']['p50_step_ms']:.3f} ms p99 "

@@ -468,13 +468,16 @@ int demsph_couple_destroy(demsph_couple* c);
 /* Per fluid particle capacity of the fluid -> ball list (default 32). */
 int demsph_couple_set_capacity(demsph_couple* c, int ballsPerFluid);
 /* nsteps coupled kick-drift-kick steps of both scenes (DEM pair rebuild
- * every 5th step as in DEMSolver.h:1384).  A list overflow inside the block
- * returns DEMSPH_ERR_CAPACITY (capacities are grown; the block is lost). */
+ * every 5th step as in DEMSolver.h:1384).  Both scenes are checkpointed on
+ * the device first; a fluid list or fluid -> ball list that outgrows its
+ * capacity inside the block grows, both scenes go back to the checkpoint
+ * and the block runs again, with the results of ample lists. */
 int demsph_couple_step(demsph_couple* c, const double* gravity, double timeStep, int nsteps);
 /* "couplingForce" "couplingTorque" (double 3 per ball, DEM insertion order,
  * last evaluation), "ballPressure" (double per ball), and
  * "fluidCouplingAcceleration" (double 3 per fluid particle, insertion order,
- * the coupling part of the last fluid momentum pass). */
+ * the coupling part of the last fluid momentum pass), "blockReplays" (one
+ * double: coupled blocks run again after a list outgrew its capacity). */
 int demsph_couple_get(demsph_couple* c, const char* field, void* out, size_t capacity, size_t* count);
 
 /* ------------------------------------------------------------ serving
@@ -530,6 +533,12 @@ int demsph_server_submit(demsph_server* v, demsph_sph* fluid, demsph_dem* dem, c
                          double timeStep, long steps, double arrival, int* jobId);
 /* Serves every submitted job; returns when all have ended.  Once per server. */
 int demsph_server_run(demsph_server* v);
+/* Progress of demsph_server_run (no counterpart in the reference): fn is
+ * called from a slot thread each time a job finishes (completed or failed),
+ * one call at a time, with the jobs finished so far, the jobs submitted and
+ * the seconds since the run started.  NULL removes it; set before the run. */
+typedef void (*demsph_server_progress_fn)(void* user, int jobsFinished, int jobsTotal, double seconds);
+int demsph_server_set_progress(demsph_server* v, demsph_server_progress_fn fn, void* user);
 int demsph_server_job_info(demsph_server* v, int jobId, demsph_job_info* info, char* error, size_t errorCap);
 /* stepsDone - 1 intervals, s. */
 int demsph_server_job_latencies(demsph_server* v, int jobId, double* out, size_t capacity, size_t* count);

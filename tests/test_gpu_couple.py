@@ -199,3 +199,66 @@ def test_coupled_job_resumes_from_exported_scenes():
         assert np.array_equal(dem.get(f), dem2.get(f)), f
     assert np.array_equal(c.get("couplingForce"), c2.get("couplingForce"))
     assert dem.clock() == dem2.clock()
+
+
+def _ball_cluster_in_dam_break():
+    """The small dam break with a 3 x 3 x 3 cluster of 1.5 mm balls (3.2 mm
+    apart) inside the column: a fluid particle next to it has all 27 balls
+    within its support 2h = 26 mm."""
+    spec = scenes.small_dam_break()
+    lo = spec.fluid_pos.min(axis=0)
+    d = scenes.DEMScene("ball cluster in water")
+    centres = scenes.lattice(3, 3, 3, 0.0032, origin=lo + np.array([0.04, 0.02, 0.04]))
+    d.add_balls(centres, [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], 0.0015, 2500.0, 0)
+    d.set_hertzian_pair(0, 0, 1e7, 1e6, 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    d.min_bound, d.max_bound, d.gravity = spec.min_bound, spec.max_bound, spec.gravity
+    return spec, d
+
+
+def test_fluid_ball_list_overflow_replays_the_block():
+    """A fluid -> ball list capacity below what the scene needs (8 entries
+    against 27 balls near one fluid particle) grows inside the coupled
+    block: both scenes go back to their device checkpoint and the block runs
+    again, so the run equals one with ample capacity bit for bit."""
+    runs = []
+    for cap in (None, 1):
+        spec, d = _ball_cluster_in_dam_break()
+        gf, gd = fluid_scene(spec), ts.DEMSceneGPU(d)
+        gc = ts.CoupledScene(gf, gd, capacity=cap)
+        for n in (6, 7):
+            gc.step(spec.dt, n, spec.gravity)
+        runs.append((gc.get("blockReplays")[0], gf.get("position"), gf.get("velocity"), gf.get("density"),
+                     gd.get("position"), gd.get("velocity"), gc.get("couplingForce")))
+        gc.close()
+    (r0, *a), (r1, *b) = runs
+    assert r0 == 0 and r1 >= 1
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.abs(a[-1]).max() > 0  # the fluid pushes on the cluster
+
+
+def test_scenes_destroyed_before_their_coupling():
+    """A scene destroyed while a coupling still holds it detaches the
+    coupling: the coupling then refuses to step, the other scene keeps
+    working on its own stream, and every object can be released in any
+    order."""
+    for first in ("fluid", "dem"):
+        spec, fluid, dem = falling_balls()
+        c = ts.CoupledScene(fluid, dem)
+        c.step(spec.dt, 3, spec.gravity)
+        if first == "fluid":
+            fluid.close()
+            with pytest.raises(ts.DemsphError):
+                c.step(spec.dt, 1, spec.gravity)
+            z0 = dem.get("position")[:, 2].copy()
+            dem.step(10)
+            assert np.all(dem.get("position")[:, 2] < z0)
+            dem.close()
+        else:
+            dem.close()
+            with pytest.raises(ts.DemsphError):
+                c.step(spec.dt, 1, spec.gravity)
+            fluid.step(spec.dt, 4)
+            assert np.all(np.isfinite(fluid.get("position")))
+            fluid.close()
+        c.close()

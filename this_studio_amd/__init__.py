@@ -678,7 +678,7 @@ class CoupledScene:
         check(self.L.demsph_couple_get(self._h, field.encode(), None, 0, C.byref(cnt)))
         out = np.empty(cnt.value, dtype=np.float64)
         check(self.L.demsph_couple_get(self._h, field.encode(), out.ctypes.data_as(_vp), cnt.value, C.byref(cnt)))
-        if field != "ballPressure":
+        if field not in ("ballPressure", "blockReplays"):
             out = out.reshape(-1, 3)
         return out
 
@@ -699,6 +699,9 @@ class ServerSummary(C.Structure):
 _SERVER_DECLARED = False
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_double)
+
+
 def _declare_server(L):
     global _SERVER_DECLARED
     if _SERVER_DECLARED:
@@ -711,6 +714,7 @@ def _declare_server(L):
         "demsph_server_submit": [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_long, C.c_double,
                                  C.POINTER(C.c_int)],
         "demsph_server_run": [_vp],
+        "demsph_server_set_progress": [_vp, PROGRESS_FN, _vp],
         "demsph_server_job_info": [_vp, C.c_int, C.POINTER(JobInfo), C.c_char_p, _sz],
         "demsph_server_job_latencies": [_vp, C.c_int, _dp, _sz, C.POINTER(_sz)],
         "demsph_server_job_result": [_vp, C.c_int, C.c_char_p, _dp, _sz, C.POINTER(_sz)],
@@ -779,6 +783,16 @@ class Server:
         if dem is not None:
             dem._h = _vp()
         return jid.value
+
+    def set_progress(self, fn) -> None:
+        """fn(jobs_finished, jobs_total, seconds) runs on a slot thread each
+        time a job finishes during run(); None removes it."""
+        if fn is None:
+            self._progress_ref = None
+            check(self.L.demsph_server_set_progress(self._h, PROGRESS_FN(), None))
+            return
+        self._progress_ref = PROGRESS_FN(lambda user, done, total, t: fn(done, total, t))
+        check(self.L.demsph_server_set_progress(self._h, self._progress_ref, None))
 
     def run(self) -> None:
         check(self.L.demsph_server_run(self._h))

@@ -652,65 +652,89 @@ struct WallArgs {
     int* overflow;
 };
 
-// count/writeBallTriangleInteractions (ballNeighborSearchKernel.cu:153-339),
-// updateBallTriangleContact (contactKernel.cu:261-418) and
-// calBallWallContactForceTorqueKernel (:420-533), fused per ball.  The
-// candidate list, its spring history and cancel flags live in registers
-// and are written back as the new history.
 // wall_contacts_ball for a ball whose cell lists at most NS candidate
 // triangles: every per-candidate value sits in a register slot indexed at
 // compile time (slot u = the cell's candidate u), so nothing goes through
-// scratch memory.  A slot is live when its triangle box is within reach
-// and capacity remains; live slots, in order, are the general path's
-// compacted candidates, so every result and its order are the same.
+// scratch memory, and the loads of all slots are issued together (boxes,
+// ids and the first history entries; then triangle corners, wall ids and
+// matched springs; then corner positions and wall states), a few memory
+// latencies per ball instead of one per dependent load.  A slot is live
+// when its triangle box is within reach and capacity remains; live slots,
+// in order, are the general path's compacted candidates, so every result
+// and its order are the same.
 template <class T, int NS>
 __device__ __forceinline__ void wall_contacts_slots(const WallArgs& a, int i, int o, const D3& ri, double rad,
                                                     int c0, int c1)
 {
+    constexpr int QH = 4;  // history entries loaded with the candidates
     const int N = a.nballs;
     const int cap = min(16, a.KW);
     const double r2 = rad * rad;
+    // candidates and the first history entries
+    double2 bx[NS], by[NS], bz[NS];
+    int ct[NS];
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        bx[u] = by[u] = bz[u] = make_double2(0.0, 0.0);
+        ct[u] = -1;
+        if (c0 + u < c1) {
+            const double2* q = a.cbox + 3 * (size_t)(c0 + u);
+            bx[u] = q[0]; by[u] = q[1]; bz[u] = q[2];
+            ct[u] = a.ctri[c0 + u];
+        }
+    }
+    const int oldc = a.hcnt[o];
+    int ht[QH];
+#pragma unroll
+    for (int q = 0; q < QH; q++) ht[q] = q < a.KW ? a.htri[(size_t)q * N + o] : -1;
     bool live[NS];
     int tri[NS], cidx[NS];
     int cnt = 0, hits = 0;
 #pragma unroll
     for (int u = 0; u < NS; u++) {
-        const int j = c0 + u;
-        bool in = false;
-        if (j < c1) {
-            const double2* q = a.cbox + 3 * (size_t)j;
-            in = box_within(ri, r2, q[0], q[1], q[2]);
-        }
+        const bool in = c0 + u < c1 && box_within(ri, r2, bx[u], by[u], bz[u]);
         live[u] = in && hits < cap;
         hits += in ? 1 : 0;
-        tri[u] = live[u] ? a.ctri[j] : -1;
+        tri[u] = live[u] ? ct[u] : -1;
         cidx[u] = cnt;
         cnt += live[u] ? 1 : 0;
     }
     if (hits > cap) atomicMax(a.overflow, hits);
-    D3 es[NS], er[NS], et[NS], cp[NS];
-    int type[NS], cancel[NS];
-    const int oldc = a.hcnt[o];
+    // corners, walls and matched springs
+    int v0[NS], v1[NS], v2[NS], wl[NS];
+    D3 es[NS], er[NS], et[NS];
 #pragma unroll
     for (int u = 0; u < NS; u++) {
         es[u] = er[u] = et[u] = D3{0, 0, 0};
+        v0[u] = v1[u] = v2[u] = wl[u] = 0;
+        if (!live[u]) continue;
+        const int t = tri[u];
+        v0[u] = a.t0[t]; v1[u] = a.t1[t]; v2[u] = a.t2[t]; wl[u] = a.twall[t];
+        int hq = -1;
+#pragma unroll
+        for (int q = QH - 1; q >= 0; q--)
+            if (q < oldc && ht[q] == t) hq = q;
+        for (int q = QH; hq < 0 && q < oldc; q++)
+            if (a.htri[(size_t)q * N + o] == t) hq = q;
+        if (hq >= 0) {
+            const size_t h = (size_t)hq * N + o;
+            es[u] = D3{a.hsx[h], a.hsy[h], a.hsz[h]};
+            er[u] = D3{a.hrx[h], a.hry[h], a.hrz[h]};
+            et[u] = D3{a.hox[h], a.hoy[h], a.hoz[h]};
+        }
+    }
+    // corner positions and classification
+    D3 p0[NS], p1[NS], p2[NS], cp[NS];
+    int type[NS], cancel[NS];
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
         cp[u] = D3{0, 0, 0};
         type[u] = ST_NONE;
         cancel[u] = 0;
-        if (!live[u]) continue;
-        for (int q = 0; q < oldc; q++) {
-            const size_t h = (size_t)q * N + o;
-            if (a.htri[h] == tri[u]) {
-                es[u] = D3{a.hsx[h], a.hsy[h], a.hsz[h]};
-                er[u] = D3{a.hrx[h], a.hry[h], a.hrz[h]};
-                et[u] = D3{a.hox[h], a.hoy[h], a.hoz[h]};
-                break;
-            }
-        }
-        const int t = tri[u];
-        const D3 p0 = ld3(a.vgx, a.vgy, a.vgz, a.t0[t]), p1 = ld3(a.vgx, a.vgy, a.vgz, a.t1[t]),
-                 p2 = ld3(a.vgx, a.vgy, a.vgz, a.t2[t]);
-        type[u] = classify(ri, rad, p0, p1, p2, cp[u]);
+        p0[u] = ld3(a.vgx, a.vgy, a.vgz, v0[u]);
+        p1[u] = ld3(a.vgx, a.vgy, a.vgz, v1[u]);
+        p2[u] = ld3(a.vgx, a.vgy, a.vgz, v2[u]);
+        if (live[u]) type[u] = classify(ri, rad, p0[u], p1[u], p2[u], cp[u]);
     }
 #pragma unroll
     for (int u = 0; u < NS; u++) {
@@ -729,9 +753,7 @@ __device__ __forceinline__ void wall_contacts_slots(const WallArgs& a, int i, in
             if (t1 == ST_NONE) continue;
             bool hit = false;
             if (t1 == ST_FACE) {
-                const int t = tri[u1];
-                const D3 q0 = ld3(a.vgx, a.vgy, a.vgz, a.t0[t]), q1 = ld3(a.vgx, a.vgy, a.vgz, a.t1[t]),
-                         q2 = ld3(a.vgx, a.vgy, a.vgz, a.t2[t]);
+                const D3 q0 = p0[u1], q1 = p1[u1], q2 = p2[u1];
                 const D3 e01 = cross(cp[u] - q0, q1 - q0), e12 = cross(cp[u] - q1, q2 - q1),
                          e02 = cross(cp[u] - q0, q2 - q0);
                 hit = dot(e01, e01) < 1.e-20 || dot(e12, e12) < 1.e-20 || dot(e02, e02) < 1.e-20;
@@ -753,16 +775,17 @@ __device__ __forceinline__ void wall_contacts_slots(const WallArgs& a, int i, in
     if (!(im < 1.e-20)) {
         const D3 vi = ld3(a.b.vx, a.b.vy, a.b.vz, i), wi = ld3(a.b.wx, a.b.wy, a.b.wz, i);
         const double m = 1. / im;
+        const int mi = a.b.mat[i];
 #pragma unroll
         for (int u = 0; u < NS; u++) {
             if (!live[u] || cancel[u]) continue;
-            const int w = a.twall[tri[u]];
+            const int w = wl[u];
             const D3 rc = cp[u];
             const D3 n = normalize(ri - rc);
             const double delta = rad - len(ri - rc);
             const D3 rj = ld3(a.wpx, a.wpy, a.wpz, w), vj = ld3(a.wvx, a.wvy, a.wvz, w), wj = ld3(a.wwx, a.wwy, a.wwz, w);
             const D3 vc = (vi + cross(wi, rc - ri)) - (vj + cross(wj, rc - rj));
-            const int ip = pair_index(a.b.mat[i], a.wmat[w], a.tb.nMat, a.tb.cap);
+            const int ip = pair_index(mi, a.wmat[w], a.tb.nMat, a.tb.cap);
             V3<T> F{0, 0, 0}, Tq{0, 0, 0}, Fs{0, 0, 0};
             contact_model<T>(a.tb, ip, m, rad, delta, cvt<T>(vc), cvt<T>(wi - wj), cvt<T>(n), a.dt, es[u], er[u], et[u],
                              F, Tq, Fs);
@@ -807,6 +830,11 @@ constexpr int kWallSlots = 4;
 // nothing there.
 constexpr int kWallSlotBalls = 32768;
 
+// count/writeBallTriangleInteractions (ballNeighborSearchKernel.cu:153-339),
+// updateBallTriangleContact (contactKernel.cu:261-418) and
+// calBallWallContactForceTorqueKernel (:420-533), fused per ball.  The
+// candidate list, its spring history and cancel flags live in registers
+// and are written back as the new history.
 template <class T, bool SLOTS = false>
 __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
 {
@@ -982,33 +1010,83 @@ __device__ __forceinline__ T at(const void* p, int i) { return ((const T*)p)[i];
 // sumObjectPointingForceTorqueFromInteraction (contactKernel.cu:683-714) and
 // the per-ball totals: owner sums - partner-side pair terms + bonds + walls.
 template <class T>
+__device__ __forceinline__ V3<T> at3(const void* x, const void* y, const void* z, int i)
+{
+    return V3<T>{at<T>(x, i), at<T>(y, i), at<T>(z, i)};
+}
+
+// Entries of each sum loaded per round trip: the loads of a batch are
+// independent, so a ball's short sums cost a few memory latencies instead of
+// one per entry; the adds still run one entry at a time in list order, and
+// entries past the end are loaded from the last valid index and not added.
+constexpr int kGatherBatch = 4;
+
+template <class T>
 __device__ __forceinline__ void gather_ball(const GatherArgs& a, int j)
 {
+    constexpr int B = kGatherBatch;
     // owner side: the sum over the ball's own entries, in list order
     V3<T> Fo{0, 0, 0}, To{0, 0, 0};
-    const int p1 = min(a.pstart[j + 1], a.pcap);
-    for (int p = a.pstart[j]; p < p1; p++) {
-        Fo = Fo + V3<T>{at<T>(a.pFx, p), at<T>(a.pFy, p), at<T>(a.pFz, p)};
-        To = To + V3<T>{at<T>(a.pIx, p), at<T>(a.pIy, p), at<T>(a.pIz, p)};
+    const int p0 = a.pstart[j], p1 = min(a.pstart[j + 1], a.pcap);
+    const int q0 = a.rstart[j], q1 = a.rstart[j + 1];
+    for (int p = p0; p < p1; p += B) {
+        V3<T> f[B], t[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int pk = min(p + k, p1 - 1);
+            f[k] = at3<T>(a.pFx, a.pFy, a.pFz, pk);
+            t[k] = at3<T>(a.pIx, a.pIy, a.pIz, pk);
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (p + k < p1) {
+                Fo = Fo + f[k];
+                To = To + t[k];
+            }
     }
     T fx = Fo.x, fy = Fo.y, fz = Fo.z;
     T tx = To.x, ty = To.y, tz = To.z;
-    for (int q = a.rstart[j]; q < a.rstart[j + 1]; q++) {
-        const int p = a.rpair[q];
-        fx -= at<T>(a.pFx, p); fy -= at<T>(a.pFy, p); fz -= at<T>(a.pFz, p);
-        tx -= at<T>(a.pRx, p); ty -= at<T>(a.pRy, p); tz -= at<T>(a.pRz, p);
+    // partner side, in ascending pair order
+    for (int q = q0; q < q1; q += B) {
+        int pp[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) pp[k] = a.rpair[min(q + k, q1 - 1)];
+        V3<T> f[B], r[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            f[k] = at3<T>(a.pFx, a.pFy, a.pFz, pp[k]);
+            r[k] = at3<T>(a.pRx, a.pRy, a.pRz, pp[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if (q + k < q1) {
+                fx -= f[k].x; fy -= f[k].y; fz -= f[k].z;
+                tx -= r[k].x; ty -= r[k].y; tz -= r[k].z;
+            }
     }
     if (a.istart) {
         const int o = a.b.orig[j];
-        for (int q = a.istart[o]; q < a.istart[o + 1]; q++) {
-            const int e = a.ibond[q];
-            const int k = e >> 1;
-            if ((e & 1) == 0) {
-                fx += at<T>(a.bFx, k); fy += at<T>(a.bFy, k); fz += at<T>(a.bFz, k);
-                tx += at<T>(a.bTix, k); ty += at<T>(a.bTiy, k); tz += at<T>(a.bTiz, k);
-            } else {
-                fx -= at<T>(a.bFx, k); fy -= at<T>(a.bFy, k); fz -= at<T>(a.bFz, k);
-                tx += at<T>(a.bTjx, k); ty += at<T>(a.bTjy, k); tz += at<T>(a.bTjz, k);
+        const int e0 = a.istart[o], e1 = a.istart[o + 1];
+        for (int q = e0; q < e1; q += B) {
+            int ee[B];
+#pragma unroll
+            for (int k = 0; k < B; k++) ee[k] = a.ibond[min(q + k, e1 - 1)];
+            V3<T> f[B], t[B];
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                const int b = ee[k] >> 1;
+                f[k] = at3<T>(a.bFx, a.bFy, a.bFz, b);
+                t[k] = (ee[k] & 1) == 0 ? at3<T>(a.bTix, a.bTiy, a.bTiz, b) : at3<T>(a.bTjx, a.bTjy, a.bTjz, b);
+            }
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                if (q + k >= e1) continue;
+                if ((ee[k] & 1) == 0) {
+                    fx += f[k].x; fy += f[k].y; fz += f[k].z;
+                } else {
+                    fx -= f[k].x; fy -= f[k].y; fz -= f[k].z;
+                }
+                tx += t[k].x; ty += t[k].y; tz += t[k].z;
             }
         }
     }
@@ -1373,8 +1451,6 @@ struct DBuf {
 struct demsph_dem;
 namespace {
 // Device checkpoint of a block of steps (defined after the state arrays).
-void dem_block_save(demsph_dem* d);
-void dem_block_restore(demsph_dem* d, int grow_pairs);
 }  // namespace
 
 struct demsph_dem {
@@ -1526,8 +1602,12 @@ struct demsph_dem {
         graphs.clear();
     }
 
+    std::vector<demsph_couple*> couples;  // couplings holding this scene
+
     ~demsph_dem()
     {
+        // couplings destroyed after this scene must not touch it
+        for (auto* c : std::vector<demsph_couple*>(couples)) couple_detach_dem(c);
         (void)hipStreamSynchronize(stream);
         drop_graphs();
         dev_free(sort_temp);
@@ -2457,6 +2537,11 @@ DemBallRefs dem_ball_refs(demsph_dem* d)
                        f.fx, f.fy, f.fz, f.tx, f.ty, f.tz};
 }
 hipStream_t dem_stream(demsph_dem* d) { return d->stream; }
+void dem_attach_couple(demsph_dem* d, demsph_couple* c) { d->couples.push_back(c); }
+void dem_detach_couple(demsph_dem* d, demsph_couple* c)
+{
+    d->couples.erase(std::remove(d->couples.begin(), d->couples.end(), c), d->couples.end());
+}
 void dem_set_stream(demsph_dem* d, hipStream_t s) { d->stream = s; }
 void dem_require_ready(demsph_dem* d)
 {
@@ -3090,11 +3175,14 @@ void dem_block_copy(demsph_dem* d, bool save)
 {
     std::vector<std::pair<void*, size_t>> segs;
     size_t total = 0;
-    dem_state_arrays(d, [&](void* p, size_t b) {
+    const auto seg = [&](void* p, size_t b) {
         if (!b) return;
         segs.push_back({p, b});
         total += (b + 255) & ~size_t(255);
-    });
+    };
+    dem_state_arrays(d, seg);
+    // a replayed block stamps its steps again from the same count
+    if (d->stamps.counter) seg(d->stamps.counter, sizeof(int));
     if (save && d->ckbuf.n < total) d->ckbuf.alloc(total + total / 4);
     unsigned char* ck = d->ckbuf.p;
     size_t off = 0;
@@ -3117,6 +3205,9 @@ void dem_block_copy(demsph_dem* d, bool save)
     }
 }
 
+}  // namespace
+
+namespace demsph {
 void dem_block_save(demsph_dem* d)
 {
     if (d->npairs < 0) throw std::logic_error("block checkpoint with an unknown pair count");
@@ -3137,7 +3228,9 @@ void dem_block_restore(demsph_dem* d, int grow_pairs)
     if ((int)d->rpair.n < std::max(d->pcap[0], d->pcap[1])) d->rpair.alloc(std::max(d->pcap[0], d->pcap[1]));
     dem_block_copy(d, false);
 }
+}  // namespace demsph
 
+namespace {
 DemStateHeader dem_state_header(const demsph_dem* d)
 {
     DemStateHeader h{};

@@ -6,6 +6,7 @@
 
 struct demsph_dem;
 struct demsph_sph;
+struct demsph_couple;
 
 namespace demsph {
 
@@ -35,6 +36,17 @@ double dem_begin_step(demsph_dem* d, double dt);
 void dem_evaluate(demsph_dem* d, double h, double dt);
 void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, double hookTime);
 void dem_end_block(demsph_dem* d);
+// A DEM scene keeps the couplings that hold it: when it is destroyed first,
+// each of them is detached (it then refuses to step and touches nothing of
+// that scene).
+void dem_attach_couple(demsph_dem* d, demsph_couple* c);
+void dem_detach_couple(demsph_dem* d, demsph_couple* c);
+void couple_detach_dem(demsph_couple* c);
+// Device checkpoint of a DEM scene for replaying a block of steps (every
+// array of the state export, the host values that size them and the step
+// stamp counter); restore grows both pair lists to at least growPairs.
+void dem_block_save(demsph_dem* d);
+void dem_block_restore(demsph_dem* d, int growPairs);
 
 // Batched serving: device bytes a staged scene will allocate (admission
 // under a memory budget), sizes, step-end timestamps and streams.

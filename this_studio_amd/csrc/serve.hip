@@ -94,6 +94,9 @@ struct demsph_server {
     size_t next = 0;
     size_t in_use = 0, peak_bytes = 0;
     int active = 0, peak_active = 0;
+    int finished_jobs = 0;
+    demsph_server_progress_fn progress = nullptr;
+    void* progress_user = nullptr;
     std::chrono::steady_clock::time_point t0;
     double wall = 0.0;
 
@@ -243,6 +246,8 @@ struct demsph_server {
                         h->error = "job needs " + std::to_string(h->bytes) + " bytes, above the memory budget";
                         h->destroy_scenes();
                         next++;
+                        finished_jobs++;
+                        if (progress) progress(progress_user, finished_jobs, (int)order.size(), now());
                         continue;
                     }
                     const double t = now();
@@ -270,6 +275,9 @@ struct demsph_server {
                 std::lock_guard<std::mutex> lk(mu);
                 in_use -= j->bytes;
                 active--;
+                finished_jobs++;
+                // called under the lock: one call at a time, in finishing order
+                if (progress) progress(progress_user, finished_jobs, (int)order.size(), now());
             }
             cv.notify_all();
         }
@@ -401,6 +409,16 @@ extern "C" int demsph_server_run(demsph_server* v)
     return guarded([&] {
         if (!v) throw InvalidArg("null server");
         v->run();
+    });
+}
+
+extern "C" int demsph_server_set_progress(demsph_server* v, demsph_server_progress_fn fn, void* user)
+{
+    return guarded([&] {
+        if (!v) throw InvalidArg("null server");
+        if (v->ran) throw InvalidArg("set the progress callback before demsph_server_run");
+        v->progress = fn;
+        v->progress_user = user;
     });
 }
 

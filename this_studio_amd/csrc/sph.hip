@@ -1384,7 +1384,7 @@ struct demsph_sph {
     float4* ck_V = nullptr;
     double *ck_vx = nullptr, *ck_vy = nullptr, *ck_vz = nullptr;
     float* ck_drho = nullptr;
-    int *ck_orig = nullptr, *ck_rank = nullptr;
+    int *ck_orig = nullptr, *ck_rank = nullptr, *ck_stamp = nullptr;
     KernelTimer timer;
     // Captured two-step graphs.  A step flips the P, V and orig buffers once
     // and drho twice, so two steps return every ping-pong index to where it
@@ -1418,7 +1418,9 @@ struct demsph_sph {
     double pref_R = 0.0;
     float4 *Pf = nullptr, *DPf = nullptr;
 
-    ~demsph_sph() { release(); }
+    std::vector<demsph_couple*> couples;  // couplings holding this scene
+
+    ~demsph_sph();
 
     void drop_graphs()
     {
@@ -1443,7 +1445,7 @@ struct demsph_sph {
         maxcount_h = nullptr;
         dfree(DP); dfree(DV); dfree(dphase); dfree(dorig); dfree(dcells); dfree(dummy_near); dfree(d_phases);
         dfree(ck_P); dfree(ck_V); dfree(ck_vx); dfree(ck_vy); dfree(ck_vz); dfree(ck_drho); dfree(ck_orig);
-        dfree(ck_rank);
+        dfree(ck_rank); dfree(ck_stamp);
         if (own_stream) stream_destroy(stream);
         stream = nullptr;
     }
@@ -1873,6 +1875,7 @@ struct demsph_sph {
         ck_P = dmalloc<FluidPos>(n); ck_V = dmalloc<float4>(n);
         ck_vx = dmalloc<double>(n); ck_vy = dmalloc<double>(n); ck_vz = dmalloc<double>(n);
         ck_drho = dmalloc<float>(n); ck_orig = dmalloc<int>(n); ck_rank = dmalloc<int>(n);
+        ck_stamp = dmalloc<int>(1);
     }
 
     void checkpoint(bool save)
@@ -1888,6 +1891,8 @@ struct demsph_sph {
         cpb(ck_drho, drho[cD], n * 4);
         cpb(ck_orig, orig[cO], n * 4);
         cpb(ck_rank, rank, n * 4);
+        // a replayed block stamps its steps again from the same count
+        if (stamps.counter) cpb(ck_stamp, stamps.counter, sizeof(int));
     }
 
     void neighbor_search()
@@ -2820,6 +2825,7 @@ struct demsph_couple {
     void* scan_temp = nullptr;
     double* Fc = nullptr;  // 6 per ball slot (force, torque) of the last evaluation
     float* acs = nullptr;  // 3 per fluid slot, coupling acceleration of the last fluid pass
+    long replays = 0;      // coupled blocks run again after a list outgrew its capacity
 
     // The DEM scene runs on the fluid's stream while coupled; it gets its
     // own stream back when the coupling ends, so that it never keeps a
@@ -2829,8 +2835,30 @@ struct demsph_couple {
 
     ~demsph_couple()
     {
+        detach_fluid();
+        if (d) {
+            if (dem_switched) dem_set_stream(d, dem_own_stream);
+            dem_detach_couple(d, this);
+        }
+    }
+
+    // The fluid scene is going away (or this coupling is): free what lives
+    // on its stream and forget it.
+    void detach_fluid()
+    {
+        if (!f) return;
         release();
-        if (dem_switched) dem_set_stream(d, dem_own_stream);
+        f->couples.erase(std::remove(f->couples.begin(), f->couples.end(), this), f->couples.end());
+        if (d && dem_switched) {
+            dem_set_stream(d, dem_own_stream);
+            dem_switched = false;
+        }
+        f = nullptr;
+    }
+
+    void require_scenes() const
+    {
+        if (!f || !d) throw InvalidArg("a scene of this coupling was destroyed");
     }
 
     void release()
@@ -2849,6 +2877,7 @@ struct demsph_couple {
 
     void setup()
     {
+        require_scenes();
         f->require_init();
         dem_require_ready(d);
         if (!dem_switched) {
@@ -2969,6 +2998,10 @@ struct demsph_couple {
                       dim3(64 * kBallsPerBlock), 0, st, c);
     }
 
+    // A block of coupled steps.  Both scenes are checkpointed on the device
+    // first; when a fluid list or the fluid-ball list outgrows its capacity
+    // the lists grow, both scenes go back to the checkpoint and the block
+    // runs again, so the results are those of ample lists.
     void step(const double* g, double dt, int nsteps)
     {
         if (!(dt > 0.0)) throw InvalidArg("time step must be positive");
@@ -2976,47 +3009,72 @@ struct demsph_couple {
         if (nsteps <= 0) return;
         const double h = 0.5 * dt;
         hipStream_t st = f->stream;
-        f->reset_overflow();
-        f->solids = view();
-        try {
-            for (int k = 0; k < nsteps; k++) {
-                const double t = dem_begin_step(d, dt);
-                f->search_enqueue();
-                prepare(st);
-                dem_evaluate(d, h, dt);
-                forces(st);
-                dem_hooks_and_half(d, true, g, h, t - h);
-                f->first_half_enqueue(dt);
-                gather(st, false);
-                f->continuity_enqueue(dt);
-                pressure(st);
-                dem_evaluate(d, h, dt);
-                forces(st);
-                dem_hooks_and_half(d, false, g, h, t);
-                f->momentum2_enqueue(dt);
-                step_stamp(f->stamps, st);
+        f->ensure_checkpoint();
+        for (int attempt = 0; attempt < 8; attempt++) {
+            f->checkpoint(true);
+            const int sP = f->cP, sV = f->cV, sD = f->cD, sO = f->cO;
+            dem_block_save(d);
+            f->reset_overflow();
+            f->solids = view();
+            try {
+                for (int k = 0; k < nsteps; k++) {
+                    const double t = dem_begin_step(d, dt);
+                    f->search_enqueue();
+                    prepare(st);
+                    dem_evaluate(d, h, dt);
+                    forces(st);
+                    dem_hooks_and_half(d, true, g, h, t - h);
+                    f->first_half_enqueue(dt);
+                    gather(st, false);
+                    f->continuity_enqueue(dt);
+                    pressure(st);
+                    dem_evaluate(d, h, dt);
+                    forces(st);
+                    dem_hooks_and_half(d, false, g, h, t);
+                    f->momentum2_enqueue(dt);
+                    step_stamp(f->stamps, st);
+                }
+            } catch (...) {
+                f->solids = SolidView();
+                throw;
             }
-        } catch (...) {
             f->solids = SolidView();
-            throw;
+            const bool grew = f->check_overflow();  // reads all three list maxima, grows the fluid lists
+            const int mfs = f->maxcount_h[2];
+            if (!grew && mfs <= K_fs) {
+                dem_end_block(d);
+                return;
+            }
+            if (mfs > K_fs) {
+                K_fs = mfs + mfs / 4 + 8;
+                ready = false;  // setup() reallocates the fluid-ball lists
+            }
+            f->cP = sP; f->cV = sV; f->cD = sD; f->cO = sO;
+            f->checkpoint(false);
+            dem_block_restore(d, 0);
+            replays++;
+            setup();
         }
-        f->solids = SolidView();
-        const int K_ff = f->K_ff, K_fd = f->K_fd;
-        const bool grew = f->check_overflow();  // reads all three list maxima, grows the fluid lists
-        const int mfs = f->maxcount_h[2];
-        if (grew)
-            throw CapacityError("fluid neighbour lists overflowed during a coupled block (capacities " +
-                                std::to_string(K_ff) + "/" + std::to_string(K_fd) +
-                                " grown for the next scene; this block's state is invalid)");
-        if (mfs > K_fs) {
-            K_fs = mfs + mfs / 4 + 8;
-            ready = false;
-            throw CapacityError("fluid-ball lists overflowed (" + std::to_string(mfs) +
-                                " balls near one fluid particle); the capacity was grown for the next scene");
-        }
-        dem_end_block(d);
+        throw std::runtime_error("coupled list capacities did not converge");
     }
 };
+
+demsph_sph::~demsph_sph()
+{
+    // couplings destroyed after this scene must not touch it
+    for (auto* c : std::vector<demsph_couple*>(couples)) c->detach_fluid();
+    release();
+}
+
+namespace demsph {
+void couple_detach_dem(demsph_couple* c)
+{
+    if (c->f && c->f->stream) (void)hipStreamSynchronize(c->f->stream);
+    if (c->dem_switched) c->dem_switched = false;  // the scene's stream is its owner's business now
+    c->d = nullptr;
+    c->ready = false;
+}
+}  // namespace demsph
 
 extern "C" int demsph_couple_create(demsph_couple** out, demsph_sph* fluid, demsph_dem* dem)
 {
@@ -3025,6 +3083,8 @@ extern "C" int demsph_couple_create(demsph_couple** out, demsph_sph* fluid, dems
         auto* c = new demsph_couple();
         c->f = fluid;
         c->d = dem;
+        fluid->couples.push_back(c);
+        dem_attach_couple(dem, c);
         *out = c;
     });
 }
@@ -3056,6 +3116,7 @@ extern "C" int demsph_couple_get(demsph_couple* c, const char* field, void* out,
 {
     return guarded([&] {
         if (!c || !field) throw InvalidArg("null argument");
+        c->require_scenes();
         if (!c->ready) throw InvalidArg("no coupled step has run");
         const std::string fld = field;
         hipStream_t st = c->f->stream;
@@ -3074,6 +3135,10 @@ extern "C" int demsph_couple_get(demsph_couple* c, const char* field, void* out,
                 for (int k = 0; k < c->ns; k++)
                     for (int q = 0; q < 3; q++) v[3 * (size_t)orig[perm[k]] + q] = fc[6 * (size_t)k + off + q];
             }
+            return emit(v, out, capacity, count);
+        }
+        if (fld == "blockReplays") {  // coupled blocks run again after a list outgrew its capacity
+            const std::vector<double> v{(double)c->replays};
             return emit(v, out, capacity, count);
         }
         if (fld == "fluidCouplingAcceleration") {
