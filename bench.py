@@ -328,7 +328,8 @@ def ice_cylinder_coupled(steps: int) -> dict:
 
 
 def serving_mix(count: int, scale: float, slots: int, autotune: str | None = None,
-                runs_wanted=("plain", "batched", "batched_autotuned"), hash_file: str | None = None) -> dict:
+                runs_wanted=("plain", "batched", "batched_autotuned"), hash_file: str | None = None,
+                block: int = 100) -> dict:
     """The batched-serving job stream (256 scenes of 2k-200k particles,
     log-uniform; 50/30/20 DEM/WCSPH/coupled; 1k-20k steps; Poisson arrivals
     of mean 50 ms) scaled to finish in seconds: `count` scenes, step counts
@@ -372,9 +373,9 @@ def serving_mix(count: int, scale: float, slots: int, autotune: str | None = Non
                 print(f"[bench] serving mix: {name}: {done}/{total} jobs finished, {t:.1f} s", file=sys.stderr,
                       flush=True)
 
-        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs, autotune=tune, progress=progress)
+        srv, ids = serving.serve(jobs, slots=sl, graphs=graphs, autotune=tune, progress=progress, block=block)
         st = srv.stats()
-        out[name] = {"slots": sl, "graphs": graphs, "autotune": tune, "wall_s": round(st["wall"], 4),
+        out[name] = {"slots": sl, "graphs": graphs, "autotune": tune, "block": block, "wall_s": round(st["wall"], 4),
                      "particle_steps_per_s": round(st["particleStepsPerSecond"], 1),
                      "p50_step_ms": round(st["p50"] * 1e3, 4), "p99_step_ms": round(st["p99"] * 1e3, 4),
                      "jobs_done": st["jobsDone"], "jobs_failed": st["jobsFailed"], "peak_active": st["peakActive"],
@@ -447,6 +448,7 @@ def main() -> int:
     ap.add_argument("--ice-steps", type=int, default=200, help="steps of the coupled ice-cylinder leg (0: skip it)")
     ap.add_argument("--mix-scale", type=float, default=0.05, help="step-count and arrival-gap scale of the mix")
     ap.add_argument("--mix-slots", type=int, default=8, help="concurrent jobs of the batched server")
+    ap.add_argument("--mix-block", type=int, default=100, help="steps per step call of a served job")
     ap.add_argument("--mix-runs", default="plain,batched,batched_autotuned",
                     help="comma-separated serving runs to make (plain, batched, batched_autotuned)")
     ap.add_argument("--mix-hashes", default=None, metavar="FILE",
@@ -625,7 +627,7 @@ def main() -> int:
     if args.mix_scenes > 0:
         mix = serving_mix(args.mix_scenes, args.mix_scale, args.mix_slots,
                           None if args.mix_autotune == "off" else args.mix_autotune,
-                          tuple(args.mix_runs.split(",")), args.mix_hashes)
+                          tuple(args.mix_runs.split(",")), args.mix_hashes, args.mix_block)
     if mix is not None and "speedup" in mix:
         print(f"[bench] serving mix: plain {mix['plain']['particle_steps_per_s']:.3e} particle-steps/s "
               f"p50 {mix['plain']['p50_step_ms']:.3f} ms p99 {mix['plain']['p99_step_ms']:.3f} ms | batched "
