@@ -225,6 +225,8 @@ typedef struct demsph_sph_tune_report {
 } demsph_sph_tune_report;
 int demsph_sph_autotune(demsph_sph* s, double dt, int trialSteps, int mode, int useCache,
                         demsph_sph_tune_report* report);
+/* Empties the process-wide autotuner caches (WCSPH choices and the DEM
+ * pair-lane choices the server's admission tuning keeps per size class). */
 int demsph_sph_autotune_clear_cache(void);
 
 /* ------------------------------------------------------------------ DEM
@@ -524,7 +526,10 @@ int demsph_server_destroy(demsph_server* v);
 int demsph_server_set_block(demsph_server* v, long steps);
 /* Autotune every job's fluid scene on admission (demsph_sph_autotune with
  * the cache on; mode -1 off (default), 0 result-invariant knobs, 1 also
- * pair lanes).  Set before demsph_server_run. */
+ * pair lanes), and every job's DEM scene (demsph_dem_autotune's "pairLanes",
+ * result-invariant, once per size class of ball count, walls, bonds,
+ * periodic axes and precision, with trialSteps of at least 5).  Set before
+ * demsph_server_run. */
 int demsph_server_set_autotune(demsph_server* v, int mode, int trialSteps);
 /* fluid and/or dem non-NULL; fluidMin/Max with a fluid, demMin/Max with a
  * DEM scene (a coupled DEM scene usually shares the fluid bounds). */

@@ -22,9 +22,12 @@
 #include <memory>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <chrono>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <set>
 #include <utility>
 #include <vector>
@@ -3668,6 +3671,47 @@ extern "C" int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value)
 // replace it.  Step timestamps and timing totals recorded before the call are
 // kept, and any failure restores the state and the knob before it returns.
 // A hook set on the scene runs during the trial steps too.
+namespace {
+using DemTuneKey = std::array<int, 5>;
+std::mutex& dem_tune_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+std::map<DemTuneKey, int>& dem_tune_cache()
+{
+    static std::map<DemTuneKey, int> c;
+    return c;
+}
+}  // namespace
+
+namespace demsph {
+void dem_autotune_cached(demsph_dem* d, const double* g, double dt, int trialSteps)
+{
+    const DemTuneKey key{(int)std::lround(4.0 * std::log2((double)d->n + 1.0)), d->nt > 0 ? 1 : 0,
+                         d->nbond > 0 ? 1 : 0, d->periodic ? 1 : 0, d->fp64 ? 1 : 0};
+    {
+        std::lock_guard<std::mutex> lk(dem_tune_mutex());
+        const auto it = dem_tune_cache().find(key);
+        if (it != dem_tune_cache().end()) {
+            d->drop_graphs();
+            d->pair_lanes_set = it->second;
+            return;
+        }
+    }
+    int lanes = 0;
+    if (demsph_dem_autotune(d, g, dt, trialSteps, &lanes) != DEMSPH_OK)
+        throw std::runtime_error(std::string("DEM autotune: ") + demsph_last_error());
+    std::lock_guard<std::mutex> lk(dem_tune_mutex());
+    dem_tune_cache()[key] = lanes;
+}
+void dem_autotune_clear_cache()
+{
+    std::lock_guard<std::mutex> lk(dem_tune_mutex());
+    dem_tune_cache().clear();
+}
+}  // namespace demsph
+
 extern "C" int demsph_dem_autotune(demsph_dem* d, const double* g, double dt, int trialSteps, int* lanes)
 {
     DEM_GUARD({

@@ -42,6 +42,11 @@ void dem_end_block(demsph_dem* d);
 void dem_attach_couple(demsph_dem* d, demsph_couple* c);
 void dem_detach_couple(demsph_dem* d, demsph_couple* c);
 void couple_detach_dem(demsph_couple* c);
+// demsph_dem_autotune once per size class in the process (ball count to a
+// quarter octave, walls, bonds, periodic axes, precision): later scenes of
+// the class take the cached "pairLanes" choice without trials.
+void dem_autotune_cached(demsph_dem* d, const double* g, double dt, int trialSteps);
+void dem_autotune_clear_cache();
 // Device checkpoint of a DEM scene for replaying a block of steps (every
 // array of the state export, the host values that size them and the step
 // stamp counter); restore grows both pair lists to at least growPairs.

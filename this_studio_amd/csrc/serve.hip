@@ -171,6 +171,8 @@ struct demsph_server {
                 dem_set_stream(j.dem, st);
                 ok(demsph_dem_initialize(j.dem, j.dmin, j.dmax));
                 ok(demsph_dem_solve_begin(j.dem));
+                // pair lanes build the same lists in the same order: every mode keeps results
+                if (autotune >= 0) dem_autotune_cached(j.dem, j.g, j.dt, std::max(5, tune_steps));
             }
             if (j.kind == 2) ok(demsph_couple_create(&j.couple, j.fluid, j.dem));
             DEMSPH_HIP(dev_malloc((void**)&sbuf, sizeof(unsigned long long) * (size_t)(j.steps + 1)));

@@ -2764,8 +2764,11 @@ extern "C" int demsph_sph_autotune(demsph_sph* s, double dt, int trialSteps, int
 extern "C" int demsph_sph_autotune_clear_cache(void)
 {
     return guarded([&] {
-        std::lock_guard<std::mutex> lk(demsph_sph::tune_mutex());
-        demsph_sph::tune_cache().clear();
+        {
+            std::lock_guard<std::mutex> lk(demsph_sph::tune_mutex());
+            demsph_sph::tune_cache().clear();
+        }
+        dem_autotune_clear_cache();
     });
 }
 
