@@ -435,6 +435,11 @@ int demsph_dem_graph_captures(demsph_dem* d, long* count);
  * else 1), 1, 2, 4 or 8.  Every setting builds the same pair lists in the
  * same order.  "pairCapacity": entries of each of the two pair lists,
  * set between initialize and solve_begin (they grow on demand from there).
+ * demsph_dem_tune_get also reads "wallCapacity", the wall-triangle
+ * candidates kept per ball (16, or 64 once a ball has found more: the step
+ * call replays from its device checkpoint with the wall history grown; more
+ * than 64 is DEMSPH_ERR_CAPACITY with the scene back at the checkpoint), and
+ * "wallReplays", the step calls run again for it.
  * "wallSlots": 1 (default) lets scenes of up to 32,768 balls evaluate
  * ball-wall contacts of cells with at most four candidate triangles in
  * registers, 0 keeps every ball on the general path (same results).

@@ -864,3 +864,49 @@ def test_full_size_cantilever_matches_oracle():
     v = o.get("vel")
     assert np.abs(v).max() > 1e-4  # the clumps and the beam are moving
     g.close()
+
+
+def _ball_on_fine_plate(square=0.001, n=20, r=0.01, gap=1e-4, v0=-0.1):
+    """A ball falling onto a plate meshed into n x n squares of side `square`
+    (two triangles each): at the impact its contact patch covers more
+    triangles boxes than the default 16 wall candidates per ball."""
+    s = scenes.DEMScene("ball on a fine plate")
+    xs = (np.arange(n + 1) - n / 2) * square
+    vx, vy = np.meshgrid(xs, xs, indexing="ij")
+    verts = np.stack([vx.ravel(), vy.ravel(), np.zeros(vx.size)], axis=1)
+    t0, t1, t2 = [], [], []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            t0 += [a, a]
+            t1 += [b, c]
+            t2 += [c, d]
+    E, nu = 1e7, 0.3
+    s.set_hertzian_pair(0, 0, E / (2 * (1 - nu * nu)), E / (4 * (2 - nu) * (1 + nu)), 0.5, 0.0, 0.0, 0.3, 0.0, 0.0)
+    s.add_balls([[0.0001, 0.0002, r + gap]], [0.0, 0.0, v0], [0, 0, 0], r, 2500.0, 0)
+    s.add_mesh_wall(verts, t0, t1, t2, [0, 0, 0], [0, 0, 0], [0, 0, 0], 0)
+    s.min_bound, s.max_bound = np.array([-0.05, -0.05, -0.05]), np.array([0.05, 0.05, 0.05])
+    s.gravity = np.array([0.0, 0.0, -9.81])
+    s.dt = 1e-5
+    return s
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_wall_capacity_grows_inside_a_step_block(fp64):
+    """The ball's impact brings more than 16 triangle candidates within
+    reach: the step call goes back to its device checkpoint, the wall
+    history grows to 64 entries per ball and the block runs again; the
+    bounce then matches the oracle (which has no per-ball limit)."""
+    s = _ball_on_fine_plate()
+    g = ts.DEMSceneGPU(s, fp64=fp64)
+    o = oracle.OracleDEM(s)
+    assert g.tuned("wallCapacity") == 16
+    for n in (150, 150, 100):
+        g.step(n)
+        o.step(n)
+    assert g.tuned("wallCapacity") == 64 and g.tuned("wallReplays") >= 1
+    tol = 1e-9 if fp64 else 1e-6
+    assert np.abs(g.get("position") - o.get("pos")).max() < tol * 0.01
+    assert np.abs(g.get("velocity") - o.get("vel")).max() < tol * 0.1 * 10
+    assert o.get("vel")[0, 2] > 0  # it bounced
+    g.close()

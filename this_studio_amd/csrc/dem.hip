@@ -655,6 +655,15 @@ struct WallArgs {
     int* overflow;
 };
 
+// Slots of the register path (the cells of a box of walls list 2 to 6
+// triangles: two per face).
+constexpr int kWallSlots = 4;
+// Wall-contact candidates per ball: the default capacity, and the larger one
+// a scene moves to (history arrays grown, the block replayed) when a ball
+// finds more triangles within reach.  More than kWallCapMax is an error.
+constexpr int kWallCap = 16;
+constexpr int kWallCapMax = 64;
+
 // wall_contacts_ball for a ball whose cell lists at most NS candidate
 // triangles: every per-candidate value sits in a register slot indexed at
 // compile time (slot u = the cell's candidate u), so nothing goes through
@@ -671,7 +680,7 @@ __device__ __forceinline__ void wall_contacts_slots(const WallArgs& a, int i, in
 {
     constexpr int QH = 4;  // history entries loaded with the candidates
     const int N = a.nballs;
-    const int cap = min(16, a.KW);
+    const int cap = min(kWallCap, a.KW);
     const double r2 = rad * rad;
     // candidates and the first history entries
     double2 bx[NS], by[NS], bz[NS];
@@ -825,9 +834,6 @@ __device__ __forceinline__ void wall_contacts_slots(const WallArgs& a, int i, in
     ((T*)a.otx)[i] = Tw.x; ((T*)a.oty)[i] = Tw.y; ((T*)a.otz)[i] = Tw.z;
 }
 
-// Slots of the register path (the cells of a box of walls list 2 to 6
-// triangles: two per face).
-constexpr int kWallSlots = 4;
 // Scenes up to this many balls put at most one wave of pair work and one of
 // wall work on each SIMD, so the register path's lower occupancy costs
 // nothing there.
@@ -838,10 +844,9 @@ constexpr int kWallSlotBalls = 32768;
 // calBallWallContactForceTorqueKernel (:420-533), fused per ball.  The
 // candidate list, its spring history and cancel flags live in registers
 // and are written back as the new history.
-template <class T, bool SLOTS = false>
+template <class T, bool SLOTS = false, int KMAX = kWallCap>
 __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
 {
-    constexpr int KMAX = 16;
     const int o = a.b.orig[i];
     const int N = a.nballs;
     const D3 ri = ld3(a.b.px, a.b.py, a.b.pz, i);
@@ -985,12 +990,12 @@ __device__ __forceinline__ void wall_contacts_ball(const WallArgs& a, int i)
     ((T*)a.ofx)[i] = Fw.x; ((T*)a.ofy)[i] = Fw.y; ((T*)a.ofz)[i] = Fw.z;
     ((T*)a.otx)[i] = Tw.x; ((T*)a.oty)[i] = Tw.y; ((T*)a.otz)[i] = Tw.z;
 }
-template <class T>
+template <class T, int KMAX>
 __global__ __launch_bounds__(kT) void k_wall_contacts(WallArgs a)
 {
     const int i = xcd_block() * kT + threadIdx.x;
     if (i >= a.b.n) return;
-    wall_contacts_ball<T>(a, i);
+    wall_contacts_ball<T, false, KMAX>(a, i);
 }
 
 // ----------------------------------------------------------------- gather
@@ -1218,7 +1223,8 @@ __global__ __launch_bounds__(kT) void k_ball_half(BallHalfArgs a)
 // overlap on the device instead of running back to back.  WALLS: 0 none,
 // 1 the general candidate path, 2 register slots for cells of up to
 // kWallSlots candidates (more registers, fewer waves per SIMD: used for
-// scenes too small to fill the device).
+// scenes too small to fill the device), 3 the general path at the larger
+// capacity kWallCapMax.
 //
 // Each range is padded to a multiple of the XCD count and every XCD takes an
 // eighth of each range (consecutive blocks of it, for L2 reuse): blocks are
@@ -1234,7 +1240,10 @@ __global__ __launch_bounds__(kT) void k_forces(PairForceArgs pa, WallArgs wa, Bo
         if (p < min(pa.p.start[pa.b.n], pa.p.cap)) pair_force_one<T>(pa, p);
     } else if (WALLS && pos < qp + qw) {
         const int i = (xcd * qw + pos - qp) * kT + threadIdx.x;
-        if (i < wa.b.n) wall_contacts_ball<T, WALLS == 2>(wa, i);
+        if (i < wa.b.n) {
+            if (WALLS == 3) wall_contacts_ball<T, false, kWallCapMax>(wa, i);
+            else wall_contacts_ball<T, WALLS == 2, kWallCap>(wa, i);
+        }
     } else if (BONDS && pos < qp + qw + qb) {
         const int k = (xcd * qb + pos - qp - qw) * kT + threadIdx.x;
         if (k < ba.nb) bond_forces_one<T>(ba, k);
@@ -1451,11 +1460,6 @@ struct DBuf {
 
 }  // namespace
 
-struct demsph_dem;
-namespace {
-// Device checkpoint of a block of steps (defined after the state arrays).
-}  // namespace
-
 struct demsph_dem {
     hipStream_t stream = nullptr;
     bool initialized = false;
@@ -1548,6 +1552,7 @@ struct demsph_dem {
     int* block_h = nullptr;  // pinned: [0] pair_over, [1] pairs of the current list
     DBuf<unsigned char> ckbuf;
     long pair_replays = 0;  // blocks replayed with grown pair lists
+    long wall_replays = 0;  // step calls run again after the wall capacity grew
     struct HostCk {
         size_t iStep;
         double time;
@@ -1915,7 +1920,7 @@ struct demsph_dem {
         DEMSPH_HIP(host_malloc((void**)&overflow_h, sizeof(int)));
         pair_over.alloc(1);
         fill_i32(pair_over.p, 1, 0, stream);
-        DEMSPH_HIP(host_malloc((void**)&block_h, 2 * sizeof(int)));
+        DEMSPH_HIP(host_malloc((void**)&block_h, 3 * sizeof(int)));
         nbond_dev = 0;
         upload_bonds();
         DEMSPH_HIP(hipStreamSynchronize(stream));
@@ -2089,7 +2094,8 @@ struct demsph_dem {
     void forces_as(int walls, bool bonds, const PairForceArgs& pa, const WallArgs& wa, const BondArgs& ba, int nbp,
                    int nbw, int nbb)
     {
-        if (walls == 2) forces_launch<T, 2>(bonds, pa, wa, ba, nbp, nbw, nbb);
+        if (walls == 3) forces_launch<T, 3>(bonds, pa, wa, ba, nbp, nbw, nbb);
+        else if (walls == 2) forces_launch<T, 2>(bonds, pa, wa, ba, nbp, nbw, nbb);
         else if (walls == 1) forces_launch<T, 1>(bonds, pa, wa, ba, nbp, nbw, nbb);
         else forces_launch<T, 0>(bonds, pa, wa, ba, nbp, nbw, nbb);
     }
@@ -2105,7 +2111,7 @@ struct demsph_dem {
         if (with_walls) {
             prepare_walls();
             wa = wall_args(wallDt);
-            walls = wall_slots_set && n <= kWallSlotBalls ? 2 : 1;
+            walls = KW > kWallCap ? 3 : wall_slots_set && n <= kWallSlotBalls ? 2 : 1;
             nbw = blocks_for(n, kT);
         }
         BondArgs ba{};
@@ -2165,8 +2171,14 @@ struct demsph_dem {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         prepare_walls();
         const WallArgs wa = wall_args(wallDt);
-        if (fp64) DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<double>, gb, tb, 0, stream, wa);
-        else DEMSPH_LAUNCH(timer, "dem_wall_contacts", k_wall_contacts<float>, gb, tb, 0, stream, wa);
+        if (KW > kWallCap) {
+            if (fp64) DEMSPH_LAUNCH(timer, "dem_wall_contacts", (k_wall_contacts<double, kWallCapMax>), gb, tb, 0, stream, wa);
+            else DEMSPH_LAUNCH(timer, "dem_wall_contacts", (k_wall_contacts<float, kWallCapMax>), gb, tb, 0, stream, wa);
+        } else if (fp64) {
+            DEMSPH_LAUNCH(timer, "dem_wall_contacts", (k_wall_contacts<double, kWallCap>), gb, tb, 0, stream, wa);
+        } else {
+            DEMSPH_LAUNCH(timer, "dem_wall_contacts", (k_wall_contacts<float, kWallCap>), gb, tb, 0, stream, wa);
+        }
     }
 
     // Per-cell triangle candidate lists (count, scan, one host read of the
@@ -2339,28 +2351,89 @@ struct demsph_dem {
         }
     }
 
-    void check_overflow()
+    // The largest wall-candidate count of the balls that overflowed the
+    // capacity since the last call (0: none), read back and cleared.
+    int take_wall_overflow()
     {
-        if (nt == 0) return;
+        if (nt == 0) return 0;
         DEMSPH_HIP(hipMemcpyAsync(overflow_h, overflow.p, sizeof(int), hipMemcpyDeviceToHost, stream));
         DEMSPH_HIP(hipStreamSynchronize(stream));
         timer.harvest();
-        if (overflow_h[0] > 0) {
-            const int c = overflow_h[0];
-            DEMSPH_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), stream));
-            throw CapacityError("a ball had " + std::to_string(c) +
-                                " wall-triangle candidates, above the per-ball capacity of 16");
+        const int c = overflow_h[0];
+        if (c > 0) fill_i32(overflow.p, 1, 0, stream);
+        return c;
+    }
+
+    // Wall-contact history arrays with room for `need` candidates per ball
+    // (entry q of ball o stays at q * n + o, so the kept entries are copied
+    // as they are).  More than kWallCapMax is an error.
+    void grow_walls(int need)
+    {
+        if (need > kWallCapMax)
+            throw CapacityError("a ball had " + std::to_string(need) +
+                                " wall-triangle candidates, above the per-ball maximum of " +
+                                std::to_string(kWallCapMax));
+        if (need <= KW) return;
+        const size_t keep = (size_t)KW * n, m = (size_t)kWallCapMax * n;
+        std::vector<void*> old;
+        const auto grow = [&](auto& b, size_t per) {
+            using E = std::remove_reference_t<decltype(*b.p)>;
+            E* q = dmalloc<E>(m * per);
+            DEMSPH_HIP(hipMemcpyAsync(q, b.p, keep * per * sizeof(E), hipMemcpyDeviceToDevice, stream));
+            old.push_back(b.p);
+            b.p = q;
+            b.n = m * per;
+        };
+        grow(htri, 1);
+        grow(hcancel, 1);
+        for (auto* b : {&hsx, &hsy, &hsz, &hrx, &hry, &hrz, &hox, &hoy, &hoz, &hov}) grow(*b, 1);
+        if (record_contacts && crec.p) grow(crec, 12);
+        // the copies read the old blocks: they are released only after them
+        DEMSPH_HIP(hipStreamSynchronize(stream));
+        for (void* q : old) dev_free(q);
+        KW = kWallCapMax;
+        drop_graphs();  // captured launches hold the old arrays and capacity
+    }
+
+    // Wall history arrays for kw candidates per ball, contents not kept (a
+    // state import overwrites them).
+    void set_wall_capacity(int kw)
+    {
+        if (nt == 0) {
+            KW = kw;
+            return;
         }
+        const size_t hn = (size_t)kw * n;
+        htri.alloc(hn); hcancel.alloc(hn);
+        for (auto* b : {&hsx, &hsy, &hsz, &hrx, &hry, &hrz, &hox, &hoy, &hoz, &hov}) b->alloc(hn);
+        if (record_contacts && crec.p) crec.alloc(12 * hn);
+        KW = kw;
+        drop_graphs();
+    }
+
+    // A wall-candidate overflow is an error here (paths that cannot replay).
+    void check_overflow()
+    {
+        const int c = take_wall_overflow();
+        if (c > KW)
+            throw CapacityError("a ball had " + std::to_string(c) + " wall-triangle candidates, above the per-ball capacity of " +
+                                std::to_string(KW) + " (this block's wall contacts are incomplete)");
     }
 
     void solve_begin()
     {
         require_init();
         neighbor_search();
-        evaluate(0.0, 0.0);
+        // the first evaluation has zero time step: running it again after the
+        // wall capacity grew changes nothing but the completed wall contacts
+        for (;;) {
+            evaluate(0.0, 0.0);
+            const int c = take_wall_overflow();
+            if (c <= KW) break;
+            grow_walls(c);
+        }
         iStep = 0;
         time = 0.0;
-        check_overflow();
     }
 
     // Steps between rebuilds can replay a captured graph when nothing in
@@ -2495,9 +2568,12 @@ struct demsph_dem {
     }
 
     // nsteps steps with one host synchronisation at the end.  A block that
-    // rebuilds the pair lists saves a device checkpoint first; a list that
-    // outgrew its capacity sends the block back to the checkpoint with
-    // lists grown to fit, and it runs again.
+    // rebuilds the pair lists, or that has walls, saves a device checkpoint
+    // first; a pair list that outgrew its capacity, or a ball with more
+    // wall-triangle candidates than the wall capacity, sends the block back
+    // to the checkpoint with the lists or the wall history grown to fit, and
+    // it runs again.  (With a host hook nothing is replayed: the rebuilds
+    // size their lists on the host, and a wall overflow is an error.)
     void step(const double* g, double dt, int nsteps)
     {
         require_init();
@@ -2505,27 +2581,38 @@ struct demsph_dem {
         if (!(dt > 0)) throw InvalidArg("time step must be positive");
         if (nsteps <= 0) return;
         const bool rebuilds = !hook_fn && nsteps >= 5 - (int)(iStep % 5);
+        const bool walls = nt > 0;
+        const bool replayable = !hook_fn && (rebuilds || walls);
         for (int attempt = 0; attempt < 8; attempt++) {
-            if (rebuilds) {
+            if (replayable) {
                 dem_block_save(this);
                 fill_i32(pair_over.p, 1, 0, stream);
             }
             run_block(g, dt, nsteps);
             DEMSPH_HIP(hipMemcpyAsync(block_h, pair_over.p, sizeof(int), hipMemcpyDeviceToHost, stream));
             DEMSPH_HIP(hipMemcpyAsync(block_h + 1, pstart[cp].p + n, sizeof(int), hipMemcpyDeviceToHost, stream));
+            if (walls) DEMSPH_HIP(hipMemcpyAsync(block_h + 2, overflow.p, sizeof(int), hipMemcpyDeviceToHost, stream));
             DEMSPH_HIP(hipStreamSynchronize(stream));
+            timer.harvest();
             const int need = rebuilds ? block_h[0] : 0;
-            if (need == 0) {
+            const int wneed = walls ? block_h[2] : 0;  // nonzero only above the capacity
+            if (wneed > 0) fill_i32(overflow.p, 1, 0, stream);
+            if (need == 0 && wneed == 0) {
                 npairs = block_h[1];
-                check_overflow();
                 return;
             }
-            // back to the checkpoint with both lists grown to fit
-            pair_replays++;
+            if (!replayable)
+                throw CapacityError("a ball had " + std::to_string(wneed) +
+                                    " wall-triangle candidates, above the per-ball capacity of " + std::to_string(KW) +
+                                    " (a scene with a host hook does not replay its steps)");
+            // back to the checkpoint with the lists or the wall history grown
+            if (need > 0) pair_replays++;
+            if (wneed > 0) wall_replays++;
             drop_graphs();
-            dem_block_restore(this, need + need / 4 + 64);
+            dem_block_restore(this, need > 0 ? need + need / 4 + 64 : 0);
+            if (wneed > 0) grow_walls(wneed);  // above the maximum: an error, the state at the checkpoint
         }
-        throw std::runtime_error("pair list capacity did not converge");
+        throw std::runtime_error("pair list and wall capacities did not converge");
     }
 };
 
@@ -2565,6 +2652,8 @@ void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, do
     d->half(first, g, h);
 }
 void dem_end_block(demsph_dem* d) { d->check_overflow(); }
+int dem_take_wall_overflow(demsph_dem* d) { return d->take_wall_overflow(); }
+void dem_grow_walls(demsph_dem* d, int need) { d->grow_walls(need); }
 size_t dem_bytes_estimate(demsph_dem* d, const double* minb, const double* maxb)
 {
     return d->bytes_estimate(minb, maxb);
@@ -3280,24 +3369,28 @@ extern "C" int demsph_dem_import_state(demsph_dem* d, const void* in, size_t byt
         std::memcpy(&h, in, sizeof h);
         if (h.magic != kDemStateMagic || h.version != 2u) throw InvalidArg("not a DEM scene state (or of another layout version)");
         if (h.n != d->n || h.nc != d->nc || h.nw != d->nw || h.nv != d->nv || h.nt != d->nt ||
-            h.nbond != d->nbond_dev || h.KW != d->KW || h.fp64 != (d->fp64 ? 1 : 0))
+            h.nbond != d->nbond_dev || h.fp64 != (d->fp64 ? 1 : 0))
             throw InvalidArg("the state belongs to a scene of other counts or precision");
+        if (h.KW != kWallCap && h.KW != kWallCapMax) throw InvalidArg("corrupt DEM state header (wall capacity)");
         if (h.npairs < 0 || (h.has_list == 0 && h.npairs != 0)) throw InvalidArg("corrupt DEM state header");
         // the size the header implies, computed before anything is resized
         const size_t np = (size_t)h.npairs;
         size_t total = sizeof h;
         {
             const bool had = d->has_list;
-            const int had_np = d->npairs;
+            const int had_np = d->npairs, had_kw = d->KW;
             d->has_list = h.has_list != 0;
             d->npairs = h.npairs;
+            d->KW = h.KW;
             dem_state_arrays(d, [&](const void*, size_t b) { total += b; });
             d->has_list = had;
             d->npairs = had_np;
+            d->KW = had_kw;
         }
         if (bytes != total) throw InvalidArg("state size does not match the scene");
         DEMSPH_HIP(hipStreamSynchronize(d->stream));
         d->drop_graphs();
+        if (h.KW != d->KW) d->set_wall_capacity(h.KW);  // the state's wall history layout
         if (h.has_list) {
             if ((size_t)d->pcap[d->cp] < np) d->alloc_pairs(d->cp, h.npairs + h.npairs / 4);
             if (d->rpair.n < np) d->rpair.alloc(std::max<size_t>(np, (size_t)d->pcap[d->cp]));
@@ -3655,6 +3748,14 @@ extern "C" int demsph_dem_tune_get(demsph_dem* d, const char* knob, int* value)
         }
         if (k == "wallSlots") {
             *value = d->wall_slots_set;
+            return;
+        }
+        if (k == "wallCapacity") {  // wall-triangle candidates per ball
+            *value = d->KW;
+            return;
+        }
+        if (k == "wallReplays") {
+            *value = (int)d->wall_replays;
             return;
         }
         if (k != "pairLanes") throw InvalidArg("unknown knob '" + k + "'");

@@ -36,6 +36,11 @@ double dem_begin_step(demsph_dem* d, double dt);
 void dem_evaluate(demsph_dem* d, double h, double dt);
 void dem_hooks_and_half(demsph_dem* d, bool first, const double* g, double h, double hookTime);
 void dem_end_block(demsph_dem* d);
+// Wall-candidate overflow since the last call (0: none; read back and
+// cleared), and the wall history grown to hold `need` candidates per ball
+// (an error above the maximum).
+int dem_take_wall_overflow(demsph_dem* d);
+void dem_grow_walls(demsph_dem* d, int need);
 // A DEM scene keeps the couplings that hold it: when it is destroyed first,
 // each of them is detached (it then refuses to step and touches nothing of
 // that scene).

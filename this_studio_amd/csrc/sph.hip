@@ -3002,9 +3002,9 @@ struct demsph_couple {
     }
 
     // A block of coupled steps.  Both scenes are checkpointed on the device
-    // first; when a fluid list or the fluid-ball list outgrows its capacity
-    // the lists grow, both scenes go back to the checkpoint and the block
-    // runs again, so the results are those of ample lists.
+    // first; when a fluid list, the fluid-ball list or the DEM wall-contact
+    // capacity overflows, it grows, both scenes go back to the checkpoint and
+    // the block runs again, so the results are those of ample capacities.
     void step(const double* g, double dt, int nsteps)
     {
         if (!(dt > 0.0)) throw InvalidArg("time step must be positive");
@@ -3044,10 +3044,8 @@ struct demsph_couple {
             f->solids = SolidView();
             const bool grew = f->check_overflow();  // reads all three list maxima, grows the fluid lists
             const int mfs = f->maxcount_h[2];
-            if (!grew && mfs <= K_fs) {
-                dem_end_block(d);
-                return;
-            }
+            const int wneed = dem_take_wall_overflow(d);  // ball-wall candidates above the DEM capacity
+            if (!grew && mfs <= K_fs && wneed == 0) return;
             if (mfs > K_fs) {
                 K_fs = mfs + mfs / 4 + 8;
                 ready = false;  // setup() reallocates the fluid-ball lists
@@ -3055,6 +3053,7 @@ struct demsph_couple {
             f->cP = sP; f->cV = sV; f->cD = sD; f->cO = sO;
             f->checkpoint(false);
             dem_block_restore(d, 0);
+            if (wneed > 0) dem_grow_walls(d, wneed);  // above the maximum: an error, both scenes at the checkpoint
             replays++;
             setup();
         }
