@@ -26,6 +26,7 @@
 #include <cmath>
 #include <chrono>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <set>
@@ -1308,11 +1309,9 @@ __global__ void k_circumcenters(WallMoveArgs a)
 
 // ----------------------------------------------------------------- hooks
 // addConstantForce (externalForceTorque.cu:4-12), per insertion index.
-__global__ void k_hook_const(Balls b, Forces f, const double* fo, double s)
+__device__ __forceinline__ void hook_const_one(const Balls& b, const Forces& f, const double* fo, double s, int i)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
     const int o = b.orig[i];
     const double ax = fo[3 * (size_t)o] * s, ay = fo[3 * (size_t)o + 1] * s, az = fo[3 * (size_t)o + 2] * s;
     f.fx[i] += ax;
@@ -1320,11 +1319,9 @@ __global__ void k_hook_const(Balls b, Forces f, const double* fo, double s)
     f.fz[i] += az;
 }
 // addGlobalDampingForceTorque (externalForceTorque.cu:26-37)
-__global__ void k_hook_damping(Balls b, Forces f, double c)
+__device__ __forceinline__ void hook_damping_one(const Balls& b, const Forces& f, double c, int i)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
     const D3 F = ld3(f.fx, f.fy, f.fz, i), T = ld3(f.tx, f.ty, f.tz, i);
     double nx, ny, nz;
     const double cf = c * hk_len(F.x, F.y, F.z);
@@ -1335,11 +1332,10 @@ __global__ void k_hook_damping(Balls b, Forces f, double c)
     st3(f.tx, f.ty, f.tz, i, D3{T.x - nx * ct, T.y - ny * ct, T.z - nz * ct});
 }
 // addBuoyancyDragKernel (externalForceTorque.cu:66-133)
-__global__ void k_hook_buoyancy(Balls b, Forces f, double rho, double cd, D3 g, D3 vf, double level)
+__device__ __forceinline__ void hook_buoyancy_one(const Balls& b, const Forces& f, double rho, double cd, D3 g, D3 vf,
+                                                  double level, int i)
 {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
     if (b.im[i] <= 0.0) return;
     const double zc = b.pz[i], r = b.r[i];
     const double dd = level - zc;
@@ -1366,6 +1362,50 @@ __global__ void k_hook_buoyancy(Balls b, Forces f, double rho, double cd, D3 g, 
     f.fx[i] += Fb.x + Fd.x;
     f.fy[i] += Fb.y + Fd.y;
     f.fz[i] += Fb.z + Fd.z;
+}
+
+__global__ void k_hook_const(Balls b, Forces f, const double* fo, double s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b.n) hook_const_one(b, f, fo, s, i);
+}
+__global__ void k_hook_damping(Balls b, Forces f, double c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b.n) hook_damping_one(b, f, c, i);
+}
+__global__ void k_hook_buoyancy(Balls b, Forces f, double rho, double cd, D3 g, D3 vf, double level)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < b.n) hook_buoyancy_one(b, f, rho, cd, g, vf, level, i);
+}
+
+// The built-in hook program applied by the gather right after a ball's
+// totals are stored (same per-ball arithmetic and order as the separate
+// hook launches): up to kMaxHooksFused entries.
+constexpr int kMaxHooksFused = 4;
+struct HookDev {
+    int kind;  // 0 constant force (scaled by s), 1 damping, 2 buoyancy and drag
+    double c, rho, cd, level, s;
+    D3 g, vf;
+    const double* fo;
+};
+struct HookTable {
+    int n;
+    HookDev op[kMaxHooksFused];
+};
+template <class T>
+__global__ __launch_bounds__(kT) void k_gather_hooks(GatherArgs a, HookTable ht)
+{
+    const int j = xcd_block() * kT + threadIdx.x;
+    if (j >= a.b.n) return;
+    gather_ball<T>(a, j);
+    for (int k = 0; k < ht.n; k++) {
+        const HookDev& h = ht.op[k];
+        if (h.kind == 0) hook_const_one(a.b, a.out, h.fo, h.s, j);
+        else if (h.kind == 1) hook_damping_one(a.b, a.out, h.c, j);
+        else hook_buoyancy_one(a.b, a.out, h.rho, h.cd, h.g, h.vf, h.level, j);
+    }
 }
 
 // Views in insertion order for a user hook (AoS x3 doubles).
@@ -2042,14 +2082,18 @@ struct demsph_dem {
     // Ball-pair and bond forces plus the reduction: calculateBallContactForceTorque
     // (DEMSolver.h:599-695); with walls, ball-wall contacts are added and the
     // walls then move (addTriangleContactForceTorque, :428-597).
-    void evaluate(double springDt, double wallDt, bool do_pairs = true, bool do_walls = true)
+    // hook_t: the time of the external-force hooks that follow this
+    // evaluation (NaN: none); the built-in hook program then runs inside the
+    // gather, and a user hook after it.
+    void evaluate(double springDt, double wallDt, bool do_pairs = true, bool do_walls = true,
+                  double hook_t = std::numeric_limits<double>::quiet_NaN())
     {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         Balls b = cur();
         const bool walls = do_walls && nt > 0;
         if (do_pairs) evaluate_pairs(springDt, walls, wallDt);
         else if (walls) evaluate_walls(wallDt);
-        gather(nt > 0 && (walls || walls_valid));
+        gather(nt > 0 && (walls || walls_valid), hook_t);
         if (walls) {
             walls_valid = true;
             if (!walls_static) move_walls(wallDt);
@@ -2228,12 +2272,46 @@ struct demsph_dem {
         return ga;
     }
 
-    void gather(bool with_walls)
+    void gather(bool with_walls, double hook_t = std::numeric_limits<double>::quiet_NaN())
     {
         const dim3 gb(blocks_for(n, kT)), tb(kT);
         const GatherArgs ga = gather_args(with_walls);
+        const bool hooked = !std::isnan(hook_t);
+        if (hooked && !hook_fn && !hooks.empty() && hooks.size() <= (size_t)kMaxHooksFused) {
+            const HookTable ht = hook_table(hook_t);
+            if (fp64) DEMSPH_LAUNCH(timer, "dem_gather", k_gather_hooks<double>, gb, tb, 0, stream, ga, ht);
+            else DEMSPH_LAUNCH(timer, "dem_gather", k_gather_hooks<float>, gb, tb, 0, stream, ga, ht);
+            return;
+        }
         if (fp64) DEMSPH_LAUNCH(timer, "dem_gather", k_gather<double>, gb, tb, 0, stream, ga);
         else DEMSPH_LAUNCH(timer, "dem_gather", k_gather<float>, gb, tb, 0, stream, ga);
+        if (hooked) run_hooks(hook_t);
+    }
+
+    // The built-in hook program at time t, for the gather (the constant
+    // force's ramp factor is taken on the host, as run_hooks does).
+    HookTable hook_table(double t) const
+    {
+        HookTable ht{};
+        ht.n = (int)hooks.size();
+        for (int k = 0; k < ht.n; k++) {
+            const HookOp& h = hooks[k];
+            HookDev& o = ht.op[k];
+            o.kind = h.kind;
+            if (h.kind == 0) {
+                double sc = 1.0;
+                if (h.ramp > 0) sc = std::min(t / h.ramp, 1.0);
+                o.s = sc;
+                o.fo = (const double*)h.f->p;
+            } else if (h.kind == 1) {
+                o.c = h.c;
+            } else {
+                o.rho = h.rho; o.cd = h.cd; o.level = h.level;
+                o.g = D3{h.g[0], h.g[1], h.g[2]};
+                o.vf = D3{h.vf[0], h.vf[1], h.vf[2]};
+            }
+        }
+        return ht;
     }
 
     void move_walls(double dt)
@@ -2555,11 +2633,9 @@ struct demsph_dem {
                 evaluate_half(true, g, h, dt);
                 evaluate_half(false, g, h, dt);
             } else {
-                evaluate(h, dt);
-                run_hooks(time - h);
+                evaluate(h, dt, true, true, time - h);  // with the hooks at t - dt/2
                 half(true, g, h);
-                evaluate(h, dt);
-                run_hooks(time);
+                evaluate(h, dt, true, true, time);
                 half(false, g, h);
             }
             step_stamp(stamps, stream);
