@@ -227,10 +227,10 @@ __device__ __forceinline__ void walk_pair_rows(const Balls& b, const GridGeom& g
 // cells and slots within a row in order), so the rows of a group, taken in
 // lane order, give the serial walk's visit order.
 // Up to kPairLanesMaxBalls balls a scene uses 8 lanes per ball (measured on
-// MI355X: C1's 4,096 balls 48 -> 44 us per step; the rebuild kernels 47 ->
-// 22 us); larger scenes have threads enough and use 1 (C3's 112,768 balls,
-// one process: 168 us per step with 1 lane, 170 with 2, 171 with 4, 173
-// with 8).  Lane q walks rows [9q/L, 9(q+1)/L).  "pairLanes" overrides it.
+// MI355X: a 4,096-ball collapse in a box, 48 -> 44 us per step; the rebuild
+// kernels 47 -> 22 us); larger scenes have threads enough and use 1 (a
+// 112,768-ball bonded beam with clumps, one process: 168 us per step with 1
+// lane, 170 with 2, 171 with 4, 173 with 8).  Lane q walks rows [9q/L, 9(q+1)/L).  "pairLanes" overrides it.
 constexpr int kPairLanesMaxBalls = 32768;
 inline int pair_walk_lanes(int n, int set) { return set > 0 ? set : (n <= kPairLanesMaxBalls ? 8 : 1); }
 template <int L>
