@@ -262,3 +262,31 @@ def test_scenes_destroyed_before_their_coupling():
             assert np.all(np.isfinite(fluid.get("position")))
             fluid.close()
         c.close()
+
+
+def test_full_size_ice_cylinder_matches_oracle():
+    """The coupled ice-cylinder workload at full size (979,470 fluid
+    particles, 466,254 dummies, 50,665 bonded ice balls, 225,861 bonds, a
+    2,048-facet cylinder): bonds bit-exact, then 5 coupled steps against the
+    FP64 oracle with the reduced scene's tolerances."""
+    import oracle
+
+    spec, d = scenes.ice_cylinder_coupled()
+    of = oracle.OracleSPH(spec)
+    od = oracle.OracleDEM(d)
+    oc = oracle.OracleCoupled(of, od)
+    gf, gd = fluid_scene(spec), ts.DEMSceneGPU(d)
+    assert np.array_equal(gd.get("bondPointed"), od.get("bond_pointed"))
+    assert np.array_equal(gd.get("bondPointing"), od.get("bond_pointing"))
+    assert od.get("bond_pointed").size == 225861
+    gc = ts.CoupledScene(gf, gd)
+    oc.step(spec.dt, 5, spec.gravity)
+    gc.step(spec.dt, 5, spec.gravity)
+    Fo, Fg = oc.get("couplingForce"), gc.get("couplingForce")
+    r = 0.004
+    weight = 917.0 * 4.0 / 3.0 * np.pi * r ** 3 * 9.81
+    assert np.abs(Fg - Fo).max() < 1e-4 * max(np.abs(Fo).max(), weight)
+    assert np.abs(gd.get("position") - od.get("pos")).max() < 1e-9
+    assert np.abs(gf.get("position") - of.get("pos")).max() < 1e-9
+    assert np.abs(gf.get("velocity") - of.get("vel")).max() < 1e-5
+    gc.close()

@@ -418,3 +418,23 @@ def test_reference_dam_break_full_series():
     assert np.abs(zmax[early] - ser[1:, 3][early]).max() < 1e-3
     assert np.abs(zmax - ser[1:, 3]).max() < 0.2
     assert np.abs(rho - ser[1:, 5]).max() < 0.05
+
+
+def test_full_size_dam_break_lists_and_steps_match_oracle():
+    """The 256,000-particle dam break at full size: the fluid-fluid and
+    fluid-dummy lists (19.5M and 0.5M entries) bit-exact against the
+    oracle's, in the reference's order, then 20 steps within the FP32
+    pair-arithmetic tolerances (positions 1e-9 m, velocities 1e-5 m/s,
+    densities 1e-3 kg/m^3)."""
+    spec = scenes.dam_break()
+    o = oracle.OracleSPH(spec)
+    s = make_gpu(spec)
+    o.neighbor_search()
+    s.neighbor_search()
+    for f, of in (("neighborCount", "count_ff"), ("dummyNeighborCount", "count_fd"),
+                  ("objectPointing", "pointing_ff"), ("dummyObjectPointing", "pointing_fd")):
+        assert np.array_equal(s.get(f), o.get(of)), f
+    assert o.get("pointing_ff").size > 19_000_000
+    o.step(spec.dt, 20)
+    s.step(spec.dt, 20)
+    _compare(s, o, 1e-9, 1e-5, 1e-3)
