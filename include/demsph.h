@@ -533,9 +533,12 @@ int demsph_server_set_block(demsph_server* v, long steps);
  * the cache on; mode -1 off (default), 0 result-invariant knobs, 1 also
  * pair lanes), and every job's DEM scene (demsph_dem_autotune's "pairLanes",
  * result-invariant, once per size class of ball count, walls, bonds,
- * periodic axes and precision, with trialSteps of at least 5).  Set before
- * demsph_server_run. */
+ * periodic axes and precision, with trialSteps of at least 5).  Jobs of
+ * fewer than minSteps steps (demsph_server_set_autotune_min_steps, default
+ * 2,000) are not tuned: a size class's trials cost some 40-50 steps of the
+ * scene.  Set before demsph_server_run. */
 int demsph_server_set_autotune(demsph_server* v, int mode, int trialSteps);
+int demsph_server_set_autotune_min_steps(demsph_server* v, long minSteps);
 /* fluid and/or dem non-NULL; fluidMin/Max with a fluid, demMin/Max with a
  * DEM scene (a coupled DEM scene usually shares the fluid bounds). */
 int demsph_server_submit(demsph_server* v, demsph_sph* fluid, demsph_dem* dem, const double* fluidMin,

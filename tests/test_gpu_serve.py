@@ -73,7 +73,7 @@ def test_autotuned_serving_keeps_results():
     same results, bit for bit, as the plain untuned path."""
     ts.SPHScene.clear_autotune_cache()
     jobs = [SMALL[0], SMALL[2], SMALL[3]]
-    srv, ids = serving.serve(jobs, slots=3, graphs=True, block=32, autotune="exact")
+    srv, ids = serving.serve(jobs, slots=3, graphs=True, block=32, autotune="exact", tune_min_steps=0)
     for job, i in zip(jobs, ids):
         assert srv.info(i)["status"] == 0, srv.info(i)["error"]
     plain, pids = serving.serve(jobs, slots=1, graphs=False)
@@ -93,7 +93,7 @@ def test_autotune_on_admission_with_eight_slots():
     kinds = ("wcsph", "dem", "wcsph", "coupled")
     jobs = [MixJob(kinds[k % 4], 2500 + 700 * k, 30 + (k % 3) * 6, 0.0, 300 + k) for k in range(16)]
     print("[serve] 16 jobs, 8 slots, autotune on admission", flush=True)
-    srv, ids = serving.serve(jobs, slots=8, graphs=True, block=16, autotune="exact")
+    srv, ids = serving.serve(jobs, slots=8, graphs=True, block=16, autotune="exact", tune_min_steps=0)
     st = srv.stats()
     assert st["jobsFailed"] == 0 and st["jobsDone"] == len(jobs), [srv.info(i)["error"] for i in ids]
     plain, pids = serving.serve(jobs, slots=1, graphs=False)

@@ -711,6 +711,7 @@ def _declare_server(L):
         "demsph_server_destroy": [_vp],
         "demsph_server_set_block": [_vp, C.c_long],
         "demsph_server_set_autotune": [_vp, C.c_int, C.c_int],
+        "demsph_server_set_autotune_min_steps": [_vp, C.c_long],
         "demsph_server_submit": [_vp, _vp, _vp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_long, C.c_double,
                                  C.POINTER(C.c_int)],
         "demsph_server_run": [_vp],
@@ -738,9 +739,10 @@ class Server:
     KINDS = ("wcsph", "dem", "coupled")
 
     def __init__(self, slots: int = 4, memory_budget: float = 288e9, graphs: bool = True, block: int = 100,
-                 autotune: str | None = None, tune_steps: int = 3):
+                 autotune: str | None = None, tune_steps: int = 3, tune_min_steps: int = 2000):
         """autotune: None (off), "exact" (result-invariant knobs) or "pairs"
-        (also pair lanes): every job's fluid scene is tuned on admission."""
+        (also pair lanes): the fluid and DEM scenes of every job of at least
+        tune_min_steps steps are tuned on admission."""
         modes = {None: -1, "exact": 0, "pairs": 1}
         if autotune not in modes:
             raise ValueError("autotune must be None, 'exact' or 'pairs'")
@@ -751,6 +753,7 @@ class Server:
         check(L.demsph_server_create(C.byref(self._h), int(slots), float(memory_budget), 1 if graphs else 0))
         check(L.demsph_server_set_block(self._h, int(block)))
         check(L.demsph_server_set_autotune(self._h, modes[autotune], int(tune_steps)))
+        check(L.demsph_server_set_autotune_min_steps(self._h, int(tune_min_steps)))
 
     def close(self) -> None:
         if self._h:

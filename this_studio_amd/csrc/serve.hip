@@ -81,6 +81,10 @@ struct demsph_server {
     bool graphs = true;
     long block = 100;  // steps per step() call
     int autotune = -1;   // WCSPH autotuner mode on admission (-1: off)
+    // Jobs shorter than this many steps are not tuned on admission: a size
+    // class's trials cost some 40-50 steps of the scene, which a short job
+    // does not earn back.
+    long tune_min_steps = 2000;
     int tune_steps = 3;  // timed steps per autotuner trial
     int device = 0;
     double clock_hz = 1.0e8;
@@ -160,7 +164,7 @@ struct demsph_server {
                 ok(demsph_sph_set_graphs(j.fluid, graphs ? 1 : 0));
                 ok(demsph_sph_initialize(j.fluid, j.fmin, j.fmax, j.g));
                 // trials restore the state; a cached size class costs nothing
-                if (autotune >= 0) ok(demsph_sph_autotune(j.fluid, j.dt, tune_steps, autotune, 1, nullptr));
+                if (tune_job(j)) ok(demsph_sph_autotune(j.fluid, j.dt, tune_steps, autotune, 1, nullptr));
                 st = sph_stream(j.fluid);
             } else {
                 DEMSPH_HIP(stream_create(&own));
@@ -172,7 +176,7 @@ struct demsph_server {
                 ok(demsph_dem_initialize(j.dem, j.dmin, j.dmax));
                 ok(demsph_dem_solve_begin(j.dem));
                 // pair lanes build the same lists in the same order: every mode keeps results
-                if (autotune >= 0) dem_autotune_cached(j.dem, j.g, j.dt, std::max(5, tune_steps));
+                if (tune_job(j)) dem_autotune_cached(j.dem, j.g, j.dt, std::max(5, tune_steps));
             }
             if (j.kind == 2) ok(demsph_couple_create(&j.couple, j.fluid, j.dem));
             DEMSPH_HIP(dev_malloc((void**)&sbuf, sizeof(unsigned long long) * (size_t)(j.steps + 1)));
@@ -232,6 +236,8 @@ struct demsph_server {
         stream_destroy(own);
         j.finished = now();
     }
+
+    bool tune_job(const Job& j) const { return autotune >= 0 && j.steps >= tune_min_steps; }
 
     void worker()
     {
@@ -393,6 +399,15 @@ extern "C" int demsph_server_set_autotune(demsph_server* v, int mode, int trialS
         if (v->ran) throw InvalidArg("the server has already run");
         v->autotune = mode;
         v->tune_steps = trialSteps;
+    });
+}
+
+extern "C" int demsph_server_set_autotune_min_steps(demsph_server* v, long steps)
+{
+    return guarded([&] {
+        if (!v) throw InvalidArg("null server");
+        if (steps < 0) throw InvalidArg("the step count must not be negative");
+        v->tune_min_steps = steps;
     });
 }
 

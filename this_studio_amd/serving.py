@@ -40,11 +40,13 @@ def submit(server: Server, job: scenes.MixJob, steps: int | None = None, arrival
 
 
 def serve(jobs: list[scenes.MixJob], slots: int, graphs: bool, memory_budget: float = 288e9,
-          block: int = 100, autotune: str | None = None, progress=None) -> tuple[Server, list[int]]:
+          block: int = 100, autotune: str | None = None, progress=None,
+          tune_min_steps: int = 2000) -> tuple[Server, list[int]]:
     """Submits every job to a new server and runs it; returns the server
     (for results) and the job ids in submission order.  progress: see
     Server.set_progress."""
-    srv = Server(slots=slots, memory_budget=memory_budget, graphs=graphs, block=block, autotune=autotune)
+    srv = Server(slots=slots, memory_budget=memory_budget, graphs=graphs, block=block, autotune=autotune,
+                 tune_min_steps=tune_min_steps)
     ids = [submit(srv, j) for j in jobs]
     if progress is not None:
         srv.set_progress(progress)
