@@ -2088,8 +2088,6 @@ struct demsph_dem {
     void evaluate(double springDt, double wallDt, bool do_pairs = true, bool do_walls = true,
                   double hook_t = std::numeric_limits<double>::quiet_NaN())
     {
-        const dim3 gb(blocks_for(n, kT)), tb(kT);
-        Balls b = cur();
         const bool walls = do_walls && nt > 0;
         if (do_pairs) evaluate_pairs(springDt, walls, wallDt);
         else if (walls) evaluate_walls(wallDt);
