@@ -147,8 +147,11 @@ struct PinnedSnap {
         }
         DEMSPH_HIP(hipEventRecord(ev, s));
     }
+    // Under the capture lock: a synchronisation from this thread while
+    // another thread captures a graph can invalidate that capture.
     void wait() const
     {
+        CaptureGuard g(capture_lock());
         DEMSPH_HIP(hipSetDevice(device));
         DEMSPH_HIP(hipEventSynchronize(ev));
     }

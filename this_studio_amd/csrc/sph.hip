@@ -129,14 +129,27 @@ __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x);
 // factor but t is loop-invariant and hoisted by the compiler.
 // Evaluated unconditionally and selected, so that a pair loop has no
 // divergent branch between its gathers and their use.
-__device__ __forceinline__ float grad_factor(float r, float h)
+struct GradK {
+    float half_invh, coef, two_h;
+};
+__device__ __forceinline__ GradK grad_k(float h)
 {
+    GradK k;
     const float invh = 1.0f / h;
-    const float t = 1.0f - 0.5f * r * invh;
+    k.half_invh = 0.5f * invh;  // exact: t below equals 1 - 0.5 r invh
     const float sigma = (float)(21.0 / (16.0 * M_PI)) * invh * invh * invh;
-    const float f = -5.0f * sigma * invh * invh * (t * t * t);
-    return (r < 1.e-20f || r >= 2.0f * h) ? 0.0f : f;
+    k.coef = -5.0f * sigma * invh * invh;
+    k.two_h = 2.0f * h;
+    return k;
 }
+__device__ __forceinline__ float grad_factor(float r, const GradK& k)
+{
+    const float t = 1.0f - r * k.half_invh;
+    const float f = k.coef * (t * t * t);
+    return (r < 1.e-20f || r >= k.two_h) ? 0.0f : f;
+}
+__device__ __forceinline__ float grad_factor(float r, float h) { return grad_factor(r, grad_k(h)); }
+
 
 // Inverse EOS for the dummy density (WCSPHIntegrationKernel.cu:132,249):
 // linear rho = P/c^2 + rho0; Tait rho = rho0 (1 + P/B)^(1/gamma).
@@ -717,75 +730,111 @@ __device__ __forceinline__ T lane_group_sum(T v)
     return v;
 }
 
-// updateWCSPHAccelerationKernel (WCSPHIntegrationKernel.cu:42-148) fused
-// with the half kick (:150-159) and, in the first half, the drift (:161-170).
-template <bool FIRST, bool UNIF, int LANES>
-__global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
+// base[j] addressed as a uniform base plus a 32-bit byte offset, so that a
+// gather costs one address instruction (records below 2^32 bytes).
+template <class T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ base, int j)
 {
-    const int g = xcd_block() * kThreads + threadIdx.x;
-    const int i = g / LANES, q = g % LANES;
-    if (i >= a.nf) return;  // uniform over a lane group
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)j * (unsigned)sizeof(T));
+}
+
+// Per-particle values of a pair pass.
+struct PairSelf {
+    double x, y, z;
+    float vx, vy, vz, w;  // w: phase bits
+    float rho, p, inv_rho;
+    Phase ph;
+    GradK gff, gfd;  // uniform scenes: kernel constants of fluid and dummy pairs
+};
+
+template <bool UNIF>
+__device__ __forceinline__ PairSelf pair_self(const SphArgs& a, int i)
+{
     const FluidPos Pi = a.P[i];
     const float4 Vi = a.V[i];
-    const double xi = Pi.x, yi = Pi.y, zi = Pi.z;
-    const float vix = Vi.x, viy = Vi.y, viz = Vi.z;
-    const Phase pi = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
-    const float rhoi = Pi.rho, PL = Pi.p;
-    const float inv_rhoi = 1.0f / rhoi;
-    float ax = 0.f, ay = 0.f, az = 0.f;
-
-    const auto load_f = [&](int j) { return FluidRec{a.P[j], a.V[j]}; };
-    const auto load_d = [&](int j) { return DummyRec{a.DP[j], a.DV[j]}; };
-    for_chunks_gathered<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, load_f, [&](const FluidRec& R, int, bool valid) {
-        const FluidPos& Pj = R.P;
-        const float4& Vj = R.V;
-        const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
-        const float rx = (float)(xi - Pj.x), ry = (float)(yi - Pj.y), rz = (float)(zi - Pj.z);
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        // e = -r_ij/|r_ij| (zero below |r|^2 = 1e-20, myVec.h:112-128)
-        const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
-        const float r = r2 * inv_r;
-        const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
-        const float UL = -(vix * rx + viy * ry + viz * rz) * inv_r;
-        const float UR = -(Vj.x * rx + Vj.y * ry + Vj.z * rz) * inv_r;
-        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + Pj.rho);
-        const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
-        const float Ps = 0.5f * (PL + Pj.p) + 0.5f * beta * rbar * (UL - UR);
-        const float mr = pj.m * inv_rhoi * rcp(Pj.rho);
-        const float kp = -2.0f * mr * Ps * f;
-        // r_hat . gradW = f * r
-        const float kv = 2.0f * (0.5f * (pi.nu + pj.nu)) * (f * r) * mr;
-        ax += kp * rx + kv * (vix - Vj.x);
-        ay += kp * ry + kv * (viy - Vj.y);
-        az += kp * rz + kv * (viz - Vj.z);
-    });
-    if (a.nd > 0) {
-        for_chunks_gathered<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, load_d, [&](const DummyRec& R, int j, bool valid) {
-            const DummyPos& Dj = R.D;
-            const float4& DVj = R.V;
-            const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
-            const float rx = (float)(xi - Dj.x), ry = (float)(yi - Dj.y), rz = (float)(zi - Dj.z);
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
-            const float r = r2 * inv_r;
-            const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
-            const float nx = Dj.nx, ny = Dj.ny, nz = DVj.w;
-            const float UL = -(nx * vix + ny * viy + nz * viz);
-            const float UR = -UL - 2.0f * (nx * DVj.x + ny * DVj.y + nz * DVj.z);
-            const float grj = fmaxf(-(a.gx * rx + a.gy * ry + a.gz * rz), 0.0f);
-            const float PR = PL + rhoi * grj;
-            const float rhoj = dummy_rho(a, pj, PR);
-            const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
-            const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
-            const float Ps = 0.5f * (PL + PR) + 0.5f * beta * rbar * (UL - UR);
-            const float mr = pj.m * inv_rhoi * rcp(rhoj);
-            const float kp = -2.0f * mr * Ps * f;
-            const float kv = 2.0f * (0.5f * (pi.nu + pj.nu)) * (f * r) * mr;
-            ax += kp * rx + kv * (vix - DVj.x);
-            ay += kp * ry + kv * (viy - DVj.y);
-            az += kp * rz + kv * (viz - DVj.z);
-        });
+    PairSelf s;
+    s.x = Pi.x; s.y = Pi.y; s.z = Pi.z;
+    s.vx = Vi.x; s.vy = Vi.y; s.vz = Vi.z; s.w = Vi.w;
+    s.ph = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
+    s.rho = Pi.rho; s.p = Pi.p;
+    s.inv_rho = 1.0f / s.rho;
+    if (UNIF) {
+        s.gff = grad_k(0.5f * (s.ph.hf + a.phf.hf));
+        s.gfd = grad_k(0.5f * (s.ph.hf + a.phd.hf));
     }
+    return s;
+}
+
+// One fluid neighbour's term of updateWCSPHAccelerationKernel
+// (WCSPHIntegrationKernel.cu:80-110).
+template <bool UNIF>
+__device__ __forceinline__ void momentum_ff(const SphArgs& a, const PairSelf& s, const FluidRec& R, bool valid,
+                                            float& ax, float& ay, float& az)
+{
+    const FluidPos& Pj = R.P;
+    const float4& Vj = R.V;
+    const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
+    const float rx = (float)(s.x - Pj.x), ry = (float)(s.y - Pj.y), rz = (float)(s.z - Pj.z);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    // e = -r_ij/|r_ij| (zero below |r|^2 = 1e-20, myVec.h:112-128)
+    const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    const float r = r2 * inv_r;
+    const float g = UNIF ? grad_factor(r, s.gff) : grad_factor(r, 0.5f * (s.ph.hf + pj.hf));
+    const float f = valid ? g : 0.0f;
+    const float dvx = s.vx - Vj.x, dvy = s.vy - Vj.y, dvz = s.vz - Vj.z;
+    // U_L - U_R with U_L = v_i . e and U_R = v_j . e
+    const float dU = -(dvx * rx + dvy * ry + dvz * rz) * inv_r;
+    const float cbar = 0.5f * (s.ph.c + pj.c);
+    const float beta = fminf(3.0f * fmaxf(dU, 0.0f), cbar);
+    // P* = (P_L + P_R)/2 + beta rho_bar (U_L - U_R)/2, rho_bar = (rho_i + rho_j)/2
+    const float Ps = 0.5f * (s.p + Pj.p) + 0.25f * beta * (s.rho + Pj.rho) * dU;
+    // f m_j / (rho_i rho_j)
+    const float fm = f * (pj.m * s.inv_rho) * rcp(Pj.rho);
+    const float kp = -2.0f * Ps * fm;
+    // 2 nu_bar (r_hat . gradW) m_j / (rho_i rho_j), r_hat . gradW = f r
+    const float kv = (s.ph.nu + pj.nu) * r * fm;
+    ax += kp * rx + kv * dvx;
+    ay += kp * ry + kv * dvy;
+    az += kp * rz + kv * dvz;
+}
+
+// One dummy neighbour's term (WCSPHIntegrationKernel.cu:112-146).
+template <bool UNIF>
+__device__ __forceinline__ void momentum_fd(const SphArgs& a, const PairSelf& s, const DummyRec& R, int j, bool valid,
+                                            float& ax, float& ay, float& az)
+{
+    const DummyPos& Dj = R.D;
+    const float4& DVj = R.V;
+    const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
+    const float rx = (float)(s.x - Dj.x), ry = (float)(s.y - Dj.y), rz = (float)(s.z - Dj.z);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    const float r = r2 * inv_r;
+    const float g = UNIF ? grad_factor(r, s.gfd) : grad_factor(r, 0.5f * (s.ph.hf + pj.hf));
+    const float f = valid ? g : 0.0f;
+    const float nx = Dj.nx, ny = Dj.ny, nz = DVj.w;
+    const float UL = -(nx * s.vx + ny * s.vy + nz * s.vz);
+    const float UR = -UL - 2.0f * (nx * DVj.x + ny * DVj.y + nz * DVj.z);
+    const float grj = fmaxf(-(a.gx * rx + a.gy * ry + a.gz * rz), 0.0f);
+    const float PR = s.p + s.rho * grj;
+    const float rhoj = dummy_rho(a, pj, PR);
+    const float cbar = 0.5f * (s.ph.c + pj.c), rbar = 0.5f * (s.rho + rhoj);
+    const float beta = fminf(3.0f * fmaxf(UL - UR, 0.0f), cbar);
+    const float Ps = 0.5f * (s.p + PR) + 0.5f * beta * rbar * (UL - UR);
+    const float mr = pj.m * s.inv_rho * rcp(rhoj);
+    const float kp = -2.0f * mr * Ps * f;
+    const float kv = 2.0f * (0.5f * (s.ph.nu + pj.nu)) * (f * r) * mr;
+    ax += kp * rx + kv * (s.vx - DVj.x);
+    ay += kp * ry + kv * (s.vy - DVj.y);
+    az += kp * rz + kv * (s.vz - DVj.z);
+}
+
+// Lane-group sum, coupled balls, gravity, half kick (:150-159) and, in the
+// first half, the drift (:161-170).
+template <bool FIRST, int LANES>
+__device__ __forceinline__ void momentum_finish(const SphArgs& a, const PairSelf& s, int i, int q, float ax, float ay,
+                                                float az)
+{
     if (LANES > 1) {
         ax = lane_group_sum<LANES>(ax);
         ay = lane_group_sum<LANES>(ay);
@@ -795,7 +844,7 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
         float cx = 0.f, cy = 0.f, cz = 0.f;
         for_chunks<LANES>(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, q, 0, [&](int j, bool valid) {
             float3 ap, av, lv;
-            fs_momentum(a, pi, xi, yi, zi, vix, viy, viz, rhoi, PL, a.SP[j], a.SV[j], a.SW[j], ap, av, lv);
+            fs_momentum(a, s.ph, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.rho, s.p, a.SP[j], a.SV[j], a.SW[j], ap, av, lv);
             cx += valid ? ap.x + av.x : 0.0f;
             cy += valid ? ap.y + av.y : 0.0f;
             cz += valid ? ap.z + av.z : 0.0f;
@@ -819,83 +868,255 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
     const double nvy = a.vy[i] + hdt * (double)ay;
     const double nvz = a.vz[i] + hdt * (double)az;
     a.ovx[i] = nvx; a.ovy[i] = nvy; a.ovz[i] = nvz;
-    a.oV[i] = make_float4((float)nvx, (float)nvy, (float)nvz, Vi.w);
+    a.oV[i] = make_float4((float)nvx, (float)nvy, (float)nvz, s.w);
     if (FIRST) {
         FluidPos o;
-        o.x = xi + a.dt * nvx;
-        o.y = yi + a.dt * nvy;
-        o.z = zi + a.dt * nvz;
-        o.rho = rhoi;
-        o.p = PL;
+        o.x = s.x + a.dt * nvx;
+        o.y = s.y + a.dt * nvy;
+        o.z = s.z + a.dt * nvz;
+        o.rho = s.rho;
+        o.p = s.p;
         a.oP[i] = o;
     }
 }
 
+// Entry-split pair loops.  Lane q of a particle's LANES lanes takes entries
+// q*E .. q*E+E-1 of every list chunk (E = 8 / LANES): the lanes of a group
+// read each chunk together and finish within one entry per chunk of each
+// other.  Against the chunk split (lane q takes whole chunks q, q + LANES,
+// ...) this balances 8 lanes over lists of about ten chunks; the first
+// kPreEntries entries of a lane are loaded with the particle's own record,
+// ahead of the gathers they index.
+template <int E>
+__device__ __forceinline__ void load_entries(const int* __restrict__ p, int* v)
+{
+    if (E == 4) {
+        const int4 x = *reinterpret_cast<const int4*>(p);
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    } else if (E == 2) {
+        const int2 x = *reinterpret_cast<const int2*>(p);
+        v[0] = x.x; v[1] = x.y;
+    } else {
+        v[0] = p[0];
+    }
+}
+
+template <int LANES, class LoadF, class UseF>
+__device__ __forceinline__ void for_entries(const int* __restrict__ list, int n, int nf, int i, int q, int safe,
+                                            LoadF&& load, UseF&& use)
+{
+    static_assert(LANES == 2 || LANES == 4 || LANES == 8, "entry-split pair loops take 2, 4 or 8 lanes");
+    constexpr int E = kChunk / LANES;
+    constexpr int CB = E >= kBatch ? 1 : kBatch / E;  // chunks per batch
+    const int* L = list + (size_t)i * kChunk + q * E;
+    const size_t cs = (size_t)nf * kChunk;
+    const int nch = (n + kChunk - 1) / kChunk;
+    for (int c0 = 0; c0 < nch; c0 += CB) {
+        int js[CB * E];
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const int c = c0 + cb;
+            int v[E];
+            if (c < nch) {
+                load_entries<E>(L + (size_t)c * cs, v);
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; e++) v[e] = safe;
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++) js[cb * E + e] = c * kChunk + q * E + e < n ? v[e] : safe;
+        }
+        decltype(load(0)) rec[CB * E];
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) rec[k] = load(js[k]);
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) use(rec[k], js[k], (c0 + k / E) * kChunk + q * E + k % E < n);
+    }
+}
+
+// The first kPreEntries list entries of a lane (E per chunk) within the
+// list capacity, loaded in one go whatever the particle's count.
+constexpr int kPreEntries = 16;
+template <int LANES>
+__device__ __forceinline__ void preload_entries(const int* __restrict__ list, int K, int nf, int i, int q,
+                                                int (&pre)[kPreEntries])
+{
+    constexpr int E = kChunk / LANES;
+    constexpr int PC = kPreEntries / E;
+    const int kch = K / kChunk;  // chunks of capacity (uniform)
+    const int* L = list + (size_t)i * kChunk + q * E;
+    const size_t cs = (size_t)nf * kChunk;
+#pragma unroll
+    for (int c = 0; c < PC; c++) {
+        if (c < kch) {
+            load_entries<E>(L + (size_t)c * cs, &pre[c * E]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) pre[c * E + e] = 0;
+        }
+    }
+}
+
+// for_entries with the first kPreEntries entries of the lane taken from
+// preload_entries.  Same entries, same order.
+template <int LANES, class LoadF, class UseF>
+__device__ __forceinline__ void for_entries_pre(const int (&pre)[kPreEntries], const int* __restrict__ list, int n,
+                                                int nf, int i, int q, int safe, LoadF&& load, UseF&& use)
+{
+    constexpr int E = kChunk / LANES;
+    constexpr int CB = E >= kBatch ? 1 : kBatch / E;  // chunks per batch
+    constexpr int PC = kPreEntries / E;               // preloaded chunks
+    const int nch = (n + kChunk - 1) / kChunk;
+#pragma unroll
+    for (int c0 = 0; c0 < PC; c0 += CB) {
+        if (c0 >= nch) break;
+        int js[CB * E];
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) {
+            const int c = c0 + k / E, e = k % E;
+            js[k] = c * kChunk + q * E + e < n ? pre[c * E + e] : safe;
+        }
+        decltype(load(0)) rec[CB * E];
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) rec[k] = load(js[k]);
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) use(rec[k], js[k], (c0 + k / E) * kChunk + q * E + k % E < n);
+    }
+    if (nch <= PC) return;
+    const int* L = list + (size_t)i * kChunk + q * E;
+    const size_t cs = (size_t)nf * kChunk;
+    for (int c0 = PC; c0 < nch; c0 += CB) {
+        int js[CB * E];
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const int c = c0 + cb;
+            int v[E];
+            load_entries<E>(L + (size_t)min(c, nch - 1) * cs, v);
+#pragma unroll
+            for (int e = 0; e < E; e++) js[cb * E + e] = c * kChunk + q * E + e < n ? v[e] : safe;
+        }
+        decltype(load(0)) rec[CB * E];
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) rec[k] = load(js[k]);
+#pragma unroll
+        for (int k = 0; k < CB * E; k++) use(rec[k], js[k], (c0 + k / E) * kChunk + q * E + k % E < n);
+    }
+}
+
+// updateWCSPHAccelerationKernel (WCSPHIntegrationKernel.cu:42-148) fused
+// with the half kick (:150-159) and, in the first half, the drift (:161-170).
+template <bool FIRST, bool UNIF, int LANES>
+__global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
+{
+    const int g = xcd_block() * kThreads + threadIdx.x;
+    const int i = g / LANES, q = g % LANES;
+    if (i >= a.nf) return;  // uniform over a lane group
+    const PairSelf s = pair_self<UNIF>(a, i);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const auto load_f = [&](int j) { return FluidRec{ld_at(a.P, j), ld_at(a.V, j)}; };
+    const auto load_d = [&](int j) { return DummyRec{ld_at(a.DP, j), ld_at(a.DV, j)}; };
+    const auto use_f = [&](const FluidRec& R, int, bool valid) { momentum_ff<UNIF>(a, s, R, valid, ax, ay, az); };
+    const auto use_d = [&](const DummyRec& R, int j, bool valid) { momentum_fd<UNIF>(a, s, R, j, valid, ax, ay, az); };
+    if constexpr (LANES >= 2) {
+        int pre[kPreEntries];
+        preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
+        for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i, load_f, use_f);
+        if (a.nd > 0) for_entries<LANES>(a.nbr_fd, min(a.cnt_fd[i], a.K_fd), a.nf, i, q, 0, load_d, use_d);
+    } else {
+        for_chunks_gathered<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, load_f, use_f);
+        if (a.nd > 0) for_chunks_gathered<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, load_d, use_d);
+    }
+    momentum_finish<FIRST, LANES>(a, s, i, q, ax, ay, az);
+}
+
+// One fluid neighbour's term of updateWCSPHDensityChangeKernel
+// (WCSPHIntegrationKernel.cu:207-231).  With gradW = f r_ij and
+// e = -r_ij/r: (v_i - v*) . gradW = f [ (v_i - v_j) . r_ij / 2 + (P_L - P_R) r / (2 rho_bar c_bar) ].
+template <bool UNIF>
+__device__ __forceinline__ float continuity_ff(const SphArgs& a, const PairSelf& s, const FluidRec& R, bool valid)
+{
+    const FluidPos& Pj = R.P;
+    const float4& Vj = R.V;
+    const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
+    const float rx = (float)(s.x - Pj.x), ry = (float)(s.y - Pj.y), rz = (float)(s.z - Pj.z);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    const float r = r2 * inv_r;
+    const float g = UNIF ? grad_factor(r, s.gff) : grad_factor(r, 0.5f * (s.ph.hf + pj.hf));
+    const float f = valid ? g : 0.0f;
+    const float cbar = 0.5f * (s.ph.c + pj.c);
+    const float dvr = (s.vx - Vj.x) * rx + (s.vy - Vj.y) * ry + (s.vz - Vj.z) * rz;
+    // (P_L - P_R) r / (rho_bar c_bar), rho_bar = (rho_i + rho_j)/2
+    const float q = (s.p - Pj.p) * r * rcp((s.rho + Pj.rho) * (0.5f * cbar));
+    return (0.5f * pj.m) * f * rcp(Pj.rho) * (dvr + q);
+}
+
+// One dummy neighbour's term (WCSPHIntegrationKernel.cu:233-262).
+template <bool UNIF>
+__device__ __forceinline__ float continuity_fd(const SphArgs& a, const PairSelf& s, const DummyRec& R, int j, bool valid)
+{
+    const DummyPos& Dj = R.D;
+    const float4& DVj = R.V;
+    const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
+    const float rx = (float)(s.x - Dj.x), ry = (float)(s.y - Dj.y), rz = (float)(s.z - Dj.z);
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    const float r = r2 * inv_r;
+    const float g = UNIF ? grad_factor(r, s.gfd) : grad_factor(r, 0.5f * (s.ph.hf + pj.hf));
+    const float f = valid ? g : 0.0f;
+    const float grj = fmaxf(-(a.gx * rx + a.gy * ry + a.gz * rz), 0.0f);
+    const float PR = s.p + s.rho * grj;
+    const float rhoj = dummy_rho(a, pj, PR);
+    const float cbar = 0.5f * (s.ph.c + pj.c), rbar = 0.5f * (s.rho + rhoj);
+    const float t = 0.5f * ((s.vx - DVj.x) * rx + (s.vy - DVj.y) * ry + (s.vz - DVj.z) * rz) +
+                    0.5f * (s.p - PR) * r * rcp(rbar * cbar);
+    return pj.m * rcp(rhoj) * f * t;
+}
+
+// Coupled balls, lane-group sum and the EOS update (:266-282).
+template <int LANES>
+__device__ __forceinline__ void continuity_finish(const SphArgs& a, const PairSelf& s, int i, int q, float d)
+{
+    if (a.ns > 0) {
+        for_chunks<LANES>(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, q, 0, [&](int j, bool valid) {
+            const float t = fs_density(a, s.ph, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.rho, s.p, a.SP[j], a.SV[j], a.SW[j]);
+            d += valid ? t : 0.0f;
+        });
+    }
+    if (LANES > 1) d = lane_group_sum<LANES>(d);
+    if (q != 0) return;
+    const float nd = a.drho[i] + (float)(a.dt * (double)(2.0f * s.rho * d));
+    a.odrho[i] = nd;
+    FluidPos o;
+    o.x = s.x; o.y = s.y; o.z = s.z;
+    eos_eval(a.eos, a.gamma, s.ph, nd, o.rho, o.p);
+    a.oP[i] = o;
+}
+
 // updateWCSPHDensityChangeKernel (WCSPHIntegrationKernel.cu:172-264) fused
-// with the EOS update (:266-282).  With gradW = f r_ij and e = -r_ij/r:
-// (v_i - v*) . gradW = f [ (v_i - v_j) . r_ij / 2 + (P_L - P_R) r / (2 rho_bar c_bar) ].
+// with the EOS update (:266-282).
 template <bool UNIF, int LANES>
 __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
 {
     const int g = xcd_block() * kThreads + threadIdx.x;
     const int i = g / LANES, q = g % LANES;
     if (i >= a.nf) return;  // uniform over a lane group
-    const FluidPos Pi = a.P[i];
-    const float4 Vi = a.V[i];
-    const double xi = Pi.x, yi = Pi.y, zi = Pi.z;
-    const float vix = Vi.x, viy = Vi.y, viz = Vi.z;
-    const Phase pi = UNIF ? a.phf : a.phases[phase_bits(Vi.w)];
-    const float rhoi = Pi.rho, PL = Pi.p;
+    const PairSelf s = pair_self<UNIF>(a, i);
     float d = 0.f;
-    const auto load_f = [&](int j) { return FluidRec{a.P[j], a.V[j]}; };
-    const auto load_d = [&](int j) { return DummyRec{a.DP[j], a.DV[j]}; };
-    for_chunks_gathered<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, load_f, [&](const FluidRec& R, int, bool valid) {
-        const FluidPos& Pj = R.P;
-        const float4& Vj = R.V;
-        const Phase pj = UNIF ? a.phf : a.phases[phase_bits(Vj.w)];
-        const float rx = (float)(xi - Pj.x), ry = (float)(yi - Pj.y), rz = (float)(zi - Pj.z);
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
-        const float r = r2 * inv_r;
-        const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
-        const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + Pj.rho);
-        const float t = 0.5f * ((vix - Vj.x) * rx + (viy - Vj.y) * ry + (viz - Vj.z) * rz) +
-                        0.5f * (PL - Pj.p) * r * rcp(rbar * cbar);
-        d += pj.m * rcp(Pj.rho) * f * t;
-    });
-    if (a.nd > 0) {
-        for_chunks_gathered<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, load_d, [&](const DummyRec& R, int j, bool valid) {
-            const DummyPos& Dj = R.D;
-            const float4& DVj = R.V;
-            const Phase pj = UNIF ? a.phd : a.phases[a.dphase[j]];
-            const float rx = (float)(xi - Dj.x), ry = (float)(yi - Dj.y), rz = (float)(zi - Dj.z);
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
-            const float r = r2 * inv_r;
-            const float f = valid ? grad_factor(r, 0.5f * (pi.hf + pj.hf)) : 0.0f;
-            const float grj = fmaxf(-(a.gx * rx + a.gy * ry + a.gz * rz), 0.0f);
-            const float PR = PL + rhoi * grj;
-            const float rhoj = dummy_rho(a, pj, PR);
-            const float cbar = 0.5f * (pi.c + pj.c), rbar = 0.5f * (rhoi + rhoj);
-            const float t = 0.5f * ((vix - DVj.x) * rx + (viy - DVj.y) * ry + (viz - DVj.z) * rz) +
-                            0.5f * (PL - PR) * r * rcp(rbar * cbar);
-            d += pj.m * rcp(rhoj) * f * t;
-        });
+    const auto load_f = [&](int j) { return FluidRec{ld_at(a.P, j), ld_at(a.V, j)}; };
+    const auto load_d = [&](int j) { return DummyRec{ld_at(a.DP, j), ld_at(a.DV, j)}; };
+    const auto use_f = [&](const FluidRec& R, int, bool valid) { d += continuity_ff<UNIF>(a, s, R, valid); };
+    const auto use_d = [&](const DummyRec& R, int j, bool valid) { d += continuity_fd<UNIF>(a, s, R, j, valid); };
+    if constexpr (LANES >= 2) {
+        int pre[kPreEntries];
+        preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
+        for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i, load_f, use_f);
+        if (a.nd > 0) for_entries<LANES>(a.nbr_fd, min(a.cnt_fd[i], a.K_fd), a.nf, i, q, 0, load_d, use_d);
+    } else {
+        for_chunks_gathered<LANES>(a.nbr_ff, a.cnt_ff[i], a.K_ff, a.nf, i, q, i, load_f, use_f);
+        if (a.nd > 0) for_chunks_gathered<LANES>(a.nbr_fd, a.cnt_fd[i], a.K_fd, a.nf, i, q, 0, load_d, use_d);
     }
-    if (a.ns > 0) {
-        for_chunks<LANES>(a.nbr_fs, a.cnt_fs[i], a.K_fs, a.nf, i, q, 0, [&](int j, bool valid) {
-            const float t = fs_density(a, pi, xi, yi, zi, vix, viy, viz, rhoi, PL, a.SP[j], a.SV[j], a.SW[j]);
-            d += valid ? t : 0.0f;
-        });
-    }
-    if (LANES > 1) d = lane_group_sum<LANES>(d);
-    if (q != 0) return;
-    const float nd = a.drho[i] + (float)(a.dt * (double)(2.0f * rhoi * d));
-    a.odrho[i] = nd;
-    FluidPos o;
-    o.x = xi; o.y = yi; o.z = zi;
-    eos_eval(a.eos, a.gamma, pi, nd, o.rho, o.p);
-    a.oP[i] = o;
+    continuity_finish<LANES>(a, s, i, q, d);
 }
 
 // ----------------------------------------------------------- coupled balls
@@ -1772,16 +1993,14 @@ struct demsph_sph {
         }
     }
 
-    // Lanes per fluid particle in the pair passes.  One lane per particle
-    // keeps every neighbour sum in list order, but each particle's chain of
-    // dependent gathers then bounds the pass; splitting the list over lanes
-    // shortens it.  The fewest of 2, 4, 8 that put 400k threads in flight
-    // (measured on MI355X: 8 lanes at 2.7k-35k particles, 4 at 110k, 2 at
-    // 256k and 741k, where 2 lanes are 20% faster than 1).
+    // Lanes per fluid particle in the pair passes: 4 or 8, the fewer that
+    // put 400k threads in flight (measured on MI355X with the entry split:
+    // 8 lanes at 32k particles, 4 at 256k and 741k, where 4 beat 2 by 5 and
+    // 9 % and 8 by 2 and 4 %).  One lane keeps every sum in list order.
     int pair_lanes() const
     {
         if (pair_lanes_set > 0) return pair_lanes_set;
-        return lanes_for(400000);
+        return lanes_for(400000, 4);
     }
 
     template <bool FIRST, bool U, int L>
@@ -2087,7 +2306,7 @@ struct demsph_sph {
         if (steps < 1 || steps > 1000) throw InvalidArg("trial steps must be in 1..1000");
         if (mode != 0 && mode != 1) throw InvalidArg("mode must be 0 (result-invariant knobs) or 1 (also pair lanes)");
         TuneOutcome out;
-        out.rules = TuneChoice{list_lanes_rule(), 1, lanes_for(400000)};
+        out.rules = TuneChoice{list_lanes_rule(), 1, lanes_for(400000, 4)};
         const TuneKey key = tune_key(mode);
         if (use_cache) {
             std::lock_guard<std::mutex> lk(tune_mutex());
