@@ -214,6 +214,14 @@ def cpu_baseline(spec, budget_s: float = 10.0) -> dict:
             "gpu_vs_oracle": parity}
 
 
+def dem_stream():
+    """A HIP stream (default priority) for a DEM scene: the scene keeps it
+    while it lives."""
+    import torch
+
+    return torch.cuda.Stream()
+
+
 def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     """The DEM granular collapse (4,096 Hertz-Mindlin spheres in a 0.3 m box
     of 12 triangles, dt = 1e-5 s): ball-steps per second over the first
@@ -224,7 +232,10 @@ def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     from this_studio_amd import scenes
 
     s = scenes.granular_collapse()
-    g = ts.DEMSceneGPU(s)
+    # a stream of its own, as a served scene has: whole rebuild cycles replay
+    # as graphs (the null stream cannot be captured)
+    st = dem_stream()
+    g = ts.DEMSceneGPU(s, stream=st.cuda_stream)
     g.synchronize()
     t0 = time.perf_counter()
     g.step(steps)
@@ -232,9 +243,11 @@ def dem_collapse(steps: int, cpu_budget: float, with_cpu: bool) -> dict:
     el = time.perf_counter() - t0
     n = g.sizes()[0]
     out = {"workload": "dem_granular_collapse", "balls": n, "triangles": g.sizes()[3], "steps": steps,
+           "graph_captures": g.graph_captures(),
            "value": round(n * steps / el, 1), "unit": "ball-steps/s", "ms_per_step": round(el / steps * 1e3, 4),
            "precision": "fp32 contact forces, fp64 positions/velocities/springs",
            "roofline": dem_roofline(g, n, el / steps * 1e3)}
+    g.close()  # before its stream goes
     if with_cpu:
         import oracle
 
@@ -268,7 +281,8 @@ def cantilever_clumps(steps: int) -> dict:
     from this_studio_amd import scenes
 
     s = scenes.cantilever_with_clumps()
-    g = ts.DEMSceneGPU(s)
+    st = dem_stream()
+    g = ts.DEMSceneGPU(s, stream=st.cuda_stream)
     g.synchronize()
     t0 = time.perf_counter()
     g.step(steps)

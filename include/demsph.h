@@ -197,7 +197,12 @@ int demsph_sph_graph_captures(demsph_sph* s, long* count);
  * "listPrefilter": 1 (default) tests list candidates first in FP32 on a
  * 16-byte copy of the positions and only those within 1e-3 of the cut (or
  * far from the grid origin) in exact FP64, 0 tests all in FP64; both give
- * the same lists.  demsph_sph_tune_get reads the value in effect. */
+ * the same lists.  "pairRecords": 1 (default) lets the pair passes of a
+ * one-phase scene gather 32-byte records holding each neighbour's position
+ * as an FP32 offset from its cell's origin (its cell coded in the list
+ * entry; the exported lists are unchanged), 0 gathers the FP64 positions;
+ * results differ by FP32 rounding.  demsph_sph_tune_get reads the value in
+ * effect ("pairRecords": 1 only where the records are in use). */
 int demsph_sph_tune(demsph_sph* s, const char* knob, int value);
 int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value);
 

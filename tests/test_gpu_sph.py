@@ -183,6 +183,32 @@ def test_lane_settings_build_identical_lists_and_agree(small):
         assert np.abs(runs[0] - base).max() < 1e-6 * small.dx, L
 
 
+def test_gather_records_agree_with_fp64_records():
+    """The pair passes gather 32-byte records holding each neighbour's
+    position as an FP32 offset from its cell's origin, its cell relative to
+    the particle's stencil centre coded in the list entry (pairRecords 1), or
+    the FP64 positions themselves (pairRecords 0).  The lists export the
+    same; the steps agree to FP32 rounding, and both stay deterministic."""
+    spec = scenes.dam_break(dx=0.01)  # 40 x 40 x 20 fluid
+    runs, lists = {}, {}
+    for mode in (1, 0, 1):
+        s = make_gpu(spec)
+        s.tune("pairRecords", mode)
+        assert s.tuned("pairRecords") == mode
+        s.neighbor_search()
+        got = [s.get(f) for f in ("neighborCount", "objectPointing")]
+        s.step(spec.dt, 40)
+        res = [s.get(f) for f in ("position", "velocity", "densityChange")]
+        if mode in runs:
+            for a, b in zip(res, runs[mode]):
+                assert np.array_equal(a, b), mode
+        runs[mode], lists[mode] = res, got
+    for a, b in zip(lists[0], lists[1]):
+        assert np.array_equal(a, b)
+    assert np.abs(runs[0][0] - runs[1][0]).max() < 1e-6 * spec.dx
+    assert np.abs(runs[0][1] - runs[1][1]).max() < 1e-4
+
+
 def test_autotune_restores_state_and_keeps_results(small):
     """Autotuning times its trials from a checkpoint and restores it: a tuned
     scene steps exactly like an untuned one.  In "exact" mode the result

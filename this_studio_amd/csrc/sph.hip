@@ -82,6 +82,26 @@ struct alignas(32) SolidPos {
 };
 static_assert(sizeof(SolidPos) == 32, "SolidPos is one 32-byte record");
 
+// The 32-byte record a pair pass gathers for a fluid neighbour (uniform
+// scenes): the position as an FP32 offset from the origin of the particle's
+// cell at the last sort, density, pressure and velocity.  Two 16-byte loads
+// per neighbour where the FP64 position, density and pressure (32 bytes)
+// plus the velocity (12) take three; the pair passes issue about one
+// vector-L1 line access per CU per cycle, so the loads per neighbour set
+// their time.  A list entry then carries the neighbour's cell relative to
+// the particle's stencil centre, 2 bits per axis above the slot, and
+// x_i - x_j = (o_i - o_j) - (c_j - c_i) cs: the offsets are below 2 cells,
+// so the difference keeps ~1e-9 m absolute (3e-7 of a typical pair
+// distance), against 3e-8 relative when rounding the FP64 difference.
+struct alignas(32) GRec {
+    float ox, oy, oz, rho;
+    float p, vx, vy, vz;
+};
+static_assert(sizeof(GRec) == 32, "GRec is one 32-byte record");
+constexpr int kCodeShift = 25;                      // entry = slot | code << 25
+constexpr int kSlotMask = (1 << kCodeShift) - 1;    // slots below 2^25
+__host__ __device__ constexpr int pack_cell(int ix, int iy, int iz) { return ix | iy << 11 | iz << 22; }
+
 __device__ __forceinline__ unsigned phase_bits(float w) { return __float_as_uint(w); }
 
 struct SphArgs {
@@ -101,6 +121,12 @@ struct SphArgs {
     // lists
     const int *nbr_ff, *cnt_ff, *nbr_fd, *cnt_fd;
     int K_ff, K_fd;
+    // gather records (null: FP64 records): read and written by the passes,
+    // each particle's packed cell at the last sort and stencil centre
+    const GRec* GRin;
+    GRec* GRout;
+    const int *pcell, *pcen;
+    GridGeom grid;
     // dummies (sorted once)
     const DummyPos* DP;
     const float4* DV;
@@ -291,6 +317,24 @@ __device__ __forceinline__ float4 pos_f32(const GridGeom& g, double R, double x,
     return make_float4((float)rx, (float)ry, (float)rz, far ? 1.f : 0.f);
 }
 
+// Origin of a packed cell, and a particle's gather record in it.
+__device__ __forceinline__ void cell_origin(const GridGeom& g, int pc, double& x, double& y, double& z)
+{
+    x = g.minx + (double)(pc & 2047) * g.cx;
+    y = g.miny + (double)((pc >> 11) & 2047) * g.cy;
+    z = g.minz + (double)((pc >> 22) & 1023) * g.cz;
+}
+__device__ __forceinline__ GRec grec(const GridGeom& g, int pc, const FluidPos& p, const float4& v)
+{
+    double x0, y0, z0;
+    cell_origin(g, pc, x0, y0, z0);
+    GRec r;
+    r.ox = (float)(p.x - x0); r.oy = (float)(p.y - y0); r.oz = (float)(p.z - z0);
+    r.rho = p.rho; r.p = p.p;
+    r.vx = v.x; r.vy = v.y; r.vz = v.z;
+    return r;
+}
+
 struct ReorderArgs {
     int n;
     long long numGrids;
@@ -310,6 +354,8 @@ struct ReorderArgs {
     float4* oPf;  // FP32 positions for the list prefilter (null: none)
     GridGeom grid;
     double R;
+    GRec* oGR;    // gather records and packed cells (null: none)
+    int* opcell;
 };
 
 __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
@@ -328,6 +374,14 @@ __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
     a.odrho[k] = a.drho[s];
     // findStartAndEnd (buildHashStartEnd.cu:14-34)
     const uint32_t h = a.skey[k];
+    if (a.oGR) {
+        const int nxy = a.grid.nx * a.grid.ny;
+        const int hh = (int)h, iz = hh / nxy, r = hh - iz * nxy, iy = r / a.grid.nx, ix = r - iy * a.grid.nx;
+        const int pc = pack_cell(ix, iy, iz);
+        a.opcell[k] = pc;
+        const float4 v = a.V[s];
+        a.oGR[k] = grec(a.grid, pc, p, v);
+    }
     if ((long long)h < a.numGrids) {
         if (k == 0 || a.skey[k - 1] != h) a.cells[h].x = k;
         if (k == a.n - 1 || a.skey[k + 1] != h) a.cells[h].y = k + 1;
@@ -360,6 +414,7 @@ struct ListArgs {
     // around cut^2 in which the exact FP64 test decides
     const float4 *Pf, *DPf;
     float band;
+    int* pcen;  // coded lists: each particle's packed stencil centre (null: plain slots)
 };
 
 enum WalkKind { kFluid = 0, kDummy = 1, kSolid = 2 };
@@ -397,7 +452,7 @@ constexpr int kWalkAhead = DEMSPH_WALK_AHEAD;
 // of each row's last batch.
 constexpr int kLaneAhead = 3;
 
-template <bool UNIF, int KIND, class PosT>
+template <bool UNIF, int KIND, class PosT, bool CODED = false>
 __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restrict__ B,
                                          const int2* __restrict__ cells, double x, double y, double z, double hA,
                                          int self, int* __restrict__ out, int K)
@@ -408,18 +463,20 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
     const size_t cstride = (size_t)a.nf * kChunk;
     const int nchunks = K / kChunk;
     const Stencil st = make_stencil(g, x, y, z);
-    int rs[9], re[9];
+    int rs[9], re[9], rb0[9], rb1[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const int dz = k / 3 - 1, dy = k % 3 - 1;
         const bool row_ok = dz >= st.loz && dz <= st.hiz && dy >= st.loy && dy <= st.hiy;
         const int row = (st.cz + dz) * g.ny * g.nx + (st.cy + dy) * g.nx + st.cx;
         int s = -1, e = -1;
+        int2 rr[3];
 #pragma unroll
         for (int dx = -1; dx <= 1; dx++) {
             const bool ok = row_ok && dx >= st.lox && dx <= st.hix;
             const int2 c = cells[ok ? row + dx : 0];
             const int2 r = ok ? c : make_int2(-1, -1);
+            rr[dx + 1] = r;
             if (r.x != -1) {
                 if (s < 0) s = r.x;
                 e = r.y;
@@ -427,12 +484,16 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
         }
         rs[k] = s < 0 ? 0 : s;
         re[k] = s < 0 ? 0 : e;
+        // a slot's x-cell in the row: -1 below b0, 0 below b1, +1 above
+        rb0[k] = rr[0].x != -1 ? rr[0].y : rs[k];
+        rb1[k] = rr[1].x != -1 ? rr[1].y : rb0[k];
     }
     Chunk buf{0, 0, 0, 0, 0, 0, 0, 0};
     int count = 0;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const int e = re[k];
+        const int rowcode = ((k / 3) << 4) | ((k % 3) << 2);  // (dz + 1) << 4 | (dy + 1) << 2
         for (int j0 = rs[k]; j0 < e; j0 += kWalkAhead) {
             double qx[kWalkAhead], qy[kWalkAhead], qz[kWalkAhead];
 #pragma unroll
@@ -451,7 +512,7 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
                 }
                 if (!within_cut(cut, x - qx[u], y - qy[u], z - qz[u])) continue;
                 const int c = count & (kChunk - 1);
-                buf.put(c, j);
+                buf.put(c, CODED ? j | (rowcode | (j >= rb0[k]) + (j >= rb1[k])) << kCodeShift : j);
                 count++;
                 if (c == kChunk - 1) {
                     const int cc = (count >> 3) - 1;
@@ -481,7 +542,7 @@ __device__ __forceinline__ int walk_list(const ListArgs& a, const PosT* __restri
 // ahead: against nine unrolled rows with all ranges loaded up front, this
 // keeps the kernel under 128 VGPRs and a tenth of the code size, and builds
 // the lists 20 % faster (256k points: 153 vs 191 us).
-template <bool UNIF, int KIND, class PosT, int L, bool PRE>
+template <bool UNIF, int KIND, class PosT, int L, bool PRE, bool CODED = false>
 __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __restrict__ B,
                                                const float4* __restrict__ Bf, const int2* __restrict__ cells,
                                                double x, double y, double z, float4 xf, double hA, int self,
@@ -527,6 +588,10 @@ __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __
             }
         }
         if (s0 < 0) { s0 = 0; e = 0; }
+        // coded lists: a slot's x-cell in the row is -1 below b0, 0 below b1, +1 above
+        const int b0 = cur[0].x != -1 ? cur[0].y : s0;
+        const int b1 = cur[1].x != -1 ? cur[1].y : b0;
+        const int rowcode = ((k / 3) << 4) | ((k % 3) << 2);  // (dz + 1) << 4 | (dy + 1) << 2
         for (int j0 = s0; j0 < e; j0 += L * kLaneAhead) {
             double qx[kLaneAhead], qy[kLaneAhead], qz[kLaneAhead];
             float4 qf[kLaneAhead];
@@ -584,7 +649,10 @@ __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __
                 const unsigned half = hi_half ? (unsigned)(bal >> 32) : (unsigned)bal;
                 const unsigned gm = __builtin_amdgcn_ubfe(half, goff, L);
                 const int kk = count + __popc(gm & ((1u << q) - 1u));
-                if (hit[u] & (kk < K)) out[(unsigned)(kk >> 3) * cstride + (unsigned)(kk & (kChunk - 1))] = j0 + u * L + q;
+                const int j = j0 + u * L + q;
+                if (hit[u] & (kk < K))
+                    out[(unsigned)(kk >> 3) * cstride + (unsigned)(kk & (kChunk - 1))] =
+                        CODED ? j | (rowcode | (j >= b0) + (j >= b1)) << kCodeShift : j;
                 count += __popc(gm);
             }
         }
@@ -592,7 +660,7 @@ __device__ __forceinline__ int walk_list_lanes(const ListArgs& a, const PosT* __
     return count;
 }
 
-template <bool UNIF, int LANES, bool PRE>
+template <bool UNIF, int LANES, bool PRE, bool CODED>
 __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
 {
     const int gt = xcd_block() * kThreads + threadIdx.x;
@@ -602,8 +670,9 @@ __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
     const float4 pf = PRE ? a.Pf[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double h = UNIF ? a.hf : a.phases[phase_bits(a.V[i].w)].h;
     int* of = a.nbr_ff + (size_t)i * kChunk;
-    const int c1 = LANES == 1 ? walk_list<UNIF, kFluid>(a, a.P, a.cells, p.x, p.y, p.z, h, i, of, a.K_ff)
-                              : walk_list_lanes<UNIF, kFluid, FluidPos, LANES, PRE>(
+    const int c1 = LANES == 1 ? walk_list<UNIF, kFluid, FluidPos, CODED>(a, a.P, a.cells, p.x, p.y, p.z, h, i, of,
+                                                                          a.K_ff)
+                              : walk_list_lanes<UNIF, kFluid, FluidPos, LANES, PRE, CODED>(
                                     a, a.P, a.Pf, a.cells, p.x, p.y, p.z, pf, h, i, of, a.K_ff, q);
     int c2 = 0;
     if (a.nd > 0 && a.dummy_near[cell_hash(a.grid, p.x, p.y, p.z)]) {
@@ -615,6 +684,10 @@ __global__ __launch_bounds__(kThreads) void k_build_lists(ListArgs a)
     if (q != 0) return;
     a.cnt_ff[i] = c1;
     a.cnt_fd[i] = c2;
+    if (CODED) {
+        const Stencil st = make_stencil(a.grid, p.x, p.y, p.z);
+        a.pcen[i] = pack_cell(st.cx, st.cy, st.cz);
+    }
     if (c1 > a.K_ff) atomicMax(&a.maxcount[0], c1);
     if (c2 > a.K_fd) atomicMax(&a.maxcount[1], c2);
 }
@@ -745,6 +818,8 @@ struct PairSelf {
     float rho, p, inv_rho;
     Phase ph;
     GradK gff, gfd;  // uniform scenes: kernel constants of fluid and dummy pairs
+    float ox, oy, oz;     // gather records: offset from the stencil centre's origin
+    float csx, csy, csz;  // cell size
 };
 
 template <bool UNIF>
@@ -762,7 +837,57 @@ __device__ __forceinline__ PairSelf pair_self(const SphArgs& a, int i)
         s.gff = grad_k(0.5f * (s.ph.hf + a.phf.hf));
         s.gfd = grad_k(0.5f * (s.ph.hf + a.phd.hf));
     }
+    if (a.GRin) {
+        double x0, y0, z0;
+        cell_origin(a.grid, a.pcen[i], x0, y0, z0);
+        s.ox = (float)(s.x - x0); s.oy = (float)(s.y - y0); s.oz = (float)(s.z - z0);
+        s.csx = (float)a.grid.cx; s.csy = (float)a.grid.cy; s.csz = (float)a.grid.cz;
+    }
     return s;
+}
+
+// momentum_ff on a gather record; code holds the neighbour's cell relative
+// to the stencil centre, 2 bits (offset + 1) per axis.
+__device__ __forceinline__ void momentum_ff_g(const SphArgs& a, const PairSelf& s, const GRec& R, int code, bool valid,
+                                              float& ax, float& ay, float& az)
+{
+    const Phase& pj = a.phf;
+    const float dcx = (float)(code & 3) - 1.0f, dcy = (float)((code >> 2) & 3) - 1.0f, dcz = (float)(code >> 4) - 1.0f;
+    const float rx = (s.ox - R.ox) - dcx * s.csx, ry = (s.oy - R.oy) - dcy * s.csy, rz = (s.oz - R.oz) - dcz * s.csz;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    const float r = r2 * inv_r;
+    const float g = grad_factor(r, s.gff);
+    const float f = valid ? g : 0.0f;
+    const float dvx = s.vx - R.vx, dvy = s.vy - R.vy, dvz = s.vz - R.vz;
+    const float dU = -(dvx * rx + dvy * ry + dvz * rz) * inv_r;
+    const float cbar = 0.5f * (s.ph.c + pj.c);
+    const float beta = fminf(3.0f * fmaxf(dU, 0.0f), cbar);
+    const float Ps = 0.5f * (s.p + R.p) + 0.25f * beta * (s.rho + R.rho) * dU;
+    const float fm = f * (pj.m * s.inv_rho) * rcp(R.rho);
+    const float kp = -2.0f * Ps * fm;
+    const float kv = (s.ph.nu + pj.nu) * r * fm;
+    ax += kp * rx + kv * dvx;
+    ay += kp * ry + kv * dvy;
+    az += kp * rz + kv * dvz;
+}
+
+// continuity_ff on a gather record.
+__device__ __forceinline__ float continuity_ff_g(const SphArgs& a, const PairSelf& s, const GRec& R, int code,
+                                                 bool valid)
+{
+    const Phase& pj = a.phf;
+    const float dcx = (float)(code & 3) - 1.0f, dcy = (float)((code >> 2) & 3) - 1.0f, dcz = (float)(code >> 4) - 1.0f;
+    const float rx = (s.ox - R.ox) - dcx * s.csx, ry = (s.oy - R.oy) - dcy * s.csy, rz = (s.oz - R.oz) - dcz * s.csz;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    const float inv_r = (r2 < 1.e-20f) ? 0.0f : rsq(r2);
+    const float r = r2 * inv_r;
+    const float g = grad_factor(r, s.gff);
+    const float f = valid ? g : 0.0f;
+    const float cbar = 0.5f * (s.ph.c + pj.c);
+    const float dvr = (s.vx - R.vx) * rx + (s.vy - R.vy) * ry + (s.vz - R.vz) * rz;
+    const float q = (s.p - R.p) * r * rcp((s.rho + R.rho) * (0.5f * cbar));
+    return (0.5f * pj.m) * f * rcp(R.rho) * (dvr + q);
 }
 
 // One fluid neighbour's term of updateWCSPHAccelerationKernel
@@ -868,7 +993,8 @@ __device__ __forceinline__ void momentum_finish(const SphArgs& a, const PairSelf
     const double nvy = a.vy[i] + hdt * (double)ay;
     const double nvz = a.vz[i] + hdt * (double)az;
     a.ovx[i] = nvx; a.ovy[i] = nvy; a.ovz[i] = nvz;
-    a.oV[i] = make_float4((float)nvx, (float)nvy, (float)nvz, s.w);
+    const float4 nV = make_float4((float)nvx, (float)nvy, (float)nvz, s.w);
+    a.oV[i] = nV;
     if (FIRST) {
         FluidPos o;
         o.x = s.x + a.dt * nvx;
@@ -877,6 +1003,7 @@ __device__ __forceinline__ void momentum_finish(const SphArgs& a, const PairSelf
         o.rho = s.rho;
         o.p = s.p;
         a.oP[i] = o;
+        if (a.GRout) a.GRout[i] = grec(a.grid, a.pcell[i], o, nV);
     }
 }
 
@@ -1005,7 +1132,7 @@ __device__ __forceinline__ void for_entries_pre(const int (&pre)[kPreEntries], c
 
 // updateWCSPHAccelerationKernel (WCSPHIntegrationKernel.cu:42-148) fused
 // with the half kick (:150-159) and, in the first half, the drift (:161-170).
-template <bool FIRST, bool UNIF, int LANES>
+template <bool FIRST, bool UNIF, int LANES, bool GR = false>
 __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
 {
     const int g = xcd_block() * kThreads + threadIdx.x;
@@ -1017,7 +1144,17 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
     const auto load_d = [&](int j) { return DummyRec{ld_at(a.DP, j), ld_at(a.DV, j)}; };
     const auto use_f = [&](const FluidRec& R, int, bool valid) { momentum_ff<UNIF>(a, s, R, valid, ax, ay, az); };
     const auto use_d = [&](const DummyRec& R, int j, bool valid) { momentum_fd<UNIF>(a, s, R, j, valid, ax, ay, az); };
-    if constexpr (LANES >= 2) {
+    if constexpr (GR) {
+        static_assert(UNIF && LANES >= 2, "gather records: uniform scenes, entry-split lanes");
+        int pre[kPreEntries];
+        preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
+        for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i,
+                               [&](int e) { return ld_at(a.GRin, e & kSlotMask); },
+                               [&](const GRec& R, int e, bool valid) {
+                                   momentum_ff_g(a, s, R, e >> kCodeShift, valid, ax, ay, az);
+                               });
+        if (a.nd > 0) for_entries<LANES>(a.nbr_fd, min(a.cnt_fd[i], a.K_fd), a.nf, i, q, 0, load_d, use_d);
+    } else if constexpr (LANES >= 2) {
         int pre[kPreEntries];
         preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
         for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i, load_f, use_f);
@@ -1091,11 +1228,12 @@ __device__ __forceinline__ void continuity_finish(const SphArgs& a, const PairSe
     o.x = s.x; o.y = s.y; o.z = s.z;
     eos_eval(a.eos, a.gamma, s.ph, nd, o.rho, o.p);
     a.oP[i] = o;
+    if (a.GRout) a.GRout[i] = grec(a.grid, a.pcell[i], o, make_float4(s.vx, s.vy, s.vz, s.w));
 }
 
 // updateWCSPHDensityChangeKernel (WCSPHIntegrationKernel.cu:172-264) fused
 // with the EOS update (:266-282).
-template <bool UNIF, int LANES>
+template <bool UNIF, int LANES, bool GR = false>
 __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
 {
     const int g = xcd_block() * kThreads + threadIdx.x;
@@ -1107,7 +1245,15 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
     const auto load_d = [&](int j) { return DummyRec{ld_at(a.DP, j), ld_at(a.DV, j)}; };
     const auto use_f = [&](const FluidRec& R, int, bool valid) { d += continuity_ff<UNIF>(a, s, R, valid); };
     const auto use_d = [&](const DummyRec& R, int j, bool valid) { d += continuity_fd<UNIF>(a, s, R, j, valid); };
-    if constexpr (LANES >= 2) {
+    if constexpr (GR) {
+        static_assert(UNIF && LANES >= 2, "gather records: uniform scenes, entry-split lanes");
+        int pre[kPreEntries];
+        preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
+        for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i,
+                               [&](int e) { return ld_at(a.GRin, e & kSlotMask); },
+                               [&](const GRec& R, int e, bool valid) { d += continuity_ff_g(a, s, R, e >> kCodeShift, valid); });
+        if (a.nd > 0) for_entries<LANES>(a.nbr_fd, min(a.cnt_fd[i], a.K_fd), a.nf, i, q, 0, load_d, use_d);
+    } else if constexpr (LANES >= 2) {
         int pre[kPreEntries];
         preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
         for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i, load_f, use_f);
@@ -1593,6 +1739,13 @@ struct demsph_sph {
     int* maxcount = nullptr;      // device [2]
     int* maxcount_h = nullptr;    // pinned [2]
     bool lists_valid = false;
+    // 32-byte gather records for the pair passes (uniform scenes): knob,
+    // whether the current lists carry cell codes, the two record buffers,
+    // each particle's packed cell at the last sort and stencil centre
+    bool pair_records = true;
+    bool lists_coded = false;
+    GRec* GR[2] = {};
+    int *pcell = nullptr, *pcen = nullptr;
     // dummies
     DummyPos* DP = nullptr;
     float4* DV = nullptr;
@@ -1661,6 +1814,7 @@ struct demsph_sph {
         dev_free(sort_temp);
         sort_temp = nullptr;
         dfree(cells); dfree(nbr_ff); dfree(cnt_ff); dfree(nbr_fd); dfree(cnt_fd); dfree(Pf); dfree(DPf);
+        dfree(GR[0]); dfree(GR[1]); dfree(pcell); dfree(pcen);
         dfree(maxcount);
         host_free(maxcount_h);
         maxcount_h = nullptr;
@@ -1703,6 +1857,7 @@ struct demsph_sph {
         a.odrho = drho[1 - cD];
         a.nbr_ff = nbr_ff; a.cnt_ff = cnt_ff; a.nbr_fd = nbr_fd; a.cnt_fd = cnt_fd;
         a.K_ff = K_ff; a.K_fd = K_fd;
+        a.pcell = pcell; a.pcen = pcen; a.grid = grid;
         a.DP = DP; a.DV = DV; a.dphase = dphase;
         a.ns = solids.ns; a.SP = solids.SP; a.SV = solids.SV; a.SW = solids.SW;
         a.nbr_fs = solids.nbr_fs; a.cnt_fs = solids.cnt_fs; a.K_fs = solids.K_fs; a.acs = solids.acs;
@@ -1754,6 +1909,7 @@ struct demsph_sph {
         const size_t rec = sizeof(FluidPos) + sizeof(float4) + 3 * sizeof(double) + sizeof(float) + sizeof(int);
         const size_t per_fluid = 2 * rec + rec + sizeof(int)             // two buffers, checkpoint, rank
                                  + sizeof(float4)                         // FP32 list-prefilter copy
+                                 + 2 * sizeof(GRec) + 2 * sizeof(int)     // gather records, cells
                                  + 2 * sizeof(int)                        // list counts
                                  + sizeof(int) * (size_t)(round_chunk(K_ff) + round_chunk(K_fd));
         const size_t per_sort = 3 * sizeof(uint32_t) + 2 * sizeof(int);
@@ -1826,6 +1982,8 @@ struct demsph_sph {
             pref_R = 500.0 * 2.0 * hmin;
         }
         cnt_ff = dmalloc<int>(n); cnt_fd = dmalloc<int>(n);
+        GR[0] = dmalloc<GRec>(n); GR[1] = dmalloc<GRec>(n);
+        pcell = dmalloc<int>(n); pcen = dmalloc<int>(n);
         maxcount = dmalloc<int>(3);
         DEMSPH_HIP(host_malloc((void**)&maxcount_h, 3 * sizeof(int)));
         alloc_lists();
@@ -1945,6 +2103,8 @@ struct demsph_sph {
         ra.odrho = drho[1 - cD];
         ra.cells = cells;
         ra.oPf = prefilter ? Pf : nullptr; ra.grid = grid; ra.R = pref_R;
+        const bool coded = records_usable();
+        ra.oGR = coded ? GR[0] : nullptr; ra.opcell = pcell;
         DEMSPH_LAUNCH(timer, "sph_reorder", k_reorder, gb, tb, 0, stream, ra);
         cP ^= 1; cV ^= 1; cD ^= 1; cO ^= 1;
         cells_track_skey = true;
@@ -1955,6 +2115,8 @@ struct demsph_sph {
         la.nbr_ff = nbr_ff; la.cnt_ff = cnt_ff; la.nbr_fd = nbr_fd; la.cnt_fd = cnt_fd;
         la.K_ff = K_ff; la.K_fd = K_fd; la.maxcount = maxcount;
         la.Pf = Pf; la.DPf = DPf; la.band = kPrefilterBand;
+        la.pcen = coded ? pcen : nullptr;
+        lists_coded = coded;
         if (uniform) lists_lanes<true>(la);
         else lists_lanes<false>(la);
         lists_valid = true;
@@ -1979,8 +2141,24 @@ struct demsph_sph {
     void lists_launch(const ListArgs& la)
     {
         const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
-        if (L > 1 && prefilter) DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, true>), gb, tb, 0, stream, la);
-        else DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, false>), gb, tb, 0, stream, la);
+        if constexpr (U) {
+            if (la.pcen) {
+                if (L > 1 && prefilter)
+                    DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, true, true>), gb, tb, 0, stream, la);
+                else DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, false, true>), gb, tb, 0, stream, la);
+                return;
+            }
+        }
+        if (L > 1 && prefilter)
+            DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, true, false>), gb, tb, 0, stream, la);
+        else DEMSPH_LAUNCH(timer, "sph_lists", (k_build_lists<U, L, false, false>), gb, tb, 0, stream, la);
+    }
+    // Gather records and coded lists need one phase, slots below 2^25 and a
+    // grid of at most 2048 x 2048 x 1024 cells, and the entry-split passes.
+    bool records_usable() const
+    {
+        return pair_records && uniform && nf > 0 && nf <= kSlotMask && grid.nx <= 2048 && grid.ny <= 2048 &&
+               grid.nz <= 1024 && pair_lanes() >= 2;
     }
     template <bool U>
     void lists_lanes(const ListArgs& la)
@@ -2007,6 +2185,12 @@ struct demsph_sph {
     void momentum_launch(const char* name, const SphArgs& a)
     {
         const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
+        if constexpr (U && L >= 2) {
+            if (a.GRin) {
+                DEMSPH_LAUNCH(timer, name, (k_momentum<FIRST, U, L, true>), gb, tb, 0, stream, a);
+                return;
+            }
+        }
         DEMSPH_LAUNCH(timer, name, (k_momentum<FIRST, U, L>), gb, tb, 0, stream, a);
     }
     template <bool FIRST, bool U>
@@ -2023,6 +2207,12 @@ struct demsph_sph {
     void continuity_launch(const SphArgs& a)
     {
         const dim3 gb(blocks_for((long long)nf * L, kThreads)), tb(kThreads);
+        if constexpr (U && L >= 2) {
+            if (a.GRin) {
+                DEMSPH_LAUNCH(timer, "sph_continuity", (k_continuity<U, L, true>), gb, tb, 0, stream, a);
+                return;
+            }
+        }
         DEMSPH_LAUNCH(timer, "sph_continuity", (k_continuity<U, L>), gb, tb, 0, stream, a);
     }
     template <bool U>
@@ -2040,6 +2230,7 @@ struct demsph_sph {
     {
         SphArgs a = args();
         a.dt = dt;
+        if (lists_coded) { a.GRin = GR[0]; a.GRout = GR[1]; }
         if (uniform) momentum_lanes<true, true>("sph_momentum_first", a);
         else momentum_lanes<true, false>("sph_momentum_first", a);
         cP ^= 1; cV ^= 1;
@@ -2049,6 +2240,7 @@ struct demsph_sph {
     {
         SphArgs a = args();
         a.dt = dt;
+        if (lists_coded) { a.GRin = GR[1]; a.GRout = GR[0]; }
         if (uniform) continuity_lanes<true>(a);
         else continuity_lanes<false>(a);
         cP ^= 1; cD ^= 1;
@@ -2058,6 +2250,7 @@ struct demsph_sph {
     {
         SphArgs a = args();
         a.dt = dt;
+        if (lists_coded) a.GRin = GR[0];
         if (uniform) momentum_lanes<false, true>("sph_momentum_second", a);
         else momentum_lanes<false, false>("sph_momentum_second", a);
         cV ^= 1;
@@ -2648,7 +2841,8 @@ extern "C" int demsph_sph_get(demsph_sph* s, const char* field, void* out, size_
                 const int i = rankh[o];
                 const int c = std::min(cnt[i], K);
                 for (int k = 0; k < c; k++) {
-                    const int j = nbr[(((size_t)(k / kChunk) * n + i) * kChunk) + (k % kChunk)];
+                    int j = nbr[(((size_t)(k / kChunk) * n + i) * kChunk) + (k % kChunk)];
+                    if (ff && s->lists_coded) j &= kSlotMask;
                     v.push_back(ff ? orig[j] : dor[j]);
                 }
             }
@@ -2931,6 +3125,12 @@ extern "C" int demsph_sph_tune(demsph_sph* s, const char* knob, int value)
             s->prefilter = value != 0;
             return;
         }
+        if (k == "pairRecords") {
+            if (value != 0 && value != 1) throw InvalidArg("pairRecords must be 0 or 1");
+            s->drop_graphs();
+            s->pair_records = value != 0;
+            return;
+        }
         if (k == "pairLanes" || k == "listLanes") {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
                 throw InvalidArg(k + " must be 0 (automatic), 1, 2, 4 or 8");
@@ -2953,6 +3153,11 @@ extern "C" int demsph_sph_tune_get(demsph_sph* s, const char* knob, int* value)
         }
         if (k == "listPrefilter") {
             *value = s->prefilter ? 1 : 0;
+            return;
+        }
+        if (k == "pairRecords") {
+            // 1 when the gather records are in use (the knob and the scene allow them)
+            *value = s->records_usable() ? 1 : 0;
             return;
         }
         throw InvalidArg("unknown tuning knob '" + k + "'");
