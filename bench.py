@@ -214,9 +214,18 @@ def cpu_baseline(spec, budget_s: float = 10.0) -> dict:
             "gpu_vs_oracle": parity}
 
 
+DEM_NULL_STREAM = False
+
+
+class _NullStream:
+    cuda_stream = None
+
+
 def dem_stream():
-    """A HIP stream (default priority) for a DEM scene: the scene keeps it
-    while it lives."""
+    """A HIP stream (default priority) for a DEM scene, which keeps it while
+    it lives; the null stream (no graphs) with --dem-null-stream."""
+    if DEM_NULL_STREAM:
+        return _NullStream()
     import torch
 
     return torch.cuda.Stream()
@@ -471,11 +480,16 @@ def main() -> int:
                     help="also serve the batched path with fluid scenes autotuned on admission")
     ap.add_argument("--no-autotune-report", action="store_true",
                     help="skip the autotuner trials on the dam break after the timed steps")
+    ap.add_argument("--dem-null-stream", action="store_true",
+                    help="run the DEM legs on the null stream, one launch at a time, as for a kernel trace "
+                         "(rocprofv3 has ended with a host-side SIGSEGV inside their cycle-graph launches)")
     ap.add_argument("--no-state-io", action="store_true",
                     help="skip timing the scene state export and import after the timed steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate HBM counters")
     args = ap.parse_args()
+    global DEM_NULL_STREAM
+    DEM_NULL_STREAM = args.dem_null_stream
     if args.copy_calibration:
         import torch
 

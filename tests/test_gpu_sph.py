@@ -209,6 +209,25 @@ def test_gather_records_agree_with_fp64_records():
     assert np.abs(runs[0][1] - runs[1][1]).max() < 1e-4
 
 
+def test_gather_records_outside_the_grid_match_oracle():
+    """With the grid's top face below the column's top row and the tank's
+    upper dummies, those particles hash into the top layer of cells
+    (clamped) while lying outside it: their gather records hold offsets of
+    more than a cell, and a stencil centre can differ from the hashed cell.
+    Both record forms still match the FP64 oracle over 50 steps."""
+    spec = scenes.small_dam_break()
+    spec.max_bound = np.array(spec.max_bound, dtype=float)
+    spec.max_bound[2] = 0.085  # the column's top row is at z = 0.09
+    o = oracle.OracleSPH(spec)
+    o.step(spec.dt, 50)
+    for mode in (1, 0):
+        s = make_gpu(spec)
+        s.tune("pairRecords", mode)
+        assert s.tuned("pairRecords") == mode
+        s.step(spec.dt, 50)
+        _compare(s, o, 1e-7, 1e-4, 1e-2)
+
+
 def test_autotune_restores_state_and_keeps_results(small):
     """Autotuning times its trials from a checkpoint and restores it: a tuned
     scene steps exactly like an untuned one.  In "exact" mode the result

@@ -385,7 +385,10 @@ struct CopySeg {
     void* dst;
     unsigned long long bytes;
 };
-constexpr int kMaxCopySegs = 160;  // the table travels as a kernel argument (< 4 KB)
+// The table travels as a kernel argument: 128 segments of 24 bytes plus the
+// count and the ~256 bytes of arguments HIP appends stay below the 4 KB
+// kernel-argument segment (160 segments made 4,104 bytes).
+constexpr int kMaxCopySegs = 128;
 struct CopyTable {
     CopySeg s[kMaxCopySegs];
     int n;
