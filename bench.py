@@ -685,7 +685,7 @@ def main() -> int:
                    "dx": spec.dx, "h": 1.3 * spec.dx, "eos": "tait_gamma7", "c0": round(spec.c0, 4),
                    "dt": spec.dt, "pairs_ff": pairs_ff, "pairs_fd": pairs_fd,
                    "step_hbm_GBps": round(step_bytes / (el / args.steps) / 1e9, 1),
-                   "precision": "fp32 pair arithmetic, fp64 position/velocity, fp32 drho"},
+                   "precision": "fp32 pair arithmetic (pair distances from fp32 offsets to cell origins), fp64 position/velocity integration, fp32 drho"},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "full_run": full,
