@@ -297,7 +297,7 @@ def test_prefilter_keeps_pairs_at_the_cut_exact():
     o.neighbor_search()
     want = o.get("pointing_ff")
     for pre in (1, 0):
-        for L in (2, 8):
+        for L in (2, 4, 8):
             s = make_gpu(spec)
             s.tune("listPrefilter", pre)
             s.tune("listLanes", L)
