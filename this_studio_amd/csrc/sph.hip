@@ -93,11 +93,21 @@ static_assert(sizeof(SolidPos) == 32, "SolidPos is one 32-byte record");
 // x_i - x_j = (o_i - o_j) - (c_j - c_i) cs: the offsets are below 2 cells,
 // so the difference keeps ~1e-9 m absolute (3e-7 of a typical pair
 // distance), against 3e-8 relative when rounding the FP64 difference.
+//
+// The two 16-byte halves of the records lie in two arrays of nf float4 (one
+// buffer: {ox, oy, oz, rho} at [0, nf), {p, vx, vy, vz} at [nf, 2 nf)): a
+// lane group's neighbours are mostly consecutive slots, so each gather
+// instruction touches half the cache lines of 32-byte records.
 struct alignas(32) GRec {
     float ox, oy, oz, rho;
     float p, vx, vy, vz;
 };
-static_assert(sizeof(GRec) == 32, "GRec is one 32-byte record");
+static_assert(sizeof(GRec) == 32, "GRec is two 16-byte halves");
+__device__ __forceinline__ void st_grec(float4* base, int half, int k, const GRec& r)
+{
+    base[k] = make_float4(r.ox, r.oy, r.oz, r.rho);
+    base[half + k] = make_float4(r.p, r.vx, r.vy, r.vz);
+}
 constexpr int kCodeShift = 25;                      // entry = slot | code << 25
 constexpr int kSlotMask = (1 << kCodeShift) - 1;    // slots below 2^25
 __host__ __device__ constexpr int pack_cell(int ix, int iy, int iz) { return ix | iy << 11 | iz << 22; }
@@ -123,8 +133,8 @@ struct SphArgs {
     int K_ff, K_fd;
     // gather records (null: FP64 records): read and written by the passes,
     // each particle's packed cell at the last sort and stencil centre
-    const GRec* GRin;
-    GRec* GRout;
+    const float4* GRin;  // split halves, nf float4 apart
+    float4* GRout;
     const int *pcell, *pcen;
     GridGeom grid;
     // dummies (sorted once)
@@ -354,7 +364,7 @@ struct ReorderArgs {
     float4* oPf;  // FP32 positions for the list prefilter (null: none)
     GridGeom grid;
     double R;
-    GRec* oGR;    // gather records and packed cells (null: none)
+    float4* oGR;  // gather records (split halves, n apart) and packed cells (null: none)
     int* opcell;
 };
 
@@ -380,7 +390,7 @@ __global__ __launch_bounds__(kThreads) void k_reorder(ReorderArgs a)
         const int pc = pack_cell(ix, iy, iz);
         a.opcell[k] = pc;
         const float4 v = a.V[s];
-        a.oGR[k] = grec(a.grid, pc, p, v);
+        st_grec(a.oGR, a.n, k, grec(a.grid, pc, p, v));
     }
     if ((long long)h < a.numGrids) {
         if (k == 0 || a.skey[k - 1] != h) a.cells[h].x = k;
@@ -810,6 +820,12 @@ __device__ __forceinline__ T ld_at(const T* __restrict__ base, int j)
 {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)j * (unsigned)sizeof(T));
 }
+// gather record j from its two halves (the second half's base is uniform too)
+__device__ __forceinline__ GRec ld_grec(const float4* __restrict__ base, int half, int j)
+{
+    const float4 a = ld_at(base, j), b = ld_at(base + half, j);
+    return GRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
 
 // Per-particle values of a pair pass.
 struct PairSelf {
@@ -1003,7 +1019,7 @@ __device__ __forceinline__ void momentum_finish(const SphArgs& a, const PairSelf
         o.rho = s.rho;
         o.p = s.p;
         a.oP[i] = o;
-        if (a.GRout) a.GRout[i] = grec(a.grid, a.pcell[i], o, nV);
+        if (a.GRout) st_grec(a.GRout, a.nf, i, grec(a.grid, a.pcell[i], o, nV));
     }
 }
 
@@ -1149,7 +1165,7 @@ __global__ __launch_bounds__(kThreads) void k_momentum(SphArgs a)
         int pre[kPreEntries];
         preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
         for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i,
-                               [&](int e) { return ld_at(a.GRin, e & kSlotMask); },
+                               [&](int e) { return ld_grec(a.GRin, a.nf, e & kSlotMask); },
                                [&](const GRec& R, int e, bool valid) {
                                    momentum_ff_g(a, s, R, e >> kCodeShift, valid, ax, ay, az);
                                });
@@ -1228,7 +1244,7 @@ __device__ __forceinline__ void continuity_finish(const SphArgs& a, const PairSe
     o.x = s.x; o.y = s.y; o.z = s.z;
     eos_eval(a.eos, a.gamma, s.ph, nd, o.rho, o.p);
     a.oP[i] = o;
-    if (a.GRout) a.GRout[i] = grec(a.grid, a.pcell[i], o, make_float4(s.vx, s.vy, s.vz, s.w));
+    if (a.GRout) st_grec(a.GRout, a.nf, i, grec(a.grid, a.pcell[i], o, make_float4(s.vx, s.vy, s.vz, s.w)));
 }
 
 // updateWCSPHDensityChangeKernel (WCSPHIntegrationKernel.cu:172-264) fused
@@ -1250,7 +1266,7 @@ __global__ __launch_bounds__(kThreads) void k_continuity(SphArgs a)
         int pre[kPreEntries];
         preload_entries<LANES>(a.nbr_ff, a.K_ff, a.nf, i, q, pre);
         for_entries_pre<LANES>(pre, a.nbr_ff, min(a.cnt_ff[i], a.K_ff), a.nf, i, q, i,
-                               [&](int e) { return ld_at(a.GRin, e & kSlotMask); },
+                               [&](int e) { return ld_grec(a.GRin, a.nf, e & kSlotMask); },
                                [&](const GRec& R, int e, bool valid) { d += continuity_ff_g(a, s, R, e >> kCodeShift, valid); });
         if (a.nd > 0) for_entries<LANES>(a.nbr_fd, min(a.cnt_fd[i], a.K_fd), a.nf, i, q, 0, load_d, use_d);
     } else if constexpr (LANES >= 2) {
@@ -1744,7 +1760,8 @@ struct demsph_sph {
     // each particle's packed cell at the last sort and stencil centre
     bool pair_records = true;
     bool lists_coded = false;
-    GRec* GR[2] = {};
+    GRec* GR[2] = {};  // nf records each, stored as split halves
+    float4* gr(int b) const { return reinterpret_cast<float4*>(GR[b]); }
     int *pcell = nullptr, *pcen = nullptr;
     // dummies
     DummyPos* DP = nullptr;
@@ -2104,7 +2121,7 @@ struct demsph_sph {
         ra.cells = cells;
         ra.oPf = prefilter ? Pf : nullptr; ra.grid = grid; ra.R = pref_R;
         const bool coded = records_usable();
-        ra.oGR = coded ? GR[0] : nullptr; ra.opcell = pcell;
+        ra.oGR = coded ? gr(0) : nullptr; ra.opcell = pcell;
         DEMSPH_LAUNCH(timer, "sph_reorder", k_reorder, gb, tb, 0, stream, ra);
         cP ^= 1; cV ^= 1; cD ^= 1; cO ^= 1;
         cells_track_skey = true;
@@ -2230,7 +2247,7 @@ struct demsph_sph {
     {
         SphArgs a = args();
         a.dt = dt;
-        if (lists_coded) { a.GRin = GR[0]; a.GRout = GR[1]; }
+        if (lists_coded) { a.GRin = gr(0); a.GRout = gr(1); }
         if (uniform) momentum_lanes<true, true>("sph_momentum_first", a);
         else momentum_lanes<true, false>("sph_momentum_first", a);
         cP ^= 1; cV ^= 1;
@@ -2240,7 +2257,7 @@ struct demsph_sph {
     {
         SphArgs a = args();
         a.dt = dt;
-        if (lists_coded) { a.GRin = GR[1]; a.GRout = GR[0]; }
+        if (lists_coded) { a.GRin = gr(1); a.GRout = gr(0); }
         if (uniform) continuity_lanes<true>(a);
         else continuity_lanes<false>(a);
         cP ^= 1; cD ^= 1;
@@ -2250,7 +2267,7 @@ struct demsph_sph {
     {
         SphArgs a = args();
         a.dt = dt;
-        if (lists_coded) a.GRin = GR[0];
+        if (lists_coded) a.GRin = gr(0);
         if (uniform) momentum_lanes<false, true>("sph_momentum_second", a);
         else momentum_lanes<false, false>("sph_momentum_second", a);
         cV ^= 1;
