@@ -87,6 +87,32 @@ def profiled_traffic(name: str):
     return None, "no profile row"
 
 
+# launches per WCSPH step of each kernel in the profile's table (2 radix passes)
+STEP_KERNELS = {"k_hash": 1, "k_radix_hist": 2, "k_radix_tile_scan": 2, "k_radix_scatter": 2, "k_reorder": 1,
+                "k_build_lists": 1, "k_momentum<true": 1, "k_continuity": 1, "k_momentum<false": 1}
+
+
+def profiled_step_traffic():
+    """HBM bytes of one dam-break step from the committed rocprofv3 summary:
+    per-launch FETCH_SIZE + WRITE_SIZE bytes (copy-calibrated) of every step
+    kernel times its launches per step, or None without the profile."""
+    if not PROFILE.exists():
+        return None
+    total, seen = 0.0, set()
+    for line in PROFILE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) != 6:
+            continue
+        for key, per_step in STEP_KERNELS.items():
+            if cells[0].startswith(key) and key not in seen:
+                try:
+                    total += (float(cells[4]) + float(cells[5])) * 1e6 * per_step
+                except ValueError:
+                    return None
+                seen.add(key)
+    return total if seen == set(STEP_KERNELS) else None
+
+
 def profiled_limiter(name: str, avg_us: float):
     """What bounds a step kernel, from the committed profile's counter table
     (SQ_WAVES, SQ_INSTS_VALU, SQ_WAIT_ANY / SQ_WAVE_CYCLES per dispatch) and
@@ -621,10 +647,17 @@ def main() -> int:
     # particle-step, at the served step time
     per_particle = step_model_bytes(nf, spec.nd)
     step_achieved = per_particle * nf / (el / args.steps) / 1e9
+    measured = None
+    step_traffic = profiled_step_traffic()
+    if step_traffic is not None:
+        measured = {"bytes": round(step_traffic), "per_fluid_particle": round(step_traffic / nf, 1),
+                    "x_model": round(step_traffic / (per_particle * nf), 2),
+                    "GBps_at_this_step_time": round(step_traffic / (el / args.steps) / 1e9, 1),
+                    "source": f"{PROFILE.name}: HBM read + write per launch x launches per step"}
     step_roofline = {"bound": "hbm", "achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(step_achieved / HBM_PEAK_GBPS, 4), "bytes_per_particle_step": round(per_particle, 1),
                      "model": "3*40 + 52 + 44 + 108*Nd/Nf compulsory bytes per fluid particle-step",
-                     "measured_traffic_per_step": None}
+                     "measured_traffic_per_step": measured}
     full = None
     if args.full_run:
         full = full_dam_break(spec)
